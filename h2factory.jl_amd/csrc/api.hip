@@ -1,0 +1,770 @@
+// C ABI of libh2g: core / workspace lifetime, validation, traversal plans and the matvec.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "h2g_internal.h"
+
+namespace h2g {
+
+void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+
+namespace {
+thread_local std::string g_last_error;
+
+constexpr int VERSION = 100;  // 0.1.0
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return H2G_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return H2G_EINVAL;
+    }
+}
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+bool is_complex_dtype(int d) { return d == H2G_C64 || d == H2G_C32; }
+
+// ------------------------------------------------------------------------------------------
+// Validation (host only, before any device call)
+// ------------------------------------------------------------------------------------------
+void validate_side(const h2g_side_desc &s, const char *name, int K) {
+    std::string n(name);
+    if (s.n_dofs < 0 || s.n_levels < 1 || s.n_boxes < 1) fail(H2G_EINVAL, n + ": empty tree");
+    if (!s.perm || !s.level_offsets || !s.parent || !s.sector || !s.dof_begin || !s.dof_end || !s.is_leaf ||
+        !s.transfers)
+        fail(H2G_EINVAL, n + ": missing array");
+    if (s.level_offsets[0] != 0 || s.level_offsets[s.n_levels] != s.n_boxes)
+        fail(H2G_EDIM, n + ": level_offsets must run from 0 to n_boxes");
+    for (int l = 0; l < s.n_levels; ++l)
+        if (s.level_offsets[l + 1] < s.level_offsets[l]) fail(H2G_EINVAL, n + ": level_offsets not monotone");
+    std::vector<uint8_t> seen(s.n_dofs, 0);
+    for (int64_t i = 0; i < s.n_dofs; ++i) {
+        if (s.perm[i] < 0 || s.perm[i] >= s.n_dofs || seen[s.perm[i]]) fail(H2G_EINVAL, n + ": perm is not a permutation");
+        seen[s.perm[i]] = 1;
+    }
+    for (int l = 0; l < s.n_levels; ++l)
+        for (int b = s.level_offsets[l]; b < s.level_offsets[l + 1]; ++b) {
+            if (s.dof_begin[b] < 0 || s.dof_end[b] < s.dof_begin[b] || s.dof_end[b] > s.n_dofs)
+                fail(H2G_EDIM, n + ": box dof range out of bounds");
+            if (l == 0) {
+                if (s.parent[b] != -1) fail(H2G_EINVAL, n + ": level-0 box with a parent");
+                continue;
+            }
+            const int p = s.parent[b];
+            if (p < s.level_offsets[l - 1] || p >= s.level_offsets[l]) fail(H2G_EINVAL, n + ": parent not on the level above");
+            if (s.sector[b] < 0 || s.sector[b] > 7) fail(H2G_EINVAL, n + ": sector out of 0..7");
+            if (s.is_leaf[p]) fail(H2G_EINVAL, n + ": a leaf has children");
+            if (s.dof_begin[b] < s.dof_begin[p] || s.dof_end[b] > s.dof_end[p]) fail(H2G_EINVAL, n + ": child range outside parent");
+            if (b > s.level_offsets[l] && s.parent[b] < s.parent[b - 1]) fail(H2G_EINVAL, n + ": children not grouped by parent");
+        }
+    if (s.level_offsets[1] != 1) fail(H2G_EINVAL, n + ": level 0 must hold exactly one root box");
+    // leaves tile the dofs
+    std::vector<uint8_t> cover(s.n_dofs, 0);
+    for (int b = 0; b < s.n_boxes; ++b)
+        if (s.is_leaf[b])
+            for (int64_t j = s.dof_begin[b]; j < s.dof_end[b]; ++j) {
+                if (cover[j]) fail(H2G_EINVAL, n + ": leaves overlap");
+                cover[j] = 1;
+            }
+    for (int64_t j = 0; j < s.n_dofs; ++j)
+        if (!cover[j]) fail(H2G_EINVAL, n + ": leaves do not cover every dof");
+    (void)K;
+}
+
+void validate_desc(const h2g_core_desc &d) {
+    if (d.dtype != H2G_F64 && d.dtype != H2G_C64) fail(H2G_EINVAL, "core dtype must be H2G_F64 or H2G_C64");
+    if (d.K < 1) fail(H2G_EINVAL, "K must be positive");
+    if (d.m2l_kind < 0 || d.m2l_kind > 3) fail(H2G_EINVAL, "unknown m2l_kind");
+    const bool toep = d.m2l_kind == H2G_M2L_TOEPLITZ || d.m2l_kind == H2G_M2L_SYMTOEPLITZ;
+    const bool sym = d.m2l_kind == H2G_M2L_SYMDENSE || d.m2l_kind == H2G_M2L_SYMTOEPLITZ;
+    if (toep) {
+        if (d.p < 2 || d.p * d.p * d.p != d.K)
+            fail(H2G_EINVAL, "Toeplitz couplings need K == p^3 with p >= 2 (ToeplitzModulation)");
+        if (d.p > 6) fail(H2G_EUNSUPPORTED, "Toeplitz couplings support p <= 6");
+    }
+    validate_side(d.test, "test", d.K);
+    const h2g_side_desc &tr = d.galerkin ? d.test : d.trial;
+    if (!d.galerkin) validate_side(d.trial, "trial", d.K);
+    if (d.n_pairs < 0 || (d.n_pairs > 0 && (!d.pair_test || !d.pair_trial || !d.pair_op)))
+        fail(H2G_EINVAL, "missing coupling pair arrays");
+    if (d.n_pairs > 0 && d.n_ops < 1) fail(H2G_EINVAL, "couplings without operators");
+    if (d.n_ops > 0 && !d.ops) fail(H2G_EINVAL, "missing coupling operators");
+    const int L = 2 * d.p - 1;
+    const int64_t plen = d.m2l_kind == H2G_M2L_SYMDENSE ? d.K : (int64_t)L * L * L;
+    if (sym) {
+        if (d.n_pairs > 0 && (!d.pair_sym || d.n_perms < 1 || !d.perms))
+            fail(H2G_EINVAL, "symmetry-reduced couplings need pair_sym and perms");
+        for (int64_t q = 0; q < (int64_t)d.n_perms * plen; ++q)
+            if (d.perms[q] < 0 || d.perms[q] >= plen)
+                fail(H2G_EDIM, "permutation entry out of range: the permutation was built for a different grid");
+        for (int q = 0; q < d.n_perms; ++q) {
+            std::vector<uint8_t> seen(plen, 0);
+            for (int64_t i = 0; i < plen; ++i) {
+                int v = d.perms[q * plen + i];
+                if (seen[v]) fail(H2G_EINVAL, "perms entry is not a permutation");
+                seen[v] = 1;
+            }
+        }
+    }
+    for (int64_t k = 0; k < d.n_pairs; ++k) {
+        const int b = d.pair_test[k], c = d.pair_trial[k];
+        if (b < 0 || b >= d.test.n_boxes || c < 0 || c >= tr.n_boxes) fail(H2G_EINVAL, "pair box out of range");
+        if (d.pair_op[k] < 0 || d.pair_op[k] >= d.n_ops) fail(H2G_EINVAL, "pair operator id out of range");
+        if (sym && (d.pair_sym[k] < 0 || d.pair_sym[k] >= d.n_perms)) fail(H2G_EINVAL, "pair permutation id out of range");
+        if (toep) {
+            int lb = 0, lc = 0;
+            while (b >= d.test.level_offsets[lb + 1]) ++lb;
+            while (c >= tr.level_offsets[lc + 1]) ++lc;
+            if (lb != lc) fail(H2G_EUNSUPPORTED, "Toeplitz couplings must join boxes of one level");
+        }
+    }
+    if (d.n_near_blocks < 0 || (d.n_near_blocks > 0 && (!d.near_test || !d.near_trial || !d.near_offset)))
+        fail(H2G_EINVAL, "missing near-field arrays");
+    for (int64_t q = 0; q < d.n_near_blocks; ++q) {
+        const int b = d.near_test[q], c = d.near_trial[q];
+        if (b < 0 || b >= d.test.n_boxes || c < 0 || c >= tr.n_boxes || !d.test.is_leaf[b] || !tr.is_leaf[c])
+            fail(H2G_EINVAL, "near block must join a test leaf and a trial leaf");
+        const int64_t nr = d.test.dof_end[b] - d.test.dof_begin[b], nc = tr.dof_end[c] - tr.dof_begin[c];
+        if (d.near_offset[q] < 0 || d.near_offset[q] + nr * nc > d.n_near_values)
+            fail(H2G_EDIM, "near block values out of range");
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Upload helpers
+// ------------------------------------------------------------------------------------------
+void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
+    S.n_dofs = d.n_dofs;
+    S.n_levels = d.n_levels;
+    S.n_boxes = d.n_boxes;
+    S.level_offsets.assign(d.level_offsets, d.level_offsets + d.n_levels + 1);
+    S.parent.assign(d.parent, d.parent + d.n_boxes);
+    S.sector.assign(d.sector, d.sector + d.n_boxes);
+    S.begin.assign(d.dof_begin, d.dof_begin + d.n_boxes);
+    S.end.assign(d.dof_end, d.dof_end + d.n_boxes);
+    S.is_leaf.assign(d.is_leaf, d.is_leaf + d.n_boxes);
+    S.perm.assign(d.perm, d.perm + d.n_dofs);
+    S.d_perm.upload_vec(S.perm);
+    S.d_begin.upload_vec(S.begin);
+    S.d_end.upload_vec(S.end);
+    const size_t es = elem_size(dtype);
+    const size_t ncoll = (size_t)K * d.n_dofs, ntr = (size_t)d.n_levels * 8 * K * K;
+    if (d.collection)
+        S.coll.upload(d.collection, ncoll * es);
+    else
+        S.coll.alloc(ncoll * es);
+    S.transfers.upload(d.transfers, ntr * es);
+    if (want32) {
+        auto narrow = [&](const void *src, size_t n, DevBuf &dst) {
+            if (dtype == H2G_F64) {
+                std::vector<float> v(n);
+                const double *s = (const double *)src;
+                for (size_t i = 0; i < n; ++i) v[i] = (float)s[i];
+                dst.upload_vec(v);
+            } else {
+                std::vector<float> v(2 * n);
+                const double *s = (const double *)src;
+                for (size_t i = 0; i < 2 * n; ++i) v[i] = (float)s[i];
+                dst.upload_vec(v);
+            }
+        };
+        if (d.collection)
+            narrow(d.collection, ncoll, S.coll32);
+        else
+            S.coll32.alloc(ncoll * es / 2);
+        narrow(d.transfers, ntr, S.transfers32);
+    }
+}
+
+int level_of(const std::vector<int32_t> &off, int b) {
+    return (int)(std::upper_bound(off.begin(), off.end(), b) - off.begin()) - 1;
+}
+
+void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, const int32_t *recv_of_pair,
+                  const int32_t *part_of_pair, const int32_t *pair_op, const int32_t *pair_sym, int64_t n_pairs) {
+    P.src = &src;
+    P.dst = &dst;
+    // M2L CSR over receiving (dst) boxes, partners sorted for a deterministic order
+    std::vector<int64_t> cnt(dst.n_boxes + 1, 0);
+    for (int64_t k = 0; k < n_pairs; ++k) cnt[recv_of_pair[k] + 1]++;
+    std::partial_sum(cnt.begin(), cnt.end(), cnt.begin());
+    std::vector<int64_t> order(n_pairs), fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t k = 0; k < n_pairs; ++k) order[fill[recv_of_pair[k]]++] = k;
+    for (int b = 0; b < dst.n_boxes; ++b)
+        std::sort(order.begin() + cnt[b], order.begin() + cnt[b + 1],
+                  [&](int64_t a, int64_t q) { return part_of_pair[a] < part_of_pair[q]; });
+    std::vector<uint8_t> has_m2l(dst.n_boxes, 0), partner(src.n_boxes, 0);
+    for (int64_t k = 0; k < n_pairs; ++k) {
+        has_m2l[recv_of_pair[k]] = 1;
+        partner[part_of_pair[k]] = 1;
+    }
+    std::vector<uint8_t> need(src.n_boxes, 0), init(dst.n_boxes, 0);
+    for (int b = 0; b < src.n_boxes; ++b) need[b] = partner[b] || (src.parent[b] >= 0 && need[src.parent[b]]);
+    for (int b = 0; b < dst.n_boxes; ++b) init[b] = has_m2l[b] || (dst.parent[b] >= 0 && init[dst.parent[b]]);
+
+    std::vector<int32_t> lists;
+    auto span_of = [&](std::vector<Span> &v, int nl, auto pred, const Side &S) {
+        v.assign(nl, Span{});
+        for (int l = 0; l < nl; ++l) {
+            v[l].begin = (int64_t)lists.size();
+            for (int b = S.level_offsets[l]; b < S.level_offsets[l + 1]; ++b)
+                if (pred(b)) lists.push_back(b);
+            v[l].count = (int32_t)((int64_t)lists.size() - v[l].begin);
+        }
+    };
+    span_of(P.p2m, src.n_levels, [&](int b) { return need[b] && src.is_leaf[b]; }, src);
+    span_of(P.m2m, src.n_levels, [&](int b) { return need[b] && !src.is_leaf[b]; }, src);
+    span_of(P.m2l_recv, dst.n_levels, [&](int b) { return (bool)has_m2l[b]; }, dst);
+    span_of(P.l2l, dst.n_levels, [&](int b) { return dst.parent[b] >= 0 && init[dst.parent[b]]; }, dst);
+    span_of(P.l2p, dst.n_levels, [&](int b) { return init[b] && dst.is_leaf[b]; }, dst);
+    P.lists.upload_vec(lists);
+
+    // M2M items: children of a box are contiguous on the next level
+    std::vector<std::vector<int32_t>> kids(src.n_boxes);
+    for (int ch = 1; ch < src.n_boxes; ++ch)
+        if (src.parent[ch] >= 0) kids[src.parent[ch]].push_back(ch);
+    std::vector<int64_t> mptr;
+    std::vector<int32_t> msrc, mmat;
+    P.m2m_ptr_base.assign(src.n_levels, 0);
+    for (int l = 0; l < src.n_levels; ++l) {
+        P.m2m_ptr_base[l] = (int64_t)mptr.size();
+        for (int i = 0; i < P.m2m[l].count; ++i) {
+            const int b = lists[P.m2m[l].begin + i];
+            mptr.push_back((int64_t)msrc.size());
+            for (int ch : kids[b]) {
+                msrc.push_back(ch);
+                mmat.push_back((l + 1) * 8 + src.sector[ch]);
+            }
+        }
+        mptr.push_back((int64_t)msrc.size());
+    }
+    P.m2m_ptr.upload_vec(mptr);
+    P.m2m_src.upload_vec(msrc);
+    P.m2m_mat.upload_vec(mmat);
+
+    // L2L items (one per target)
+    std::vector<int32_t> lsrc, lmat;
+    std::vector<uint8_t> lacc;
+    std::vector<int64_t> lptr;
+    P.l2l_base.assign(dst.n_levels, 0);
+    for (int l = 0; l < dst.n_levels; ++l) {
+        P.l2l_base[l] = (int64_t)lptr.size();
+        for (int i = 0; i < P.l2l[l].count; ++i) {
+            const int b = lists[P.l2l[l].begin + i];
+            lptr.push_back((int64_t)lsrc.size());
+            lsrc.push_back(dst.parent[b]);
+            lmat.push_back(l * 8 + dst.sector[b]);
+            lacc.push_back(has_m2l[b]);
+        }
+        lptr.push_back((int64_t)lsrc.size());
+    }
+    P.l2l_src.upload_vec(lsrc);
+    P.l2l_mat.upload_vec(lmat);
+    P.l2l_acc.upload_vec(lacc);  // indexed by item: one item per L2L target
+    P.l2l_ptr.upload_vec(lptr);
+
+    // M2L items
+    std::vector<int64_t> iptr;
+    std::vector<int32_t> isrc, iop, isym;
+    P.m2l_base.assign(dst.n_levels, 0);
+    P.min_level = dst.n_levels;
+    P.max_level = -1;
+    P.max_recv_level_boxes = 0;
+    P.max_src_level_boxes = 0;
+    for (int l = 0; l < dst.n_levels; ++l) {
+        P.m2l_base[l] = (int64_t)iptr.size();
+        for (int i = 0; i < P.m2l_recv[l].count; ++i) {
+            const int b = lists[P.m2l_recv[l].begin + i];
+            iptr.push_back((int64_t)isrc.size());
+            for (int64_t q = cnt[b]; q < cnt[b + 1]; ++q) {
+                const int64_t k = order[q];
+                isrc.push_back(part_of_pair[k]);
+                iop.push_back(pair_op[k]);
+                if (pair_sym) isym.push_back(pair_sym[k]);
+            }
+        }
+        iptr.push_back((int64_t)isrc.size());
+        if (P.m2l_recv[l].count > 0) {
+            P.min_level = std::min(P.min_level, l);
+            P.max_level = std::max(P.max_level, l);
+            P.max_recv_level_boxes = std::max<int64_t>(P.max_recv_level_boxes, P.m2l_recv[l].count);
+            if (l < src.n_levels)
+                P.max_src_level_boxes =
+                    std::max<int64_t>(P.max_src_level_boxes, src.level_offsets[l + 1] - src.level_offsets[l]);
+        }
+    }
+    P.m2l_ptr.upload_vec(iptr);
+    P.m2l_src.upload_vec(isrc);
+    P.m2l_op.upload_vec(iop);
+    if (pair_sym) P.m2l_sym.upload_vec(isym);
+
+    // output mask: dofs of initialised dst leaves
+    std::vector<uint8_t> mask(dst.n_dofs, 0);
+    for (int b = 0; b < dst.n_boxes; ++b)
+        if (init[b] && dst.is_leaf[b])
+            for (int64_t j = dst.begin[b]; j < dst.end[b]; ++j) mask[j] = 1;
+    P.out_mask.upload_vec(mask);
+}
+
+void build_near(Core &c, const h2g_core_desc &d) {
+    const h2g_side_desc &tr = d.galerkin ? d.test : d.trial;
+    const int64_t nb = d.n_near_blocks;
+    c.n_near_blocks = nb;
+    c.n_near_values = d.n_near_values;
+    if (d.near_values)
+        c.near_vals.upload(d.near_values, (size_t)d.n_near_values * elem_size(d.dtype));
+    else
+        c.near_vals.alloc((size_t)d.n_near_values * elem_size(d.dtype));
+    std::vector<int64_t> rb(nb), cb(nb), off(nb);
+    std::vector<int32_t> nr(nb), nc(nb);
+    for (int64_t q = 0; q < nb; ++q) {
+        const int b = d.near_test[q], cc = d.near_trial[q];
+        rb[q] = d.test.dof_begin[b];
+        cb[q] = tr.dof_begin[cc];
+        nr[q] = (int32_t)(d.test.dof_end[b] - d.test.dof_begin[b]);
+        nc[q] = (int32_t)(tr.dof_end[cc] - tr.dof_begin[cc]);
+        off[q] = d.near_offset[q];
+    }
+    c.near_blk_rb.upload_vec(rb);
+    c.near_blk_cb.upload_vec(cb);
+    c.near_blk_nr.upload_vec(nr);
+    c.near_blk_nc.upload_vec(nc);
+    c.near_blk_off.upload_vec(off);
+    auto csr = [&](const int32_t *key, int nkeys, DevBuf &tgt, DevBuf &ptr, DevBuf &blk, int32_t &ntgt) {
+        std::vector<int64_t> cnt(nkeys + 1, 0);
+        for (int64_t q = 0; q < nb; ++q) cnt[key[q] + 1]++;
+        std::partial_sum(cnt.begin(), cnt.end(), cnt.begin());
+        std::vector<int32_t> order(nb);
+        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+        for (int64_t q = 0; q < nb; ++q) order[fill[key[q]]++] = (int32_t)q;
+        std::vector<int32_t> targets;
+        std::vector<int64_t> p;
+        for (int k = 0; k < nkeys; ++k)
+            if (cnt[k + 1] > cnt[k]) {
+                targets.push_back(k);
+                p.push_back(cnt[k]);
+            }
+        p.push_back(nb);
+        ntgt = (int32_t)targets.size();
+        tgt.upload_vec(targets);
+        ptr.upload_vec(p);
+        blk.upload_vec(order);
+    };
+    csr(d.near_test, d.test.n_boxes, c.near_fwd_tgt, c.near_fwd_ptr, c.near_fwd_blk, c.n_near_fwd_tgt);
+    csr(d.near_trial, tr.n_boxes, c.near_rev_tgt, c.near_rev_ptr, c.near_rev_blk, c.n_near_rev_tgt);
+}
+
+// ------------------------------------------------------------------------------------------
+// Far pass
+// ------------------------------------------------------------------------------------------
+bool is_toeplitz(const Core &c) { return c.m2l_kind == H2G_M2L_TOEPLITZ || c.m2l_kind == H2G_M2L_SYMTOEPLITZ; }
+
+void far_pass(Workspace &w, int op, int R) {
+    Core &c = *w.core;
+    const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
+    const Side &src = *P.src, &dst = *P.dst;
+    const bool mixed = w.opts.precision == H2G_PREC_MIXED;
+    const int cd = mixed ? (c.dtype == H2G_F64 ? H2G_F32 : H2G_C32) : c.dtype;
+    const bool conj = op == H2G_OP_C;
+    const int32_t *L = P.lists.as<int32_t>();
+    hipStream_t s = w.stream;
+    const void *scoll = mixed ? src.coll32.ptr : src.coll.ptr;
+    const void *dcoll = mixed ? dst.coll32.ptr : dst.coll.ptr;
+    const void *str = mixed ? src.transfers32.ptr : src.transfers.ptr;
+    const void *dtr = mixed ? dst.transfers32.ptr : dst.transfers.ptr;
+    const void *ops = mixed ? c.ops32.ptr : c.ops.ptr;
+    const int K = c.K;
+
+    // upward pass
+    for (int l = src.n_levels - 1; l >= 0; --l) {
+        launch_p2m(cd, conj, L + P.p2m[l].begin, P.p2m[l].count, src.d_begin.as<int64_t>(), src.d_end.as<int64_t>(),
+                   scoll, K, w.xs.ptr, R, w.mom.ptr, s);
+        launch_boxgemm(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin, P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l],
+                       P.m2m_src.as<int32_t>(), P.m2m_mat.as<int32_t>(), nullptr, nullptr, nullptr, str, K, w.mom.ptr,
+                       w.mom.ptr, R, 0, nullptr, s);
+    }
+    // couplings
+    const int opA = op == H2G_OP_N ? 0 : (op == H2G_OP_T ? 1 : 2);
+    const bool sym = c.m2l_kind == H2G_M2L_SYMDENSE || c.m2l_kind == H2G_M2L_SYMTOEPLITZ;
+    for (int l = 0; l < dst.n_levels; ++l) {
+        const int n = P.m2l_recv[l].count;
+        if (n == 0) continue;
+        const int32_t *recv = L + P.m2l_recv[l].begin;
+        const int64_t *iptr = P.m2l_ptr.as<int64_t>() + P.m2l_base[l];
+        if (!is_toeplitz(c)) {
+            launch_boxgemm(cd, opA, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
+                           sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), c.perms_inv.as<int32_t>(),
+                           ops, K, w.mom.ptr, w.loc.ptr, R, 0, nullptr, s);
+        } else {
+            // forward / adjoint: fft in, ifft out; transpose swaps the two plans
+            const bool inv_in = op == H2G_OP_T;
+            if (l >= src.n_levels) continue;
+            const int box0 = src.level_offsets[l];
+            const int nsrc = src.level_offsets[l + 1] - box0;
+            launch_fft_in(cd, inv_in, nullptr, nsrc, box0, w.mom.ptr, K, c.p, c.L, R, w.spec_in.ptr, c.twiddle.ptr, s);
+            launch_hadamard(cd, conj, n, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
+                            sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), ops, c.L3, box0,
+                            w.spec_in.ptr, w.spec_out.ptr, R, s);
+            launch_fft_out(cd, !inv_in, recv, n, w.spec_out.ptr, K, c.p, c.L, R, w.loc.ptr, c.twiddle.ptr, s);
+        }
+    }
+    // downward pass
+    const cplx<double> a = conj ? conjv(c.alpha) : c.alpha;
+    for (int l = 0; l < dst.n_levels; ++l) {
+        launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
+                       P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(), P.l2l_mat.as<int32_t>(),
+                       nullptr, nullptr, nullptr, dtr, K, w.loc.ptr, w.loc.ptr, R, 2, P.l2l_acc.as<uint8_t>(), s);
+        launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(), dst.d_end.as<int64_t>(),
+                   dcoll, K, w.loc.ptr, R, w.ys.ptr, a, s);
+    }
+}
+
+}  // namespace
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int64_t Core::device_bytes() const {
+    int64_t b = 0;
+    auto side = [&](const Side &S) {
+        b += S.d_perm.bytes + S.d_begin.bytes + S.d_end.bytes + S.coll.bytes + S.coll32.bytes + S.transfers.bytes +
+             S.transfers32.bytes;
+    };
+    side(test);
+    if (!galerkin) side(trial_own);
+    b += ops.bytes + ops32.bytes + perms.bytes + perms_inv.bytes + near_vals.bytes;
+    return b;
+}
+
+int64_t Workspace::device_bytes() const {
+    return xs.bytes + ys.bytes + mom.bytes + loc.bytes + spec_in.bytes + spec_out.bytes + xhost.bytes + yhost.bytes;
+}
+
+}  // namespace h2g
+
+using namespace h2g;
+
+extern "C" {
+
+int h2g_version(void) { return VERSION; }
+
+int h2g_last_error(char *buf, size_t len) {
+    if (!buf || len == 0) return H2G_EINVAL;
+    std::snprintf(buf, len, "%s", g_last_error.c_str());
+    return H2G_OK;
+}
+
+int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
+    return guarded([&] {
+        if (!desc || !out) fail(H2G_EINVAL, "null argument");
+        *out = nullptr;
+        validate_desc(*desc);
+        const h2g_core_desc &d = *desc;
+        int ndev = 0;
+        H2G_HIP(hipGetDeviceCount(&ndev));
+        if (d.device < 0 || d.device >= ndev) fail(H2G_EINVAL, "device ordinal out of range");
+        H2G_HIP(hipSetDevice(d.device));
+        auto *c = new h2g_core();
+        try {
+            c->device = d.device;
+            c->dtype = d.dtype;
+            c->K = d.K;
+            c->p = d.p;
+            c->L = d.p > 0 ? 2 * d.p - 1 : 0;
+            c->L3 = c->L * c->L * c->L;
+            c->galerkin = d.galerkin != 0;
+            c->m2l_kind = d.m2l_kind;
+            c->n_pairs = d.n_pairs;
+            c->n_ops = d.n_ops;
+            c->n_perms = d.n_perms;
+            c->alpha = {d.alpha_re, d.alpha_im};
+            c->precisions = d.precisions ? d.precisions : (1 << H2G_PREC_PLAIN);
+            const bool want32 = (c->precisions & (1 << H2G_PREC_MIXED)) != 0;
+            copy_side(c->test, d.test, d.dtype, d.K, want32);
+            if (c->galerkin) {
+                c->trial = &c->test;
+            } else {
+                copy_side(c->trial_own, d.trial, d.dtype, d.K, want32);
+                c->trial = &c->trial_own;
+            }
+            const bool toep = is_toeplitz(*c);
+            const size_t opelems = toep ? (size_t)d.n_ops * c->L3 : (size_t)d.n_ops * d.K * d.K;
+            const int opdtype = toep ? H2G_C64 : d.dtype;
+            c->ops.upload(d.ops, opelems * elem_size(opdtype));
+            if (want32) {
+                const size_t nreal = opelems * (opdtype == H2G_C64 ? 2 : 1);
+                std::vector<float> v(nreal);
+                const double *s = (const double *)d.ops;
+                for (size_t i = 0; i < nreal; ++i) v[i] = (float)s[i];
+                c->ops32.upload_vec(v);
+            }
+            if (d.n_perms > 0) {
+                const int64_t plen = d.m2l_kind == H2G_M2L_SYMDENSE ? d.K : c->L3;
+                std::vector<int32_t> pv(d.perms, d.perms + (size_t)d.n_perms * plen), pinv(pv.size());
+                for (int q = 0; q < d.n_perms; ++q)
+                    for (int64_t i = 0; i < plen; ++i) pinv[q * plen + pv[q * plen + i]] = (int32_t)i;
+                c->perms.upload_vec(pv);
+                c->perms_inv.upload_vec(pinv);
+            }
+            if (c->L > 0) {
+                std::vector<cplx<double>> tw(c->L);
+                for (int i = 0; i < c->L; ++i) {
+                    const double ang = -2.0 * M_PI * (double)i / (double)c->L;
+                    tw[i] = {std::cos(ang), std::sin(ang)};
+                }
+                c->twiddle.upload_vec(tw);
+            }
+            build_near(*c, d);
+            build_orient(*c, c->fwd, *c->trial, c->test, d.pair_test, d.pair_trial, d.pair_op, d.pair_sym, d.n_pairs);
+            build_orient(*c, c->rev, c->test, *c->trial, d.pair_trial, d.pair_test, d.pair_op, d.pair_sym, d.n_pairs);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int h2g_core_retain(h2g_core *core) {
+    if (!core) return H2G_EINVAL;
+    core->refs.fetch_add(1);
+    return H2G_OK;
+}
+
+int h2g_core_release(h2g_core *core) {
+    if (!core) return H2G_EINVAL;
+    if (core->refs.fetch_sub(1) == 1) {
+        (void)hipSetDevice(core->device);
+        delete core;
+    }
+    return H2G_OK;
+}
+
+int h2g_core_size(const h2g_core *core, int64_t *rows, int64_t *cols) {
+    if (!core || !rows || !cols) return H2G_EINVAL;
+    *rows = core->test.n_dofs;
+    *cols = core->trial->n_dofs;
+    return H2G_OK;
+}
+
+int h2g_core_device_bytes(const h2g_core *core, int64_t *bytes) {
+    if (!core || !bytes) return H2G_EINVAL;
+    *bytes = core->device_bytes();
+    return H2G_OK;
+}
+
+int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
+    return guarded([&] {
+        if (!core || !out) fail(H2G_EINVAL, "null argument");
+        *out = nullptr;
+        h2g_ws_opts o{};
+        o.precision = H2G_PREC_PLAIN;
+        o.max_rhs = 1;
+        if (opts) o = *opts;
+        if (o.max_rhs < 1) o.max_rhs = 1;
+        if (o.precision != H2G_PREC_PLAIN && o.precision != H2G_PREC_MIXED) fail(H2G_EINVAL, "unknown precision");
+        if (!(core->precisions & (1 << o.precision)))
+            fail(H2G_EINVAL, "the core was built without this precision (h2g_core_desc.precisions)");
+        H2G_HIP(hipSetDevice(core->device));
+        auto *w = new h2g_ws();
+        try {
+            w->core = core;
+            w->opts = o;
+            w->R_cap = o.max_rhs;
+            H2G_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+            const Core &c = *core;
+            const int64_t nmax = std::max(c.test.n_dofs, c.trial->n_dofs);
+            const int64_t bmax = std::max(c.test.n_boxes, c.trial->n_boxes);
+            const size_t se = elem_size(c.dtype);
+            const bool mixed = o.precision == H2G_PREC_MIXED;
+            const size_t ce = mixed ? se / 2 : se;
+            w->xs.alloc((size_t)nmax * w->R_cap * se);
+            w->ys.alloc((size_t)nmax * w->R_cap * se);
+            w->mom.alloc((size_t)bmax * c.K * w->R_cap * ce);
+            w->loc.alloc((size_t)bmax * c.K * w->R_cap * ce);
+            if (is_toeplitz(c)) {
+                const size_t cc = mixed ? 8 : 16;
+                const int64_t ns = std::max(c.fwd.max_src_level_boxes, c.rev.max_src_level_boxes);
+                const int64_t nr = std::max(c.fwd.max_recv_level_boxes, c.rev.max_recv_level_boxes);
+                w->spec_in.alloc((size_t)ns * w->R_cap * c.L3 * cc);
+                w->spec_out.alloc((size_t)nr * w->R_cap * c.L3 * cc);
+            }
+        } catch (...) {
+            if (w->stream) (void)hipStreamDestroy(w->stream);
+            delete w;
+            throw;
+        }
+        h2g_core_retain(core);
+        *out = w;
+    });
+}
+
+int h2g_ws_destroy(h2g_ws *ws) {
+    if (!ws) return H2G_EINVAL;
+    (void)hipSetDevice(ws->core->device);
+    if (ws->stream) (void)hipStreamSynchronize(ws->stream);
+    for (auto &kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &e : ws->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (ws->stream) (void)hipStreamDestroy(ws->stream);
+    h2g_core *c = static_cast<h2g_core *>(ws->core);
+    delete ws;
+    return h2g_core_release(c);
+}
+
+int h2g_ws_device_bytes(const h2g_ws *ws, int64_t *bytes) {
+    if (!ws || !bytes) return H2G_EINVAL;
+    *bytes = ws->device_bytes();
+    return H2G_OK;
+}
+
+void *h2g_ws_stream(h2g_ws *ws) { return ws ? (void *)ws->stream : nullptr; }
+
+int h2g_ws_acquire(h2g_ws *ws) {
+    if (!ws) return H2G_EINVAL;
+    bool expected = false;
+    if (!ws->busy.compare_exchange_strong(expected, true)) {
+        set_error("ConcurrentWorkspaceError: the workspace is already in use");
+        return H2G_EBUSY;
+    }
+    return H2G_OK;
+}
+
+int h2g_ws_release(h2g_ws *ws) {
+    if (!ws) return H2G_EINVAL;
+    ws->busy.store(false);
+    return H2G_OK;
+}
+
+int h2g_ws_sync(h2g_ws *ws) {
+    if (!ws) return H2G_EINVAL;
+    return guarded([&] { H2G_HIP(hipStreamSynchronize(ws->stream)); });
+}
+
+int h2g_ws_timing(h2g_ws *ws, h2g_timing_record *out, int32_t max, int32_t *n) {
+    if (!ws || !n) return H2G_EINVAL;
+    *n = (int32_t)ws->timings.size();
+    if (out)
+        for (int32_t i = 0; i < std::min(max, *n); ++i) out[i] = ws->timings[i];
+    return H2G_OK;
+}
+
+int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const void *x, int64_t ldx, void *y,
+               int64_t ldy, const void *alpha, const void *beta, int flags) {
+    if (!ws) {
+        set_error("null workspace");
+        return H2G_EINVAL;
+    }
+    int rc = h2g_ws_acquire(ws);
+    if (rc != H2G_OK) return rc;
+    rc = guarded([&] {
+        Core &c = *ws->core;
+        if (op < 0 || op > 2) fail(H2G_EINVAL, "op must be H2G_OP_N, H2G_OP_T or H2G_OP_C");
+        if (which < 1 || which > 3) fail(H2G_EINVAL, "which must be H2G_MUL, H2G_FARMUL or H2G_NEARMUL");
+        if (dtype < 0 || dtype > 3) fail(H2G_EINVAL, "unknown vector dtype");
+        if (c.dtype == H2G_F64 && is_complex_dtype(dtype))
+            fail(H2G_EINVAL, "a real map takes real vectors");
+        if ((alpha == nullptr) != (beta == nullptr)) fail(H2G_EINVAL, "give both alpha and beta or neither");
+        if (nrhs < 0) fail(H2G_EINVAL, "nrhs must be >= 0");
+        const int64_t n_in = op == H2G_OP_N ? c.trial->n_dofs : c.test.n_dofs;
+        const int64_t n_out = op == H2G_OP_N ? c.test.n_dofs : c.trial->n_dofs;
+        if (nrhs > 0 && (ldx < n_in || ldy < n_out)) fail(H2G_EDIM, "leading dimension smaller than the vector length");
+        if (nrhs > 0 && (!x || !y)) fail(H2G_EINVAL, "null x or y");
+        cplx<double> a{1.0, 0.0}, b{0.0, 0.0};
+        const bool five = alpha != nullptr;
+        if (five) {
+            const bool host = flags & H2G_HOST_PTRS;
+            auto rd = [&](const void *p) -> cplx<double> {
+                if (!host) {
+                    // scalars are host-side in every mode; device scalars are not supported
+                }
+                switch (dtype) {
+                case H2G_F64: return {*(const double *)p, 0.0};
+                case H2G_F32: return {(double)*(const float *)p, 0.0};
+                case H2G_C64: return {((const double *)p)[0], ((const double *)p)[1]};
+                default: return {(double)((const float *)p)[0], (double)((const float *)p)[1]};
+                }
+            };
+            a = rd(alpha);
+            b = rd(beta);
+        }
+        if (nrhs == 0) return;
+        H2G_HIP(hipSetDevice(c.device));
+        hipStream_t s = ws->stream;
+        const size_t xe = elem_size(dtype);
+        const void *dx = x;
+        void *dy = y;
+        int64_t lx = ldx, ly = ldy;
+        const bool host = flags & H2G_HOST_PTRS;
+        const bool reads_y = five && (b.re != 0.0 || b.im != 0.0 || which == H2G_FARMUL);
+        if (host) {
+            if (ws->xhost.bytes < (size_t)(n_in * nrhs) * xe) ws->xhost.alloc((size_t)(n_in * nrhs) * xe);
+            if (ws->yhost.bytes < (size_t)(n_out * nrhs) * xe) ws->yhost.alloc((size_t)(n_out * nrhs) * xe);
+            H2G_HIP(hipMemcpy2DAsync(ws->xhost.ptr, n_in * xe, x, ldx * xe, n_in * xe, nrhs, hipMemcpyHostToDevice, s));
+            if (reads_y)
+                H2G_HIP(hipMemcpy2DAsync(ws->yhost.ptr, n_out * xe, y, ldy * xe, n_out * xe, nrhs, hipMemcpyHostToDevice, s));
+            dx = ws->xhost.ptr;
+            dy = ws->yhost.ptr;
+            lx = n_in;
+            ly = n_out;
+        }
+        const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
+        const Side &src = *P.src, &dst = *P.dst;
+        const size_t se = elem_size(c.dtype);
+        for (int64_t c0 = 0; c0 < nrhs; c0 += ws->R_cap) {
+            const int R = (int)std::min<int64_t>(ws->R_cap, nrhs - c0);
+            launch_gather(c.dtype, dtype, (const char *)dx + (size_t)c0 * lx * xe, lx, src.d_perm.as<int64_t>(),
+                          src.n_dofs, R, ws->xs.ptr, s);
+            auto body = [&]() {
+                launch_fill_zero(ws->ys.ptr, (size_t)dst.n_dofs * R * se, s);
+                if (which & H2G_NEARMUL) launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, s);
+                if (which & H2G_FARMUL) far_pass(*ws, op, R);
+            };
+            if (ws->opts.use_graph) {
+                GraphKey key{op, which, R};
+                auto it = ws->graphs.find(key);
+                if (it == ws->graphs.end()) {
+                    hipGraph_t g;
+                    H2G_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    try {
+                        body();
+                    } catch (...) {
+                        (void)hipStreamEndCapture(s, &g);
+                        throw;
+                    }
+                    H2G_HIP(hipStreamEndCapture(s, &g));
+                    hipGraphExec_t ge;
+                    H2G_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                    (void)hipGraphDestroy(g);
+                    it = ws->graphs.emplace(key, ge).first;
+                }
+                H2G_HIP(hipGraphLaunch(it->second, s));
+            } else {
+                body();
+            }
+            const uint8_t *mask = (which == H2G_FARMUL && five) ? P.out_mask.as<uint8_t>() : nullptr;
+            launch_scatter(c.dtype, dtype, ws->ys.ptr, dst.d_perm.as<int64_t>(), dst.n_dofs, R,
+                           (char *)dy + (size_t)c0 * ly * xe, ly, a, five ? b : cplx<double>{0.0, 0.0}, mask, s);
+        }
+        if (host) {
+            H2G_HIP(hipMemcpy2DAsync(y, ldy * xe, ws->yhost.ptr, n_out * xe, n_out * xe, nrhs, hipMemcpyDeviceToHost, s));
+            H2G_HIP(hipStreamSynchronize(s));
+        } else if (!(flags & H2G_ASYNC)) {
+            H2G_HIP(hipStreamSynchronize(s));
+        }
+    });
+    h2g_ws_release(ws);
+    return rc;
+}
+
+}  // extern "C"

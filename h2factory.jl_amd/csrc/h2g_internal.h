@@ -1,0 +1,288 @@
+// Internal declarations of libh2g: scalar types, error plumbing, device-side core and
+// workspace structures, kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "h2g.h"
+
+namespace h2g {
+
+// ------------------------------------------------------------------------------------------
+// Scalars: real (double / float) and interleaved complex (cplx<double> / cplx<float>).
+// ------------------------------------------------------------------------------------------
+template <class R>
+struct cplx {
+    R re, im;
+};
+
+template <class T> struct real_of { using type = T; };
+template <class R> struct real_of<cplx<R>> { using type = R; };
+template <class T> struct is_complex { static constexpr bool value = false; };
+template <class R> struct is_complex<cplx<R>> { static constexpr bool value = true; };
+template <class T> struct complex_of { using type = cplx<T>; };
+template <class R> struct complex_of<cplx<R>> { using type = cplx<R>; };
+
+__host__ __device__ inline double conjv(double a) { return a; }
+__host__ __device__ inline float conjv(float a) { return a; }
+template <class R>
+__host__ __device__ inline cplx<R> conjv(cplx<R> a) { return {a.re, -a.im}; }
+
+template <class R>
+__host__ __device__ inline cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <class R>
+__host__ __device__ inline cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <class R>
+__host__ __device__ inline cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class R>
+__host__ __device__ inline cplx<R> operator*(R a, cplx<R> b) { return {a * b.re, a * b.im}; }
+template <class R>
+__host__ __device__ inline cplx<R> operator*(cplx<R> a, R b) { return {a.re * b, a.im * b}; }
+template <class R>
+__host__ __device__ inline cplx<R> &operator+=(cplx<R> &a, cplx<R> b) { a.re += b.re; a.im += b.im; return a; }
+
+// acc += a * b (real or complex, same type)
+__device__ inline void fmacc(double &acc, double a, double b) { acc = fma(a, b, acc); }
+__device__ inline void fmacc(float &acc, float a, float b) { acc = fmaf(a, b, acc); }
+template <class R>
+__device__ inline void fmacc(cplx<R> &acc, cplx<R> a, cplx<R> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+// acc(complex) += a(real or complex) * b(complex)
+template <class R>
+__device__ inline void fmacc(cplx<R> &acc, R a, cplx<R> b) {
+    acc.re = fma(a, b.re, acc.re);
+    acc.im = fma(a, b.im, acc.im);
+}
+
+template <class T> __host__ __device__ inline T zero_v() { return T(0); }
+template <> __host__ __device__ inline cplx<double> zero_v<cplx<double>>() { return {0.0, 0.0}; }
+template <> __host__ __device__ inline cplx<float> zero_v<cplx<float>>() { return {0.0f, 0.0f}; }
+
+// conversions between element types (complex -> real keeps the real part)
+template <class To> struct cvt;
+template <> struct cvt<double> {
+    __host__ __device__ static double from(double a) { return a; }
+    __host__ __device__ static double from(float a) { return a; }
+    template <class R> __host__ __device__ static double from(cplx<R> a) { return (double)a.re; }
+};
+template <> struct cvt<float> {
+    __host__ __device__ static float from(double a) { return (float)a; }
+    __host__ __device__ static float from(float a) { return a; }
+    template <class R> __host__ __device__ static float from(cplx<R> a) { return (float)a.re; }
+};
+template <class R> struct cvt<cplx<R>> {
+    __host__ __device__ static cplx<R> from(double a) { return {(R)a, (R)0}; }
+    __host__ __device__ static cplx<R> from(float a) { return {(R)a, (R)0}; }
+    template <class S> __host__ __device__ static cplx<R> from(cplx<S> a) { return {(R)a.re, (R)a.im}; }
+};
+
+// scalar * value for the alpha/beta of the entry points (alpha carried as c64)
+template <class T> __device__ inline T scale_by(cplx<double> s, T v);
+template <> __device__ inline double scale_by<double>(cplx<double> s, double v) { return s.re * v; }
+template <> __device__ inline float scale_by<float>(cplx<double> s, float v) { return (float)(s.re * v); }
+template <> __device__ inline cplx<double> scale_by<cplx<double>>(cplx<double> s, cplx<double> v) { return s * v; }
+template <> __device__ inline cplx<float> scale_by<cplx<float>>(cplx<double> s, cplx<float> v) {
+    cplx<double> r = s * cplx<double>{(double)v.re, (double)v.im};
+    return {(float)r.re, (float)r.im};
+}
+
+// ------------------------------------------------------------------------------------------
+// Errors
+// ------------------------------------------------------------------------------------------
+void set_error(const std::string &msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+#define H2G_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::h2g::Error{H2G_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)};  \
+    } while (0)
+
+inline size_t elem_size(int dtype) {
+    switch (dtype) {
+    case H2G_F64: return 8;
+    case H2G_C64: return 16;
+    case H2G_F32: return 4;
+    case H2G_C32: return 8;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Device buffers
+// ------------------------------------------------------------------------------------------
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw Error{H2G_ENOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed: " + hipGetErrorString(e)};
+        }
+        bytes = n;
+    }
+    void upload(const void *host, size_t n) {
+        alloc(n);
+        if (n) H2G_HIP(hipMemcpy(ptr, host, n, hipMemcpyHostToDevice));
+    }
+    template <class T> void upload_vec(const std::vector<T> &v) { upload(v.data(), v.size() * sizeof(T)); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Per-level list: [begin, begin+count) into a concatenated int32 device array.
+struct Span {
+    int64_t begin = 0;
+    int32_t count = 0;
+};
+
+// One side of the map.
+struct Side {
+    int64_t n_dofs = 0;
+    int n_levels = 0, n_boxes = 0;
+    std::vector<int32_t> level_offsets, parent, sector;
+    std::vector<int64_t> begin, end;
+    std::vector<uint8_t> is_leaf;
+    std::vector<int64_t> perm;
+    DevBuf d_perm, d_begin, d_end;
+    DevBuf coll, coll32;            // K x n_dofs core type / reduced type
+    DevBuf transfers, transfers32;  // [n_levels][8] K x K
+    int level_of(int b) const {
+        int l = 0;
+        while (l + 1 < (int)level_offsets.size() && b >= level_offsets[l + 1]) ++l;
+        return l;
+    }
+};
+
+// Work lists of one orientation (forward: trial -> test; reverse: test -> trial).
+struct OrientPlan {
+    const Side *src = nullptr, *dst = nullptr;
+    DevBuf lists;                       // concatenated int32 box lists
+    std::vector<Span> p2m, m2m, l2l, l2p, m2l_recv;  // per level
+    // M2M items: CSR over m2m targets (all levels concatenated); item = (src child box, matrix index)
+    DevBuf m2m_ptr, m2m_src, m2m_mat;
+    std::vector<int64_t> m2m_ptr_base;  // per level: offset of that level's targets into m2m_ptr
+    // L2L items: one per target: (parent box, matrix index, accumulate flag)
+    DevBuf l2l_ptr, l2l_src, l2l_mat, l2l_acc;
+    std::vector<int64_t> l2l_base;
+    // M2L items: CSR over receiving boxes (concatenated by level)
+    DevBuf m2l_ptr, m2l_src, m2l_op, m2l_sym;
+    std::vector<int64_t> m2l_base;
+    int min_level = 0, max_level = -1;
+    int64_t max_src_level_boxes = 0, max_recv_level_boxes = 0;
+    DevBuf out_mask;                    // uint8 per tree-ordered output dof: leaf initialised
+};
+
+struct Core {
+    std::atomic<int> refs{1};
+    int device = 0;
+    int dtype = H2G_F64;
+    int K = 0, p = 0, L = 0, L3 = 0;
+    bool galerkin = true;
+    Side test, trial_own;
+    const Side *trial = nullptr;
+    int m2l_kind = H2G_M2L_DENSE;
+    int64_t n_pairs = 0, n_ops = 0;
+    int n_perms = 0;
+    DevBuf ops, ops32;        // dense: core type / reduced; toeplitz: c64 spectra / c32
+    DevBuf perms, perms_inv;  // int32
+    // near: blocks
+    int64_t n_near_blocks = 0, n_near_values = 0;
+    DevBuf near_vals;
+    DevBuf near_fwd_ptr, near_fwd_blk;   // CSR over test leaves (by list index)
+    DevBuf near_rev_ptr, near_rev_blk;   // CSR over trial leaves
+    DevBuf near_fwd_tgt, near_rev_tgt;   // target leaf lists
+    int32_t n_near_fwd_tgt = 0, n_near_rev_tgt = 0;
+    DevBuf near_blk_rb, near_blk_cb, near_blk_nr, near_blk_nc, near_blk_off;  // per block
+    cplx<double> alpha{1.0, 0.0};
+    int precisions = 1;
+    OrientPlan fwd, rev;
+    DevBuf twiddle;  // c64 [L]: exp(-2 pi i n / L)
+    int64_t device_bytes() const;
+};
+
+struct GraphKey {
+    int op, which, nrhs;
+    bool operator<(const GraphKey &o) const {
+        return std::tie(op, which, nrhs) < std::tie(o.op, o.which, o.nrhs);
+    }
+};
+
+struct Workspace {
+    Core *core = nullptr;
+    h2g_ws_opts opts{};
+    std::atomic<bool> busy{false};
+    hipStream_t stream = nullptr;
+    int R_cap = 1;
+    DevBuf xs, ys;        // tree-ordered FP64/C64, [n][R]
+    DevBuf mom, loc;      // [box][R][K] compute type
+    DevBuf spec_in, spec_out;  // toeplitz spectra [slot][R][L3] complex compute type
+    DevBuf xhost, yhost;  // device staging for host pointers
+    DevBuf scal;          // unused placeholder
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    std::vector<h2g_timing_record> timings;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    int64_t device_bytes() const;
+};
+
+// ------------------------------------------------------------------------------------------
+// Kernel launchers (kernels.hip), all on `s`.
+// T = compute scalar, S = storage scalar of xs/ys (double or cplx<double>).
+// ------------------------------------------------------------------------------------------
+void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, const int64_t *perm, int64_t n,
+                   int R, void *xs, hipStream_t s);
+void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, int64_t n, int R, void *y,
+                    int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s);
+void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s);
+void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s);
+void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, hipStream_t s);
+// out[t] = (acc ? out[t] : 0) + sum_items opA(mats[item_mat]) * in[item_src]  (optionally permuted)
+// acc_mode: 0 overwrite, 1 accumulate, 2 accumulate when acc_flags[item_ptr[t]] (first item of t)
+// opA: 0 = A, 1 = A^T, 2 = A^H, 3 = conj(A)
+void launch_boxgemm(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
+                    const int32_t *item_src, const int32_t *item_mat, const int32_t *item_sym,
+                    const int32_t *perms, const int32_t *perms_inv, const void *mats, int K, const void *in,
+                    void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s);
+void launch_fft_in(int cdtype, bool inverse, const int32_t *boxes, int n, int slot_base, const void *mom, int K,
+                   int p, int L, int R, void *spec, const void *twiddle, hipStream_t s);
+void launch_hadamard(int cdtype, bool conj, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
+                     const int32_t *item_op, const int32_t *item_sym, const int32_t *perms, const void *ops,
+                     int L3, int src_slot_base, const void *spec_in, void *spec_out, int R, hipStream_t s);
+void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
+                    int R, void *loc, const void *twiddle, hipStream_t s);
+
+}  // namespace h2g
+
+struct h2g_core : h2g::Core {};
+struct h2g_ws : h2g::Workspace {};

@@ -1,0 +1,490 @@
+// Device kernels of libh2g.
+//
+// Buffers in tree order, R = right-hand sides of the current pass:
+//   xs, ys   [dof][R]        FP64 / ComplexF64 (near accumulation and final sum stay FP64)
+//   mom, loc [box][R][K]     compute type (FP64/C64 plain, FP32/C32 mixed)
+//   spectra  [slot][R][L^3]  complex compute type, frequency x fastest
+#include <type_traits>
+
+#include "h2g_internal.h"
+
+namespace h2g {
+
+namespace {
+
+constexpr int TPB = 256;
+
+template <class T>
+using store_t = typename std::conditional<is_complex<T>::value, cplx<double>, double>::type;
+
+template <class T>
+__device__ inline T maybe_conj(T v, bool c) {
+    return c ? conjv(v) : v;
+}
+
+inline unsigned grid1(int64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// ------------------------------------------------------------------------------------------
+// gather: xs[i][r] = x[perm[i] + r*ldx]
+// ------------------------------------------------------------------------------------------
+template <class S, class X>
+__global__ void gather_kernel(const X *__restrict__ x, int64_t ldx, const int64_t *__restrict__ perm, int64_t n,
+                              int R, S *__restrict__ xs) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * R) return;
+    int64_t i = idx / R;
+    int r = (int)(idx - i * R);
+    xs[idx] = cvt<S>::from(x[perm[i] + (int64_t)r * ldx]);
+}
+
+// ------------------------------------------------------------------------------------------
+// scatter: y[perm[i] + r*ldy] = a*ys[i][r] + b*y (b == 0: y is not read); masked rows skipped
+// ------------------------------------------------------------------------------------------
+template <class S, class Y>
+__global__ void scatter_kernel(const S *__restrict__ ys, const int64_t *__restrict__ perm, int64_t n, int R,
+                               Y *__restrict__ y, int64_t ldy, cplx<double> a, cplx<double> b,
+                               const uint8_t *__restrict__ mask) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * R) return;
+    int64_t i = idx / R;
+    int r = (int)(idx - i * R);
+    if (mask && !mask[i]) return;
+    int64_t o = perm[i] + (int64_t)r * ldy;
+    S v = scale_by<S>(a, ys[idx]);
+    if (b.re != 0.0 || b.im != 0.0) v = v + scale_by<S>(b, cvt<S>::from(y[o]));
+    y[o] = cvt<Y>::from(v);
+}
+
+// ------------------------------------------------------------------------------------------
+// near field: one workgroup per target leaf; blocks are column-major nr x nc.
+//   forward:   ys[rb + i] += B[i, j] * xs[cb + j]
+//   transpose: ys[cb + j] += op(B[i, j]) * xs[rb + i]
+// ------------------------------------------------------------------------------------------
+template <class S, bool TRANS>
+__global__ void near_kernel(const int32_t *__restrict__ tgt, const int64_t *__restrict__ ptr,
+                            const int32_t *__restrict__ blk, const int64_t *__restrict__ rb,
+                            const int64_t *__restrict__ cb, const int32_t *__restrict__ nr,
+                            const int32_t *__restrict__ nc, const int64_t *__restrict__ off,
+                            const S *__restrict__ vals, const S *__restrict__ xs, S *__restrict__ ys, int R,
+                            bool conj) {
+    const int t = blockIdx.x;
+    const int64_t b0 = ptr[t], b1 = ptr[t + 1];
+    if (b0 == b1) return;
+    const int first = blk[b0];
+    const int n_out = TRANS ? nc[first] : nr[first];
+    const int64_t out0 = TRANS ? cb[first] : rb[first];
+    for (int o = threadIdx.x; o < n_out * R; o += blockDim.x) {
+        const int row = o / R;
+        const int r = o - row * R;
+        S acc = zero_v<S>();
+        for (int64_t q = b0; q < b1; ++q) {
+            const int bl = blk[q];
+            const int m = nr[bl], n = nc[bl];
+            const S *B = vals + off[bl];
+            if (!TRANS) {
+                const S *xin = xs + cb[bl] * R + r;
+                for (int j = 0; j < n; ++j) fmacc(acc, B[row + (int64_t)m * j], xin[(int64_t)j * R]);
+            } else {
+                const S *xin = xs + rb[bl] * R + r;
+                const S *col = B + (int64_t)m * row;
+                for (int i = 0; i < m; ++i) fmacc(acc, maybe_conj(col[i], conj), xin[(int64_t)i * R]);
+            }
+        }
+        ys[(out0 + row) * R + r] = acc;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// P2M: mom[leaf][r][k] = sum_j opC(C[k, b0 + j]) * xs[b0 + j][r]
+// ------------------------------------------------------------------------------------------
+template <class T>
+__global__ void p2m_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
+                           const int64_t *__restrict__ end, const T *__restrict__ coll, int K,
+                           const store_t<T> *__restrict__ xs, int R, T *__restrict__ mom, bool conj) {
+    const int b = leaves[blockIdx.x];
+    const int64_t j0 = begin[b], j1 = end[b];
+    for (int o = threadIdx.x; o < K * R; o += blockDim.x) {
+        const int r = o / K;
+        const int k = o - r * K;
+        T acc = zero_v<T>();
+        for (int64_t j = j0; j < j1; ++j)
+            fmacc(acc, maybe_conj(coll[k + (int64_t)K * j], conj), cvt<T>::from(xs[j * R + r]));
+        mom[(int64_t)b * K * R + o] = acc;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// L2P: ys[b0 + j][r] += alpha * sum_k opC(C[k, b0 + j]) * loc[leaf][r][k]
+// ------------------------------------------------------------------------------------------
+template <class T>
+__global__ void l2p_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
+                           const int64_t *__restrict__ end, const T *__restrict__ coll, int K,
+                           const T *__restrict__ loc, int R, store_t<T> *__restrict__ ys, cplx<double> alpha,
+                           bool conj) {
+    using S = store_t<T>;
+    const int b = leaves[blockIdx.x];
+    const int64_t j0 = begin[b], n = end[b] - j0;
+    const T *l = loc + (int64_t)b * K * R;
+    for (int64_t o = threadIdx.x; o < n * R; o += blockDim.x) {
+        const int64_t j = o / R;
+        const int r = (int)(o - j * R);
+        const T *c = coll + (j0 + j) * K;
+        const T *lr = l + (int64_t)r * K;
+        T acc = zero_v<T>();
+        for (int k = 0; k < K; ++k) fmacc(acc, maybe_conj(c[k], conj), lr[k]);
+        const int64_t yo = (j0 + j) * R + r;
+        ys[yo] = ys[yo] + scale_by<S>(alpha, cvt<S>::from(acc));
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// box GEMM: out[t] = (acc ? out[t] : 0) + sum_items opA(A_item) * in[src_item]
+// opA: 0 A, 1 A^T, 2 A^H, 3 conj(A).  With a permutation pi (symmetry-reduced dense):
+//   out[k] = sum_j opA(canonical)[pi_inv[k], j] * in[pi[j]]
+// One workgroup per (target, RHS chunk of RB); threads over rows k.
+// ------------------------------------------------------------------------------------------
+constexpr int RB = 8;
+
+template <class T>
+__device__ inline T load_op(const T *A, int K, int row, int col, int opA) {
+    switch (opA) {
+    case 0: return A[row + (int64_t)K * col];
+    case 1: return A[col + (int64_t)K * row];
+    case 2: return conjv(A[col + (int64_t)K * row]);
+    default: return conjv(A[row + (int64_t)K * col]);
+    }
+}
+
+template <class T>
+__global__ void boxgemm_kernel(int opA, const int32_t *__restrict__ targets, const int64_t *__restrict__ item_ptr,
+                               const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_mat,
+                               const int32_t *__restrict__ item_sym, const int32_t *__restrict__ perms,
+                               const int32_t *__restrict__ perms_inv, const T *__restrict__ mats, int K,
+                               const T *__restrict__ in, T *__restrict__ out, int R, int acc_mode,
+                               const uint8_t *__restrict__ acc_flags) {
+    const int ti = blockIdx.x;
+    const int t = targets[ti];
+    const int r0 = blockIdx.y * RB;
+    const int nr = min(RB, R - r0);
+    const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
+    const bool acc_in = acc_mode == 1 || (acc_mode == 2 && acc_flags[i0]);
+    T *o = out + (int64_t)t * K * R;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        T acc[RB];
+        for (int q = 0; q < RB; ++q) acc[q] = zero_v<T>();
+        if (acc_in)
+            for (int q = 0; q < nr; ++q) acc[q] = o[(int64_t)(r0 + q) * K + k];
+        for (int64_t it = i0; it < i1; ++it) {
+            const T *A = mats + (int64_t)item_mat[it] * K * K;
+            const T *x = in + (int64_t)item_src[it] * K * R + (int64_t)r0 * K;
+            const int sym = item_sym ? item_sym[it] : -1;
+            const int32_t *pg = sym >= 0 ? perms + (int64_t)sym * K : nullptr;
+            const int row = sym >= 0 ? perms_inv[(int64_t)sym * K + k] : k;
+            for (int j = 0; j < K; ++j) {
+                const T a = load_op(A, K, row, j, opA);
+                const int jj = pg ? pg[j] : j;
+                for (int q = 0; q < RB; ++q)
+                    if (q < nr) fmacc(acc[q], a, x[(int64_t)q * K + jj]);
+            }
+        }
+        for (int q = 0; q < nr; ++q) o[(int64_t)(r0 + q) * K + k] = acc[q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Toeplitz modulation: pad -> 3-D DFT (fft_in) and 3-D inverse DFT -> crop (fft_out).
+// Separable stages over LDS; lengths L = 2p-1 are odd, so each stage is a direct DFT of the
+// non-zero / kept entries only.  `inverse` selects exp(+2 pi i/L) and the 1/L^3 scale.
+// ------------------------------------------------------------------------------------------
+template <class T>
+__global__ void fft_in_kernel(int box0, const T *__restrict__ mom, int K, int p, int L, int R,
+                              typename complex_of<T>::type *__restrict__ spec,
+                              const cplx<double> *__restrict__ twiddle, bool inverse) {
+    using C = typename complex_of<T>::type;
+    using Rl = typename real_of<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    C *m = tw + L;
+    C *b1 = m + p * p * p;
+    C *b2 = b1 + L * p * p;
+    const int slot = blockIdx.x;
+    const int r = blockIdx.y;
+    const int box = box0 + slot;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        cplx<double> w = twiddle[i];
+        tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
+    }
+    const T *src = mom + ((int64_t)box * R + r) * K;
+    for (int i = threadIdx.x; i < p * p * p; i += blockDim.x) m[i] = cvt<C>::from(src[i]);
+    __syncthreads();
+    // x: b1[fx + L*(iy + p*iz)] = sum_ix m[ix + p*(iy + p*iz)] w^(fx*ix)
+    for (int i = threadIdx.x; i < L * p * p; i += blockDim.x) {
+        const int fx = i % L, rest = i / L;
+        C acc = zero_v<C>();
+        for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], m[ix + p * rest]);
+        b1[i] = acc;
+    }
+    __syncthreads();
+    // y: b2[fx + L*(fy + L*iz)] = sum_iy b1[fx + L*(iy + p*iz)] w^(fy*iy)
+    for (int i = threadIdx.x; i < L * L * p; i += blockDim.x) {
+        const int fx = i % L, fy = (i / L) % L, iz = i / (L * L);
+        C acc = zero_v<C>();
+        for (int iy = 0; iy < p; ++iy) fmacc(acc, tw[(fy * iy) % L], b1[fx + L * (iy + p * iz)]);
+        b2[i] = acc;
+    }
+    __syncthreads();
+    const int L3 = L * L * L;
+    const Rl scale = inverse ? (Rl)(1.0 / (double)L3) : (Rl)1;
+    C *dst = spec + ((int64_t)slot * R + r) * L3;
+    for (int i = threadIdx.x; i < L3; i += blockDim.x) {
+        const int fxy = i % (L * L), fz = i / (L * L);
+        C acc = zero_v<C>();
+        for (int iz = 0; iz < p; ++iz) fmacc(acc, tw[(fz * iz) % L], b2[fxy + L * L * iz]);
+        dst[i] = acc * scale;
+    }
+}
+
+template <class T>
+__global__ void fft_out_kernel(const int32_t *__restrict__ boxes, const typename complex_of<T>::type *__restrict__ spec,
+                               int K, int p, int L, int R, T *__restrict__ loc,
+                               const cplx<double> *__restrict__ twiddle, bool inverse) {
+    using C = typename complex_of<T>::type;
+    using Rl = typename real_of<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    const int L3 = L * L * L;
+    C *s = tw + L;
+    C *a1 = s + L3;
+    C *a2 = a1 + L * L * p;
+    const int idx = blockIdx.x;
+    const int r = blockIdx.y;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        cplx<double> w = twiddle[i];
+        tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
+    }
+    const C *src = spec + ((int64_t)idx * R + r) * L3;
+    for (int i = threadIdx.x; i < L3; i += blockDim.x) s[i] = src[i];
+    __syncthreads();
+    // z: a1[fxy + L*L*iz] = sum_fz s[fxy + L*L*fz] w^(fz*iz)
+    for (int i = threadIdx.x; i < L * L * p; i += blockDim.x) {
+        const int fxy = i % (L * L), iz = i / (L * L);
+        C acc = zero_v<C>();
+        for (int fz = 0; fz < L; ++fz) fmacc(acc, tw[(fz * iz) % L], s[fxy + L * L * fz]);
+        a1[i] = acc;
+    }
+    __syncthreads();
+    // y: a2[fx + L*(iy + p*iz)] = sum_fy a1[fx + L*(fy + L*iz)] w^(fy*iy)
+    for (int i = threadIdx.x; i < L * p * p; i += blockDim.x) {
+        const int fx = i % L, iy = (i / L) % p, iz = i / (L * p);
+        C acc = zero_v<C>();
+        for (int fy = 0; fy < L; ++fy) fmacc(acc, tw[(fy * iy) % L], a1[fx + L * (fy + L * iz)]);
+        a2[i] = acc;
+    }
+    __syncthreads();
+    const Rl scale = inverse ? (Rl)(1.0 / (double)L3) : (Rl)1;
+    T *dst = loc + ((int64_t)boxes[idx] * R + r) * K;
+    for (int i = threadIdx.x; i < p * p * p; i += blockDim.x) {
+        const int ix = i % p, rest = i / p;
+        C acc = zero_v<C>();
+        for (int fx = 0; fx < L; ++fx) fmacc(acc, tw[(fx * ix) % L], a2[fx + L * rest]);
+        dst[i] = cvt<T>::from(acc * scale);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Hadamard accumulation over a receiving box's partners:
+//   spec_out[recv][r][f] = sum_items transform(D_op[perm? perm[f] : f]) * spec_in[src][r][f]
+// ------------------------------------------------------------------------------------------
+template <class C, class D>
+__global__ void hadamard_kernel(const int64_t *__restrict__ item_ptr, const int32_t *__restrict__ item_src,
+                                const int32_t *__restrict__ item_op, const int32_t *__restrict__ item_sym,
+                                const int32_t *__restrict__ perms, const D *__restrict__ ops, int L3,
+                                int src_slot_base, const C *__restrict__ spec_in, C *__restrict__ spec_out, int R,
+                                bool conj) {
+    const int recv = blockIdx.x;
+    const int r = blockIdx.y;
+    const int64_t i0 = item_ptr[recv], i1 = item_ptr[recv + 1];
+    for (int f = threadIdx.x; f < L3; f += blockDim.x) {
+        C acc = zero_v<C>();
+        for (int64_t it = i0; it < i1; ++it) {
+            const int sym = item_sym ? item_sym[it] : -1;
+            const int ff = sym >= 0 ? perms[(int64_t)sym * L3 + f] : f;
+            const C d = cvt<C>::from(ops[(int64_t)item_op[it] * L3 + ff]);
+            const C x = spec_in[((int64_t)(item_src[it] - src_slot_base) * R + r) * L3 + f];
+            fmacc(acc, maybe_conj(d, conj), x);
+        }
+        spec_out[((int64_t)recv * R + r) * L3 + f] = acc;
+    }
+}
+
+__global__ void fill_zero_kernel(uint32_t *__restrict__ p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------------
+#define H2G_CHECK_LAUNCH()                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = hipGetLastError();                                                   \
+        if (e_ != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define DISPATCH_T(cd, ...)                                                   \
+    switch (cd) {                                                             \
+    case H2G_F64: { using T = double; __VA_ARGS__; } break;                   \
+    case H2G_C64: { using T = cplx<double>; __VA_ARGS__; } break;             \
+    case H2G_F32: { using T = float; __VA_ARGS__; } break;                    \
+    case H2G_C32: { using T = cplx<float>; __VA_ARGS__; } break;             \
+    default: throw Error{H2G_EINVAL, "unknown element type"};                 \
+    }
+
+#define DISPATCH_S(cd, ...)                                                   \
+    switch (cd) {                                                             \
+    case H2G_F64: { using S = double; __VA_ARGS__; } break;                   \
+    case H2G_C64: { using S = cplx<double>; __VA_ARGS__; } break;             \
+    default: throw Error{H2G_EINVAL, "core element type must be F64 or C64"}; \
+    }
+
+#define DISPATCH_X(dt, ...)                                                   \
+    switch (dt) {                                                             \
+    case H2G_F64: { using X = double; __VA_ARGS__; } break;                   \
+    case H2G_C64: { using X = cplx<double>; __VA_ARGS__; } break;             \
+    case H2G_F32: { using X = float; __VA_ARGS__; } break;                    \
+    case H2G_C32: { using X = cplx<float>; __VA_ARGS__; } break;             \
+    default: throw Error{H2G_EINVAL, "unknown vector element type"};          \
+    }
+
+// Zero fill by kernel (usable under stream capture, unlike a memset node).
+void launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
+    const int64_t n = (int64_t)(bytes / 4);
+    if (n == 0) return;
+    fill_zero_kernel<<<grid1(n), TPB, 0, s>>>((uint32_t *)p, n);
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, const int64_t *perm, int64_t n, int R,
+                   void *xs, hipStream_t s) {
+    if (n * R == 0) return;
+    DISPATCH_S(dtype_core, DISPATCH_X(dtype_x, {
+        gather_kernel<S, X><<<grid1(n * R), TPB, 0, s>>>((const X *)x, ldx, perm, n, R, (S *)xs);
+    }));
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, int64_t n, int R, void *y,
+                    int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s) {
+    if (n * R == 0) return;
+    DISPATCH_S(dtype_core, DISPATCH_X(dtype_y, {
+        scatter_kernel<S, X><<<grid1(n * R), TPB, 0, s>>>((const S *)ys, perm, n, R, (X *)y, ldy, a, b, mask);
+    }));
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
+    const bool trans = op != H2G_OP_N;
+    const int ntgt = trans ? c.n_near_rev_tgt : c.n_near_fwd_tgt;
+    if (ntgt == 0) return;
+    const int32_t *tgt = trans ? c.near_rev_tgt.as<int32_t>() : c.near_fwd_tgt.as<int32_t>();
+    const int64_t *ptr = trans ? c.near_rev_ptr.as<int64_t>() : c.near_fwd_ptr.as<int64_t>();
+    const int32_t *blk = trans ? c.near_rev_blk.as<int32_t>() : c.near_fwd_blk.as<int32_t>();
+    DISPATCH_S(dtype_core, {
+        if (trans)
+            near_kernel<S, true><<<ntgt, TPB, 0, s>>>(tgt, ptr, blk, c.near_blk_rb.as<int64_t>(),
+                                                      c.near_blk_cb.as<int64_t>(), c.near_blk_nr.as<int32_t>(),
+                                                      c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(),
+                                                      c.near_vals.as<S>(), (const S *)xs, (S *)ys, R,
+                                                      op == H2G_OP_C);
+        else
+            near_kernel<S, false><<<ntgt, TPB, 0, s>>>(tgt, ptr, blk, c.near_blk_rb.as<int64_t>(),
+                                                       c.near_blk_cb.as<int64_t>(), c.near_blk_nr.as<int32_t>(),
+                                                       c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(),
+                                                       c.near_vals.as<S>(), (const S *)xs, (S *)ys, R, false);
+    });
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
+    if (n == 0) return;
+    DISPATCH_T(cdtype, {
+        p2m_kernel<T><<<n, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const store_t<T> *)xs, R,
+                                        (T *)mom, conj);
+    });
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, hipStream_t s) {
+    if (n == 0) return;
+    DISPATCH_T(cdtype, {
+        l2p_kernel<T><<<n, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R,
+                                        (store_t<T> *)ys, alpha, conj);
+    });
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_boxgemm(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
+                    const int32_t *item_src, const int32_t *item_mat, const int32_t *item_sym,
+                    const int32_t *perms, const int32_t *perms_inv, const void *mats, int K, const void *in,
+                    void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
+    if (n_targets == 0) return;
+    dim3 grid(n_targets, (R + RB - 1) / RB);
+    DISPATCH_T(cdtype, {
+        boxgemm_kernel<T><<<grid, TPB, 0, s>>>(opA, targets, item_ptr, item_src, item_mat, item_sym, perms,
+                                               perms_inv, (const T *)mats, K, (const T *)in, (T *)out, R,
+                                               acc_mode, acc_flags);
+    });
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_fft_in(int cdtype, bool inverse, const int32_t *, int n, int box0, const void *mom, int K, int p,
+                   int L, int R, void *spec, const void *twiddle, hipStream_t s) {
+    if (n == 0) return;
+    dim3 grid(n, R);
+    DISPATCH_T(cdtype, {
+        using C = typename complex_of<T>::type;
+        size_t sh = sizeof(C) * (size_t)(L + p * p * p + L * p * p + L * L * p);
+        fft_in_kernel<T><<<grid, TPB, sh, s>>>(box0, (const T *)mom, K, p, L, R, (C *)spec,
+                                               (const cplx<double> *)twiddle, inverse);
+    });
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_hadamard(int cdtype, bool conj, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
+                     const int32_t *item_op, const int32_t *item_sym, const int32_t *perms, const void *ops,
+                     int L3, int src_slot_base, const void *spec_in, void *spec_out, int R, hipStream_t s) {
+    if (n_recv == 0) return;
+    dim3 grid(n_recv, R);
+    switch (cdtype) {
+    case H2G_F64:
+    case H2G_C64:
+        hadamard_kernel<cplx<double>, cplx<double>><<<grid, TPB, 0, s>>>(
+            item_ptr, item_src, item_op, item_sym, perms, (const cplx<double> *)ops, L3, src_slot_base,
+            (const cplx<double> *)spec_in, (cplx<double> *)spec_out, R, conj);
+        break;
+    default:
+        hadamard_kernel<cplx<float>, cplx<float>><<<grid, TPB, 0, s>>>(
+            item_ptr, item_src, item_op, item_sym, perms, (const cplx<float> *)ops, L3, src_slot_base,
+            (const cplx<float> *)spec_in, (cplx<float> *)spec_out, R, conj);
+    }
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
+                    int R, void *loc, const void *twiddle, hipStream_t s) {
+    if (n == 0) return;
+    dim3 grid(n, R);
+    DISPATCH_T(cdtype, {
+        using C = typename complex_of<T>::type;
+        size_t sh = sizeof(C) * (size_t)(L + L * L * L + L * L * p + L * p * p);
+        fft_out_kernel<T><<<grid, TPB, sh, s>>>(boxes, (const C *)spec, K, p, L, R, (T *)loc,
+                                                (const cplx<double> *)twiddle, inverse);
+    });
+    H2G_CHECK_LAUNCH();
+}
+
+}  // namespace h2g

@@ -1,0 +1,239 @@
+/*
+ * h2g -- MI355X-native hierarchical (H2) matrix-vector product behind a C ABI.
+ *
+ * One shared library (libh2g.so) replaces the per-node traversal that H2Factory.jl runs for
+ * LinearAlgebra.mul! on an H2 map.  The method (tree, leaf collections, transfers, couplings,
+ * near operator) is described once in an h2g_core_desc and uploaded into a read-only core;
+ * each h2g_ws workspace owns the mutable device buffers of one matvec at a time.
+ *
+ * All arrays are plain host pointers in the core description; x and y of h2g_matvec are
+ * device pointers unless H2G_HOST_PTRS is set.  Matrices are column-major (Julia layout).
+ * Complex values are interleaved (re, im) pairs.
+ */
+#ifndef H2G_H
+#define H2G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Each maps to the Julia exception the reference raises for the same
+ * condition: H2G_EBUSY  -> ConcurrentWorkspaceError (src/maps/workspace.jl:284-343),
+ * H2G_EINVAL -> ArgumentError (src/maps/workspace.jl:124-135,
+ *               src/farinteractions/i2otranslations/i2oprotocol.jl:180-203),
+ * H2G_EDIM   -> DimensionMismatch (src/farinteractions/i2otranslations/symmetryreducedcollection.jl:19-26). */
+enum {
+    H2G_OK = 0,
+    H2G_EBUSY = 1,
+    H2G_EINVAL = 2,
+    H2G_EDIM = 3,
+    H2G_ENOMEM = 4,
+    H2G_EHIP = 5,
+    H2G_EUNSUPPORTED = 6
+};
+
+/* Element types. */
+enum { H2G_F64 = 0, H2G_C64 = 1, H2G_F32 = 2, H2G_C32 = 3 };
+
+/* Orientation of the product: A, transpose(A), adjoint(A)
+ * (src/maps/traits.jl:4-53, LinearMaps.TransposeMap / AdjointMap). */
+enum { H2G_OP_N = 0, H2G_OP_T = 1, H2G_OP_C = 2 };
+
+/* Which product h2g_matvec computes.
+ *   H2G_MUL     mul!(y, A, x[, a, b])        src/maps/abstracth2map.jl:42-86
+ *   H2G_FARMUL  farmul!(y, A, x[, a, b])     src/farinteractions/farmul.jl:8-70
+ *   H2G_NEARMUL nearmul!(y, A, x[, a, b])    src/nearinteractions/nearmul.jl:8-14 */
+enum { H2G_MUL = 3, H2G_FARMUL = 2, H2G_NEARMUL = 1 };
+
+/* Storage of the couplings (I2O / M2L).
+ *   DENSE        one K x K matrix per translation id
+ *                (src/farinteractions/i2otranslations/i2otranslationcollection.jl:157-221,351-409)
+ *   TOEPLITZ     one L^3 spectrum D = fft(kappa) per id, L = 2p-1, kappa in wrap order
+ *                (src/farinteractions/i2otranslations/toeplitzi2ocollection.jl:9-193,
+ *                 src/moments/toeplitz/toeplitzmul.jl:86-305)
+ *   SYMDENSE     canonical K x K matrices + grid permutation per pair
+ *                (src/farinteractions/i2otranslations/symmetryreduceddense.jl:16-293)
+ *   SYMTOEPLITZ  canonical spectra + spectrum permutation per pair
+ *                (src/farinteractions/i2otranslations/symmetryreducedcollection.jl:11-35,356-387) */
+enum { H2G_M2L_DENSE = 0, H2G_M2L_TOEPLITZ = 1, H2G_M2L_SYMDENSE = 2, H2G_M2L_SYMTOEPLITZ = 3 };
+
+/* Arithmetic of the far field in a workspace.
+ *   PLAIN  everything in the core's type (Float64 / ComplexF64): the drop-in.
+ *   MIXED  P2M, M2M, M2L, L2L, L2P in FP32 (ComplexF32); near field, the near+far sum and all
+ *          precomputed spectra stay FP64. */
+enum { H2G_PREC_PLAIN = 0, H2G_PREC_MIXED = 1 };
+
+/* Flags of h2g_matvec. */
+enum {
+    H2G_HOST_PTRS = 1, /* x, y, alpha, beta point to host memory (copied in / out)         */
+    H2G_ASYNC = 2      /* return after enqueueing on the workspace stream (device ptrs only) */
+};
+
+/* One side (test = rows/targets, trial = columns/sources) of the map: its octree and its leaf
+ * data.  Boxes are level-contiguous; the children of a box are contiguous on the next level.
+ * Replaces the H2Trees tree plus testmomentcollections / trialmomentcollections and the I2I
+ * level translator that the core accessors return (docs/src/implementing/required.md:23-104,
+ * src/maps/leveltranslator.jl:100-158). */
+typedef struct h2g_side_desc {
+    int64_t n_dofs;               /* degrees of freedom of this side                          */
+    int32_t n_levels;             /* levels 0 .. n_levels-1                                    */
+    int32_t n_boxes;
+    const int64_t *perm;          /* [n_dofs] tree position -> original dof (values(tree, leaf)) */
+    const int32_t *level_offsets; /* [n_levels+1] boxes of level l: [off[l], off[l+1])         */
+    const int32_t *parent;        /* [n_boxes] parent box, -1 for the root                     */
+    const int32_t *sector;        /* [n_boxes] child index 0..7 in its parent, -1 for the root  */
+    const int64_t *dof_begin;     /* [n_boxes] tree-order dof range of the box's subtree        */
+    const int64_t *dof_end;
+    const uint8_t *is_leaf;       /* [n_boxes]                                                  */
+    const void *collection;       /* K x n_dofs (NULL: filled later by h2g_method_fill),
+                                     column j = leaf basis at tree-ordered dof j:
+                                     P2M m = C x (linkmomenttocollection.jl:56-63), L2P y = C^T m
+                                     (linkmomenttocollection.jl:386-411).  Core type.           */
+    const void *transfers;        /* [n_levels][8] K x K: transfers[l][s] maps a child of sector s
+                                     at level l to its parent (leveltranslator.jl:111-116).
+                                     Entries of level 0 are ignored.  Core type.               */
+} h2g_side_desc;
+
+/* Everything a method supplies to a GPU core.  Replaces the core accessors of an H2Factory
+ * GalerkinH2Core / PetrovGalerkinH2Core (src/maps/core.jl:17-54). */
+typedef struct h2g_core_desc {
+    int32_t dtype;                /* H2G_F64 or H2G_C64                                        */
+    int32_t K;                    /* moment length per box                                     */
+    int32_t p;                    /* nodes per axis for the Toeplitz kinds (K == p^3), else 0  */
+    int32_t galerkin;             /* 1: one tree for both sides, `trial` is ignored            */
+    h2g_side_desc test;
+    h2g_side_desc trial;
+
+    /* couplings: forward pairs (receiving test box, translating trial box) */
+    int32_t m2l_kind;
+    int64_t n_pairs;
+    const int32_t *pair_test;
+    const int32_t *pair_trial;
+    const int32_t *pair_op;       /* translation id (canonical id for the SYM kinds)           */
+    const int32_t *pair_sym;      /* permutation id (SYM kinds), else NULL                    */
+    int64_t n_ops;
+    const void *ops;              /* DENSE/SYMDENSE: n_ops K x K, core type.
+                                     TOEPLITZ/SYMTOEPLITZ: n_ops L^3 ComplexF64 spectra, x fastest */
+    int32_t n_perms;
+    const int32_t *perms;         /* SYMDENSE: [n_perms][K] grid permutation (0-based):
+                                       t[perm[i], perm[j]] = canonical[i, j]
+                                     SYMTOEPLITZ: [n_perms][L^3]: D_pair[j] = D_canon[perm[j]]  */
+
+    /* near field: block-sparse over (test leaf, trial leaf) pairs; block b is the
+     * n_rows x n_cols column-major matrix at near_values + near_offset[b]
+     * (the stored operator nearmul! multiplies, src/nearinteractions/nearmul.jl:8-14) */
+    int64_t n_near_blocks;
+    const int32_t *near_test;
+    const int32_t *near_trial;
+    const int64_t *near_offset;
+    int64_t n_near_values;
+    const void *near_values;      /* core type (NULL: filled later by h2g_method_fill)        */
+
+    double alpha_re, alpha_im;    /* the core's far-field prefactor alpha(A)                   */
+    int32_t precisions;           /* bitmask (1 << H2G_PREC_*) of workspace precisions to support */
+    int32_t device;               /* HIP device ordinal                                        */
+} h2g_core_desc;
+
+typedef struct h2g_core h2g_core;
+typedef struct h2g_ws h2g_ws;
+
+/* Workspace options (a workspace is the mutable half of an H2Factory map: H2Workspace,
+ * src/maps/workspace.jl:36-100). */
+typedef struct h2g_ws_opts {
+    int32_t precision;            /* H2G_PREC_PLAIN or H2G_PREC_MIXED                         */
+    int32_t max_rhs;              /* columns per pass; wider x is processed in column blocks  */
+    int32_t use_graph;            /* capture each (op, parts, nrhs) pass into a hipGraph      */
+    int32_t timing;               /* record per-phase / per-level device times                */
+} h2g_ws_opts;
+
+/* One per-phase timing record, shaped like the reference's TimingRecord
+ * (src/diagnostics/timing.jl:11-88). */
+typedef struct h2g_timing_record {
+    int32_t phase;                /* 0 gather, 1 near, 2 p2m, 3 m2m, 4 m2l, 5 l2l, 6 l2p, 7 scatter */
+    int32_t level;                /* -1 for whole-vector phases                               */
+    float ms;
+} h2g_timing_record;
+
+/* --- core ------------------------------------------------------------------------------ */
+/* Builds the read-only device core (GalerkinH2Core / PetrovGalerkinH2Core). */
+int h2g_core_create(const h2g_core_desc *desc, h2g_core **out);
+/* Reference counting: a core is freed when its creator and every workspace released it. */
+int h2g_core_retain(h2g_core *core);
+int h2g_core_release(h2g_core *core);
+/* size(A) (src/maps/abstracth2map.jl:88-98). */
+int h2g_core_size(const h2g_core *core, int64_t *rows, int64_t *cols);
+/* Device bytes held by the core (admission against a memory budget). */
+int h2g_core_device_bytes(const h2g_core *core, int64_t *bytes);
+
+/* --- workspace ------------------------------------------------------------------------- */
+/* H2Workspace(core; ...) (src/maps/workspace.jl:81-100); similarworkspace is a second call
+ * with the same core (src/maps/workspace.jl:109-113). */
+int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out);
+int h2g_ws_destroy(h2g_ws *ws);
+int h2g_ws_device_bytes(const h2g_ws *ws, int64_t *bytes);
+/* The workspace's HIP stream (hipStream_t). */
+void *h2g_ws_stream(h2g_ws *ws);
+
+/* y = op(A) x (alpha == beta == NULL) or y = alpha op(A) x + beta y, for `which` in
+ * H2G_MUL / H2G_FARMUL / H2G_NEARMUL, exactly as the reference's 3- and 5-argument methods.
+ * x is n_in x nrhs (leading dimension ldx), y is n_out x nrhs (ldy), column-major, in the
+ * original dof order, of element type `dtype` (F64/C64 for a plain map; F32/C32 are
+ * accepted and widened on entry / rounded on exit).  alpha and beta point to one scalar of
+ * the complex-or-real type matching dtype.  Returns H2G_EBUSY when the workspace is in use
+ * (acquireworkspace!, src/maps/workspace.jl:304-314). */
+int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs,
+               const void *x, int64_t ldx, void *y, int64_t ldy,
+               const void *alpha, const void *beta, int flags);
+
+/* Exclusive ownership of a workspace across several calls: acquireworkspace! /
+ * releaseworkspace! (src/maps/workspace.jl:304-317).  h2g_matvec on a held workspace
+ * returns H2G_EBUSY, like a nested withworkspace in the reference. */
+int h2g_ws_acquire(h2g_ws *ws);
+int h2g_ws_release(h2g_ws *ws);
+
+/* Waits for the workspace stream (after H2G_ASYNC calls). */
+int h2g_ws_sync(h2g_ws *ws);
+
+/* Timing records of the last matvec (opts.timing != 0). */
+int h2g_ws_timing(h2g_ws *ws, h2g_timing_record *out, int32_t max, int32_t *n);
+
+/* --- method assembly on the device ---------------------------------------------------- */
+/* Fills, on the device, the near-field values and/or the leaf collections of a core that was
+ * created with near_values == NULL and/or collection == NULL, for the point-kernel
+ * interpolation method the benchmarks use: kernel 0 = Laplace 1/(4 pi r), 1 = Helmholtz
+ * exp(i k r)/(4 pi r) (0 on the diagonal); collections are tensor Lagrange bases on the
+ * `nodes` (reference coordinates in [-1, 1]) of each leaf box (center, half width).
+ * The reference leaves this to the downstream method package (near assemblers,
+ * src/nearinteractions/strategies.jl:67-81; collection builders behind
+ * testmomentcollections / trialmomentcollections). */
+typedef struct h2g_method_side_desc {
+    const double *points;         /* [n_dofs][3] original order                               */
+    const double *box_center;     /* [n_boxes][3]                                             */
+    const double *box_halfwidth;  /* [n_boxes]                                                */
+} h2g_method_side_desc;
+
+typedef struct h2g_method_desc {
+    int32_t kernel;
+    double wavenumber;
+    int32_t p;
+    const double *nodes;          /* [p]                                                      */
+    h2g_method_side_desc test;
+    h2g_method_side_desc trial;   /* ignored for a Galerkin core                              */
+    int32_t fill_near;
+    int32_t fill_collections;
+} h2g_method_desc;
+
+int h2g_method_fill(h2g_core *core, const h2g_method_desc *desc);
+
+/* Last error message of the calling thread. */
+int h2g_last_error(char *buf, size_t len);
+/* Library version (major * 10000 + minor * 100 + patch). */
+int h2g_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H2G_H */

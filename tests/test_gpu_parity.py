@@ -1,0 +1,185 @@
+"""HIP path vs the oracle on identical core data (FP64 / ComplexF64 drop-in, rtol 1e-12).
+
+Every product runs through libh2g.so; the oracle traverses the same arrays on the CPU.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import build as OB
+from oracle import h2 as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def relerr(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def make_map(core, **ws):
+    import h2g
+    from convert import to_arrays
+
+    arrays = to_arrays(core)
+    c = h2g.GpuH2Core(arrays, precisions=("plain", "mixed"))
+    return h2g.H2Workspace(c, **ws)
+
+
+def orient(A, op):
+    import h2g
+
+    return {"N": A, "T": h2g.transpose(A), "C": h2g.adjoint(A)}[op]
+
+
+RNG = np.random.default_rng(7)
+PTS_A = RNG.random((260, 3))
+PTS_B = RNG.random((190, 3))
+
+
+@pytest.mark.parametrize("kind", ["dense", "toeplitz", "symdense", "symtoeplitz"])
+@pytest.mark.parametrize("petrov", [False, True])
+def test_random_core_all_products(gpu, kind, petrov):
+    """Random complex operators: every orientation x product x 3/5-argument form."""
+    import h2g
+
+    core = OB.random_core(PTS_A, PTS_B if petrov else None, p=2, m2l_kind=kind, leaf_size=8, seed=11,
+                          alpha=0.8 - 0.3j)
+    A = make_map(core, max_rhs=2)
+    rng = np.random.default_rng(5)
+    a, b = 1.3 - 0.4j, -0.6 + 0.2j
+    for op in "NTC":
+        M = orient(A, op)
+        n_out, n_in = M.shape
+        for nrhs in (1, 3):
+            shp_in = (n_in,) if nrhs == 1 else (n_in, nrhs)
+            shp_out = (n_out,) if nrhs == 1 else (n_out, nrhs)
+            x = rng.standard_normal(shp_in) + 1j * rng.standard_normal(shp_in)
+            y0 = rng.standard_normal(shp_out) + 1j * rng.standard_normal(shp_out)
+            for fn, ref in ((h2g.mul_, O.mul), (h2g.farmul_, O.farmul), (h2g.nearmul_, O.nearmul)):
+                y = np.zeros(shp_out, dtype=np.complex128)
+                fn(y, M, x)
+                assert relerr(y, ref(core, x, op)) < RTOL, (op, nrhs, fn.__name__, "3-arg")
+                y = y0.copy()
+                fn(y, M, x, a, b)
+                assert relerr(y, ref(core, x, op, a, b, y0)) < RTOL, (op, nrhs, fn.__name__, "5-arg")
+
+
+def test_farmul5_leaves_uninitialised_rows(gpu):
+    """farmul!(y, A, x, a, b) writes only leaves with a far contribution
+    (src/farinteractions/traversalhelpers.jl:103-112)."""
+    import h2g
+
+    pts = np.random.default_rng(2).random((70, 3))
+    core = OB.random_core(pts, p=2, m2l_kind="dense", leaf_size=8, seed=3)
+    _, mask = O.far_apply(core, np.ones(70, dtype=np.complex128), "N")
+    assert (~mask).any() and mask.any()
+    A = make_map(core)
+    x = np.random.default_rng(1).standard_normal(70) + 0j
+    y0 = np.arange(70) + 1j
+    y = y0.copy()
+    h2g.farmul_(y, A, x, 2.0, 3.0)
+    assert np.array_equal(y[~mask], y0[~mask])
+    assert relerr(y, O.farmul(core, x, "N", 2.0, 3.0, y0)) < RTOL
+
+
+def test_worked_example(gpu):
+    """docs/src/implementing/example.md:205-256: forward, transpose, adjoint, Matrix(A) and
+    scaled mul! in all orientations equal the dense kernel matrix to rtol 1e-12."""
+    import h2g
+
+    core, s, F = OB.worked_example_core()
+    A = make_map(core, max_rhs=4)
+    n = len(s)
+    j = np.arange(1, n + 1)
+    x = np.sin(j) + 1j * np.cos(2 * j)
+    assert relerr(A @ x, F @ x) < RTOL
+    assert relerr(h2g.transpose(A) @ x, F.T @ x) < RTOL
+    assert relerr(h2g.adjoint(A) @ x, F.conj().T @ x) < RTOL
+    assert relerr(h2g.Matrix(A), F) < RTOL
+    for M, R in ((A, F), (h2g.transpose(A), F.T), (h2g.adjoint(A), F.conj().T)):
+        out = np.full(n, 2.0 + 1j)
+        expected = (2 - 1j) * (R @ x) + (3 + 2j) * out
+        h2g.mul_(out, M, x, 2 - 1j, 3 + 2j)
+        assert relerr(out, expected) < RTOL
+    B = h2g.similarworkspace(A)
+    assert B.core is A.core and B.handle.value != A.handle.value
+    assert relerr(B @ x, F @ x) < RTOL
+
+
+@pytest.mark.parametrize("kind,p,nodes", [("dense", 4, "cheb"), ("toeplitz", 4, "equi"),
+                                          ("symtoeplitz", 4, "equi"), ("symdense", 3, "equi")])
+def test_laplace_method(gpu, kind, p, nodes):
+    """Real Laplace interpolation maps: GPU == oracle to 1e-12, forward and transpose."""
+    import h2g
+
+    pts = np.random.default_rng(4).random((3000, 3))
+    core = OB.build_core(pts, p=p, nodekind=nodes, m2l_kind=kind, leaf_size=64)
+    A = make_map(core, max_rhs=2)
+    x = np.random.default_rng(9).standard_normal((3000, 2))
+    for op in "NT":
+        y = np.zeros((3000, 2))
+        h2g.mul_(y, orient(A, op), x)
+        assert relerr(y, O.mul(core, x, op).real) < RTOL, op
+
+
+def test_helmholtz_complex(gpu):
+    import h2g
+
+    pts = np.random.default_rng(8).standard_normal((1500, 3))
+    pts = 0.5 + 0.45 * pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=10.0,
+                         leaf_size=32)
+    A = make_map(core)
+    x = np.random.default_rng(2).standard_normal(1500) + 1j * np.random.default_rng(3).standard_normal(1500)
+    for op in "NTC":
+        y = np.zeros(1500, dtype=np.complex128)
+        h2g.mul_(y, orient(A, op), x)
+        assert relerr(y, O.mul(core, x, op)) < RTOL, op
+
+
+def test_busy_flag(gpu):
+    """A held workspace rejects products (ConcurrentWorkspaceError), views share the flag."""
+    import h2g
+
+    core, s, F = OB.worked_example_core()
+    A = make_map(core)
+    x = np.ones(len(s), dtype=np.complex128)
+    A.acquire()
+    with pytest.raises(h2g.ConcurrentWorkspaceError):
+        A @ x
+    with pytest.raises(h2g.ConcurrentWorkspaceError):
+        h2g.transpose(A) @ x
+    A.release()
+    assert relerr(A @ x, F @ x) < RTOL
+
+
+def test_graph_replays_match_eager(gpu):
+    """hipGraph-captured passes replayed several times, eager work in between, equal eager."""
+    import h2g
+
+    core = OB.random_core(PTS_A, p=2, m2l_kind="toeplitz", leaf_size=8, seed=2)
+    A = make_map(core, max_rhs=3)
+    G = make_map(core, max_rhs=3, use_graph=True)
+    rng = np.random.default_rng(0)
+    for it in range(4):
+        x = rng.standard_normal((260, 3)) + 1j * rng.standard_normal((260, 3))
+        ye, yg = A @ x, G @ x
+        assert relerr(yg, ye) < 1e-15, it
+        assert relerr(h2g.adjoint(G) @ x, h2g.adjoint(A) @ x) < 1e-15, it
+
+
+def test_device_tensors(gpu):
+    """torch tensors on the GPU go through the same entry point in place."""
+    import torch
+
+    import h2g
+
+    core, s, F = OB.worked_example_core()
+    A = make_map(core, max_rhs=2)
+    x = np.random.default_rng(0).standard_normal((32, 2)) + 0j
+    xt = torch.tensor(x, device="cuda").t().contiguous().t()
+    yt = torch.zeros((32, 2), dtype=torch.complex128, device="cuda").t().contiguous().t()
+    h2g.mul_(yt, A, xt)
+    assert relerr(yt.cpu().numpy(), F @ x) < RTOL
