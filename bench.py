@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""H2 matvec benchmark: N = 2,000,000 points uniform in [0,1]^3 (seed 0), Laplace kernel,
+equispaced p = 6 interpolation (K = 216), Toeplitz M2L (11^3 spectra), leaf size <= 64,
+32 right-hand sides of N(0,1) FP32 entries (seed 1).
+
+One step = one multi-RHS product Y = A X with all 32 columns (near field + far field) in the
+batched mixed-precision form (FP32 far field, FP64 near field and final sum, hipGraph replay).
+The same run times the plain FP64 form one right-hand side at a time, checks the batched
+result against it, reports the error against the O(N^2) direct sum on 1,000 sampled targets,
+measures the dominant kernel's per-launch time with HIP events for the roofline figure and
+times the CPU oracle on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "h2factory.jl_amd"))
+
+PHASES = ["gather", "near", "p2m", "m2m", "m2l_fft_in", "m2l_apply", "m2l_fft_out", "l2l", "l2p", "scatter"]
+PEAK = {"hbm": 8000.0, "fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2_000_000)
+    ap.add_argument("--nrhs", type=int, default=32)
+    ap.add_argument("--p", type=int, default=6)
+    ap.add_argument("--m2l", default="toeplitz")
+    ap.add_argument("--plain-rhs", type=int, default=2, help="columns timed in the plain FP64 form")
+    ap.add_argument("--cpu-n", type=int, default=60_000, help="points of the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    return ap.parse_args()
+
+
+def algorithmic_work(arrays, nrhs, phase, level, mixed):
+    """Algorithmic bytes (HBM-bound phases) or flops (compute phases) of one launch."""
+    import numpy as np
+
+    tr = arrays.test.tree
+    K, p = arrays.K, arrays.p
+    L3 = (2 * p - 1) ** 3
+    cb = 4 if mixed else 8
+    if phase == "near":
+        n = tr.points.shape[0]
+        return "hbm", arrays.n_near_values * 8.0 + 2 * n * nrhs * 8.0
+    if phase == "m2l_apply":
+        lv = tr.level[arrays.pairs[:, 0]]
+        npairs = int(np.sum(lv == level))
+        if arrays.m2l_kind in ("toeplitz", "symtoeplitz"):
+            return "flop", npairs * L3 * 8.0 * nrhs
+        return "flop", npairs * 2.0 * K * K * nrhs
+    if phase in ("p2m", "l2p"):
+        leaves = tr.leaves()
+        n = int(np.sum((tr.end - tr.begin)[leaves[tr.level[leaves] == level]]))
+        return "hbm", n * K * cb + n * nrhs * 8.0 + n * K * 0
+    if phase in ("m2l_fft_in", "m2l_fft_out"):
+        nb = int(tr.level_offsets[level + 1] - tr.level_offsets[level])
+        return "hbm", nb * nrhs * (L3 * 2 * cb + K * cb)
+    if phase in ("m2m", "l2l"):
+        nb = int(tr.level_offsets[level + 1] - tr.level_offsets[level])
+        return "flop", nb * 2.0 * K * K * nrhs
+    n = tr.points.shape[0]
+    return "hbm", 2.0 * n * nrhs * 8
+
+
+def cpu_baseline(n, seed_pts=0, seed_rhs=1, p=6):
+    """The oracle (NumPy port of the reference traversal, FP64, one thread) on a bounded sample."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    sys.path.insert(0, ROOT)
+    from oracle import build as OB
+    from oracle import h2 as O
+
+    pts = np.random.default_rng(seed_pts).random((n, 3))
+    x = np.random.default_rng(seed_rhs).standard_normal(n)
+    core = OB.build_core(pts, p=p, nodekind="equi", m2l_kind="toeplitz", leaf_size=64)
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        O.mul(core, x)
+        dt = time.perf_counter() - t0
+    return dt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
+              "of this path is a design in DESIGN.md, not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import h2g
+    from h2g import method as PM
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    N, R = args.n, args.nrhs
+
+    t0 = time.perf_counter()
+    pts = np.random.default_rng(0).random((N, 3))
+    arrays = PM.build_map(pts, p=args.p, nodekind="equi", m2l_kind=args.m2l, leaf_size=64)
+    t_build = time.perf_counter() - t0
+    log(f"map built on host in {t_build:.1f}s: {arrays.stats}")
+    t0 = time.perf_counter()
+    core = h2g.GpuH2Core(arrays, precisions=("plain", "mixed"))
+    torch.cuda.synchronize()
+    t_core = time.perf_counter() - t0
+    log(f"core on device in {t_core:.1f}s, {core.device_bytes() / 1e9:.1f} GB")
+
+    g = torch.Generator(device="cpu").manual_seed(1)
+    X = torch.randn((R, N), generator=g, dtype=torch.float32).to(dev).t()  # column-major N x R
+    Y = torch.zeros((R, N), dtype=torch.float32, device=dev).t()
+
+    A = h2g.H2Workspace(core, max_rhs=R, precision="mixed", use_graph=not args.no_graph)
+    log(f"batched workspace {A.device_bytes() / 1e9:.1f} GB")
+    for i in range(args.warmup):
+        h2g.mul_(Y, A, X)
+        log(f"warmup {i} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        h2g.mul_(Y, A, X)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = 1e3 * dt / args.steps
+    value = R * args.steps / dt
+    log(f"batched: {ms_per_step:.2f} ms per {R}-RHS product, {value:.1f} RHS/s")
+
+    # plain FP64 form, one right-hand side at a time
+    P = h2g.H2Workspace(core, max_rhs=1, precision="plain")
+    ncol = max(1, min(args.plain_rhs, R))
+    X64 = X[:, :ncol].double().t().contiguous().t()
+    Y64 = torch.zeros((ncol, N), dtype=torch.float64, device=dev).t()
+    h2g.mul_(Y64[:, :1], P, X64[:, :1])  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in range(ncol):
+        h2g.mul_(Y64[:, j:j + 1], P, X64[:, j:j + 1])
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / ncol
+    plain_rhs_per_s = 1.0 / t_plain
+    err_vs_plain = float(torch.linalg.norm(Y[:, :ncol].double() - Y64) / torch.linalg.norm(Y64))
+    log(f"plain FP64: {1e3 * t_plain:.2f} ms per RHS ({plain_rhs_per_s:.1f} RHS/s); "
+        f"batched vs plain rel err {err_vs_plain:.2e}")
+
+    # O(N^2) direct sum on 1,000 sampled targets (FP64, on the GPU)
+    rows = torch.from_numpy(np.random.default_rng(2).choice(N, 1000, replace=False)).to(dev)
+    P_d = torch.from_numpy(pts).to(dev)
+    yd = torch.zeros((1000, ncol), dtype=torch.float64, device=dev)
+    for c0 in range(0, N, 250_000):
+        d = torch.cdist(P_d[rows], P_d[c0:c0 + 250_000])
+        G = torch.where(d > 0, 1.0 / (4 * np.pi * torch.where(d > 0, d, torch.ones_like(d))), torch.zeros_like(d))
+        yd += G @ X64[c0:c0 + 250_000]
+    err_direct_plain = float(torch.linalg.norm(Y64[rows] - yd) / torch.linalg.norm(yd))
+    err_direct_batched = float(torch.linalg.norm(Y[rows, :ncol].double() - yd) / torch.linalg.norm(yd))
+    log(f"direct sum on 1000 targets: plain {err_direct_plain:.2e}, batched {err_direct_batched:.2e}")
+
+    # per-phase device times (HIP events on the workspace stream), same product, eager
+    Tm = h2g.H2Workspace(core, max_rhs=R, precision="mixed", timing=True)
+    h2g.mul_(Y, Tm, X)
+    recs = []
+    for _ in range(2):
+        h2g.mul_(Y, Tm, X)
+        recs.append(Tm.timings())
+    per = {}
+    for rr in recs:
+        for (ph, lvl, ms) in rr:
+            per.setdefault((PHASES[ph], lvl), []).append(ms)
+    totals = {}
+    for (ph, lvl), v in per.items():
+        totals[ph] = totals.get(ph, 0.0) + float(np.mean(v))
+    log("per-phase ms (sum over levels): " + ", ".join(f"{k}={v:.2f}" for k, v in sorted(totals.items(), key=lambda t: -t[1])))
+    (dph, dlv), dms = max(((k, float(np.mean(v))) for k, v in per.items()), key=lambda t: t[1])
+    kind, work = algorithmic_work(arrays, R, dph, dlv, True)
+    if kind == "hbm":
+        achieved = work / (dms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK["hbm"], "unit": "GB/s",
+                "frac": round(achieved / PEAK["hbm"], 4), "traffic": None}
+    else:
+        achieved = work / (dms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK["fp32"], 4), "traffic": None}
+    roof["kernel"] = f"{dph} level {dlv}"
+    roof["launch_ms"] = round(dms, 3)
+
+    cpu = None
+    if not args.no_cpu:
+        dt_cpu = cpu_baseline(args.cpu_n)
+        cpu_rhs_s = 1.0 / (dt_cpu * (N / args.cpu_n))
+        cpu = {"value": round(cpu_rhs_s, 6), "unit": "RHS/s", "cores": 1, "kind": "port",
+               "sample": f"oracle (NumPy, FP64) forward product, 1 RHS, N={args.cpu_n} same distribution/p/leaf size, "
+                         f"{dt_cpu:.1f}s, scaled linearly to N={N}"}
+        log(f"cpu oracle sample: {dt_cpu:.1f}s at N={args.cpu_n}")
+
+    out = {
+        "metric": "H2 matvec RHS/s, N=2M p=6 32 RHS, rel.err<=1e-6 vs FP64, vs plain; p99 latency",
+        "value": round(value, 2), "unit": "RHS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"laplace equispaced p={args.p} {args.m2l} M2L, N={N} uniform cube, leaf<=64, "
+                               f"{R} RHS FP32 batched (FP64 near + final sum)",
+                   "n": N, "nrhs": R, "p": args.p, "m2l": args.m2l, "leaf_size": 64},
+        "plain_fp64_rhs_per_s": round(plain_rhs_per_s, 2),
+        "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
+        "rel_err_vs_plain_fp64": err_vs_plain,
+        "rel_err_vs_direct_sum_1000": {"plain": err_direct_plain, "batched": err_direct_batched},
+        "phase_ms": {k: round(v, 3) for k, v in totals.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "build_s": {"host_plan": round(t_build, 1), "device_core": round(t_core, 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

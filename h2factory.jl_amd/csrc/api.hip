@@ -386,11 +386,20 @@ void far_pass(Workspace &w, int op, int R) {
 
     // upward pass
     for (int l = src.n_levels - 1; l >= 0; --l) {
-        launch_p2m(cd, conj, L + P.p2m[l].begin, P.p2m[l].count, src.d_begin.as<int64_t>(), src.d_end.as<int64_t>(),
-                   scoll, K, w.xs.ptr, R, w.mom.ptr, s);
-        launch_boxgemm(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin, P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l],
-                       P.m2m_src.as<int32_t>(), P.m2m_mat.as<int32_t>(), nullptr, nullptr, nullptr, str, K, w.mom.ptr,
-                       w.mom.ptr, R, 0, nullptr, s);
+        if (P.p2m[l].count) {
+            w.mark_begin(2, l);
+            launch_p2m(cd, conj, L + P.p2m[l].begin, P.p2m[l].count, src.d_begin.as<int64_t>(),
+                       src.d_end.as<int64_t>(), scoll, K, w.xs.ptr, R, w.mom.ptr, s);
+            w.mark_end();
+        }
+        if (P.m2m[l].count) {
+            w.mark_begin(3, l);
+            launch_boxgemm(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
+                           P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
+                           P.m2m_mat.as<int32_t>(), nullptr, nullptr, nullptr, str, K, w.mom.ptr, w.mom.ptr, R, 0,
+                           nullptr, s);
+            w.mark_end();
+        }
     }
     // couplings
     const int opA = op == H2G_OP_N ? 0 : (op == H2G_OP_T ? 1 : 2);
@@ -401,36 +410,70 @@ void far_pass(Workspace &w, int op, int R) {
         const int32_t *recv = L + P.m2l_recv[l].begin;
         const int64_t *iptr = P.m2l_ptr.as<int64_t>() + P.m2l_base[l];
         if (!is_toeplitz(c)) {
+            w.mark_begin(5, l);
             launch_boxgemm(cd, opA, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
                            sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), c.perms_inv.as<int32_t>(),
                            ops, K, w.mom.ptr, w.loc.ptr, R, 0, nullptr, s);
+            w.mark_end();
         } else {
             // forward / adjoint: fft in, ifft out; transpose swaps the two plans
             const bool inv_in = op == H2G_OP_T;
             if (l >= src.n_levels) continue;
             const int box0 = src.level_offsets[l];
             const int nsrc = src.level_offsets[l + 1] - box0;
+            w.mark_begin(4, l);
             launch_fft_in(cd, inv_in, nullptr, nsrc, box0, w.mom.ptr, K, c.p, c.L, R, w.spec_in.ptr, c.twiddle.ptr, s);
+            w.mark_end();
+            w.mark_begin(5, l);
             launch_hadamard(cd, conj, n, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
                             sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), ops, c.L3, box0,
                             w.spec_in.ptr, w.spec_out.ptr, R, s);
+            w.mark_end();
+            w.mark_begin(6, l);
             launch_fft_out(cd, !inv_in, recv, n, w.spec_out.ptr, K, c.p, c.L, R, w.loc.ptr, c.twiddle.ptr, s);
+            w.mark_end();
         }
     }
     // downward pass
     const cplx<double> a = conj ? conjv(c.alpha) : c.alpha;
     for (int l = 0; l < dst.n_levels; ++l) {
-        launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
-                       P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(), P.l2l_mat.as<int32_t>(),
-                       nullptr, nullptr, nullptr, dtr, K, w.loc.ptr, w.loc.ptr, R, 2, P.l2l_acc.as<uint8_t>(), s);
-        launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(), dst.d_end.as<int64_t>(),
-                   dcoll, K, w.loc.ptr, R, w.ys.ptr, a, s);
+        if (P.l2l[l].count) {
+            w.mark_begin(7, l);
+            launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
+                           P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(), P.l2l_mat.as<int32_t>(),
+                           nullptr, nullptr, nullptr, dtr, K, w.loc.ptr, w.loc.ptr, R, 2, P.l2l_acc.as<uint8_t>(), s);
+            w.mark_end();
+        }
+        if (P.l2p[l].count) {
+            w.mark_begin(8, l);
+            launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(),
+                       dst.d_end.as<int64_t>(), dcoll, K, w.loc.ptr, R, w.ys.ptr, a, s);
+            w.mark_end();
+        }
     }
 }
 
 }  // namespace
 
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+void Workspace::mark_begin(int phase, int level) {
+    if (!opts.timing) return;
+    if (n_events_used == events.size()) {
+        hipEvent_t a, b;
+        H2G_HIP(hipEventCreate(&a));
+        H2G_HIP(hipEventCreate(&b));
+        events.emplace_back(a, b);
+    }
+    H2G_HIP(hipEventRecord(events[n_events_used].first, stream));
+    timings.push_back(h2g_timing_record{phase, level, 0.0f});
+}
+
+void Workspace::mark_end() {
+    if (!opts.timing) return;
+    H2G_HIP(hipEventRecord(events[n_events_used].second, stream));
+    ++n_events_used;
+}
 
 int64_t Core::device_bytes() const {
     int64_t b = 0;
@@ -721,16 +764,25 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
         const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
         const Side &src = *P.src, &dst = *P.dst;
         const size_t se = elem_size(c.dtype);
+        const bool use_graph = ws->opts.use_graph && !ws->opts.timing;
+        ws->timings.clear();
+        ws->n_events_used = 0;
         for (int64_t c0 = 0; c0 < nrhs; c0 += ws->R_cap) {
             const int R = (int)std::min<int64_t>(ws->R_cap, nrhs - c0);
+            ws->mark_begin(0, -1);
             launch_gather(c.dtype, dtype, (const char *)dx + (size_t)c0 * lx * xe, lx, src.d_perm.as<int64_t>(),
                           src.n_dofs, R, ws->xs.ptr, s);
+            ws->mark_end();
             auto body = [&]() {
                 launch_fill_zero(ws->ys.ptr, (size_t)dst.n_dofs * R * se, s);
-                if (which & H2G_NEARMUL) launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, s);
+                if (which & H2G_NEARMUL) {
+                    ws->mark_begin(1, -1);
+                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, s);
+                    ws->mark_end();
+                }
                 if (which & H2G_FARMUL) far_pass(*ws, op, R);
             };
-            if (ws->opts.use_graph) {
+            if (use_graph) {
                 GraphKey key{op, which, R};
                 auto it = ws->graphs.find(key);
                 if (it == ws->graphs.end()) {
@@ -753,15 +805,19 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
                 body();
             }
             const uint8_t *mask = (which == H2G_FARMUL && five) ? P.out_mask.as<uint8_t>() : nullptr;
+            ws->mark_begin(9, -1);
             launch_scatter(c.dtype, dtype, ws->ys.ptr, dst.d_perm.as<int64_t>(), dst.n_dofs, R,
                            (char *)dy + (size_t)c0 * ly * xe, ly, a, five ? b : cplx<double>{0.0, 0.0}, mask, s);
+            ws->mark_end();
         }
         if (host) {
             H2G_HIP(hipMemcpy2DAsync(y, ldy * xe, ws->yhost.ptr, n_out * xe, n_out * xe, nrhs, hipMemcpyDeviceToHost, s));
             H2G_HIP(hipStreamSynchronize(s));
-        } else if (!(flags & H2G_ASYNC)) {
+        } else if (!(flags & H2G_ASYNC) || ws->opts.timing) {
             H2G_HIP(hipStreamSynchronize(s));
         }
+        for (size_t i = 0; i < ws->n_events_used && i < ws->timings.size(); ++i)
+            H2G_HIP(hipEventElapsedTime(&ws->timings[i].ms, ws->events[i].first, ws->events[i].second));
     });
     h2g_ws_release(ws);
     return rc;

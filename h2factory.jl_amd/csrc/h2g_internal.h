@@ -250,6 +250,10 @@ struct Workspace {
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<h2g_timing_record> timings;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    size_t n_events_used = 0;
+    // records [start, stop) of one phase when opts.timing is set (eager passes only)
+    void mark_begin(int phase, int level);
+    void mark_end();
     int64_t device_bytes() const;
 };
 

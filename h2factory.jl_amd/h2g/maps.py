@@ -252,6 +252,14 @@ class H2Workspace:
     def sync(self):
         _raise(H.lib().h2g_ws_sync(self.handle))
 
+    def timings(self):
+        """Per-phase device times of the last product: (phase, level, ms) records."""
+        n = C.c_int32()
+        _raise(H.lib().h2g_ws_timing(self.handle, None, 0, C.byref(n)))
+        arr = (H.TimingRecord * max(n.value, 1))()
+        _raise(H.lib().h2g_ws_timing(self.handle, arr, n.value, C.byref(n)))
+        return [(r.phase, r.level, r.ms) for r in arr[:n.value]]
+
     def close(self):
         if getattr(self, "handle", None):
             H.lib().h2g_ws_destroy(self.handle)

@@ -224,7 +224,14 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
         keyarr = np.concatenate([lv[:, None], canon], axis=1)
     else:
         keyarr = np.concatenate([lv[:, None], offs], axis=1)
-    uniq, pair_op = np.unique(keyarr, axis=0, return_inverse=True)
+    # one integer per (level, offset): offsets lie in [-3, 3]^3 for the standard lists; the
+    # generic path keeps np.unique over rows
+    if len(keyarr) and np.abs(keyarr[:, 1:]).max() <= 3:
+        code = ((keyarr[:, 0] * 8 + keyarr[:, 1] + 3) * 8 + keyarr[:, 2] + 3) * 8 + keyarr[:, 3] + 3
+        ucode, first, pair_op = np.unique(code, return_index=True, return_inverse=True)
+        uniq = keyarr[first]
+    else:
+        uniq, pair_op = np.unique(keyarr, axis=0, return_inverse=True)
     pair_op = pair_op.reshape(-1)
     ops = []
     for l in np.unique(uniq[:, 0]) if len(uniq) else []:

@@ -146,13 +146,16 @@ typedef struct h2g_ws_opts {
     int32_t precision;            /* H2G_PREC_PLAIN or H2G_PREC_MIXED                         */
     int32_t max_rhs;              /* columns per pass; wider x is processed in column blocks  */
     int32_t use_graph;            /* capture each (op, parts, nrhs) pass into a hipGraph      */
-    int32_t timing;               /* record per-phase / per-level device times                */
+    int32_t timing;               /* record per-phase / per-level device times (HIP events on the
+                                     workspace stream; passes are then not captured in graphs) */
 } h2g_ws_opts;
 
 /* One per-phase timing record, shaped like the reference's TimingRecord
  * (src/diagnostics/timing.jl:11-88). */
 typedef struct h2g_timing_record {
-    int32_t phase;                /* 0 gather, 1 near, 2 p2m, 3 m2m, 4 m2l, 5 l2l, 6 l2p, 7 scatter */
+    int32_t phase;                /* 0 gather, 1 near, 2 p2m, 3 m2m, 4 m2l modulate-in (FFT),
+                                     5 m2l apply (Hadamard / dense GEMM), 6 m2l modulate-out (iFFT),
+                                     7 l2l, 8 l2p, 9 scatter                                    */
     int32_t level;                /* -1 for whole-vector phases                               */
     float ms;
 } h2g_timing_record;
