@@ -6,7 +6,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["api.hip", "kernels.hip", "method.hip", "m2l_mfma.hip"]
+SOURCES = ["api.hip", "kernels.hip", "method.hip", "m2l_lattice.hip", "m2l_mfma.hip"]
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <complex>
 
 #include "h2g_internal.h"
 
@@ -67,6 +68,11 @@ void validate_side(const h2g_side_desc &s, const char *name, int K) {
             if (b > s.level_offsets[l] && s.parent[b] < s.parent[b - 1]) fail(H2G_EINVAL, n + ": children not grouped by parent");
         }
     if (s.level_offsets[1] != 1) fail(H2G_EINVAL, n + ": level 0 must hold exactly one root box");
+    if (s.coord)
+        for (int b = 1; b < s.n_boxes; ++b)
+            for (int a = 0; a < 3; ++a)
+                if (s.coord[3 * b + a] != 2 * s.coord[3 * s.parent[b] + a] + ((s.sector[b] >> a) & 1))
+                    fail(H2G_EINVAL, n + ": box coordinate is not 2*parent + sector bits");
     // leaves tile the dofs
     std::vector<uint8_t> cover(s.n_dofs, 0);
     for (int b = 0; b < s.n_boxes; ++b)
@@ -142,6 +148,46 @@ void validate_desc(const h2g_core_desc &d) {
 // ------------------------------------------------------------------------------------------
 // Upload helpers
 // ------------------------------------------------------------------------------------------
+// Kronecker factors of every transfer (T = Az (x) Ay (x) Ax, p x p each), if they exist.
+template <class V>
+bool extract_factors(const V *tr, int n_levels, int K, int p, std::vector<V> &fac) {
+    auto mag = [](const V &v) { return std::abs(v); };
+    fac.assign((size_t)n_levels * 8 * 3 * p * p, V(0));
+    for (int l = 1; l < n_levels; ++l)
+        for (int s = 0; s < 8; ++s) {
+            const V *T = tr + ((size_t)l * 8 + s) * K * K;
+            size_t best = 0;
+            double mx = 0.0;
+            for (size_t i = 0; i < (size_t)K * K; ++i)
+                if (mag(T[i]) > mx) mx = mag(T[i]), best = i;
+            if (mx == 0.0) return false;
+            const int ks = (int)(best % K), js = (int)(best / K);
+            const int kx = ks % p, ky = (ks / p) % p, kz = ks / (p * p);
+            const int jx = js % p, jy = (js / p) % p, jz = js / (p * p);
+            auto at = [&](int a, int b, int c, int d, int e, int f) {
+                return T[(a + p * (b + p * c)) + (size_t)K * (d + p * (e + p * f))];
+            };
+            const V piv = at(kx, ky, kz, jx, jy, jz);
+            V *F = fac.data() + ((size_t)l * 8 + s) * 3 * p * p;
+            for (int a = 0; a < p; ++a)
+                for (int b = 0; b < p; ++b) {
+                    F[a + p * b] = at(a, ky, kz, b, jy, jz);
+                    F[p * p + a + p * b] = at(kx, a, kz, jx, b, jz) / piv;
+                    F[2 * p * p + a + p * b] = at(kx, ky, a, jx, jy, b) / piv;
+                }
+            for (int c = 0; c < p; ++c)
+                for (int f = 0; f < p; ++f)
+                    for (int b = 0; b < p; ++b)
+                        for (int e = 0; e < p; ++e)
+                            for (int a = 0; a < p; ++a)
+                                for (int d = 0; d < p; ++d) {
+                                    const V v = F[a + p * d] * F[p * p + b + p * e] * F[2 * p * p + c + p * f];
+                                    if (mag(v - at(a, b, c, d, e, f)) > 1e-12 * mx) return false;
+                                }
+        }
+    return true;
+}
+
 void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
     S.n_dofs = d.n_dofs;
     S.n_levels = d.n_levels;
@@ -153,6 +199,7 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
     S.end.assign(d.dof_end, d.dof_end + d.n_boxes);
     S.is_leaf.assign(d.is_leaf, d.is_leaf + d.n_boxes);
     S.perm.assign(d.perm, d.perm + d.n_dofs);
+    if (d.coord) S.coord.assign(d.coord, d.coord + 3 * (size_t)d.n_boxes);
     S.d_perm.upload_vec(S.perm);
     S.d_begin.upload_vec(S.begin);
     S.d_end.upload_vec(S.end);
@@ -163,6 +210,35 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
     else
         S.coll.alloc(ncoll * es);
     S.transfers.upload(d.transfers, ntr * es);
+    {
+        int p = 1;
+        while (p * p * p < K) ++p;
+        if (p * p * p == K && p > 1 && d.n_levels > 1) {
+            bool ok;
+            size_t nf;
+            if (dtype == H2G_F64) {
+                std::vector<double> fac;
+                ok = extract_factors((const double *)d.transfers, d.n_levels, K, p, fac);
+                if (ok) S.factors.upload_vec(fac);
+                nf = fac.size();
+            } else {
+                std::vector<std::complex<double>> fac;
+                ok = extract_factors((const std::complex<double> *)d.transfers, d.n_levels, K, p, fac);
+                if (ok) S.factors.upload_vec(fac);
+                nf = fac.size() * 2;
+            }
+            if (ok) {
+                S.separable = true;
+                S.sep_p = p;
+                if (want32) {
+                    std::vector<double> host(nf);
+                    H2G_HIP(hipMemcpy(host.data(), S.factors.ptr, nf * sizeof(double), hipMemcpyDeviceToHost));
+                    std::vector<float> f32(host.begin(), host.end());
+                    S.factors32.upload_vec(f32);
+                }
+            }
+        }
+    }
     if (want32) {
         auto narrow = [&](const void *src, size_t n, DevBuf &dst) {
             if (dtype == H2G_F64) {
@@ -188,6 +264,10 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
 int level_of(const std::vector<int32_t> &off, int b) {
     return (int)(std::upper_bound(off.begin(), off.end(), b) - off.begin()) - 1;
 }
+
+void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<int32_t> &lists,
+                         const std::vector<int64_t> &iptr_level, const std::vector<int32_t> &isrc,
+                         const std::vector<int32_t> &iop, const std::vector<int32_t> &isym, bool sym);
 
 void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, const int32_t *recv_of_pair,
                   const int32_t *part_of_pair, const int32_t *pair_op, const int32_t *pair_sym, int64_t n_pairs) {
@@ -306,6 +386,13 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
     P.m2l_src.upload_vec(isrc);
     P.m2l_op.upload_vec(iop);
     if (pair_sym) P.m2l_sym.upload_vec(isym);
+    P.lattice = std::vector<LatticeLevel>(dst.n_levels);
+    if (c.m2l_kind == H2G_M2L_TOEPLITZ || c.m2l_kind == H2G_M2L_SYMTOEPLITZ)
+        for (int l = 0; l < dst.n_levels; ++l) {
+            if (P.m2l_recv[l].count == 0) continue;
+            std::vector<int64_t> lvl(iptr.begin() + P.m2l_base[l], iptr.begin() + P.m2l_base[l] + P.m2l_recv[l].count + 1);
+            build_lattice_level(c, P, l, lists, lvl, isrc, iop, isym, pair_sym != nullptr);
+        }
 
     // output mask: dofs of initialised dst leaves
     std::vector<uint8_t> mask(dst.n_dofs, 0);
@@ -313,6 +400,105 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
         if (init[b] && dst.is_leaf[b])
             for (int64_t j = dst.begin[b]; j < dst.end[b]; ++j) mask[j] = 1;
     P.out_mask.upload_vec(mask);
+}
+
+// Parent-block tables for the lattice M2L kernels of one level; ok = false when the level's
+// couplings do not follow the parent-neighbour lattice with one operator per offset.
+void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<int32_t> &lists,
+                         const std::vector<int64_t> &iptr_level, const std::vector<int32_t> &isrc,
+                         const std::vector<int32_t> &iop, const std::vector<int32_t> &isym, bool sym) {
+    LatticeLevel &LL = P.lattice[l];
+    LL.ok = false;
+    const Side &src = *P.src, &dst = *P.dst;
+    if (src.coord.empty() || dst.coord.empty() || l >= src.n_levels || l < 1) return;
+    const int n = P.m2l_recv[l].count;
+    if (n == 0) return;
+    auto cd = [&](const Side &S, int b, int a) { return S.coord[3 * (size_t)b + a]; };
+    std::vector<int32_t> op_tab(343, -1), sym_tab(343, -1);
+    // blocks = receiving parents
+    std::map<int, int> block_of;
+    std::vector<int> parents;
+    for (int i = 0; i < n; ++i) {
+        const int b = lists[P.m2l_recv[l].begin + i];
+        const int pa = dst.parent[b];
+        if (pa < 0) return;
+        if (!block_of.count(pa)) {
+            block_of[pa] = (int)parents.size();
+            parents.push_back(pa);
+        }
+    }
+    const int nb = (int)parents.size();
+    std::vector<int32_t> tgt((size_t)nb * 8, -1);
+    int64_t n_items = 0;
+    for (int i = 0; i < n; ++i) {
+        const int b = lists[P.m2l_recv[l].begin + i];
+        tgt[(size_t)block_of[dst.parent[b]] * 8 + dst.sector[b]] = i;
+        for (int64_t it = iptr_level[i]; it < iptr_level[i + 1]; ++it) {
+            const int cc = isrc[it];
+            int o[3], dq[3];
+            for (int a = 0; a < 3; ++a) {
+                o[a] = cd(src, cc, a) - cd(dst, b, a);
+                dq[a] = cd(src, src.parent[cc], a) - cd(dst, dst.parent[b], a);
+                if (o[a] < -3 || o[a] > 3 || dq[a] < -1 || dq[a] > 1) return;
+            }
+            if (std::max({std::abs(o[0]), std::abs(o[1]), std::abs(o[2])}) <= 1) return;
+            const int oi = (o[0] + 3) + 7 * (o[1] + 3) + 49 * (o[2] + 3);
+            const int sv = sym ? isym[it] : -1;
+            if (op_tab[oi] < 0) {
+                op_tab[oi] = iop[it];
+                sym_tab[oi] = sv;
+            } else if (op_tab[oi] != iop[it] || sym_tab[oi] != sv) {
+                return;
+            }
+            ++n_items;
+        }
+    }
+    // source slots by coordinate lookup
+    std::map<std::tuple<int, int, int>, int> src_at;
+    for (int b = src.level_offsets[l]; b < src.level_offsets[l + 1]; ++b)
+        src_at[{cd(src, b, 0), cd(src, b, 1), cd(src, b, 2)}] = b - src.level_offsets[l];
+    std::vector<int32_t> bsrc((size_t)216 * nb, -1);
+    std::vector<int32_t> tq_op(8 * 216, -1), tq_sym(8 * 216, -1);
+    for (int q = 0; q < 216; ++q) {
+        const int dn = q / 8, s = q % 8;
+        const int d[3] = {dn % 3 - 1, (dn / 3) % 3 - 1, dn / 9 - 1};
+        for (int t = 0; t < 8; ++t) {
+            int o[3];
+            for (int a = 0; a < 3; ++a) o[a] = 2 * d[a] + ((s >> a) & 1) - ((t >> a) & 1);
+            if (std::max({std::abs(o[0]), std::abs(o[1]), std::abs(o[2])}) <= 1) continue;
+            const int oi = (o[0] + 3) + 7 * (o[1] + 3) + 49 * (o[2] + 3);
+            tq_op[t * 216 + q] = op_tab[oi];
+            tq_sym[t * 216 + q] = sym_tab[oi];
+        }
+    }
+    int64_t implied = 0;
+    for (int k = 0; k < nb; ++k) {
+        const int pa = parents[k];
+        for (int q = 0; q < 216; ++q) {
+            const int dn = q / 8, s = q % 8;
+            const int cx = 2 * (cd(dst, pa, 0) + dn % 3 - 1) + (s & 1);
+            const int cy = 2 * (cd(dst, pa, 1) + (dn / 3) % 3 - 1) + ((s >> 1) & 1);
+            const int cz = 2 * (cd(dst, pa, 2) + dn / 9 - 1) + ((s >> 2) & 1);
+            auto it = src_at.find({cx, cy, cz});
+            if (it == src_at.end()) continue;
+            bsrc[(size_t)q * nb + k] = it->second;
+            for (int t = 0; t < 8; ++t) {
+                if (tgt[(size_t)k * 8 + t] < 0) continue;
+                int o[3] = {2 * (dn % 3 - 1) + (s & 1) - (t & 1), 2 * ((dn / 3) % 3 - 1) + ((s >> 1) & 1) - ((t >> 1) & 1),
+                            2 * (dn / 9 - 1) + ((s >> 2) & 1) - ((t >> 2) & 1)};
+                if (std::max({std::abs(o[0]), std::abs(o[1]), std::abs(o[2])}) <= 1) continue;
+                if (tq_op[t * 216 + q] < 0) return;  // a lattice partner with no coupling
+                ++implied;
+            }
+        }
+    }
+    if (implied != n_items) return;
+    LL.n_blocks = nb;
+    LL.blk_src.upload_vec(bsrc);
+    LL.blk_tgt.upload_vec(tgt);
+    LL.tq_op.upload_vec(tq_op);
+    if (sym) LL.tq_sym.upload_vec(tq_sym);
+    LL.ok = true;
 }
 
 void build_near(Core &c, const h2g_core_desc &d) {
@@ -394,10 +580,16 @@ void far_pass(Workspace &w, int op, int R) {
         }
         if (P.m2m[l].count) {
             w.mark_begin(3, l);
-            launch_boxgemm(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
-                           P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
-                           P.m2m_mat.as<int32_t>(), nullptr, nullptr, nullptr, str, K, w.mom.ptr, w.mom.ptr, R, 0,
-                           nullptr, s);
+            if (src.separable)
+                launch_sep_transfer(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
+                                    P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
+                                    P.m2m_mat.as<int32_t>(), mixed ? src.factors32.ptr : src.factors.ptr, src.sep_p,
+                                    w.mom.ptr, w.mom.ptr, R, 0, nullptr, s);
+            else
+                launch_boxgemm(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
+                               P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
+                               P.m2m_mat.as<int32_t>(), nullptr, nullptr, nullptr, str, K, w.mom.ptr, w.mom.ptr, R, 0,
+                               nullptr, s);
             w.mark_end();
         }
     }
@@ -421,6 +613,24 @@ void far_pass(Workspace &w, int op, int R) {
             if (l >= src.n_levels) continue;
             const int box0 = src.level_offsets[l];
             const int nsrc = src.level_offsets[l + 1] - box0;
+            const LatticeLevel &LL = P.lattice[l];
+            if (LL.ok && R <= 256) {
+                const bool half = c.half_spectrum;
+                w.mark_begin(4, l);
+                launch_fft_in_fm(cd, half, inv_in, box0, nsrc, w.mom.ptr, K, c.p, c.L, R, w.spec_in.ptr,
+                                 c.twiddle.ptr, s);
+                w.mark_end();
+                w.mark_begin(5, l);
+                launch_hadamard_lat(cd, conj, half, LL.n_blocks, LL.blk_src.as<int32_t>(), LL.blk_tgt.as<int32_t>(),
+                                    LL.tq_op.as<int32_t>(), sym ? LL.tq_sym.as<int32_t>() : nullptr,
+                                    c.perms.as<int32_t>(), ops, c.L, nsrc, n, w.spec_in.ptr, w.spec_out.ptr, R, s);
+                w.mark_end();
+                w.mark_begin(6, l);
+                launch_fft_out_fm(cd, half, !inv_in, recv, n, w.spec_out.ptr, K, c.p, c.L, R, w.loc.ptr,
+                                  c.twiddle.ptr, s);
+                w.mark_end();
+                continue;
+            }
             w.mark_begin(4, l);
             launch_fft_in(cd, inv_in, nullptr, nsrc, box0, w.mom.ptr, K, c.p, c.L, R, w.spec_in.ptr, c.twiddle.ptr, s);
             w.mark_end();
@@ -439,9 +649,16 @@ void far_pass(Workspace &w, int op, int R) {
     for (int l = 0; l < dst.n_levels; ++l) {
         if (P.l2l[l].count) {
             w.mark_begin(7, l);
-            launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
-                           P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(), P.l2l_mat.as<int32_t>(),
-                           nullptr, nullptr, nullptr, dtr, K, w.loc.ptr, w.loc.ptr, R, 2, P.l2l_acc.as<uint8_t>(), s);
+            if (dst.separable)
+                launch_sep_transfer(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
+                                    P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),
+                                    P.l2l_mat.as<int32_t>(), mixed ? dst.factors32.ptr : dst.factors.ptr, dst.sep_p,
+                                    w.loc.ptr, w.loc.ptr, R, 2, P.l2l_acc.as<uint8_t>(), s);
+            else
+                launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
+                               P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),
+                               P.l2l_mat.as<int32_t>(), nullptr, nullptr, nullptr, dtr, K, w.loc.ptr, w.loc.ptr, R, 2,
+                               P.l2l_acc.as<uint8_t>(), s);
             w.mark_end();
         }
         if (P.l2p[l].count) {
@@ -548,6 +765,23 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
                 const double *s = (const double *)d.ops;
                 for (size_t i = 0; i < nreal; ++i) v[i] = (float)s[i];
                 c->ops32.upload_vec(v);
+            }
+            if (toep && d.dtype == H2G_F64) {
+                // real map: real-to-complex modulation when every spectrum is Hermitian
+                const cplx<double> *D = (const cplx<double> *)d.ops;
+                const int L = c->L;
+                double mx = 0.0, dev = 0.0;
+                for (int64_t o = 0; o < d.n_ops; ++o)
+                    for (int z = 0; z < L; ++z)
+                        for (int y = 0; y < L; ++y)
+                            for (int x = 0; x < L; ++x) {
+                                const cplx<double> a = D[o * c->L3 + x + L * (y + L * z)];
+                                const cplx<double> b =
+                                    D[o * c->L3 + (L - x) % L + L * ((L - y) % L + L * ((L - z) % L))];
+                                mx = std::max(mx, std::hypot(a.re, a.im));
+                                dev = std::max(dev, std::hypot(a.re - b.re, a.im + b.im));
+                            }
+                c->half_spectrum = dev <= 1e-12 * mx;
             }
             if (d.n_perms > 0) {
                 const int64_t plen = d.m2l_kind == H2G_M2L_SYMDENSE ? d.K : c->L3;

@@ -172,9 +172,13 @@ struct Side {
     std::vector<int64_t> begin, end;
     std::vector<uint8_t> is_leaf;
     std::vector<int64_t> perm;
+    std::vector<int32_t> coord;  // [n_boxes][3] or empty
     DevBuf d_perm, d_begin, d_end;
     DevBuf coll, coll32;            // K x n_dofs core type / reduced type
     DevBuf transfers, transfers32;  // [n_levels][8] K x K
+    bool separable = false;         // every transfer is Az (x) Ay (x) Ax
+    int sep_p = 0;
+    DevBuf factors, factors32;      // [n_levels*8][3][p][p]
     int level_of(int b) const {
         int l = 0;
         while (l + 1 < (int)level_offsets.size() && b >= level_offsets[l + 1]) ++l;
@@ -182,8 +186,19 @@ struct Side {
     }
 };
 
+// Parent-block tables of one level for the lattice M2L kernels.
+struct LatticeLevel {
+    bool ok = false;
+    int n_blocks = 0;
+    DevBuf blk_src;  // int32 [216][n_blocks]: source slot (box - level offset) or -1
+    DevBuf blk_tgt;  // int32 [n_blocks][8]: index into the level's receiving list or -1
+    DevBuf tq_op;    // int32 [8][216]
+    DevBuf tq_sym;   // int32 [8][216] (symmetry-reduced kinds)
+};
+
 // Work lists of one orientation (forward: trial -> test; reverse: test -> trial).
 struct OrientPlan {
+    std::vector<LatticeLevel> lattice;  // per dst level
     const Side *src = nullptr, *dst = nullptr;
     DevBuf lists;                       // concatenated int32 box lists
     std::vector<Span> p2m, m2m, l2l, l2p, m2l_recv;  // per level
@@ -226,6 +241,7 @@ struct Core {
     int precisions = 1;
     OrientPlan fwd, rev;
     DevBuf twiddle;  // c64 [L]: exp(-2 pi i n / L)
+    bool half_spectrum = false;  // real map with Hermitian spectra: real-to-complex modulation
     int64_t device_bytes() const;
 };
 
@@ -283,6 +299,17 @@ void launch_hadamard(int cdtype, bool conj, int n_recv, const int64_t *item_ptr,
                      const int32_t *item_op, const int32_t *item_sym, const int32_t *perms, const void *ops,
                      int L3, int src_slot_base, const void *spec_in, void *spec_out, int R, hipStream_t s);
 void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
+                         const int32_t *item_src, const int32_t *item_mat, const void *factors, int p, const void *in,
+                         void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s);
+void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, const void *mom, int K, int p, int L,
+                      int R, void *S, const void *twiddle, hipStream_t s);
+void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,
+                       int p, int L, int R, void *loc, const void *twiddle, hipStream_t s);
+void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const int32_t *blk_src,
+                         const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
+                         const void *ops, int L, int n_src, int n_recv, const void *S, void *O, int R,
+                         hipStream_t s);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
                     int R, void *loc, const void *twiddle, hipStream_t s);
 

@@ -4,6 +4,7 @@
 //   xs, ys   [dof][R]        FP64 / ComplexF64 (near accumulation and final sum stay FP64)
 //   mom, loc [box][R][K]     compute type (FP64/C64 plain, FP32/C32 mixed)
 //   spectra  [slot][R][L^3]  complex compute type, frequency x fastest
+#include <algorithm>
 #include <type_traits>
 
 #include "h2g_internal.h"
@@ -59,81 +60,197 @@ __global__ void scatter_kernel(const S *__restrict__ ys, const int64_t *__restri
 // near field: one workgroup per target leaf; blocks are column-major nr x nc.
 //   forward:   ys[rb + i] += B[i, j] * xs[cb + j]
 //   transpose: ys[cb + j] += op(B[i, j]) * xs[rb + i]
+// Wide products (R >= 8): a thread owns (row, 8 consecutive RHS); lanes of a wave run over rows
+// so each B load is coalesced and each x load is a broadcast.
+// Narrow products: a thread owns (row, column group); partial sums are reduced through LDS.
 // ------------------------------------------------------------------------------------------
+constexpr int NRC = 8;
+
 template <class S, bool TRANS>
-__global__ void near_kernel(const int32_t *__restrict__ tgt, const int64_t *__restrict__ ptr,
-                            const int32_t *__restrict__ blk, const int64_t *__restrict__ rb,
-                            const int64_t *__restrict__ cb, const int32_t *__restrict__ nr,
-                            const int32_t *__restrict__ nc, const int64_t *__restrict__ off,
-                            const S *__restrict__ vals, const S *__restrict__ xs, S *__restrict__ ys, int R,
-                            bool conj) {
+__global__ void near_wide_kernel(const int32_t *__restrict__ ptr_tgt, const int64_t *__restrict__ ptr,
+                                 const int32_t *__restrict__ blk, const int64_t *__restrict__ rb,
+                                 const int64_t *__restrict__ cb, const int32_t *__restrict__ nr,
+                                 const int32_t *__restrict__ nc, const int64_t *__restrict__ off,
+                                 const S *__restrict__ vals, const S *__restrict__ xs, S *__restrict__ ys, int R,
+                                 bool conj) {
     const int t = blockIdx.x;
     const int64_t b0 = ptr[t], b1 = ptr[t + 1];
     if (b0 == b1) return;
     const int first = blk[b0];
     const int n_out = TRANS ? nc[first] : nr[first];
     const int64_t out0 = TRANS ? cb[first] : rb[first];
-    for (int o = threadIdx.x; o < n_out * R; o += blockDim.x) {
-        const int row = o / R;
-        const int r = o - row * R;
-        S acc = zero_v<S>();
-        for (int64_t q = b0; q < b1; ++q) {
-            const int bl = blk[q];
+    const int nchunk = (R + NRC - 1) / NRC;
+    for (int o = threadIdx.x; o < n_out * nchunk; o += blockDim.x) {
+        const int row = o % n_out;
+        const int r0 = (o / n_out) * NRC;
+        const int nrr = min(NRC, R - r0);
+        S acc[NRC];
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) acc[q] = zero_v<S>();
+        for (int64_t qq = b0; qq < b1; ++qq) {
+            const int bl = blk[qq];
             const int m = nr[bl], n = nc[bl];
             const S *B = vals + off[bl];
             if (!TRANS) {
-                const S *xin = xs + cb[bl] * R + r;
-                for (int j = 0; j < n; ++j) fmacc(acc, B[row + (int64_t)m * j], xin[(int64_t)j * R]);
+                const S *xin = xs + cb[bl] * R + r0;
+                for (int j = 0; j < n; ++j) {
+                    const S a = B[row + (int64_t)m * j];
+                    const S *xr = xin + (int64_t)j * R;
+#pragma unroll
+                    for (int q = 0; q < NRC; ++q)
+                        if (q < nrr) fmacc(acc[q], a, xr[q]);
+                }
             } else {
-                const S *xin = xs + rb[bl] * R + r;
+                const S *xin = xs + rb[bl] * R + r0;
                 const S *col = B + (int64_t)m * row;
-                for (int i = 0; i < m; ++i) fmacc(acc, maybe_conj(col[i], conj), xin[(int64_t)i * R]);
+                for (int i = 0; i < m; ++i) {
+                    const S a = maybe_conj(col[i], conj);
+                    const S *xr = xin + (int64_t)i * R;
+#pragma unroll
+                    for (int q = 0; q < NRC; ++q)
+                        if (q < nrr) fmacc(acc[q], a, xr[q]);
+                }
             }
         }
-        ys[(out0 + row) * R + r] = acc;
+        S *yo = ys + (out0 + row) * R + r0;
+        for (int q = 0; q < nrr; ++q) yo[q] = acc[q];
+    }
+}
+
+template <class S, bool TRANS>
+__global__ void near_narrow_kernel(const int32_t *__restrict__ ptr_tgt, const int64_t *__restrict__ ptr,
+                                   const int32_t *__restrict__ blk, const int64_t *__restrict__ rb,
+                                   const int64_t *__restrict__ cb, const int32_t *__restrict__ nr,
+                                   const int32_t *__restrict__ nc, const int64_t *__restrict__ off,
+                                   const S *__restrict__ vals, const S *__restrict__ xs, S *__restrict__ ys, int R,
+                                   bool conj) {
+    __shared__ S part[256];
+    const int t = blockIdx.x;
+    const int64_t b0 = ptr[t], b1 = ptr[t + 1];
+    if (b0 == b1) return;
+    const int first = blk[b0];
+    const int n_out = TRANS ? nc[first] : nr[first];
+    const int64_t out0 = TRANS ? cb[first] : rb[first];
+    const int G = max(1, (int)blockDim.x / max(n_out, 1));  // column groups per row
+    for (int r = 0; r < R; ++r) {
+        for (int rbase = 0; rbase < n_out; rbase += (int)blockDim.x) {
+            const int rows = min(n_out - rbase, (int)blockDim.x);
+            const int g = min(G, max(1, (int)blockDim.x / rows));
+            const int row = rbase + threadIdx.x % rows;
+            const int jg = threadIdx.x / rows;
+            S acc = zero_v<S>();
+            if (jg < g) {
+                for (int64_t qq = b0; qq < b1; ++qq) {
+                    const int bl = blk[qq];
+                    const int m = nr[bl], n = nc[bl];
+                    const S *B = vals + off[bl];
+                    if (!TRANS) {
+                        const S *xin = xs + cb[bl] * R + r;
+                        for (int j = jg; j < n; j += g) fmacc(acc, B[row + (int64_t)m * j], xin[(int64_t)j * R]);
+                    } else {
+                        const S *xin = xs + rb[bl] * R + r;
+                        const S *col = B + (int64_t)m * row;
+                        for (int i = jg; i < m; i += g) fmacc(acc, maybe_conj(col[i], conj), xin[(int64_t)i * R]);
+                    }
+                }
+            }
+            part[threadIdx.x] = acc;
+            __syncthreads();
+            if (threadIdx.x < rows) {
+                S sum = part[threadIdx.x];
+                for (int q = 1; q < g; ++q) sum = sum + part[threadIdx.x + q * rows];
+                ys[(out0 + rbase + threadIdx.x) * R + r] = sum;
+            }
+            __syncthreads();
+        }
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // P2M: mom[leaf][r][k] = sum_j opC(C[k, b0 + j]) * xs[b0 + j][r]
+// A thread owns (k, 8 RHS); the leaf's x block (converted to the compute type) is staged in
+// LDS in chunks of 64 points, so each C load (coalesced over k) feeds 8 FMAs.
 // ------------------------------------------------------------------------------------------
+constexpr int TRC = 8;  // RHS per thread
+
 template <class T>
 __global__ void p2m_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                            const int64_t *__restrict__ end, const T *__restrict__ coll, int K,
-                           const store_t<T> *__restrict__ xs, int R, T *__restrict__ mom, bool conj) {
+                           const store_t<T> *__restrict__ xs, int R, T *__restrict__ mom, bool conj, int PCH) {
+    extern __shared__ unsigned char smem_raw[];
+    T *xb = reinterpret_cast<T *>(smem_raw);  // [PCH][R]
     const int b = leaves[blockIdx.x];
-    const int64_t j0 = begin[b], j1 = end[b];
-    for (int o = threadIdx.x; o < K * R; o += blockDim.x) {
-        const int r = o / K;
-        const int k = o - r * K;
-        T acc = zero_v<T>();
-        for (int64_t j = j0; j < j1; ++j)
-            fmacc(acc, maybe_conj(coll[k + (int64_t)K * j], conj), cvt<T>::from(xs[j * R + r]));
-        mom[(int64_t)b * K * R + o] = acc;
+    const int64_t j0 = begin[b], n = end[b] - j0;
+    const int nch = (R + TRC - 1) / TRC;
+    const int ntask = K * nch;
+    T *m = mom + (int64_t)b * K * R;
+    for (int task0 = 0; task0 < ntask; task0 += blockDim.x) {
+        const int task = task0 + threadIdx.x;
+        const int k = task % K, r0 = (task / K) * TRC;
+        const int nr = min(TRC, R - r0);
+        T acc[TRC];
+#pragma unroll
+        for (int q = 0; q < TRC; ++q) acc[q] = zero_v<T>();
+        for (int64_t c0 = 0; c0 < n; c0 += PCH) {
+            const int cn = (int)min((int64_t)PCH, n - c0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < cn * R; e += blockDim.x) xb[e] = cvt<T>::from(xs[(j0 + c0) * R + e]);
+            __syncthreads();
+            if (task < ntask) {
+                const T *c = coll + (j0 + c0) * K + k;
+                for (int j = 0; j < cn; ++j) {
+                    const T a = maybe_conj(c[(int64_t)j * K], conj);
+                    const T *xr = xb + j * R + r0;
+#pragma unroll
+                    for (int q = 0; q < TRC; ++q)
+                        if (q < nr) fmacc(acc[q], a, xr[q]);
+                }
+            }
+        }
+        if (task < ntask)
+            for (int q = 0; q < nr; ++q) m[(int64_t)(r0 + q) * K + k] = acc[q];
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // L2P: ys[b0 + j][r] += alpha * sum_k opC(C[k, b0 + j]) * loc[leaf][r][k]
+// The leaf's local moments are staged in LDS as [k][r] (r fastest); a thread owns
+// (point, 8 RHS) and reads its collection column contiguously.
 // ------------------------------------------------------------------------------------------
+constexpr int LRC = 8;  // RHS per LDS stage
+
 template <class T>
 __global__ void l2p_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                            const int64_t *__restrict__ end, const T *__restrict__ coll, int K,
                            const T *__restrict__ loc, int R, store_t<T> *__restrict__ ys, cplx<double> alpha,
                            bool conj) {
     using S = store_t<T>;
+    extern __shared__ unsigned char smem_raw[];
+    T *lb = reinterpret_cast<T *>(smem_raw);  // [K][LRC]
     const int b = leaves[blockIdx.x];
     const int64_t j0 = begin[b], n = end[b] - j0;
     const T *l = loc + (int64_t)b * K * R;
-    for (int64_t o = threadIdx.x; o < n * R; o += blockDim.x) {
-        const int64_t j = o / R;
-        const int r = (int)(o - j * R);
-        const T *c = coll + (j0 + j) * K;
-        const T *lr = l + (int64_t)r * K;
-        T acc = zero_v<T>();
-        for (int k = 0; k < K; ++k) fmacc(acc, maybe_conj(c[k], conj), lr[k]);
-        const int64_t yo = (j0 + j) * R + r;
-        ys[yo] = ys[yo] + scale_by<S>(alpha, cvt<S>::from(acc));
+    for (int r0 = 0; r0 < R; r0 += LRC) {
+        const int nr = min(LRC, R - r0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < K * LRC; e += blockDim.x) {
+            const int k = e / LRC, q = e - k * LRC;
+            lb[e] = q < nr ? l[(int64_t)(r0 + q) * K + k] : zero_v<T>();
+        }
+        __syncthreads();
+        for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+            const T *c = coll + (j0 + j) * K;
+            T acc[LRC];
+#pragma unroll
+            for (int q = 0; q < LRC; ++q) acc[q] = zero_v<T>();
+            for (int k = 0; k < K; ++k) {
+                const T a = maybe_conj(c[k], conj);
+#pragma unroll
+                for (int q = 0; q < LRC; ++q) fmacc(acc[q], a, lb[k * LRC + q]);
+            }
+            S *yo = ys + (j0 + j) * R + r0;
+            for (int q = 0; q < nr; ++q) yo[q] = yo[q] + scale_by<S>(alpha, cvt<S>::from(acc[q]));
+        }
     }
 }
 
@@ -391,18 +508,21 @@ void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys
     const int32_t *tgt = trans ? c.near_rev_tgt.as<int32_t>() : c.near_fwd_tgt.as<int32_t>();
     const int64_t *ptr = trans ? c.near_rev_ptr.as<int64_t>() : c.near_fwd_ptr.as<int64_t>();
     const int32_t *blk = trans ? c.near_rev_blk.as<int32_t>() : c.near_fwd_blk.as<int32_t>();
+    const bool wide = R >= NRC;
     DISPATCH_S(dtype_core, {
-        if (trans)
-            near_kernel<S, true><<<ntgt, TPB, 0, s>>>(tgt, ptr, blk, c.near_blk_rb.as<int64_t>(),
-                                                      c.near_blk_cb.as<int64_t>(), c.near_blk_nr.as<int32_t>(),
-                                                      c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(),
-                                                      c.near_vals.as<S>(), (const S *)xs, (S *)ys, R,
-                                                      op == H2G_OP_C);
-        else
-            near_kernel<S, false><<<ntgt, TPB, 0, s>>>(tgt, ptr, blk, c.near_blk_rb.as<int64_t>(),
-                                                       c.near_blk_cb.as<int64_t>(), c.near_blk_nr.as<int32_t>(),
-                                                       c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(),
-                                                       c.near_vals.as<S>(), (const S *)xs, (S *)ys, R, false);
+        auto args = [&](auto kern) {
+            kern<<<ntgt, TPB, 0, s>>>(tgt, ptr, blk, c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(),
+                                      c.near_blk_nr.as<int32_t>(), c.near_blk_nc.as<int32_t>(),
+                                      c.near_blk_off.as<int64_t>(), c.near_vals.as<S>(), (const S *)xs, (S *)ys, R,
+                                      op == H2G_OP_C);
+        };
+        if (wide) {
+            if (trans) args(near_wide_kernel<S, true>);
+            else args(near_wide_kernel<S, false>);
+        } else {
+            if (trans) args(near_narrow_kernel<S, true>);
+            else args(near_narrow_kernel<S, false>);
+        }
     });
     H2G_CHECK_LAUNCH();
 }
@@ -411,8 +531,9 @@ void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64
                 const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
     if (n == 0) return;
     DISPATCH_T(cdtype, {
-        p2m_kernel<T><<<n, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const store_t<T> *)xs, R,
-                                        (T *)mom, conj);
+        const int pch = std::max(1, std::min(64, (int)(49152 / (sizeof(T) * (size_t)R))));
+        p2m_kernel<T><<<n, TPB, sizeof(T) * pch * (size_t)R, s>>>(leaves, begin, end, (const T *)coll, K,
+                                                                 (const store_t<T> *)xs, R, (T *)mom, conj, pch);
     });
     H2G_CHECK_LAUNCH();
 }
@@ -421,8 +542,8 @@ void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64
                 const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, hipStream_t s) {
     if (n == 0) return;
     DISPATCH_T(cdtype, {
-        l2p_kernel<T><<<n, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R,
-                                        (store_t<T> *)ys, alpha, conj);
+        l2p_kernel<T><<<n, TPB, sizeof(T) * LRC * (size_t)K, s>>>(leaves, begin, end, (const T *)coll, K,
+                                                                 (const T *)loc, R, (store_t<T> *)ys, alpha, conj);
     });
     H2G_CHECK_LAUNCH();
 }
@@ -483,6 +604,116 @@ void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const
         size_t sh = sizeof(C) * (size_t)(L + L * L * L + L * L * p + L * p * p);
         fft_out_kernel<T><<<grid, TPB, sh, s>>>(boxes, (const C *)spec, K, p, L, R, (T *)loc,
                                                 (const cplx<double> *)twiddle, inverse);
+    });
+    H2G_CHECK_LAUNCH();
+}
+
+
+// ------------------------------------------------------------------------------------------
+// Separable level transfers (tensor-product M2M / L2L, the reference's SeparableI2I,
+// src/maps/leveltranslator.jl:168-265): T = Az (x) Ay (x) Ax with p x p factors, applied as
+// three contractions in LDS.  factors: [n_levels*8][3][p][p] column-major per factor (x, y, z).
+// opA: 0 T, 1 T^T, 2 T^H, 3 conj(T); out[t] = (acc ? out[t] : 0) + sum_items op(T_item) in[src].
+// ------------------------------------------------------------------------------------------
+namespace {
+// right-hand sides per LDS pass: keeps 4 K-long buffers under 64 KB at p = 6
+template <class T> constexpr int sjr_of() { return sizeof(T) >= 16 ? 4 : 8; }
+
+template <class T>
+__device__ inline T sep_factor(const T *F, int p, int row, int col, int opA) {
+    // F column-major p x p
+    switch (opA) {
+    case 0: return F[row + p * col];
+    case 1: return F[col + p * row];
+    case 2: return conjv(F[col + p * row]);
+    default: return conjv(F[row + p * col]);
+    }
+}
+
+template <class T, int SJR>
+__global__ void sep_transfer_kernel(int opA, const int32_t *__restrict__ targets, const int64_t *__restrict__ item_ptr,
+                                    const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_mat,
+                                    const T *__restrict__ factors, int p, const T *__restrict__ in,
+                                    T *__restrict__ out, int R, int acc_mode, const uint8_t *__restrict__ acc_flags) {
+    extern __shared__ unsigned char smem_raw[];
+    const int K = p * p * p;
+    T *m = reinterpret_cast<T *>(smem_raw);  // [SJR][K]
+    T *t1 = m + SJR * K;
+    T *t2 = t1 + SJR * K;
+    T *acc = t2 + SJR * K;
+    T *fs = acc + SJR * K;  // [3][p][p] factors of the current item
+    const int ti = blockIdx.x;
+    const int t = targets[ti];
+    const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
+    const bool acc_in = acc_mode == 1 || (acc_mode == 2 && acc_flags[i0]);
+    T *o = out + (int64_t)t * K * R;
+    for (int r0 = 0; r0 < R; r0 += SJR) {
+        const int nr = min(SJR, R - r0);
+        for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
+            const int q = e / K, k = e - q * K;
+            acc[e] = (acc_in && q < nr) ? o[(int64_t)(r0 + q) * K + k] : zero_v<T>();
+        }
+        for (int64_t it = i0; it < i1; ++it) {
+            __syncthreads();
+            const T *F = factors + (int64_t)item_mat[it] * 3 * p * p;
+            for (int e = threadIdx.x; e < 3 * p * p; e += blockDim.x) {
+                const int ax = e / (p * p), rc = e - ax * p * p;
+                fs[e] = sep_factor(F + ax * p * p, p, rc % p, rc / p, opA);  // fs[ax][row + p*col]
+            }
+            const T *x = in + (int64_t)item_src[it] * K * R + (int64_t)r0 * K;
+            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
+                const int q = e / K;
+                m[e] = q < nr ? x[e] : zero_v<T>();
+            }
+            __syncthreads();
+            // x: t1[q][kx + p*(jy + p*jz)] = sum_jx Fx[kx, jx] m[q][jx + p*(jy + p*jz)]
+            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
+                const int q = e / K, k = e - q * K;
+                const int kx = k % p, rest = k / p;
+                T a = zero_v<T>();
+                for (int j = 0; j < p; ++j) fmacc(a, fs[kx + p * j], m[q * K + j + p * rest]);
+                t1[e] = a;
+            }
+            __syncthreads();
+            // y
+            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
+                const int q = e / K, k = e - q * K;
+                const int kx = k % p, ky = (k / p) % p, kz = k / (p * p);
+                T a = zero_v<T>();
+                for (int j = 0; j < p; ++j) fmacc(a, fs[p * p + ky + p * j], t1[q * K + kx + p * (j + p * kz)]);
+                t2[e] = a;
+            }
+            __syncthreads();
+            // z, accumulated
+            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
+                const int q = e / K, k = e - q * K;
+                const int kxy = k % (p * p), kz = k / (p * p);
+                T a = acc[e];
+                for (int j = 0; j < p; ++j) fmacc(a, fs[2 * p * p + kz + p * j], t2[q * K + kxy + p * p * j]);
+                acc[e] = a;
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
+            const int q = e / K, k = e - q * K;
+            if (q < nr) o[(int64_t)(r0 + q) * K + k] = acc[e];
+        }
+        __syncthreads();
+    }
+}
+}  // namespace
+
+void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
+                         const int32_t *item_src, const int32_t *item_mat, const void *factors, int p, const void *in,
+                         void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
+    if (n_targets == 0) return;
+    const int K = p * p * p;
+    DISPATCH_T(cdtype, {
+        constexpr int SJR = sjr_of<T>();
+        const size_t sh = sizeof(T) * (size_t)(4 * SJR * K + 3 * p * p);
+        sep_transfer_kernel<T, SJR><<<n_targets, TPB, sh, s>>>(opA, targets, item_ptr, item_src, item_mat,
+                                                          (const T *)factors, p, (const T *)in, (T *)out, R,
+                                                          acc_mode, acc_flags);
     });
     H2G_CHECK_LAUNCH();
 }

@@ -1,0 +1,331 @@
+// Lattice-blocked Toeplitz M2L.
+//
+// When the couplings of a level follow the parent-neighbour lattice (every receiving box b
+// translates exactly the children of the 27 neighbours of its parent that are not adjacent to
+// b), the level's M2L is a stencil over parent blocks: block = one receiving parent P, its 8
+// child targets t and 216 source slots q = (neighbour d of P, child sector s).  The coupling
+// of (t, q) depends only on the level and the offset 2d + bits(s) - bits(t).
+//
+// Spectra are frequency-major: S[f][slot][r] (slot = source box index within the level) and
+// O[f][recv][r], so that one workgroup works on one frequency f:
+//   1. E_f[t][q] = transform(D[op(t,q)][perm_sym(t,q)(f)]) is expanded into LDS (symmetry-
+//      reduced storage is read through its permutation here, once per frequency);
+//   2. each lane owns (block, r) and accumulates all 8 targets over the 216 slots; the
+//      source value is loaded once and used by up to 8 targets, E_f is an LDS broadcast.
+// For a real map with Hermitian spectra only the half spectrum fx < (L+1)/2 is stored
+// (real-to-complex modulation); complex maps keep the full L^3 spectrum.
+#include <algorithm>
+
+#include "h2g_internal.h"
+
+namespace h2g {
+
+namespace {
+
+constexpr int TPB = 256;
+
+template <class T>
+__device__ inline T maybe_conj2(T v, bool c) {
+    return c ? conjv(v) : v;
+}
+
+// --------------------------------------------------------------------------------------------
+// Modulate-in: pad + 3-D DFT of JB (box, rhs) jobs per workgroup, frequency-major output.
+//   S[f][slot][r], f = fx + FX*(fy + L*fz), FX = HALF ? (L+1)/2 : L.
+// --------------------------------------------------------------------------------------------
+template <class T, bool HALF, int JB>
+__global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, int K, int p, int L, int R,
+                                 typename complex_of<T>::type *__restrict__ S,
+                                 const cplx<double> *__restrict__ twiddle, bool inverse) {
+    using C = typename complex_of<T>::type;
+    using Rl = typename real_of<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    const int FX = HALF ? (L + 1) / 2 : L;
+    const int P3 = p * p * p;
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    C *m = tw + L;                 // [JB][p^3]
+    C *b1 = m + JB * P3;           // [JB][FX*p*p]
+    C *b2 = b1 + JB * FX * p * p;  // [JB][FX*L*p]
+    const int64_t njobs = (int64_t)nbox * R;
+    const int64_t job0 = (int64_t)blockIdx.x * JB;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        cplx<double> w = twiddle[i];
+        tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
+    }
+    // jobs are (slot, r) with r fastest so that consecutive jobs write consecutive addresses
+    for (int i = threadIdx.x; i < JB * P3; i += blockDim.x) {
+        const int j = i / P3, k = i - j * P3;
+        const int64_t job = job0 + j;
+        C v = zero_v<C>();
+        if (job < njobs) {
+            const int64_t slot = job / R;
+            const int r = (int)(job - slot * R);
+            v = cvt<C>::from(mom[((int64_t)(box0 + slot) * R + r) * K + k]);
+        }
+        m[i] = v;
+    }
+    __syncthreads();
+    const int n1 = FX * p * p;
+    for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
+        const int j = i / n1, e = i - j * n1;
+        const int fx = e % FX, rest = e / FX;
+        C acc = zero_v<C>();
+        const C *mj = m + j * P3 + p * rest;
+        for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], mj[ix]);
+        b1[i] = acc;
+    }
+    __syncthreads();
+    const int n2 = FX * L * p;
+    for (int i = threadIdx.x; i < JB * n2; i += blockDim.x) {
+        const int j = i / n2, e = i - j * n2;
+        const int fx = e % FX, fy = (e / FX) % L, iz = e / (FX * L);
+        C acc = zero_v<C>();
+        const C *bj = b1 + j * n1;
+        for (int iy = 0; iy < p; ++iy) fmacc(acc, tw[(fy * iy) % L], bj[fx + FX * (iy + p * iz)]);
+        b2[i] = acc;
+    }
+    __syncthreads();
+    const int F = FX * L * L;
+    const Rl scale = inverse ? (Rl)(1.0 / (double)(L * L * L)) : (Rl)1;
+    const int64_t stride_f = njobs;  // S[f] slice = nbox * R
+    for (int i = threadIdx.x; i < JB * F; i += blockDim.x) {
+        const int j = i % JB, f = i / JB;  // jobs fastest -> coalesced writes
+        const int64_t job = job0 + j;
+        if (job >= njobs) continue;
+        const int fxy = f % (FX * L), fz = f / (FX * L);
+        C acc = zero_v<C>();
+        const C *bj = b2 + j * n2;
+        for (int iz = 0; iz < p; ++iz) fmacc(acc, tw[(fz * iz) % L], bj[fxy + FX * L * iz]);
+        S[(int64_t)f * stride_f + job] = acc * scale;
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// Modulate-out: 3-D inverse DFT + crop of JB (recv, rhs) jobs, from O[f][recv][r]; writes
+// (overwrites) loc[box][r][k].  HALF: real output from the half spectrum.
+// --------------------------------------------------------------------------------------------
+template <class T, bool HALF, int JB>
+__global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nrecv,
+                                  const typename complex_of<T>::type *__restrict__ O, int K, int p, int L, int R,
+                                  T *__restrict__ loc, const cplx<double> *__restrict__ twiddle, bool inverse) {
+    using C = typename complex_of<T>::type;
+    using Rl = typename real_of<T>::type;
+    extern __shared__ unsigned char smem_raw[];
+    const int FX = HALF ? (L + 1) / 2 : L;
+    const int F = FX * L * L;
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    C *s = tw + L;              // [JB][F]
+    C *a1 = s + JB * F;         // [JB][FX*L*p]
+    C *a2 = a1 + JB * FX * L * p;  // [JB][FX*p*p]
+    const int64_t njobs = (int64_t)nrecv * R;
+    const int64_t job0 = (int64_t)blockIdx.x * JB;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        cplx<double> w = twiddle[i];
+        tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
+    }
+    for (int i = threadIdx.x; i < JB * F; i += blockDim.x) {
+        const int j = i % JB, f = i / JB;
+        const int64_t job = job0 + j;
+        s[j * F + f] = job < njobs ? O[(int64_t)f * njobs + job] : zero_v<C>();
+    }
+    __syncthreads();
+    const int n1 = FX * L * p;
+    for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
+        const int j = i / n1, e = i - j * n1;
+        const int fxy = e % (FX * L), iz = e / (FX * L);
+        C acc = zero_v<C>();
+        const C *sj = s + j * F;
+        for (int fz = 0; fz < L; ++fz) fmacc(acc, tw[(fz * iz) % L], sj[fxy + FX * L * fz]);
+        a1[i] = acc;
+    }
+    __syncthreads();
+    const int n2 = FX * p * p;
+    for (int i = threadIdx.x; i < JB * n2; i += blockDim.x) {
+        const int j = i / n2, e = i - j * n2;
+        const int fx = e % FX, iy = (e / FX) % p, iz = e / (FX * p);
+        C acc = zero_v<C>();
+        const C *aj = a1 + j * n1;
+        for (int fy = 0; fy < L; ++fy) fmacc(acc, tw[(fy * iy) % L], aj[fx + FX * (fy + L * iz)]);
+        a2[i] = acc;
+    }
+    __syncthreads();
+    const int P3 = p * p * p;
+    const Rl scale = inverse ? (Rl)(1.0 / (double)(L * L * L)) : (Rl)1;
+    for (int i = threadIdx.x; i < JB * P3; i += blockDim.x) {
+        const int j = i / P3, k = i - j * P3;
+        const int64_t job = job0 + j;
+        if (job >= njobs) continue;
+        const int ix = k % p, rest = k / p;
+        const C *aj = a2 + j * n2 + FX * rest;
+        C acc = zero_v<C>();
+        if (HALF) {
+            // Hermitian spectrum: y = a2[0] + 2 Re sum_{fx>=1} a2[fx] w^(fx*ix)
+            C tail = zero_v<C>();
+            for (int fx = 1; fx < FX; ++fx) fmacc(tail, tw[(fx * ix) % L], aj[fx]);
+            acc = C{aj[0].re + (Rl)2 * tail.re, (Rl)0};
+        } else {
+            for (int fx = 0; fx < FX; ++fx) fmacc(acc, tw[(fx * ix) % L], aj[fx]);
+        }
+        const int64_t slot = job / R;
+        const int r = (int)(job - slot * R);
+        loc[((int64_t)recv_boxes[slot] * R + r) * K + k] = cvt<T>::from(acc * scale);
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// Lattice Hadamard: one frequency per workgroup (blockIdx.x), lanes over (block, r).
+// blk_src: [216][n_blocks] source slot or -1; blk_tgt: [n_blocks][8] receiving index or -1.
+// tq_op / tq_sym: [8][216] level tables (-1: no coupling).
+// --------------------------------------------------------------------------------------------
+// C: storage type of the spectra (cplx<float> / cplx<double>); accumulation is in FP64, which
+// runs at the same vector rate as unpacked FP32 on CDNA4.  D: storage type of the couplings.
+template <class C, class D>
+__global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ blk_src,
+                                    const int32_t *__restrict__ blk_tgt, const int32_t *__restrict__ tq_op,
+                                    const int32_t *__restrict__ tq_sym, const int32_t *__restrict__ perms,
+                                    const D *__restrict__ ops, int L, bool half, int n_src, int n_recv,
+                                    const C *__restrict__ S, C *__restrict__ O, int R, bool conj) {
+    using A = cplx<double>;
+    __shared__ A E[8 * 216];
+    const int f = blockIdx.x;
+    const int L3 = L * L * L;
+    const int FX = half ? (L + 1) / 2 : L;
+    const int fx = f % FX, fy = (f / FX) % L, fz = f / (FX * L);
+    const int ffull = fx + L * (fy + L * fz);
+    for (int i = threadIdx.x; i < 8 * 216; i += blockDim.x) {
+        const int op = tq_op[i];
+        A v = zero_v<A>();
+        if (op >= 0) {
+            const int sym = tq_sym ? tq_sym[i] : -1;
+            const int ff = sym >= 0 ? perms[(int64_t)sym * L3 + ffull] : ffull;
+            v = maybe_conj2(cvt<A>::from(ops[(int64_t)op * L3 + ff]), conj);
+        }
+        E[i] = v;
+    }
+    __syncthreads();
+    const int per_wg = blockDim.x / R;  // blocks per workgroup pass (R <= blockDim.x)
+    const int bl = threadIdx.x / R, r = threadIdx.x - bl * R;
+    if (bl >= per_wg) return;
+    const C *Sf = S + (int64_t)f * n_src * R;
+    C *Of = O + (int64_t)f * n_recv * R;
+    for (int b = blockIdx.y * per_wg + bl; b < n_blocks; b += gridDim.y * per_wg) {
+        A acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = zero_v<A>();
+        for (int q = 0; q < 216; ++q) {
+            const int src = blk_src[(int64_t)q * n_blocks + b];
+            if (src < 0) continue;
+            const A s = cvt<A>::from(Sf[(int64_t)src * R + r]);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) fmacc(acc[t], E[t * 216 + q], s);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int rv = blk_tgt[b * 8 + t];
+            if (rv >= 0) Of[(int64_t)rv * R + r] = cvt<C>::from(acc[t]);
+        }
+    }
+}
+
+template <int JB, class T, bool HALF>
+size_t fft_in_smem(int p, int L) {
+    using C = typename complex_of<T>::type;
+    const int FX = HALF ? (L + 1) / 2 : L;
+    return sizeof(C) * (size_t)(L + JB * (p * p * p + FX * p * p + FX * L * p));
+}
+
+template <int JB, class T, bool HALF>
+size_t fft_out_smem(int p, int L) {
+    using C = typename complex_of<T>::type;
+    const int FX = HALF ? (L + 1) / 2 : L;
+    return sizeof(C) * (size_t)(L + JB * (FX * L * L + FX * L * p + FX * p * p));
+}
+
+}  // namespace
+
+#define H2G_CHECK_LAUNCH2()                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e_)}; \
+    } while (0)
+
+// jobs per workgroup chosen so that LDS stays below 64 KB at p = 6
+template <class T, bool HALF>
+static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, void *S, const void *tw,
+                        bool inverse, hipStream_t s) {
+    constexpr int JB = (sizeof(typename real_of<T>::type) == 4 ? 8 : 4) / (HALF ? 1 : 2);
+    const int64_t njobs = (int64_t)nbox * R;
+    const unsigned grid = (unsigned)((njobs + JB - 1) / JB);
+    fft_in_fm_kernel<T, HALF, JB><<<grid, TPB, fft_in_smem<JB, T, HALF>(p, L), s>>>(
+        box0, nbox, (const T *)mom, K, p, L, R, (typename complex_of<T>::type *)S, (const cplx<double> *)tw, inverse);
+    H2G_CHECK_LAUNCH2();
+}
+
+template <class T, bool HALF>
+static void fft_out_fm_t(const int32_t *recv, int nrecv, const void *O, int K, int p, int L, int R, void *loc,
+                         const void *tw, bool inverse, hipStream_t s) {
+    constexpr int JB = (sizeof(typename real_of<T>::type) == 4 ? 4 : 2) / (HALF ? 1 : 2);
+    const int64_t njobs = (int64_t)nrecv * R;
+    const unsigned grid = (unsigned)((njobs + JB - 1) / JB);
+    fft_out_fm_kernel<T, HALF, JB><<<grid, TPB, fft_out_smem<JB, T, HALF>(p, L), s>>>(
+        recv, nrecv, (const typename complex_of<T>::type *)O, K, p, L, R, (T *)loc, (const cplx<double> *)tw,
+        inverse);
+    H2G_CHECK_LAUNCH2();
+}
+
+void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, const void *mom, int K, int p, int L,
+                      int R, void *S, const void *twiddle, hipStream_t s) {
+    if (nbox == 0) return;
+    switch (cdtype) {
+    case H2G_F64:
+        if (half) fft_in_fm_t<double, true>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+        else fft_in_fm_t<double, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+        break;
+    case H2G_F32:
+        if (half) fft_in_fm_t<float, true>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+        else fft_in_fm_t<float, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+        break;
+    case H2G_C64: fft_in_fm_t<cplx<double>, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s); break;
+    default: fft_in_fm_t<cplx<float>, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+    }
+}
+
+void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,
+                       int p, int L, int R, void *loc, const void *twiddle, hipStream_t s) {
+    if (nrecv == 0) return;
+    switch (cdtype) {
+    case H2G_F64:
+        if (half) fft_out_fm_t<double, true>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s);
+        else fft_out_fm_t<double, false>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s);
+        break;
+    case H2G_F32:
+        if (half) fft_out_fm_t<float, true>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s);
+        else fft_out_fm_t<float, false>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s);
+        break;
+    case H2G_C64: fft_out_fm_t<cplx<double>, false>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s); break;
+    default: fft_out_fm_t<cplx<float>, false>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s);
+    }
+}
+
+void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const int32_t *blk_src,
+                         const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
+                         const void *ops, int L, int n_src, int n_recv, const void *S, void *O, int R,
+                         hipStream_t s) {
+    if (n_blocks == 0) return;
+    const int FX = half ? (L + 1) / 2 : L;
+    const int F = FX * L * L;
+    const int per_wg = TPB / R;
+    const int chunks = std::max(1, std::min((n_blocks + per_wg - 1) / per_wg, 64));
+    dim3 grid(F, chunks);
+    if (cdtype == H2G_F64 || cdtype == H2G_C64)
+        hadamard_lat_kernel<cplx<double>, cplx<double>><<<grid, TPB, 0, s>>>(
+            n_blocks, blk_src, blk_tgt, tq_op, tq_sym, perms, (const cplx<double> *)ops, L, half, n_src, n_recv,
+            (const cplx<double> *)S, (cplx<double> *)O, R, conj);
+    else
+        hadamard_lat_kernel<cplx<float>, cplx<float>><<<grid, TPB, 0, s>>>(
+            n_blocks, blk_src, blk_tgt, tq_op, tq_sym, perms, (const cplx<float> *)ops, L, half, n_src, n_recv,
+            (const cplx<float> *)S, (cplx<float> *)O, R, conj);
+    H2G_CHECK_LAUNCH2();
+}
+
+}  // namespace h2g

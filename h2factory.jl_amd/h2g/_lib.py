@@ -40,7 +40,7 @@ class SideDesc(C.Structure):
         ("n_dofs", C.c_int64), ("n_levels", C.c_int32), ("n_boxes", C.c_int32),
         ("perm", I64P), ("level_offsets", I32P), ("parent", I32P), ("sector", I32P),
         ("dof_begin", I64P), ("dof_end", I64P), ("is_leaf", U8P),
-        ("collection", VP), ("transfers", VP),
+        ("collection", VP), ("transfers", VP), ("coord", I32P),
     ]
 
 

@@ -64,8 +64,9 @@ def _i64(a):
 class GpuH2Core:
     """Read-only device core built from a ``MapArrays`` description."""
 
-    def __init__(self, arrays: MapArrays, precisions=("plain",), device: int = 0, fill=True):
+    def __init__(self, arrays: MapArrays, precisions=("plain",), device: int = 0, fill=True, lattice=True):
         self.arrays = arrays
+        self.use_coords = lattice
         self._keep = []
         d = H.CoreDesc()
         cdt = np.complex128 if arrays.dtype == "c64" else np.float64
@@ -118,7 +119,8 @@ class GpuH2Core:
         arrs = dict(perm=_i64(t.perm), lo=_i32(t.level_offsets), par=_i32(t.parent), sec=_i32(t.sector),
                     b=_i64(t.begin), e=_i64(t.end), leaf=np.ascontiguousarray(t.is_leaf, dtype=np.uint8),
                     coll=None if s.collection is None else np.ascontiguousarray(s.collection, dtype=cdt),
-                    tr=np.ascontiguousarray(s.transfers, dtype=cdt))
+                    tr=np.ascontiguousarray(s.transfers, dtype=cdt),
+                    coord=None if getattr(t, "coord", None) is None else np.ascontiguousarray(t.coord, dtype=np.int32))
         self._keep.append(arrs)
         sd.n_dofs = t.points.shape[0]
         sd.n_levels = t.n_levels
@@ -132,6 +134,7 @@ class GpuH2Core:
         sd.is_leaf = H.ptr(arrs["leaf"], C.c_uint8)
         sd.collection = H.ptr(arrs["coll"]) if arrs["coll"] is not None else None
         sd.transfers = H.ptr(arrs["tr"])
+        sd.coord = H.ptr(arrs["coord"], C.c_int32) if (arrs["coord"] is not None and self.use_coords) else None
         return sd
 
     def _fill(self):

@@ -95,6 +95,10 @@ typedef struct h2g_side_desc {
     const void *transfers;        /* [n_levels][8] K x K: transfers[l][s] maps a child of sector s
                                      at level l to its parent (leveltranslator.jl:111-116).
                                      Entries of level 0 are ignored.  Core type.               */
+    const int32_t *coord;         /* optional [n_boxes][3] integer lattice coordinate of each box
+                                     at its level (child = 2*parent + sector bits).  When given,
+                                     levels whose couplings follow the parent-neighbour lattice run
+                                     the lattice-blocked M2L kernels; NULL keeps the generic ones. */
 } h2g_side_desc;
 
 /* Everything a method supplies to a GPU core.  Replaces the core accessors of an H2Factory

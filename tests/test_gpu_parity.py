@@ -183,3 +183,39 @@ def test_device_tensors(gpu):
     yt = torch.zeros((32, 2), dtype=torch.complex128, device="cuda").t().contiguous().t()
     h2g.mul_(yt, A, xt)
     assert relerr(yt.cpu().numpy(), F @ x) < RTOL
+
+
+@pytest.mark.parametrize("kind", ["toeplitz", "symtoeplitz"])
+@pytest.mark.parametrize("real", [True, False])
+def test_lattice_toeplitz_matches_generic(gpu, kind, real):
+    """The lattice-blocked Toeplitz kernels (frequency-major spectra, half spectra for real maps)
+    against the generic per-pair kernels and the oracle, every orientation, 1..33 RHS."""
+    import h2g
+    from convert import to_arrays
+
+    pts = np.random.default_rng(12).random((4000, 3))
+    if real:
+        core = OB.build_core(pts, p=4, nodekind="equi", m2l_kind=kind, leaf_size=32)
+    else:
+        core = OB.random_core(pts[:900], p=2, m2l_kind=kind, leaf_size=8, seed=4)
+    arrays = to_arrays(core)
+    lat = h2g.H2Workspace(h2g.GpuH2Core(arrays, precisions=("plain", "mixed")), max_rhs=33)
+    gen = h2g.H2Workspace(h2g.GpuH2Core(arrays, precisions=("plain",), lattice=False), max_rhs=33)
+    n = arrays.test.tree.points.shape[0]
+    rng = np.random.default_rng(3)
+    for nrhs in (1, 5, 33):
+        x = rng.standard_normal((n, nrhs))
+        if not real:
+            x = x + 1j * rng.standard_normal((n, nrhs))
+        for op in "NTC":
+            yl = orient(lat, op) @ x
+            yg = orient(gen, op) @ x
+            ref = O.mul(core, x, op)
+            ref = ref.real if real else ref
+            assert relerr(yl, ref) < RTOL, (op, nrhs)
+            assert relerr(yl, yg) < RTOL, (op, nrhs)
+    mixed = h2g.H2Workspace(lat.core, max_rhs=8, precision="mixed")
+    x = rng.standard_normal((n, 8)) + (0 if real else 1j * rng.standard_normal((n, 8)))
+    for op in "NT":
+        ref = O.mul(core, x, op)
+        assert relerr(orient(mixed, op) @ x, ref.real if real else ref) < 1e-5, op
