@@ -188,6 +188,14 @@ def main():
     for (ph, lvl), v in per.items():
         totals[ph] = totals.get(ph, 0.0) + float(np.mean(v))
     log("per-phase ms (sum over levels): " + ", ".join(f"{k}={v:.2f}" for k, v in sorted(totals.items(), key=lambda t: -t[1])))
+    Tp = h2g.H2Workspace(core, max_rhs=1, precision="plain", timing=True)
+    h2g.mul_(Y64[:, :1], Tp, X64[:, :1])
+    h2g.mul_(Y64[:, :1], Tp, X64[:, :1])
+    ptot = {}
+    for (ph, lvl, ms) in Tp.timings():
+        ptot[PHASES[ph]] = ptot.get(PHASES[ph], 0.0) + ms
+    log("plain per-phase ms (1 RHS): " + ", ".join(f"{k}={v:.2f}" for k, v in sorted(ptot.items(), key=lambda t: -t[1])))
+    del Tp
     (dph, dlv), dms = max(((k, float(np.mean(v))) for k, v in per.items()), key=lambda t: t[1])
     kind, work = algorithmic_work(arrays, R, dph, dlv, True)
     if kind == "hbm":
@@ -223,6 +231,7 @@ def main():
         "rel_err_vs_plain_fp64": err_vs_plain,
         "rel_err_vs_direct_sum_1000": {"plain": err_direct_plain, "batched": err_direct_batched},
         "phase_ms": {k: round(v, 3) for k, v in totals.items()},
+        "plain_phase_ms": {k: round(v, 3) for k, v in ptot.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
         "build_s": {"host_plan": round(t_build, 1), "device_core": round(t_core, 1)},

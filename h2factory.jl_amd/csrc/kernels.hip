@@ -64,7 +64,9 @@ __global__ void scatter_kernel(const S *__restrict__ ys, const int64_t *__restri
 // so each B load is coalesced and each x load is a broadcast.
 // Narrow products: a thread owns (row, column group); partial sums are reduced through LDS.
 // ------------------------------------------------------------------------------------------
-constexpr int NRC = 8;
+constexpr int NRC = 8;  // RHS at which the wide kernel takes over
+
+constexpr int NRG = 4;  // rows per thread in the wide kernel
 
 template <class S, bool TRANS>
 __global__ void near_wide_kernel(const int32_t *__restrict__ ptr_tgt, const int64_t *__restrict__ ptr,
@@ -73,47 +75,46 @@ __global__ void near_wide_kernel(const int32_t *__restrict__ ptr_tgt, const int6
                                  const int32_t *__restrict__ nc, const int64_t *__restrict__ off,
                                  const S *__restrict__ vals, const S *__restrict__ xs, S *__restrict__ ys, int R,
                                  bool conj) {
+    // a thread owns NRG rows x one RHS; lanes run over r, so x loads are coalesced and the
+    // NRG block entries of a column are wave-uniform broadcasts
     const int t = blockIdx.x;
     const int64_t b0 = ptr[t], b1 = ptr[t + 1];
     if (b0 == b1) return;
     const int first = blk[b0];
     const int n_out = TRANS ? nc[first] : nr[first];
     const int64_t out0 = TRANS ? cb[first] : rb[first];
-    const int nchunk = (R + NRC - 1) / NRC;
-    for (int o = threadIdx.x; o < n_out * nchunk; o += blockDim.x) {
-        const int row = o % n_out;
-        const int r0 = (o / n_out) * NRC;
-        const int nrr = min(NRC, R - r0);
-        S acc[NRC];
+    const int ngroups = (n_out + NRG - 1) / NRG;
+    for (int o = threadIdx.x; o < ngroups * R; o += blockDim.x) {
+        const int r = o % R;
+        const int g0 = (o / R) * NRG;
+        const int ng = min(NRG, n_out - g0);
+        S acc[NRG];
 #pragma unroll
-        for (int q = 0; q < NRC; ++q) acc[q] = zero_v<S>();
+        for (int q = 0; q < NRG; ++q) acc[q] = zero_v<S>();
         for (int64_t qq = b0; qq < b1; ++qq) {
             const int bl = blk[qq];
             const int m = nr[bl], n = nc[bl];
             const S *B = vals + off[bl];
             if (!TRANS) {
-                const S *xin = xs + cb[bl] * R + r0;
+                const S *xin = xs + cb[bl] * R + r;
                 for (int j = 0; j < n; ++j) {
-                    const S a = B[row + (int64_t)m * j];
-                    const S *xr = xin + (int64_t)j * R;
+                    const S xv = xin[(int64_t)j * R];
+                    const S *Bc = B + (int64_t)m * j + g0;
 #pragma unroll
-                    for (int q = 0; q < NRC; ++q)
-                        if (q < nrr) fmacc(acc[q], a, xr[q]);
+                    for (int q = 0; q < NRG; ++q)
+                        if (q < ng) fmacc(acc[q], Bc[q], xv);
                 }
             } else {
-                const S *xin = xs + rb[bl] * R + r0;
-                const S *col = B + (int64_t)m * row;
+                const S *xin = xs + rb[bl] * R + r;
                 for (int i = 0; i < m; ++i) {
-                    const S a = maybe_conj(col[i], conj);
-                    const S *xr = xin + (int64_t)i * R;
+                    const S xv = xin[(int64_t)i * R];
 #pragma unroll
-                    for (int q = 0; q < NRC; ++q)
-                        if (q < nrr) fmacc(acc[q], a, xr[q]);
+                    for (int q = 0; q < NRG; ++q)
+                        if (q < ng) fmacc(acc[q], maybe_conj(B[i + (int64_t)m * (g0 + q)], conj), xv);
                 }
             }
         }
-        S *yo = ys + (out0 + row) * R + r0;
-        for (int q = 0; q < nrr; ++q) yo[q] = acc[q];
+        for (int q = 0; q < ng; ++q) ys[(out0 + g0 + q) * R + r] = acc[q];
     }
 }
 

@@ -41,10 +41,8 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
     using Rl = typename real_of<T>::type;
     extern __shared__ unsigned char smem_raw[];
     const int FX = HALF ? (L + 1) / 2 : L;
-    const int P3 = p * p * p;
     C *tw = reinterpret_cast<C *>(smem_raw);
-    C *m = tw + L;                 // [JB][p^3]
-    C *b1 = m + JB * P3;           // [JB][FX*p*p]
+    C *b1 = tw + L;                // [JB][FX*p*p]
     C *b2 = b1 + JB * FX * p * p;  // [JB][FX*L*p]
     const int64_t njobs = (int64_t)nbox * R;
     const int64_t job0 = (int64_t)blockIdx.x * JB;
@@ -52,26 +50,20 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
         cplx<double> w = twiddle[i];
         tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
     }
-    // jobs are (slot, r) with r fastest so that consecutive jobs write consecutive addresses
-    for (int i = threadIdx.x; i < JB * P3; i += blockDim.x) {
-        const int j = i / P3, k = i - j * P3;
-        const int64_t job = job0 + j;
-        C v = zero_v<C>();
-        if (job < njobs) {
-            const int64_t slot = job / R;
-            const int r = (int)(job - slot * R);
-            v = cvt<C>::from(mom[((int64_t)(box0 + slot) * R + r) * K + k]);
-        }
-        m[i] = v;
-    }
     __syncthreads();
+    // jobs are (slot, r) with r fastest; a job's moment is K contiguous values of mom
     const int n1 = FX * p * p;
     for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
         const int j = i / n1, e = i - j * n1;
         const int fx = e % FX, rest = e / FX;
+        const int64_t job = job0 + j;
         C acc = zero_v<C>();
-        const C *mj = m + j * P3 + p * rest;
-        for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], mj[ix]);
+        if (job < njobs) {
+            const int64_t slot = job / R;
+            const int r = (int)(job - slot * R);
+            const T *mj = mom + ((int64_t)(box0 + slot) * R + r) * K + p * rest;
+            for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], cvt<C>::from(mj[ix]));
+        }
         b1[i] = acc;
     }
     __syncthreads();
@@ -87,22 +79,22 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
     __syncthreads();
     const int F = FX * L * L;
     const Rl scale = inverse ? (Rl)(1.0 / (double)(L * L * L)) : (Rl)1;
-    const int64_t stride_f = njobs;  // S[f] slice = nbox * R
     for (int i = threadIdx.x; i < JB * F; i += blockDim.x) {
-        const int j = i % JB, f = i / JB;  // jobs fastest -> coalesced writes
+        const int j = i % JB, f = i / JB;  // jobs fastest -> contiguous writes per frequency
         const int64_t job = job0 + j;
         if (job >= njobs) continue;
         const int fxy = f % (FX * L), fz = f / (FX * L);
         C acc = zero_v<C>();
         const C *bj = b2 + j * n2;
         for (int iz = 0; iz < p; ++iz) fmacc(acc, tw[(fz * iz) % L], bj[fxy + FX * L * iz]);
-        S[(int64_t)f * stride_f + job] = acc * scale;
+        S[(int64_t)f * njobs + job] = acc * scale;
     }
 }
 
 // --------------------------------------------------------------------------------------------
 // Modulate-out: 3-D inverse DFT + crop of JB (recv, rhs) jobs, from O[f][recv][r]; writes
-// (overwrites) loc[box][r][k].  HALF: real output from the half spectrum.
+// (overwrites) loc[box][r][k].  HALF: real output from the half spectrum.  The first stage
+// reads the spectrum from global memory (each value is used p times, from cache).
 // --------------------------------------------------------------------------------------------
 template <class T, bool HALF, int JB>
 __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nrecv,
@@ -112,10 +104,8 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
     using Rl = typename real_of<T>::type;
     extern __shared__ unsigned char smem_raw[];
     const int FX = HALF ? (L + 1) / 2 : L;
-    const int F = FX * L * L;
     C *tw = reinterpret_cast<C *>(smem_raw);
-    C *s = tw + L;              // [JB][F]
-    C *a1 = s + JB * F;         // [JB][FX*L*p]
+    C *a1 = tw + L;                // [JB][FX*L*p]
     C *a2 = a1 + JB * FX * L * p;  // [JB][FX*p*p]
     const int64_t njobs = (int64_t)nrecv * R;
     const int64_t job0 = (int64_t)blockIdx.x * JB;
@@ -123,20 +113,17 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
         cplx<double> w = twiddle[i];
         tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
     }
-    for (int i = threadIdx.x; i < JB * F; i += blockDim.x) {
-        const int j = i % JB, f = i / JB;
-        const int64_t job = job0 + j;
-        s[j * F + f] = job < njobs ? O[(int64_t)f * njobs + job] : zero_v<C>();
-    }
     __syncthreads();
     const int n1 = FX * L * p;
     for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
-        const int j = i / n1, e = i - j * n1;
+        const int j = i % JB, e = i / JB;  // jobs fastest -> contiguous reads per frequency
         const int fxy = e % (FX * L), iz = e / (FX * L);
+        const int64_t job = job0 + j;
         C acc = zero_v<C>();
-        const C *sj = s + j * F;
-        for (int fz = 0; fz < L; ++fz) fmacc(acc, tw[(fz * iz) % L], sj[fxy + FX * L * fz]);
-        a1[i] = acc;
+        if (job < njobs)
+            for (int fz = 0; fz < L; ++fz)
+                fmacc(acc, tw[(fz * iz) % L], O[(int64_t)(fxy + FX * L * fz) * njobs + job]);
+        a1[j * n1 + e] = acc;
     }
     __syncthreads();
     const int n2 = FX * p * p;
@@ -173,7 +160,9 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
 }
 
 // --------------------------------------------------------------------------------------------
-// Lattice Hadamard: one frequency per workgroup (blockIdx.x), lanes over (block, r).
+// Lattice Hadamard: one frequency per workgroup (blockIdx.y), lanes over (block, r); the
+// block chunks of one frequency are consecutive workgroups (blockIdx.x fastest), so the
+// frequency slice they share stays in L2 while they run.
 // blk_src: [216][n_blocks] source slot or -1; blk_tgt: [n_blocks][8] receiving index or -1.
 // tq_op / tq_sym: [8][216] level tables (-1: no coupling).
 // --------------------------------------------------------------------------------------------
@@ -187,7 +176,7 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
                                     const C *__restrict__ S, C *__restrict__ O, int R, bool conj) {
     using A = cplx<double>;
     __shared__ A E[8 * 216];
-    const int f = blockIdx.x;
+    const int f = blockIdx.y;
     const int L3 = L * L * L;
     const int FX = half ? (L + 1) / 2 : L;
     const int fx = f % FX, fy = (f / FX) % L, fz = f / (FX * L);
@@ -208,7 +197,7 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
     if (bl >= per_wg) return;
     const C *Sf = S + (int64_t)f * n_src * R;
     C *Of = O + (int64_t)f * n_recv * R;
-    for (int b = blockIdx.y * per_wg + bl; b < n_blocks; b += gridDim.y * per_wg) {
+    for (int b = blockIdx.x * per_wg + bl; b < n_blocks; b += gridDim.x * per_wg) {
         A acc[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) acc[t] = zero_v<A>();
@@ -228,17 +217,40 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
 }
 
 template <int JB, class T, bool HALF>
-size_t fft_in_smem(int p, int L) {
+size_t fft_smem(int p, int L) {
     using C = typename complex_of<T>::type;
     const int FX = HALF ? (L + 1) / 2 : L;
-    return sizeof(C) * (size_t)(L + JB * (p * p * p + FX * p * p + FX * L * p));
+    return sizeof(C) * (size_t)(L + JB * (FX * p * p + FX * L * p));
 }
 
-template <int JB, class T, bool HALF>
-size_t fft_out_smem(int p, int L) {
-    using C = typename complex_of<T>::type;
-    const int FX = HALF ? (L + 1) / 2 : L;
-    return sizeof(C) * (size_t)(L + JB * (FX * L * L + FX * L * p + FX * p * p));
+// jobs per workgroup: 16 for FP32 half spectra at p = 6 (78 KB of LDS), fewer for wider types
+template <class T, bool HALF>
+constexpr int fft_jobs() {
+    return (sizeof(typename real_of<T>::type) == 4 ? 16 : 8) / (HALF ? 1 : 2) / (is_complex<T>::value ? 1 : 1);
+}
+
+// LDS a workgroup may allocate on the current device (queried once per device)
+size_t device_lds_limit() {
+    static size_t cache[64] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) return 65536;
+    if (!cache[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        cache[dev] = (size_t)v;
+    }
+    return cache[dev];
+}
+
+template <class K>
+bool allow_lds(K kernel, size_t bytes) {
+    if (bytes > device_lds_limit()) return false;
+    if (bytes > 65536)
+        return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+               hipSuccess;
+    return true;
 }
 
 }  // namespace
@@ -249,27 +261,41 @@ size_t fft_out_smem(int p, int L) {
         if (e_ != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e_)}; \
     } while (0)
 
-// jobs per workgroup chosen so that LDS stays below 64 KB at p = 6
 template <class T, bool HALF>
 static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, void *S, const void *tw,
                         bool inverse, hipStream_t s) {
-    constexpr int JB = (sizeof(typename real_of<T>::type) == 4 ? 8 : 4) / (HALF ? 1 : 2);
+    constexpr int JB = fft_jobs<T, HALF>();
+    constexpr int JH = JB / 2;
     const int64_t njobs = (int64_t)nbox * R;
-    const unsigned grid = (unsigned)((njobs + JB - 1) / JB);
-    fft_in_fm_kernel<T, HALF, JB><<<grid, TPB, fft_in_smem<JB, T, HALF>(p, L), s>>>(
-        box0, nbox, (const T *)mom, K, p, L, R, (typename complex_of<T>::type *)S, (const cplx<double> *)tw, inverse);
+    const size_t sh = fft_smem<JB, T, HALF>(p, L);
+    if (allow_lds(fft_in_fm_kernel<T, HALF, JB>, sh)) {
+        fft_in_fm_kernel<T, HALF, JB><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+            box0, nbox, (const T *)mom, K, p, L, R, (typename complex_of<T>::type *)S, (const cplx<double> *)tw,
+            inverse);
+    } else {
+        fft_in_fm_kernel<T, HALF, JH><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(p, L), s>>>(
+            box0, nbox, (const T *)mom, K, p, L, R, (typename complex_of<T>::type *)S, (const cplx<double> *)tw,
+            inverse);
+    }
     H2G_CHECK_LAUNCH2();
 }
 
 template <class T, bool HALF>
 static void fft_out_fm_t(const int32_t *recv, int nrecv, const void *O, int K, int p, int L, int R, void *loc,
                          const void *tw, bool inverse, hipStream_t s) {
-    constexpr int JB = (sizeof(typename real_of<T>::type) == 4 ? 4 : 2) / (HALF ? 1 : 2);
+    constexpr int JB = fft_jobs<T, HALF>();
+    constexpr int JH = JB / 2;
     const int64_t njobs = (int64_t)nrecv * R;
-    const unsigned grid = (unsigned)((njobs + JB - 1) / JB);
-    fft_out_fm_kernel<T, HALF, JB><<<grid, TPB, fft_out_smem<JB, T, HALF>(p, L), s>>>(
-        recv, nrecv, (const typename complex_of<T>::type *)O, K, p, L, R, (T *)loc, (const cplx<double> *)tw,
-        inverse);
+    const size_t sh = fft_smem<JB, T, HALF>(p, L);
+    if (allow_lds(fft_out_fm_kernel<T, HALF, JB>, sh)) {
+        fft_out_fm_kernel<T, HALF, JB><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+            recv, nrecv, (const typename complex_of<T>::type *)O, K, p, L, R, (T *)loc, (const cplx<double> *)tw,
+            inverse);
+    } else {
+        fft_out_fm_kernel<T, HALF, JH><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(p, L), s>>>(
+            recv, nrecv, (const typename complex_of<T>::type *)O, K, p, L, R, (T *)loc, (const cplx<double> *)tw,
+            inverse);
+    }
     H2G_CHECK_LAUNCH2();
 }
 
@@ -315,8 +341,8 @@ void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const i
     const int FX = half ? (L + 1) / 2 : L;
     const int F = FX * L * L;
     const int per_wg = TPB / R;
-    const int chunks = std::max(1, std::min((n_blocks + per_wg - 1) / per_wg, 64));
-    dim3 grid(F, chunks);
+    const int chunks = std::max(1, std::min((n_blocks + per_wg - 1) / per_wg, 128));
+    dim3 grid(chunks, F);
     if (cdtype == H2G_F64 || cdtype == H2G_C64)
         hadamard_lat_kernel<cplx<double>, cplx<double>><<<grid, TPB, 0, s>>>(
             n_blocks, blk_src, blk_tgt, tq_op, tq_sym, perms, (const cplx<double> *)ops, L, half, n_src, n_recv,
