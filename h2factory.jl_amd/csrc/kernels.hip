@@ -97,7 +97,22 @@ __global__ void near_wide_kernel(const int32_t *__restrict__ ptr_tgt, const int6
             const S *B = vals + off[bl];
             if (!TRANS) {
                 const S *xin = xs + cb[bl] * R + r;
-                for (int j = 0; j < n; ++j) {
+                int j = 0;
+                for (; j + 4 <= n; j += 4) {
+                    S xv[4], bv[4][NRG];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        xv[u] = xin[(int64_t)(j + u) * R];
+                        const S *Bc = B + (int64_t)m * (j + u) + g0;
+#pragma unroll
+                        for (int q = 0; q < NRG; ++q) bv[u][q] = q < ng ? Bc[q] : zero_v<S>();
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int q = 0; q < NRG; ++q) fmacc(acc[q], bv[u][q], xv[u]);
+                }
+                for (; j < n; ++j) {
                     const S xv = xin[(int64_t)j * R];
                     const S *Bc = B + (int64_t)m * j + g0;
 #pragma unroll
@@ -106,7 +121,22 @@ __global__ void near_wide_kernel(const int32_t *__restrict__ ptr_tgt, const int6
                 }
             } else {
                 const S *xin = xs + rb[bl] * R + r;
-                for (int i = 0; i < m; ++i) {
+                int i = 0;
+                for (; i + 4 <= m; i += 4) {
+                    S xv[4], bv[4][NRG];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        xv[u] = xin[(int64_t)(i + u) * R];
+#pragma unroll
+                        for (int q = 0; q < NRG; ++q)
+                            bv[u][q] = q < ng ? maybe_conj(B[i + u + (int64_t)m * (g0 + q)], conj) : zero_v<S>();
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int q = 0; q < NRG; ++q) fmacc(acc[q], bv[u][q], xv[u]);
+                }
+                for (; i < m; ++i) {
                     const S xv = xin[(int64_t)i * R];
 #pragma unroll
                     for (int q = 0; q < NRG; ++q)

@@ -33,14 +33,15 @@ __device__ inline T maybe_conj2(T v, bool c) {
 // Modulate-in: pad + 3-D DFT of JB (box, rhs) jobs per workgroup, frequency-major output.
 //   S[f][slot][r], f = fx + FX*(fy + L*fz), FX = HALF ? (L+1)/2 : L.
 // --------------------------------------------------------------------------------------------
-template <class T, bool HALF, int JB>
-__global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, int K, int p, int L, int R,
+template <class T, bool HALF, int JB, int P>
+__global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, int K, int R,
                                  typename complex_of<T>::type *__restrict__ S,
                                  const cplx<double> *__restrict__ twiddle, bool inverse) {
     using C = typename complex_of<T>::type;
     using Rl = typename real_of<T>::type;
     extern __shared__ unsigned char smem_raw[];
-    const int FX = HALF ? (L + 1) / 2 : L;
+    constexpr int p = P, L = 2 * P - 1;
+    constexpr int FX = HALF ? (L + 1) / 2 : L;
     C *tw = reinterpret_cast<C *>(smem_raw);
     C *b1 = tw + L;                // [JB][FX*p*p]
     C *b2 = b1 + JB * FX * p * p;  // [JB][FX*L*p]
@@ -52,7 +53,7 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
     }
     __syncthreads();
     // jobs are (slot, r) with r fastest; a job's moment is K contiguous values of mom
-    const int n1 = FX * p * p;
+    constexpr int n1 = FX * p * p;
     for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
         const int j = i / n1, e = i - j * n1;
         const int fx = e % FX, rest = e / FX;
@@ -67,7 +68,7 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
         b1[i] = acc;
     }
     __syncthreads();
-    const int n2 = FX * L * p;
+    constexpr int n2 = FX * L * p;
     for (int i = threadIdx.x; i < JB * n2; i += blockDim.x) {
         const int j = i / n2, e = i - j * n2;
         const int fx = e % FX, fy = (e / FX) % L, iz = e / (FX * L);
@@ -77,7 +78,7 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
         b2[i] = acc;
     }
     __syncthreads();
-    const int F = FX * L * L;
+    constexpr int F = FX * L * L;
     const Rl scale = inverse ? (Rl)(1.0 / (double)(L * L * L)) : (Rl)1;
     for (int i = threadIdx.x; i < JB * F; i += blockDim.x) {
         const int j = i % JB, f = i / JB;  // jobs fastest -> contiguous writes per frequency
@@ -96,14 +97,15 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
 // (overwrites) loc[box][r][k].  HALF: real output from the half spectrum.  The first stage
 // reads the spectrum from global memory (each value is used p times, from cache).
 // --------------------------------------------------------------------------------------------
-template <class T, bool HALF, int JB>
+template <class T, bool HALF, int JB, int P>
 __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nrecv,
-                                  const typename complex_of<T>::type *__restrict__ O, int K, int p, int L, int R,
+                                  const typename complex_of<T>::type *__restrict__ O, int K, int R,
                                   T *__restrict__ loc, const cplx<double> *__restrict__ twiddle, bool inverse) {
     using C = typename complex_of<T>::type;
     using Rl = typename real_of<T>::type;
     extern __shared__ unsigned char smem_raw[];
-    const int FX = HALF ? (L + 1) / 2 : L;
+    constexpr int p = P, L = 2 * P - 1;
+    constexpr int FX = HALF ? (L + 1) / 2 : L;
     C *tw = reinterpret_cast<C *>(smem_raw);
     C *a1 = tw + L;                // [JB][FX*L*p]
     C *a2 = a1 + JB * FX * L * p;  // [JB][FX*p*p]
@@ -114,7 +116,7 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
         tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
     }
     __syncthreads();
-    const int n1 = FX * L * p;
+    constexpr int n1 = FX * L * p;
     for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
         const int j = i % JB, e = i / JB;  // jobs fastest -> contiguous reads per frequency
         const int fxy = e % (FX * L), iz = e / (FX * L);
@@ -126,7 +128,7 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
         a1[j * n1 + e] = acc;
     }
     __syncthreads();
-    const int n2 = FX * p * p;
+    constexpr int n2 = FX * p * p;
     for (int i = threadIdx.x; i < JB * n2; i += blockDim.x) {
         const int j = i / n2, e = i - j * n2;
         const int fx = e % FX, iy = (e / FX) % p, iz = e / (FX * p);
@@ -136,7 +138,7 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
         a2[i] = acc;
     }
     __syncthreads();
-    const int P3 = p * p * p;
+    constexpr int P3 = p * p * p;
     const Rl scale = inverse ? (Rl)(1.0 / (double)(L * L * L)) : (Rl)1;
     for (int i = threadIdx.x; i < JB * P3; i += blockDim.x) {
         const int j = i / P3, k = i - j * P3;
@@ -197,16 +199,26 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
     if (bl >= per_wg) return;
     const C *Sf = S + (int64_t)f * n_src * R;
     C *Of = O + (int64_t)f * n_recv * R;
+    constexpr int U = 8;  // slots in flight per lane
     for (int b = blockIdx.x * per_wg + bl; b < n_blocks; b += gridDim.x * per_wg) {
         A acc[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) acc[t] = zero_v<A>();
-        for (int q = 0; q < 216; ++q) {
-            const int src = blk_src[(int64_t)q * n_blocks + b];
-            if (src < 0) continue;
-            const A s = cvt<A>::from(Sf[(int64_t)src * R + r]);
+        int src[U];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) fmacc(acc[t], E[t * 216 + q], s);
+        for (int u = 0; u < U; ++u) src[u] = blk_src[(int64_t)u * n_blocks + b];
+        for (int q0 = 0; q0 < 216; q0 += U) {
+            A sv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) sv[u] = src[u] >= 0 ? cvt<A>::from(Sf[(int64_t)src[u] * R + r]) : zero_v<A>();
+            if (q0 + U < 216) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) src[u] = blk_src[(int64_t)(q0 + U + u) * n_blocks + b];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int t = 0; t < 8; ++t) fmacc(acc[t], E[t * 216 + q0 + u], sv[u]);
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -261,42 +273,70 @@ bool allow_lds(K kernel, size_t bytes) {
         if (e_ != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e_)}; \
     } while (0)
 
-template <class T, bool HALF>
-static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, void *S, const void *tw,
-                        bool inverse, hipStream_t s) {
+template <class T, bool HALF, int P>
+static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, void *S, const void *tw, bool inverse,
+                        hipStream_t s) {
     constexpr int JB = fft_jobs<T, HALF>();
     constexpr int JH = JB / 2;
+    constexpr int L = 2 * P - 1;
     const int64_t njobs = (int64_t)nbox * R;
-    const size_t sh = fft_smem<JB, T, HALF>(p, L);
-    if (allow_lds(fft_in_fm_kernel<T, HALF, JB>, sh)) {
-        fft_in_fm_kernel<T, HALF, JB><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
-            box0, nbox, (const T *)mom, K, p, L, R, (typename complex_of<T>::type *)S, (const cplx<double> *)tw,
-            inverse);
+    const size_t sh = fft_smem<JB, T, HALF>(P, L);
+    using C = typename complex_of<T>::type;
+    if (allow_lds(fft_in_fm_kernel<T, HALF, JB, P>, sh)) {
+        fft_in_fm_kernel<T, HALF, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+            box0, nbox, (const T *)mom, K, R, (C *)S, (const cplx<double> *)tw, inverse);
     } else {
-        fft_in_fm_kernel<T, HALF, JH><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(p, L), s>>>(
-            box0, nbox, (const T *)mom, K, p, L, R, (typename complex_of<T>::type *)S, (const cplx<double> *)tw,
-            inverse);
+        fft_in_fm_kernel<T, HALF, JH, P><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(P, L), s>>>(
+            box0, nbox, (const T *)mom, K, R, (C *)S, (const cplx<double> *)tw, inverse);
+    }
+    H2G_CHECK_LAUNCH2();
+}
+
+template <class T, bool HALF, int P>
+static void fft_out_fm_p(const int32_t *recv, int nrecv, const void *O, int K, int R, void *loc, const void *tw,
+                         bool inverse, hipStream_t s) {
+    constexpr int JB = fft_jobs<T, HALF>();
+    constexpr int JH = JB / 2;
+    constexpr int L = 2 * P - 1;
+    const int64_t njobs = (int64_t)nrecv * R;
+    const size_t sh = fft_smem<JB, T, HALF>(P, L);
+    using C = typename complex_of<T>::type;
+    if (allow_lds(fft_out_fm_kernel<T, HALF, JB, P>, sh)) {
+        fft_out_fm_kernel<T, HALF, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+            recv, nrecv, (const C *)O, K, R, (T *)loc, (const cplx<double> *)tw, inverse);
+    } else {
+        fft_out_fm_kernel<T, HALF, JH, P><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(P, L), s>>>(
+            recv, nrecv, (const C *)O, K, R, (T *)loc, (const cplx<double> *)tw, inverse);
     }
     H2G_CHECK_LAUNCH2();
 }
 
 template <class T, bool HALF>
+static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, void *S, const void *tw,
+                        bool inverse, hipStream_t s) {
+    switch (p) {
+    case 2: fft_in_fm_p<T, HALF, 2>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
+    case 3: fft_in_fm_p<T, HALF, 3>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
+    case 4: fft_in_fm_p<T, HALF, 4>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
+    case 5: fft_in_fm_p<T, HALF, 5>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
+    case 6: fft_in_fm_p<T, HALF, 6>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
+    default: throw Error{H2G_EUNSUPPORTED, "Toeplitz modulation supports 2 <= p <= 6"};
+    }
+    (void)L;
+}
+
+template <class T, bool HALF>
 static void fft_out_fm_t(const int32_t *recv, int nrecv, const void *O, int K, int p, int L, int R, void *loc,
                          const void *tw, bool inverse, hipStream_t s) {
-    constexpr int JB = fft_jobs<T, HALF>();
-    constexpr int JH = JB / 2;
-    const int64_t njobs = (int64_t)nrecv * R;
-    const size_t sh = fft_smem<JB, T, HALF>(p, L);
-    if (allow_lds(fft_out_fm_kernel<T, HALF, JB>, sh)) {
-        fft_out_fm_kernel<T, HALF, JB><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
-            recv, nrecv, (const typename complex_of<T>::type *)O, K, p, L, R, (T *)loc, (const cplx<double> *)tw,
-            inverse);
-    } else {
-        fft_out_fm_kernel<T, HALF, JH><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(p, L), s>>>(
-            recv, nrecv, (const typename complex_of<T>::type *)O, K, p, L, R, (T *)loc, (const cplx<double> *)tw,
-            inverse);
+    switch (p) {
+    case 2: fft_out_fm_p<T, HALF, 2>(recv, nrecv, O, K, R, loc, tw, inverse, s); break;
+    case 3: fft_out_fm_p<T, HALF, 3>(recv, nrecv, O, K, R, loc, tw, inverse, s); break;
+    case 4: fft_out_fm_p<T, HALF, 4>(recv, nrecv, O, K, R, loc, tw, inverse, s); break;
+    case 5: fft_out_fm_p<T, HALF, 5>(recv, nrecv, O, K, R, loc, tw, inverse, s); break;
+    case 6: fft_out_fm_p<T, HALF, 6>(recv, nrecv, O, K, R, loc, tw, inverse, s); break;
+    default: throw Error{H2G_EUNSUPPORTED, "Toeplitz modulation supports 2 <= p <= 6"};
     }
-    H2G_CHECK_LAUNCH2();
+    (void)L;
 }
 
 void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, const void *mom, int K, int p, int L,
