@@ -493,8 +493,23 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
         }
     }
     if (implied != n_items) return;
+    // compact each block's present sources to the front (the kernel loops over them only)
+    std::vector<int32_t> csrc((size_t)216 * nb, -1), cq((size_t)216 * nb, 0), nq(nb, 0);
+    for (int k = 0; k < nb; ++k) {
+        int c = 0;
+        for (int q = 0; q < 216; ++q) {
+            const int v = bsrc[(size_t)q * nb + k];
+            if (v < 0) continue;
+            csrc[(size_t)c * nb + k] = v;
+            cq[(size_t)c * nb + k] = q;
+            ++c;
+        }
+        nq[k] = c;
+    }
     LL.n_blocks = nb;
-    LL.blk_src.upload_vec(bsrc);
+    LL.blk_src.upload_vec(csrc);
+    LL.blk_q.upload_vec(cq);
+    LL.blk_nq.upload_vec(nq);
     LL.blk_tgt.upload_vec(tgt);
     LL.tq_op.upload_vec(tq_op);
     if (sym) LL.tq_sym.upload_vec(tq_sym);
@@ -621,7 +636,8 @@ void far_pass(Workspace &w, int op, int R) {
                                  c.twiddle.ptr, s);
                 w.mark_end();
                 w.mark_begin(5, l);
-                launch_hadamard_lat(cd, conj, half, LL.n_blocks, LL.blk_src.as<int32_t>(), LL.blk_tgt.as<int32_t>(),
+                launch_hadamard_lat(cd, conj, half, LL.n_blocks, LL.blk_src.as<int32_t>(), LL.blk_q.as<int32_t>(),
+                                    LL.blk_nq.as<int32_t>(), LL.blk_tgt.as<int32_t>(),
                                     LL.tq_op.as<int32_t>(), sym ? LL.tq_sym.as<int32_t>() : nullptr,
                                     c.perms.as<int32_t>(), ops, c.L, nsrc, n, w.spec_in.ptr, w.spec_out.ptr, R, s);
                 w.mark_end();

@@ -190,7 +190,9 @@ struct Side {
 struct LatticeLevel {
     bool ok = false;
     int n_blocks = 0;
-    DevBuf blk_src;  // int32 [216][n_blocks]: source slot (box - level offset) or -1
+    DevBuf blk_src;  // int32 [216][n_blocks]: present sources first (compacted), then -1
+    DevBuf blk_q;    // int32 [216][n_blocks]: slot id q of each compacted entry
+    DevBuf blk_nq;   // int32 [n_blocks]: number of present sources
     DevBuf blk_tgt;  // int32 [n_blocks][8]: index into the level's receiving list or -1
     DevBuf tq_op;    // int32 [8][216]
     DevBuf tq_sym;   // int32 [8][216] (symmetry-reduced kinds)
@@ -307,7 +309,7 @@ void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, c
 void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,
                        int p, int L, int R, void *loc, const void *twiddle, hipStream_t s);
 void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const int32_t *blk_src,
-                         const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
+                         const int32_t *blk_q, const int32_t *blk_nq, const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
                          const void *ops, int L, int n_src, int n_recv, const void *S, void *O, int R,
                          hipStream_t s);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,

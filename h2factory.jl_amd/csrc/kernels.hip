@@ -532,7 +532,10 @@ void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *
     H2G_CHECK_LAUNCH();
 }
 
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s);
+
 void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
+    if (launch_near_mfma(op, c, xs, ys, R, s)) return;
     const bool trans = op != H2G_OP_N;
     const int ntgt = trans ? c.n_near_rev_tgt : c.n_near_fwd_tgt;
     if (ntgt == 0) return;
@@ -749,4 +752,126 @@ void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targ
     H2G_CHECK_LAUNCH();
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Near field on FP64 MFMA (real maps, R >= 16): Y_t (n_out x R) += B (n_out x n_in) X (n_in x R)
+// with v_mfma_f64_16x16x4f64.  One workgroup (4 waves) per target leaf; a wave owns 16-row
+// output tiles across all 16-column RHS tiles.  Operand lane layouts of the 16x16x4 FP64
+// instruction: A[m][k] in lane m + 16k, B[k][n] in lane n + 16k, D[m][n] in lane n + 16(m/4),
+// element m % 4.
+// ------------------------------------------------------------------------------------------
+namespace {
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline int mf_a_row(int lane) { return lane & 15; }
+__device__ inline int mf_a_k(int lane) { return lane >> 4; }
+__device__ inline int mf_d_row(int lane, int v) { return 4 * (lane >> 4) + v; }
+
+template <bool TRANS>
+__global__ void near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t *__restrict__ blk,
+                                 const int64_t *__restrict__ rb, const int64_t *__restrict__ cb,
+                                 const int32_t *__restrict__ nr, const int32_t *__restrict__ nc,
+                                 const int64_t *__restrict__ off, const double *__restrict__ vals,
+                                 const double *__restrict__ xs, double *__restrict__ ys, int R) {
+    constexpr int MAXCT = 4;  // up to 64 RHS per pass
+    const int t = blockIdx.x;
+    const int64_t b0 = ptr[t], b1 = ptr[t + 1];
+    if (b0 == b1) return;
+    const int first = blk[b0];
+    const int n_out = TRANS ? nc[first] : nr[first];
+    const int64_t out0 = TRANS ? cb[first] : rb[first];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int n_rt = (n_out + 15) / 16;
+    for (int c0 = 0; c0 < R; c0 += 16 * MAXCT) {
+        const int n_ct = min(MAXCT, (R - c0 + 15) / 16);
+        for (int rt = wave; rt < n_rt; rt += nwaves) {
+            double4_t acc[MAXCT];
+#pragma unroll
+            for (int c = 0; c < MAXCT; ++c) acc[c] = double4_t{0.0, 0.0, 0.0, 0.0};
+            const int arow = rt * 16 + mf_a_row(lane);  // output row this lane feeds in A
+            const int ak = mf_a_k(lane);                // k offset within the step
+            const int bn = lane & 15, bk = lane >> 4;   // B operand: column n, k offset
+            for (int64_t q = b0; q < b1; ++q) {
+                const int bl = blk[q];
+                const int m = nr[bl], n = nc[bl];
+                const double *B = vals + off[bl];
+                const int kdim = TRANS ? m : n;
+                const double *xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
+                for (int k0 = 0; k0 < kdim; k0 += 4) {
+                    const int kk = k0 + ak;
+                    double a = 0.0;
+                    if (arow < n_out && kk < kdim) a = TRANS ? B[kk + (int64_t)m * arow] : B[arow + (int64_t)m * kk];
+                    const int kb = k0 + bk;
+#pragma unroll
+                    for (int c = 0; c < MAXCT; ++c) {
+                        if (c < n_ct) {
+                            const int col = c0 + 16 * c + bn;
+                            const double bv = (kb < kdim && col < R) ? xin[(int64_t)kb * R + col] : 0.0;
+                            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[c], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < MAXCT; ++c) {
+                if (c >= n_ct) continue;
+                const int col = c0 + 16 * c + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = rt * 16 + mf_d_row(lane, v);
+                    if (row < n_out && col < R) ys[(out0 + row) * R + col] = acc[c][v];
+                }
+            }
+        }
+    }
+}
+
+// probe of the operand / result lane layout used above: D = A (16x4) * B (4x16)
+__global__ void mfma_f64_probe_kernel(const double *__restrict__ A, const double *__restrict__ B,
+                                      double *__restrict__ D) {
+    const int lane = threadIdx.x & 63;
+    const double a = A[mf_a_row(lane) * 4 + mf_a_k(lane)];
+    const double b = B[(lane >> 4) * 16 + (lane & 15)];
+    double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) D[mf_d_row(lane, v) * 16 + (lane & 15)] = acc[v];
+}
+}  // namespace
+
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
+    if (c.dtype != H2G_F64 || R < 16) return false;
+    const bool trans = op != H2G_OP_N;
+    const int ntgt = trans ? c.n_near_rev_tgt : c.n_near_fwd_tgt;
+    if (ntgt == 0) return true;
+    const int64_t *ptr = trans ? c.near_rev_ptr.as<int64_t>() : c.near_fwd_ptr.as<int64_t>();
+    const int32_t *blk = trans ? c.near_rev_blk.as<int32_t>() : c.near_fwd_blk.as<int32_t>();
+    auto go = [&](auto kern) {
+        kern<<<ntgt, TPB, 0, s>>>(ptr, blk, c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(),
+                                  c.near_blk_nr.as<int32_t>(), c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(),
+                                  c.near_vals.as<double>(), (const double *)xs, (double *)ys, R);
+    };
+    if (trans) go(near_mfma_kernel<true>);
+    else go(near_mfma_kernel<false>);
+    H2G_CHECK_LAUNCH();
+    return true;
+}
+
 }  // namespace h2g
+
+// Test hook: runs the MFMA layout probe on host arrays (A 16x4, B 4x16 row-major; D 16x16).
+extern "C" int h2g_probe_mfma_f64(const double *A, const double *B, double *D) {
+    using namespace h2g;
+    try {
+        DevBuf a, b, d;
+        a.upload(A, 64 * sizeof(double));
+        b.upload(B, 64 * sizeof(double));
+        d.alloc(256 * sizeof(double));
+        mfma_f64_probe_kernel<<<1, 64>>>(a.as<double>(), b.as<double>(), d.as<double>());
+        H2G_HIP(hipGetLastError());
+        H2G_HIP(hipMemcpy(D, d.ptr, 256 * sizeof(double), hipMemcpyDeviceToHost));
+        return H2G_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+}

@@ -172,6 +172,7 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
 // runs at the same vector rate as unpacked FP32 on CDNA4.  D: storage type of the couplings.
 template <class C, class D>
 __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ blk_src,
+                                    const int32_t *__restrict__ blk_q, const int32_t *__restrict__ blk_nq,
                                     const int32_t *__restrict__ blk_tgt, const int32_t *__restrict__ tq_op,
                                     const int32_t *__restrict__ tq_sym, const int32_t *__restrict__ perms,
                                     const D *__restrict__ ops, int L, bool half, int n_src, int n_recv,
@@ -204,21 +205,32 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
         A acc[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) acc[t] = zero_v<A>();
-        int src[U];
+        // present sources are compacted to the front of the block's column of blk_src
+        const int nq = blk_nq[b];
+        int src[U], qs[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) src[u] = blk_src[(int64_t)u * n_blocks + b];
-        for (int q0 = 0; q0 < 216; q0 += U) {
+        for (int u = 0; u < U; ++u) {
+            src[u] = u < nq ? blk_src[(int64_t)u * n_blocks + b] : -1;
+            qs[u] = u < nq ? blk_q[(int64_t)u * n_blocks + b] : 0;
+        }
+        for (int i0 = 0; i0 < nq; i0 += U) {
             A sv[U];
+            int qc[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) sv[u] = src[u] >= 0 ? cvt<A>::from(Sf[(int64_t)src[u] * R + r]) : zero_v<A>();
-            if (q0 + U < 216) {
+            for (int u = 0; u < U; ++u) {
+                sv[u] = src[u] >= 0 ? cvt<A>::from(Sf[(int64_t)src[u] * R + r]) : zero_v<A>();
+                qc[u] = qs[u];
+            }
 #pragma unroll
-                for (int u = 0; u < U; ++u) src[u] = blk_src[(int64_t)(q0 + U + u) * n_blocks + b];
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + U + u;
+                src[u] = i < nq ? blk_src[(int64_t)i * n_blocks + b] : -1;
+                qs[u] = i < nq ? blk_q[(int64_t)i * n_blocks + b] : 0;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int t = 0; t < 8; ++t) fmacc(acc[t], E[t * 216 + q0 + u], sv[u]);
+                for (int t = 0; t < 8; ++t) fmacc(acc[t], E[t * 216 + qc[u]], sv[u]);
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -374,7 +386,7 @@ void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv,
 }
 
 void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const int32_t *blk_src,
-                         const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
+                         const int32_t *blk_q, const int32_t *blk_nq, const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
                          const void *ops, int L, int n_src, int n_recv, const void *S, void *O, int R,
                          hipStream_t s) {
     if (n_blocks == 0) return;
@@ -385,11 +397,11 @@ void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const i
     dim3 grid(chunks, F);
     if (cdtype == H2G_F64 || cdtype == H2G_C64)
         hadamard_lat_kernel<cplx<double>, cplx<double>><<<grid, TPB, 0, s>>>(
-            n_blocks, blk_src, blk_tgt, tq_op, tq_sym, perms, (const cplx<double> *)ops, L, half, n_src, n_recv,
+            n_blocks, blk_src, blk_q, blk_nq, blk_tgt, tq_op, tq_sym, perms, (const cplx<double> *)ops, L, half, n_src, n_recv,
             (const cplx<double> *)S, (cplx<double> *)O, R, conj);
     else
         hadamard_lat_kernel<cplx<float>, cplx<float>><<<grid, TPB, 0, s>>>(
-            n_blocks, blk_src, blk_tgt, tq_op, tq_sym, perms, (const cplx<float> *)ops, L, half, n_src, n_recv,
+            n_blocks, blk_src, blk_q, blk_nq, blk_tgt, tq_op, tq_sym, perms, (const cplx<float> *)ops, L, half, n_src, n_recv,
             (const cplx<float> *)S, (cplx<float> *)O, R, conj);
     H2G_CHECK_LAUNCH2();
 }

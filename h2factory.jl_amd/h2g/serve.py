@@ -1,0 +1,242 @@
+"""Serving loop: resident H2 maps, continuous per-map batching, memory-budget admission.
+
+An iterative solver issues one matvec at a time per map; many solvers (or many right-hand
+sides of one) run against a few resident maps.  The server keeps the built cores in HBM and
+turns the incoming stream into multi-RHS products:
+
+* a map is admitted when its core plus its workspaces fit the HBM budget
+  (``AdmissionError`` otherwise); each map owns ``ws_per_map`` workspaces of ``max_batch``
+  columns, so a batch needs no new memory once its map is admitted;
+* requests queue per (map, orientation); whenever a workspace of that map is free, up to
+  ``max_batch`` queued requests form one product -- requests that arrived while earlier
+  batches were on the GPU join the next batch (continuous batching);
+* each workspace runs on its own HIP stream (``h2g_matvec`` with ``H2G_ASYNC``): host->device
+  copy, product and device->host copy are enqueued back to back, so batches of different
+  workspaces and maps overlap on the device while the host forms the next batch;
+* completion is polled with events; latency = completion time - arrival time.
+
+Each result is the product of its own column: the kernels treat the columns of a batch
+independently, so a request's result is the same alone and in the mix (the near-field kernel
+for narrow batches sums in a different order than the one for wide batches; the difference is
+at the FP64 rounding level).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as H
+from .maps import GpuH2Core, H2Workspace, _raise
+
+OPS = {"N": H.H2G_OP_N, "T": H.H2G_OP_T, "C": H.H2G_OP_C}
+
+
+class AdmissionError(RuntimeError):
+    """A map does not fit the server's HBM budget."""
+
+
+@dataclass
+class Request:
+    map_name: str
+    op: str
+    x: np.ndarray
+    t_arrival: float = 0.0
+    t_done: float = None
+    result: np.ndarray = None
+    batch_size: int = 0
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_arrival
+
+
+@dataclass
+class _Slot:
+    ws: H2Workspace
+    stream: object
+    xdev: object
+    ydev: object
+    xhost: object
+    yhost: object
+    event: object = None
+    batch: list = field(default_factory=list)
+
+
+class _ResidentMap:
+    def __init__(self, name, core, n_ws, max_batch, precision, dtype):
+        import torch
+
+        self.name = name
+        self.core = core
+        self.queues = {op: deque() for op in OPS}
+        self.slots = []
+        n_out, n_in = core.shape
+        n = max(n_out, n_in)
+        for _ in range(n_ws):
+            ws = H2Workspace(core, max_rhs=max_batch, precision=precision, use_graph=True)
+            st = torch.cuda.ExternalStream(ws.stream())
+            xd = torch.zeros((max_batch, n), dtype=dtype, device="cuda")
+            yd = torch.zeros((max_batch, n), dtype=dtype, device="cuda")
+            xh = torch.zeros((max_batch, n), dtype=dtype).pin_memory()
+            yh = torch.zeros((max_batch, n), dtype=dtype).pin_memory()
+            self.slots.append(_Slot(ws, st, xd, yd, xh, yh))
+        self.n = n
+
+    def device_bytes(self):
+        return self.core.device_bytes() + sum(s.ws.device_bytes() + 2 * s.xdev.numel() * s.xdev.element_size()
+                                              for s in self.slots)
+
+
+def estimate_core_bytes(arrays, precisions=("plain", "mixed")):
+    """Device bytes of a core before it is built (admission check)."""
+    es = 16 if arrays.dtype == "c64" else 8
+    n = arrays.test.tree.points.shape[0] + (0 if arrays.galerkin else arrays.trial.tree.points.shape[0])
+    b = arrays.n_near_values * es
+    coll = n * arrays.K * es
+    b += coll * (1.5 if "mixed" in precisions else 1.0)
+    b += arrays.ops.nbytes * (1.5 if "mixed" in precisions else 1.0)
+    b += arrays.test.transfers.nbytes * 1.5
+    b += len(arrays.near_pairs) * 40 + len(arrays.pairs) * 24 + n * 24
+    return int(b)
+
+
+class Server:
+    """Continuous-batching request loop over resident maps."""
+
+    def __init__(self, budget_bytes: int, max_batch: int = 32, ws_per_map: int = 2, precision: str = "mixed"):
+        self.budget = int(budget_bytes)
+        self.max_batch = max_batch
+        self.ws_per_map = ws_per_map
+        self.precision = precision
+        self.maps = {}
+        self.done = []
+
+    def used_bytes(self):
+        return sum(m.device_bytes() for m in self.maps.values())
+
+    def add_map(self, name, arrays):
+        """Admit and build a map, or raise AdmissionError when it does not fit the budget."""
+        import torch
+
+        est = estimate_core_bytes(arrays)
+        n = arrays.test.tree.points.shape[0]
+        es = 16 if arrays.dtype == "c64" else 8
+        ws_est = self.ws_per_map * (n * self.max_batch * (4 * es) + arrays.test.tree.n_boxes * arrays.K *
+                                    self.max_batch * es + 2 * 726 * arrays.test.tree.n_boxes * self.max_batch * es)
+        if self.used_bytes() + est + ws_est > self.budget:
+            raise AdmissionError(f"map {name!r} needs ~{(est + ws_est) / 1e9:.1f} GB, "
+                                 f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
+        core = GpuH2Core(arrays, precisions=("plain", self.precision))
+        dtype = torch.complex64 if arrays.dtype == "c64" else torch.float32
+        rm = _ResidentMap(name, core, self.ws_per_map, self.max_batch, self.precision, dtype)
+        self.maps[name] = rm
+        if self.used_bytes() > self.budget:
+            del self.maps[name]
+            raise AdmissionError(f"map {name!r} exceeded the budget once built")
+        return rm
+
+    def remove_map(self, name):
+        rm = self.maps.pop(name)
+        for s in rm.slots:
+            s.stream.synchronize()
+
+    def submit(self, req: Request):
+        if req.map_name not in self.maps:
+            raise KeyError(req.map_name)
+        self.maps[req.map_name].queues[req.op].append(req)
+
+    def pending(self):
+        return sum(len(q) for m in self.maps.values() for q in m.queues.values()) + \
+            sum(len(s.batch) for m in self.maps.values() for s in m.slots)
+
+    def _launch(self, rm: _ResidentMap, slot: _Slot, op: str):
+        import torch
+
+        q = rm.queues[op]
+        batch = [q.popleft() for _ in range(min(self.max_batch, len(q)))]
+        nb = len(batch)
+        n_out, n_in = rm.core.shape
+        if op != "N":
+            n_out, n_in = n_in, n_out
+        for j, r in enumerate(batch):
+            slot.xhost[j, :n_in].copy_(torch.from_numpy(np.ascontiguousarray(r.x)))
+        with torch.cuda.stream(slot.stream):
+            slot.xdev[:nb, :n_in].copy_(slot.xhost[:nb, :n_in], non_blocking=True)
+        xd = slot.xdev.t()
+        yd = slot.ydev.t()
+        dt = H.H2G_C32 if slot.xdev.dtype == torch.complex64 else H.H2G_F32
+        _raise(H.lib().h2g_matvec(slot.ws.handle, OPS[op], H.H2G_MUL, dt, nb, C.c_void_p(xd.data_ptr()),
+                                  slot.xdev.shape[1], C.c_void_p(yd.data_ptr()), slot.ydev.shape[1], None, None,
+                                  H.H2G_ASYNC))
+        with torch.cuda.stream(slot.stream):
+            slot.yhost[:nb, :n_out].copy_(slot.ydev[:nb, :n_out], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(slot.stream)
+        slot.event = ev
+        for r in batch:
+            r.batch_size = nb
+        slot.batch = batch
+        slot.n_out = n_out
+
+    def _harvest(self, slot: _Slot, now: float):
+        for j, r in enumerate(slot.batch):
+            r.result = slot.yhost[j, :slot.n_out].numpy().copy()
+            r.t_done = now
+            self.done.append(r)
+        slot.batch = []
+        slot.event = None
+
+    def step(self):
+        """One scheduler pass: harvest finished batches, launch new ones on free workspaces."""
+        now = time.perf_counter()
+        progressed = False
+        for rm in self.maps.values():
+            for slot in rm.slots:
+                if slot.event is not None and slot.event.query():
+                    self._harvest(slot, time.perf_counter())
+                    progressed = True
+            for slot in rm.slots:
+                if slot.event is not None:
+                    continue
+                # the orientation with the oldest waiting request goes first
+                heads = [(q[0].t_arrival, op) for op, q in rm.queues.items() if q]
+                if not heads:
+                    break
+                _, op = min(heads)
+                self._launch(rm, slot, op)
+                progressed = True
+        return progressed
+
+    def drain(self, timeout_s=600.0):
+        t0 = time.perf_counter()
+        while self.pending():
+            if not self.step():
+                time.sleep(20e-6)
+            if time.perf_counter() - t0 > timeout_s:
+                raise TimeoutError("requests still pending")
+
+    def run_stream(self, requests, rate_hz, seed=0):
+        """Replay ``requests`` with Poisson arrivals of mean rate ``rate_hz`` (seeded)."""
+        rng = np.random.default_rng(seed)
+        gaps = rng.exponential(1.0 / rate_hz, size=len(requests))
+        t0 = time.perf_counter()
+        arrivals = t0 + np.cumsum(gaps)
+        i = 0
+        while i < len(requests) or self.pending():
+            now = time.perf_counter()
+            while i < len(requests) and arrivals[i] <= now:
+                requests[i].t_arrival = arrivals[i]
+                self.submit(requests[i])
+                i += 1
+            if not self.step():
+                time.sleep(20e-6)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in requests])
+        return {"requests": len(requests), "wall_s": wall, "matvecs_per_s": len(requests) / wall,
+                "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "mean_batch": float(np.mean([r.batch_size for r in requests]))}

@@ -219,3 +219,35 @@ def test_lattice_toeplitz_matches_generic(gpu, kind, real):
     for op in "NT":
         ref = O.mul(core, x, op)
         assert relerr(orient(mixed, op) @ x, ref.real if real else ref) < 1e-5, op
+
+
+def test_mfma_f64_lane_layout(gpu):
+    """The lane layout the FP64 MFMA near kernel assumes for v_mfma_f64_16x16x4f64."""
+    import ctypes as C
+
+    from h2g import _lib as H
+
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    D = np.zeros((16, 16))
+    lib = H.lib()
+    lib.h2g_probe_mfma_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    assert lib.h2g_probe_mfma_f64(H.ptr(A), H.ptr(B), H.ptr(D)) == 0
+    assert np.allclose(D, A @ B, rtol=1e-14, atol=1e-14)
+
+
+def test_wide_near_field_all_orientations(gpu):
+    """Near field alone at 16..40 RHS (FP64 MFMA path for real maps) against the oracle."""
+    import h2g
+
+    pts = np.random.default_rng(21).random((2500, 3))
+    core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=40)
+    A = make_map(core, max_rhs=40)
+    rng = np.random.default_rng(1)
+    for nrhs in (16, 21, 40):
+        x = rng.standard_normal((2500, nrhs))
+        for op in "NT":
+            y = np.zeros((2500, nrhs))
+            h2g.nearmul_(y, orient(A, op), x)
+            assert relerr(y, O.nearmul(core, x, op)) < RTOL, (op, nrhs)

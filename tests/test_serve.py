@@ -1,0 +1,70 @@
+"""Serving loop: admission against the HBM budget, continuous batching, and each request's
+result the same alone and in the mix."""
+
+import numpy as np
+import pytest
+
+from oracle import h2 as O
+
+
+def _maps():
+    from h2g import method as PM
+
+    rng = np.random.default_rng(0)
+    cube = rng.random((3000, 3))
+    sph = rng.standard_normal((1600, 3))
+    sph = 0.5 + 0.45 * sph / np.linalg.norm(sph, axis=1, keepdims=True)
+    lap = PM.build_map(cube, p=4, nodekind="equi", m2l_kind="toeplitz", leaf_size=32, host_values=True)
+    hel = PM.build_map(sph, p=3, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=2.0,
+                       leaf_size=32, host_values=True)
+    return lap, hel
+
+
+def test_admission_rejects_over_budget():
+    """A map that does not fit the budget is refused before anything is built."""
+    from h2g.serve import AdmissionError, Server
+
+    lap, _ = _maps()
+    srv = Server(budget_bytes=1_000_000)
+    with pytest.raises(AdmissionError):
+        srv.add_map("laplace", lap)
+    assert not srv.maps
+
+
+@pytest.mark.gpu
+def test_continuous_batching_same_alone_and_in_mix(gpu):
+    import h2g
+    from h2g.serve import Request, Server
+
+    lap, hel = _maps()
+    srv = Server(budget_bytes=50 * 2 ** 30, max_batch=8, ws_per_map=2)
+    srv.add_map("lap", lap)
+    srv.add_map("hel", hel)
+    rng = np.random.default_rng(1)
+    reqs = []
+    for i in range(60):
+        name = "lap" if i % 3 else "hel"
+        op = "NTC"[i % 3] if name == "hel" else "NT"[i % 2]
+        n = 3000 if name == "lap" else 1600
+        x = rng.standard_normal(n).astype(np.float32)
+        if name == "hel":
+            x = (x + 1j * rng.standard_normal(n)).astype(np.complex64)
+        reqs.append(Request(name, op, x))
+    stats = srv.run_stream(reqs, rate_hz=20000.0, seed=3)
+    assert stats["requests"] == 60 and all(r.result is not None for r in reqs)
+    assert stats["mean_batch"] > 1.0  # requests were batched
+    # alone: the same request as a one-column product on a fresh workspace of the same map
+    alone = {name: h2g.H2Workspace(srv.maps[name].core, max_rhs=1, precision="mixed") for name in ("lap", "hel")}
+    for r in reqs:
+        A = alone[r.map_name]
+        M = {"N": A, "T": h2g.transpose(A), "C": h2g.adjoint(A)}[r.op]
+        y = M @ r.x
+        assert np.linalg.norm(r.result - y) <= 1e-6 * np.linalg.norm(y), (r.map_name, r.op, r.batch_size)
+    # and the served results match the FP64 plain product to the mixed-precision tolerance
+    plain = h2g.H2Workspace(srv.maps["lap"].core, max_rhs=1, precision="plain")
+    for r in reqs[:6]:
+        if r.map_name != "lap":
+            continue
+        M = plain if r.op == "N" else h2g.transpose(plain)
+        y64 = M @ r.x.astype(np.float64)
+        assert np.linalg.norm(r.result - y64) <= 1e-5 * np.linalg.norm(y64)
