@@ -245,42 +245,43 @@ __global__ void p2m_kernel(const int32_t *__restrict__ leaves, const int64_t *__
 
 // ------------------------------------------------------------------------------------------
 // L2P: ys[b0 + j][r] += alpha * sum_k opC(C[k, b0 + j]) * loc[leaf][r][k]
-// The leaf's local moments are staged in LDS as [k][r] (r fastest); a thread owns
-// (point, 8 RHS) and reads its collection column contiguously.
+// The leaf's local moments are staged in LDS transposed, [k][r]; a thread owns one (point, r)
+// pair, so a wave spans a few points x consecutive RHS: collection loads are wave-broadcasts
+// and the LDS reads are conflict-free.
 // ------------------------------------------------------------------------------------------
-constexpr int LRC = 8;  // RHS per LDS stage
-
 template <class T>
 __global__ void l2p_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                            const int64_t *__restrict__ end, const T *__restrict__ coll, int K,
                            const T *__restrict__ loc, int R, store_t<T> *__restrict__ ys, cplx<double> alpha,
-                           bool conj) {
+                           bool conj, int RC) {
     using S = store_t<T>;
     extern __shared__ unsigned char smem_raw[];
-    T *lb = reinterpret_cast<T *>(smem_raw);  // [K][LRC]
+    T *lb = reinterpret_cast<T *>(smem_raw);  // [K][RC]
     const int b = leaves[blockIdx.x];
     const int64_t j0 = begin[b], n = end[b] - j0;
     const T *l = loc + (int64_t)b * K * R;
-    for (int r0 = 0; r0 < R; r0 += LRC) {
-        const int nr = min(LRC, R - r0);
+    for (int r0 = 0; r0 < R; r0 += RC) {
+        const int nr = min(RC, R - r0);
         __syncthreads();
-        for (int e = threadIdx.x; e < K * LRC; e += blockDim.x) {
-            const int k = e / LRC, q = e - k * LRC;
-            lb[e] = q < nr ? l[(int64_t)(r0 + q) * K + k] : zero_v<T>();
+        for (int e = threadIdx.x; e < K * nr; e += blockDim.x) {
+            const int q = e / K, k = e - q * K;  // coalesced read of loc[r0+q][k]
+            lb[k * RC + q] = l[(int64_t)(r0 + q) * K + k];
         }
         __syncthreads();
-        for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+        for (int64_t o = threadIdx.x; o < n * nr; o += blockDim.x) {
+            const int64_t j = o / nr;
+            const int q = (int)(o - j * nr);
             const T *c = coll + (j0 + j) * K;
-            T acc[LRC];
-#pragma unroll
-            for (int q = 0; q < LRC; ++q) acc[q] = zero_v<T>();
-            for (int k = 0; k < K; ++k) {
-                const T a = maybe_conj(c[k], conj);
-#pragma unroll
-                for (int q = 0; q < LRC; ++q) fmacc(acc[q], a, lb[k * LRC + q]);
+            T acc0 = zero_v<T>(), acc1 = zero_v<T>();
+            int k = 0;
+            for (; k + 2 <= K; k += 2) {
+                fmacc(acc0, maybe_conj(c[k], conj), lb[k * RC + q]);
+                fmacc(acc1, maybe_conj(c[k + 1], conj), lb[(k + 1) * RC + q]);
             }
-            S *yo = ys + (j0 + j) * R + r0;
-            for (int q = 0; q < nr; ++q) yo[q] = yo[q] + scale_by<S>(alpha, cvt<S>::from(acc[q]));
+            if (k < K) fmacc(acc0, maybe_conj(c[k], conj), lb[k * RC + q]);
+            const T acc = acc0 + acc1;
+            S *yo = ys + (j0 + j) * R + r0 + q;
+            *yo = *yo + scale_by<S>(alpha, cvt<S>::from(acc));
         }
     }
 }
@@ -576,8 +577,10 @@ void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64
                 const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, hipStream_t s) {
     if (n == 0) return;
     DISPATCH_T(cdtype, {
-        l2p_kernel<T><<<n, TPB, sizeof(T) * LRC * (size_t)K, s>>>(leaves, begin, end, (const T *)coll, K,
-                                                                 (const T *)loc, R, (store_t<T> *)ys, alpha, conj);
+        // RHS per LDS pass: all of them when K x R fits in 48 KB
+        const int rc = std::max(1, std::min(R, (int)(49152 / (sizeof(T) * (size_t)K))));
+        l2p_kernel<T><<<n, TPB, sizeof(T) * rc * (size_t)K, s>>>(leaves, begin, end, (const T *)coll, K,
+                                                                (const T *)loc, R, (store_t<T> *)ys, alpha, conj, rc);
     });
     H2G_CHECK_LAUNCH();
 }
@@ -757,15 +760,15 @@ void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targ
 // Near field on FP64 MFMA (real maps, R >= 16): Y_t (n_out x R) += B (n_out x n_in) X (n_in x R)
 // with v_mfma_f64_16x16x4f64.  One workgroup (4 waves) per target leaf; a wave owns 16-row
 // output tiles across all 16-column RHS tiles.  Operand lane layouts of the 16x16x4 FP64
-// instruction: A[m][k] in lane m + 16k, B[k][n] in lane n + 16k, D[m][n] in lane n + 16(m/4),
-// element m % 4.
+// instruction (checked by the layout probe below on gfx950): A[m][k] in lane m + 16k,
+// B[k][n] in lane n + 16k, D[m][n] in lane n + 16(m % 4), element m / 4.
 // ------------------------------------------------------------------------------------------
 namespace {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 __device__ inline int mf_a_row(int lane) { return lane & 15; }
 __device__ inline int mf_a_k(int lane) { return lane >> 4; }
-__device__ inline int mf_d_row(int lane, int v) { return 4 * (lane >> 4) + v; }
+__device__ inline int mf_d_row(int lane, int v) { return (lane >> 4) + 4 * v; }
 
 template <bool TRANS>
 __global__ void near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t *__restrict__ blk,
