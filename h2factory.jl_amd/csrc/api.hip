@@ -944,6 +944,15 @@ int h2g_ws_sync(h2g_ws *ws) {
     return guarded([&] { H2G_HIP(hipStreamSynchronize(ws->stream)); });
 }
 
+int h2g_ws_query(h2g_ws *ws) {
+    if (!ws) return H2G_EINVAL;
+    const hipError_t e = hipStreamQuery(ws->stream);
+    if (e == hipSuccess) return H2G_OK;
+    if (e == hipErrorNotReady) return H2G_EAGAIN;
+    set_error(std::string("hipStreamQuery: ") + hipGetErrorString(e));
+    return H2G_EHIP;
+}
+
 int h2g_ws_timing(h2g_ws *ws, h2g_timing_record *out, int32_t max, int32_t *n) {
     if (!ws || !n) return H2G_EINVAL;
     *n = (int32_t)ws->timings.size();
@@ -1001,8 +1010,11 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
         const bool host = flags & H2G_HOST_PTRS;
         const bool reads_y = five && (b.re != 0.0 || b.im != 0.0 || which == H2G_FARMUL);
         if (host) {
-            if (ws->xhost.bytes < (size_t)(n_in * nrhs) * xe) ws->xhost.alloc((size_t)(n_in * nrhs) * xe);
-            if (ws->yhost.bytes < (size_t)(n_out * nrhs) * xe) ws->yhost.alloc((size_t)(n_out * nrhs) * xe);
+            // staging sized for the workspace's column capacity, so later calls do not reallocate
+            // (a reallocation would wait for in-flight asynchronous products)
+            const int64_t cols = std::max<int64_t>(nrhs, ws->R_cap);
+            if (ws->xhost.bytes < (size_t)(n_in * nrhs) * xe) ws->xhost.alloc((size_t)(n_in * cols) * xe);
+            if (ws->yhost.bytes < (size_t)(n_out * nrhs) * xe) ws->yhost.alloc((size_t)(n_out * cols) * xe);
             H2G_HIP(hipMemcpy2DAsync(ws->xhost.ptr, n_in * xe, x, ldx * xe, n_in * xe, nrhs, hipMemcpyHostToDevice, s));
             if (reads_y)
                 H2G_HIP(hipMemcpy2DAsync(ws->yhost.ptr, n_out * xe, y, ldy * xe, n_out * xe, nrhs, hipMemcpyHostToDevice, s));
@@ -1062,7 +1074,7 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
         }
         if (host) {
             H2G_HIP(hipMemcpy2DAsync(y, ldy * xe, ws->yhost.ptr, n_out * xe, n_out * xe, nrhs, hipMemcpyDeviceToHost, s));
-            H2G_HIP(hipStreamSynchronize(s));
+            if (!(flags & H2G_ASYNC) || ws->opts.timing) H2G_HIP(hipStreamSynchronize(s));
         } else if (!(flags & H2G_ASYNC) || ws->opts.timing) {
             H2G_HIP(hipStreamSynchronize(s));
         }

@@ -33,6 +33,18 @@ __device__ inline T maybe_conj2(T v, bool c) {
 // Modulate-in: pad + 3-D DFT of JB (box, rhs) jobs per workgroup, frequency-major output.
 //   S[f][slot][r], f = fx + FX*(fy + L*fz), FX = HALF ? (L+1)/2 : L.
 // --------------------------------------------------------------------------------------------
+// Each stage is a batch of 1-D DFTs along one axis: a thread loads one input line (p or L
+// values) into registers and writes the whole output line, with the L twiddles in registers.
+template <class C, int L>
+__device__ inline void load_twiddles(C (&tw)[L], const cplx<double> *__restrict__ twiddle, bool inverse) {
+    using Rl = typename real_of<C>::type;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const cplx<double> w = twiddle[i];
+        tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
+    }
+}
+
 template <class T, bool HALF, int JB, int P>
 __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, int K, int R,
                                  typename complex_of<T>::type *__restrict__ S,
@@ -42,60 +54,75 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
     extern __shared__ unsigned char smem_raw[];
     constexpr int p = P, L = 2 * P - 1;
     constexpr int FX = HALF ? (L + 1) / 2 : L;
-    C *tw = reinterpret_cast<C *>(smem_raw);
-    C *b1 = tw + L;                // [JB][FX*p*p]
-    C *b2 = b1 + JB * FX * p * p;  // [JB][FX*L*p]
+    constexpr int n1 = FX * p * p, n2 = FX * L * p;
+    C *b1 = reinterpret_cast<C *>(smem_raw);  // [JB][FX*p*p]
+    C *b2 = b1 + JB * n1;                     // [JB][FX*L*p]
+    C tw[L];
+    load_twiddles(tw, twiddle, inverse);
     const int64_t njobs = (int64_t)nbox * R;
     const int64_t job0 = (int64_t)blockIdx.x * JB;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) {
-        cplx<double> w = twiddle[i];
-        tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
-    }
-    __syncthreads();
-    // jobs are (slot, r) with r fastest; a job's moment is K contiguous values of mom
-    constexpr int n1 = FX * p * p;
-    for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
-        const int j = i / n1, e = i - j * n1;
-        const int fx = e % FX, rest = e / FX;
+    // x: line (iy, iz) of a job -> b1[fx + FX*(iy + p*iz)], fx < FX
+    for (int i = threadIdx.x; i < JB * p * p; i += blockDim.x) {
+        const int j = i / (p * p), rest = i - j * (p * p);
         const int64_t job = job0 + j;
-        C acc = zero_v<C>();
+        C in[p];
         if (job < njobs) {
             const int64_t slot = job / R;
             const int r = (int)(job - slot * R);
             const T *mj = mom + ((int64_t)(box0 + slot) * R + r) * K + p * rest;
-            for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], cvt<C>::from(mj[ix]));
+#pragma unroll
+            for (int ix = 0; ix < p; ++ix) in[ix] = cvt<C>::from(mj[ix]);
+        } else {
+#pragma unroll
+            for (int ix = 0; ix < p; ++ix) in[ix] = zero_v<C>();
         }
-        b1[i] = acc;
+#pragma unroll
+        for (int fx = 0; fx < FX; ++fx) {
+            C acc = zero_v<C>();
+#pragma unroll
+            for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], in[ix]);
+            b1[j * n1 + fx + FX * rest] = acc;
+        }
     }
     __syncthreads();
-    constexpr int n2 = FX * L * p;
-    for (int i = threadIdx.x; i < JB * n2; i += blockDim.x) {
-        const int j = i / n2, e = i - j * n2;
-        const int fx = e % FX, fy = (e / FX) % L, iz = e / (FX * L);
-        C acc = zero_v<C>();
-        const C *bj = b1 + j * n1;
-        for (int iy = 0; iy < p; ++iy) fmacc(acc, tw[(fy * iy) % L], bj[fx + FX * (iy + p * iz)]);
-        b2[i] = acc;
+    // y: line (fx, iz) -> b2[fx + FX*(fy + L*iz)], fy < L
+    for (int i = threadIdx.x; i < JB * FX * p; i += blockDim.x) {
+        const int j = i / (FX * p), e = i - j * (FX * p);
+        const int fx = e % FX, iz = e / FX;
+        C in[p];
+#pragma unroll
+        for (int iy = 0; iy < p; ++iy) in[iy] = b1[j * n1 + fx + FX * (iy + p * iz)];
+#pragma unroll
+        for (int fy = 0; fy < L; ++fy) {
+            C acc = zero_v<C>();
+#pragma unroll
+            for (int iy = 0; iy < p; ++iy) fmacc(acc, tw[(fy * iy) % L], in[iy]);
+            b2[j * n2 + fx + FX * (fy + L * iz)] = acc;
+        }
     }
     __syncthreads();
-    constexpr int F = FX * L * L;
+    // z: line (fx, fy) -> S[f][job], f = fx + FX*(fy + L*fz); jobs fastest for contiguous writes
     const Rl scale = inverse ? (Rl)(1.0 / (double)(L * L * L)) : (Rl)1;
-    for (int i = threadIdx.x; i < JB * F; i += blockDim.x) {
-        const int j = i % JB, f = i / JB;  // jobs fastest -> contiguous writes per frequency
+    for (int i = threadIdx.x; i < JB * FX * L; i += blockDim.x) {
+        const int j = i % JB, fxy = i / JB;
         const int64_t job = job0 + j;
         if (job >= njobs) continue;
-        const int fxy = f % (FX * L), fz = f / (FX * L);
-        C acc = zero_v<C>();
-        const C *bj = b2 + j * n2;
-        for (int iz = 0; iz < p; ++iz) fmacc(acc, tw[(fz * iz) % L], bj[fxy + FX * L * iz]);
-        S[(int64_t)f * njobs + job] = acc * scale;
+        C in[p];
+#pragma unroll
+        for (int iz = 0; iz < p; ++iz) in[iz] = b2[j * n2 + fxy + FX * L * iz];
+#pragma unroll
+        for (int fz = 0; fz < L; ++fz) {
+            C acc = zero_v<C>();
+#pragma unroll
+            for (int iz = 0; iz < p; ++iz) fmacc(acc, tw[(fz * iz) % L], in[iz]);
+            S[(int64_t)(fxy + FX * L * fz) * njobs + job] = acc * scale;
+        }
     }
 }
 
 // --------------------------------------------------------------------------------------------
 // Modulate-out: 3-D inverse DFT + crop of JB (recv, rhs) jobs, from O[f][recv][r]; writes
-// (overwrites) loc[box][r][k].  HALF: real output from the half spectrum.  The first stage
-// reads the spectrum from global memory (each value is used p times, from cache).
+// (overwrites) loc[box][r][k].  HALF: real output from the half spectrum.
 // --------------------------------------------------------------------------------------------
 template <class T, bool HALF, int JB, int P>
 __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nrecv,
@@ -106,58 +133,72 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
     extern __shared__ unsigned char smem_raw[];
     constexpr int p = P, L = 2 * P - 1;
     constexpr int FX = HALF ? (L + 1) / 2 : L;
-    C *tw = reinterpret_cast<C *>(smem_raw);
-    C *a1 = tw + L;                // [JB][FX*L*p]
-    C *a2 = a1 + JB * FX * L * p;  // [JB][FX*p*p]
+    constexpr int n1 = FX * L * p, n2 = FX * p * p;
+    C *a1 = reinterpret_cast<C *>(smem_raw);  // [JB][FX*L*p]
+    C *a2 = a1 + JB * n1;                     // [JB][FX*p*p]
+    C tw[L];
+    load_twiddles(tw, twiddle, inverse);
     const int64_t njobs = (int64_t)nrecv * R;
     const int64_t job0 = (int64_t)blockIdx.x * JB;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) {
-        cplx<double> w = twiddle[i];
-        tw[i] = C{(Rl)w.re, (Rl)(inverse ? -w.im : w.im)};
-    }
-    __syncthreads();
-    constexpr int n1 = FX * L * p;
-    for (int i = threadIdx.x; i < JB * n1; i += blockDim.x) {
-        const int j = i % JB, e = i / JB;  // jobs fastest -> contiguous reads per frequency
-        const int fxy = e % (FX * L), iz = e / (FX * L);
+    // z: line (fx, fy) of the spectrum -> a1[fxy + FX*L*iz], iz < p
+    for (int i = threadIdx.x; i < JB * FX * L; i += blockDim.x) {
+        const int j = i % JB, fxy = i / JB;
         const int64_t job = job0 + j;
-        C acc = zero_v<C>();
-        if (job < njobs)
-            for (int fz = 0; fz < L; ++fz)
-                fmacc(acc, tw[(fz * iz) % L], O[(int64_t)(fxy + FX * L * fz) * njobs + job]);
-        a1[j * n1 + e] = acc;
+        C in[L];
+#pragma unroll
+        for (int fz = 0; fz < L; ++fz) in[fz] = job < njobs ? O[(int64_t)(fxy + FX * L * fz) * njobs + job] : zero_v<C>();
+#pragma unroll
+        for (int iz = 0; iz < p; ++iz) {
+            C acc = zero_v<C>();
+#pragma unroll
+            for (int fz = 0; fz < L; ++fz) fmacc(acc, tw[(fz * iz) % L], in[fz]);
+            a1[j * n1 + fxy + FX * L * iz] = acc;
+        }
     }
     __syncthreads();
-    constexpr int n2 = FX * p * p;
-    for (int i = threadIdx.x; i < JB * n2; i += blockDim.x) {
-        const int j = i / n2, e = i - j * n2;
-        const int fx = e % FX, iy = (e / FX) % p, iz = e / (FX * p);
-        C acc = zero_v<C>();
-        const C *aj = a1 + j * n1;
-        for (int fy = 0; fy < L; ++fy) fmacc(acc, tw[(fy * iy) % L], aj[fx + FX * (fy + L * iz)]);
-        a2[i] = acc;
+    // y: line (fx, iz) -> a2[fx + FX*(iy + p*iz)], iy < p
+    for (int i = threadIdx.x; i < JB * FX * p; i += blockDim.x) {
+        const int j = i / (FX * p), e = i - j * (FX * p);
+        const int fx = e % FX, iz = e / FX;
+        C in[L];
+#pragma unroll
+        for (int fy = 0; fy < L; ++fy) in[fy] = a1[j * n1 + fx + FX * (fy + L * iz)];
+#pragma unroll
+        for (int iy = 0; iy < p; ++iy) {
+            C acc = zero_v<C>();
+#pragma unroll
+            for (int fy = 0; fy < L; ++fy) fmacc(acc, tw[(fy * iy) % L], in[fy]);
+            a2[j * n2 + fx + FX * (iy + p * iz)] = acc;
+        }
     }
     __syncthreads();
-    constexpr int P3 = p * p * p;
+    // x: line (iy, iz) -> loc[... + ix + p*(iy + p*iz)], ix < p
     const Rl scale = inverse ? (Rl)(1.0 / (double)(L * L * L)) : (Rl)1;
-    for (int i = threadIdx.x; i < JB * P3; i += blockDim.x) {
-        const int j = i / P3, k = i - j * P3;
+    for (int i = threadIdx.x; i < JB * p * p; i += blockDim.x) {
+        const int j = i / (p * p), rest = i - j * (p * p);
         const int64_t job = job0 + j;
         if (job >= njobs) continue;
-        const int ix = k % p, rest = k / p;
-        const C *aj = a2 + j * n2 + FX * rest;
-        C acc = zero_v<C>();
-        if (HALF) {
-            // Hermitian spectrum: y = a2[0] + 2 Re sum_{fx>=1} a2[fx] w^(fx*ix)
-            C tail = zero_v<C>();
-            for (int fx = 1; fx < FX; ++fx) fmacc(tail, tw[(fx * ix) % L], aj[fx]);
-            acc = C{aj[0].re + (Rl)2 * tail.re, (Rl)0};
-        } else {
-            for (int fx = 0; fx < FX; ++fx) fmacc(acc, tw[(fx * ix) % L], aj[fx]);
-        }
+        C in[FX];
+#pragma unroll
+        for (int fx = 0; fx < FX; ++fx) in[fx] = a2[j * n2 + fx + FX * rest];
         const int64_t slot = job / R;
         const int r = (int)(job - slot * R);
-        loc[((int64_t)recv_boxes[slot] * R + r) * K + k] = cvt<T>::from(acc * scale);
+        T *dst = loc + ((int64_t)recv_boxes[slot] * R + r) * K + p * rest;
+#pragma unroll
+        for (int ix = 0; ix < p; ++ix) {
+            C acc = zero_v<C>();
+            if (HALF) {
+                // Hermitian spectrum: y = a2[0] + 2 Re sum_{fx>=1} a2[fx] w^(fx*ix)
+                C tail = zero_v<C>();
+#pragma unroll
+                for (int fx = 1; fx < FX; ++fx) fmacc(tail, tw[(fx * ix) % L], in[fx]);
+                acc = C{in[0].re + (Rl)2 * tail.re, (Rl)0};
+            } else {
+#pragma unroll
+                for (int fx = 0; fx < FX; ++fx) fmacc(acc, tw[(fx * ix) % L], in[fx]);
+            }
+            dst[ix] = cvt<T>::from(acc * scale);
+        }
     }
 }
 
@@ -244,7 +285,7 @@ template <int JB, class T, bool HALF>
 size_t fft_smem(int p, int L) {
     using C = typename complex_of<T>::type;
     const int FX = HALF ? (L + 1) / 2 : L;
-    return sizeof(C) * (size_t)(L + JB * (FX * p * p + FX * L * p));
+    return sizeof(C) * (size_t)(JB * (FX * p * p + FX * L * p));
 }
 
 // jobs per workgroup: 16 for FP32 half spectra at p = 6 (78 KB of LDS), fewer for wider types

@@ -13,7 +13,7 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib", "libh2g.so")
 
-H2G_OK, H2G_EBUSY, H2G_EINVAL, H2G_EDIM, H2G_ENOMEM, H2G_EHIP, H2G_EUNSUPPORTED = range(7)
+H2G_OK, H2G_EBUSY, H2G_EINVAL, H2G_EDIM, H2G_ENOMEM, H2G_EHIP, H2G_EUNSUPPORTED, H2G_EAGAIN = range(8)
 H2G_F64, H2G_C64, H2G_F32, H2G_C32 = range(4)
 H2G_OP_N, H2G_OP_T, H2G_OP_C = range(3)
 H2G_NEARMUL, H2G_FARMUL, H2G_MUL = 1, 2, 3
@@ -25,7 +25,7 @@ EXPORTED = [
     "h2g_core_create", "h2g_core_retain", "h2g_core_release", "h2g_core_size", "h2g_core_device_bytes",
     "h2g_ws_create", "h2g_ws_destroy", "h2g_ws_device_bytes", "h2g_ws_stream", "h2g_ws_acquire",
     "h2g_ws_release", "h2g_matvec", "h2g_ws_sync", "h2g_ws_timing", "h2g_last_error", "h2g_version",
-    "h2g_method_fill",
+    "h2g_method_fill", "h2g_ws_query",
 ]
 
 VP = C.c_void_p
@@ -108,6 +108,7 @@ def lib():
         L.h2g_matvec.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.c_int64, VP, C.c_int64, VP, C.c_int64,
                                  VP, VP, C.c_int]
         L.h2g_ws_sync.argtypes = [VP]
+        L.h2g_ws_query.argtypes = [VP]
         L.h2g_ws_timing.argtypes = [VP, C.POINTER(TimingRecord), C.c_int32, I32P]
         L.h2g_last_error.argtypes = [C.c_char_p, C.c_size_t]
         L.h2g_method_fill.argtypes = [VP, C.POINTER(MethodDesc)]

@@ -10,10 +10,14 @@ turns the incoming stream into multi-RHS products:
 * requests queue per (map, orientation); whenever a workspace of that map is free, up to
   ``max_batch`` queued requests form one product -- requests that arrived while earlier
   batches were on the GPU join the next batch (continuous batching);
-* each workspace runs on its own HIP stream (``h2g_matvec`` with ``H2G_ASYNC``): host->device
-  copy, product and device->host copy are enqueued back to back, so batches of different
-  workspaces and maps overlap on the device while the host forms the next batch;
-* completion is polled with events; latency = completion time - arrival time.
+* each workspace runs on its own HIP stream (``h2g_matvec`` with ``H2G_HOST_PTRS |
+  H2G_ASYNC`` on page-locked buffers): host->device copy, product and device->host copy are
+  enqueued back to back by the library, so batches of different workspaces and maps overlap
+  on the device while the host forms the next batch;
+* completion is polled with ``h2g_ws_query``; latency = completion time - arrival time;
+* nothing else records work on a workspace's stream, and ``close`` waits for every product
+  before it releases the pinned buffers and destroys the workspaces (a stream must not be
+  destroyed while events recorded on it are still alive).
 
 Each result is the product of its own column: the kernels treat the columns of a batch
 independently, so a request's result is the same alone and in the mix (the near-field kernel
@@ -23,7 +27,6 @@ at the FP64 rounding level).
 
 from __future__ import annotations
 
-import ctypes as C
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -58,19 +61,24 @@ class Request:
 @dataclass
 class _Slot:
     ws: H2Workspace
-    stream: object
-    xdev: object
-    ydev: object
-    xhost: object
-    yhost: object
-    event: object = None
+    xhost: np.ndarray      # pinned, column-major n x max_batch
+    yhost: np.ndarray
+    keep: tuple            # the pinned allocations backing xhost / yhost
     batch: list = field(default_factory=list)
+    n_out: int = 0
+
+
+def _pinned(shape, dtype):
+    """Page-locked host array (column-major) and the tensor that owns its memory."""
+    import torch
+
+    tdt = {np.dtype(np.float32): torch.float32, np.dtype(np.complex64): torch.complex64}[np.dtype(dtype)]
+    t = torch.zeros((shape[1], shape[0]), dtype=tdt).pin_memory()
+    return t.numpy().T, t
 
 
 class _ResidentMap:
     def __init__(self, name, core, n_ws, max_batch, precision, dtype):
-        import torch
-
         self.name = name
         self.core = core
         self.queues = {op: deque() for op in OPS}
@@ -79,17 +87,31 @@ class _ResidentMap:
         n = max(n_out, n_in)
         for _ in range(n_ws):
             ws = H2Workspace(core, max_rhs=max_batch, precision=precision, use_graph=True)
-            st = torch.cuda.ExternalStream(ws.stream())
-            xd = torch.zeros((max_batch, n), dtype=dtype, device="cuda")
-            yd = torch.zeros((max_batch, n), dtype=dtype, device="cuda")
-            xh = torch.zeros((max_batch, n), dtype=dtype).pin_memory()
-            yh = torch.zeros((max_batch, n), dtype=dtype).pin_memory()
-            self.slots.append(_Slot(ws, st, xd, yd, xh, yh))
+            xh, tx = _pinned((n, max_batch), dtype)
+            yh, ty = _pinned((n, max_batch), dtype)
+            self.slots.append(_Slot(ws, xh, yh, (tx, ty)))
         self.n = n
+        self.dtype = dtype
+
+    def busy(self, slot):
+        if not slot.batch:
+            return False
+        rc = H.lib().h2g_ws_query(slot.ws.handle)
+        if rc == H.H2G_EAGAIN:
+            return True
+        _raise(rc)
+        return False
 
     def device_bytes(self):
-        return self.core.device_bytes() + sum(s.ws.device_bytes() + 2 * s.xdev.numel() * s.xdev.element_size()
-                                              for s in self.slots)
+        return self.core.device_bytes() + sum(s.ws.device_bytes() for s in self.slots)
+
+    def close(self):
+        # every product must have finished before the pinned buffers or the workspaces go away
+        for s in self.slots:
+            _raise(H.lib().h2g_ws_sync(s.ws.handle))
+        for s in self.slots:
+            s.ws.close()
+        self.slots = []
 
 
 def estimate_core_bytes(arrays, precisions=("plain", "mixed")):
@@ -132,18 +154,21 @@ class Server:
             raise AdmissionError(f"map {name!r} needs ~{(est + ws_est) / 1e9:.1f} GB, "
                                  f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
         core = GpuH2Core(arrays, precisions=("plain", self.precision))
-        dtype = torch.complex64 if arrays.dtype == "c64" else torch.float32
+        dtype = np.complex64 if arrays.dtype == "c64" else np.float32
         rm = _ResidentMap(name, core, self.ws_per_map, self.max_batch, self.precision, dtype)
         self.maps[name] = rm
         if self.used_bytes() > self.budget:
+            rm.close()
             del self.maps[name]
             raise AdmissionError(f"map {name!r} exceeded the budget once built")
         return rm
 
     def remove_map(self, name):
-        rm = self.maps.pop(name)
-        for s in rm.slots:
-            s.stream.synchronize()
+        self.maps.pop(name).close()
+
+    def close(self):
+        for name in list(self.maps):
+            self.remove_map(name)
 
     def submit(self, req: Request):
         if req.map_name not in self.maps:
@@ -155,8 +180,6 @@ class Server:
             sum(len(s.batch) for m in self.maps.values() for s in m.slots)
 
     def _launch(self, rm: _ResidentMap, slot: _Slot, op: str):
-        import torch
-
         q = rm.queues[op]
         batch = [q.popleft() for _ in range(min(self.max_batch, len(q)))]
         nb = len(batch)
@@ -164,20 +187,12 @@ class Server:
         if op != "N":
             n_out, n_in = n_in, n_out
         for j, r in enumerate(batch):
-            slot.xhost[j, :n_in].copy_(torch.from_numpy(np.ascontiguousarray(r.x)))
-        with torch.cuda.stream(slot.stream):
-            slot.xdev[:nb, :n_in].copy_(slot.xhost[:nb, :n_in], non_blocking=True)
-        xd = slot.xdev.t()
-        yd = slot.ydev.t()
-        dt = H.H2G_C32 if slot.xdev.dtype == torch.complex64 else H.H2G_F32
-        _raise(H.lib().h2g_matvec(slot.ws.handle, OPS[op], H.H2G_MUL, dt, nb, C.c_void_p(xd.data_ptr()),
-                                  slot.xdev.shape[1], C.c_void_p(yd.data_ptr()), slot.ydev.shape[1], None, None,
-                                  H.H2G_ASYNC))
-        with torch.cuda.stream(slot.stream):
-            slot.yhost[:nb, :n_out].copy_(slot.ydev[:nb, :n_out], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(slot.stream)
-        slot.event = ev
+            slot.xhost[:n_in, j] = r.x
+        dt = H.H2G_C32 if rm.dtype == np.complex64 else H.H2G_F32
+        # host -> device copy, product and device -> host copy, all on the workspace stream
+        _raise(H.lib().h2g_matvec(slot.ws.handle, OPS[op], H.H2G_MUL, dt, nb, H.ptr(slot.xhost), slot.xhost.shape[0],
+                                  H.ptr(slot.yhost), slot.yhost.shape[0], None, None,
+                                  H.H2G_HOST_PTRS | H.H2G_ASYNC))
         for r in batch:
             r.batch_size = nb
         slot.batch = batch
@@ -185,23 +200,21 @@ class Server:
 
     def _harvest(self, slot: _Slot, now: float):
         for j, r in enumerate(slot.batch):
-            r.result = slot.yhost[j, :slot.n_out].numpy().copy()
+            r.result = slot.yhost[:slot.n_out, j].copy()
             r.t_done = now
             self.done.append(r)
         slot.batch = []
-        slot.event = None
 
     def step(self):
         """One scheduler pass: harvest finished batches, launch new ones on free workspaces."""
-        now = time.perf_counter()
         progressed = False
         for rm in self.maps.values():
             for slot in rm.slots:
-                if slot.event is not None and slot.event.query():
+                if slot.batch and not rm.busy(slot):
                     self._harvest(slot, time.perf_counter())
                     progressed = True
             for slot in rm.slots:
-                if slot.event is not None:
+                if slot.batch:
                     continue
                 # the orientation with the oldest waiting request goes first
                 heads = [(q[0].t_arrival, op) for op, q in rm.queues.items() if q]

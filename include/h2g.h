@@ -32,7 +32,8 @@ enum {
     H2G_EDIM = 3,
     H2G_ENOMEM = 4,
     H2G_EHIP = 5,
-    H2G_EUNSUPPORTED = 6
+    H2G_EUNSUPPORTED = 6,
+    H2G_EAGAIN = 7     /* h2g_ws_query: work of the workspace is still in flight */
 };
 
 /* Element types. */
@@ -68,8 +69,10 @@ enum { H2G_PREC_PLAIN = 0, H2G_PREC_MIXED = 1 };
 
 /* Flags of h2g_matvec. */
 enum {
-    H2G_HOST_PTRS = 1, /* x, y, alpha, beta point to host memory (copied in / out)         */
-    H2G_ASYNC = 2      /* return after enqueueing on the workspace stream (device ptrs only) */
+    H2G_HOST_PTRS = 1, /* x, y point to host memory (staged through the workspace)          */
+    H2G_ASYNC = 2      /* return after enqueueing on the workspace stream; with
+                          H2G_HOST_PTRS the host buffers must be pinned and stay alive until
+                          h2g_ws_query (or h2g_ws_sync) reports the workspace idle             */
 };
 
 /* One side (test = rows/targets, trial = columns/sources) of the map: its octree and its leaf
@@ -203,6 +206,8 @@ int h2g_ws_release(h2g_ws *ws);
 
 /* Waits for the workspace stream (after H2G_ASYNC calls). */
 int h2g_ws_sync(h2g_ws *ws);
+/* H2G_OK when every product enqueued on the workspace has finished, H2G_EAGAIN otherwise. */
+int h2g_ws_query(h2g_ws *ws);
 
 /* Timing records of the last matvec (opts.timing != 0). */
 int h2g_ws_timing(h2g_ws *ws, h2g_timing_record *out, int32_t max, int32_t *n);

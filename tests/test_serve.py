@@ -68,3 +68,5 @@ def test_continuous_batching_same_alone_and_in_mix(gpu):
         M = plain if r.op == "N" else h2g.transpose(plain)
         y64 = M @ r.x.astype(np.float64)
         assert np.linalg.norm(r.result - y64) <= 1e-5 * np.linalg.norm(y64)
+    srv.close()
+    assert not srv.maps
