@@ -213,7 +213,7 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
     {
         int p = 1;
         while (p * p * p < K) ++p;
-        if (p * p * p == K && p > 1 && d.n_levels > 1) {
+        if (p * p * p == K && p > 1 && p <= 6 && d.n_levels > 1) {
             bool ok;
             size_t nf;
             if (dtype == H2G_F64) {

@@ -653,8 +653,6 @@ void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const
 // opA: 0 T, 1 T^T, 2 T^H, 3 conj(T); out[t] = (acc ? out[t] : 0) + sum_items op(T_item) in[src].
 // ------------------------------------------------------------------------------------------
 namespace {
-// right-hand sides per LDS pass: keeps 4 K-long buffers under 64 KB at p = 6
-template <class T> constexpr int sjr_of() { return sizeof(T) >= 16 ? 4 : 8; }
 
 template <class T>
 __device__ inline T sep_factor(const T *F, int p, int row, int col, int opA) {
@@ -667,90 +665,146 @@ __device__ inline T sep_factor(const T *F, int p, int row, int col, int opA) {
     }
 }
 
-template <class T, int SJR>
-__global__ void sep_transfer_kernel(int opA, const int32_t *__restrict__ targets, const int64_t *__restrict__ item_ptr,
-                                    const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_mat,
-                                    const T *__restrict__ factors, int p, const T *__restrict__ in,
-                                    T *__restrict__ out, int R, int acc_mode, const uint8_t *__restrict__ acc_flags) {
+// Jobs are (target, rhs) pairs, JB per workgroup (several targets per workgroup when R is
+// small).  Each stage is a batch of length-p line transforms: a thread loads one line into
+// registers and writes the transformed line (6 LDS reads per 36 FMAs at p = 6).
+template <class T, int JB, int P>
+__global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__restrict__ targets,
+                                    const int64_t *__restrict__ item_ptr, const int32_t *__restrict__ item_src,
+                                    const int32_t *__restrict__ item_mat, const T *__restrict__ factors,
+                                    const T *__restrict__ in, T *__restrict__ out, int R, int acc_mode,
+                                    const uint8_t *__restrict__ acc_flags) {
+    constexpr int p = P, K = P * P * P, LINES = P * P;
     extern __shared__ unsigned char smem_raw[];
-    const int K = p * p * p;
-    T *m = reinterpret_cast<T *>(smem_raw);  // [SJR][K]
-    T *t1 = m + SJR * K;
-    T *t2 = t1 + SJR * K;
-    T *acc = t2 + SJR * K;
-    T *fs = acc + SJR * K;  // [3][p][p] factors of the current item
-    const int ti = blockIdx.x;
-    const int t = targets[ti];
-    const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
-    const bool acc_in = acc_mode == 1 || (acc_mode == 2 && acc_flags[i0]);
-    T *o = out + (int64_t)t * K * R;
-    for (int r0 = 0; r0 < R; r0 += SJR) {
-        const int nr = min(SJR, R - r0);
-        for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
-            const int q = e / K, k = e - q * K;
-            acc[e] = (acc_in && q < nr) ? o[(int64_t)(r0 + q) * K + k] : zero_v<T>();
+    T *t1 = reinterpret_cast<T *>(smem_raw);  // [JB][K]
+    T *t2 = t1 + JB * K;                      // [JB][K]
+    T *acc = t2 + JB * K;                     // [JB][K]
+    __shared__ int s_item0[JB], s_nitem[JB], s_tgt[JB], s_r[JB], s_max;
+    const int64_t job0 = (int64_t)blockIdx.x * JB;
+    const int64_t njobs = (int64_t)n_targets * R;
+    if (threadIdx.x < JB) {
+        const int64_t job = job0 + threadIdx.x;
+        if (job < njobs) {
+            const int ti = (int)(job / R);
+            s_tgt[threadIdx.x] = ti;
+            s_r[threadIdx.x] = (int)(job - (int64_t)ti * R);
+            s_item0[threadIdx.x] = (int)item_ptr[ti];
+            s_nitem[threadIdx.x] = (int)(item_ptr[ti + 1] - item_ptr[ti]);
+        } else {
+            s_tgt[threadIdx.x] = -1;
+            s_nitem[threadIdx.x] = 0;
         }
-        for (int64_t it = i0; it < i1; ++it) {
-            __syncthreads();
-            const T *F = factors + (int64_t)item_mat[it] * 3 * p * p;
-            for (int e = threadIdx.x; e < 3 * p * p; e += blockDim.x) {
-                const int ax = e / (p * p), rc = e - ax * p * p;
-                fs[e] = sep_factor(F + ax * p * p, p, rc % p, rc / p, opA);  // fs[ax][row + p*col]
-            }
-            const T *x = in + (int64_t)item_src[it] * K * R + (int64_t)r0 * K;
-            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
-                const int q = e / K;
-                m[e] = q < nr ? x[e] : zero_v<T>();
-            }
-            __syncthreads();
-            // x: t1[q][kx + p*(jy + p*jz)] = sum_jx Fx[kx, jx] m[q][jx + p*(jy + p*jz)]
-            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
-                const int q = e / K, k = e - q * K;
-                const int kx = k % p, rest = k / p;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int mx = 0;
+        for (int j = 0; j < JB; ++j) mx = max(mx, s_nitem[j]);
+        s_max = mx;
+    }
+    for (int e = threadIdx.x; e < JB * K; e += blockDim.x) {
+        const int j = e / K, k = e - j * K;
+        T v = zero_v<T>();
+        const int ti = s_tgt[j];
+        if (ti >= 0) {
+            const bool acc_in = acc_mode == 1 || (acc_mode == 2 && acc_flags[s_item0[j]]);
+            if (acc_in) v = out[((int64_t)targets[ti] * R + s_r[j]) * K + k];
+        }
+        acc[e] = v;
+    }
+    __syncthreads();
+    const int nmax = s_max;
+    for (int it = 0; it < nmax; ++it) {
+        // x: input line (jy, jz) -> t1[kx + p*(jy + p*jz)]
+        for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
+            const int j = e / LINES, line = e - j * LINES;
+            if (it >= s_nitem[j]) continue;
+            const int64_t itg = s_item0[j] + it;
+            const T *F = factors + (int64_t)item_mat[itg] * 3 * p * p;
+            const T *x = in + ((int64_t)item_src[itg] * R + s_r[j]) * K + p * line;
+            T v[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) v[i] = x[i];
+#pragma unroll
+            for (int o = 0; o < p; ++o) {
                 T a = zero_v<T>();
-                for (int j = 0; j < p; ++j) fmacc(a, fs[kx + p * j], m[q * K + j + p * rest]);
-                t1[e] = a;
-            }
-            __syncthreads();
-            // y
-            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
-                const int q = e / K, k = e - q * K;
-                const int kx = k % p, ky = (k / p) % p, kz = k / (p * p);
-                T a = zero_v<T>();
-                for (int j = 0; j < p; ++j) fmacc(a, fs[p * p + ky + p * j], t1[q * K + kx + p * (j + p * kz)]);
-                t2[e] = a;
-            }
-            __syncthreads();
-            // z, accumulated
-            for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
-                const int q = e / K, k = e - q * K;
-                const int kxy = k % (p * p), kz = k / (p * p);
-                T a = acc[e];
-                for (int j = 0; j < p; ++j) fmacc(a, fs[2 * p * p + kz + p * j], t2[q * K + kxy + p * p * j]);
-                acc[e] = a;
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a, sep_factor(F, p, o, i, opA), v[i]);
+                t1[j * K + o + p * line] = a;
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < SJR * K; e += blockDim.x) {
-            const int q = e / K, k = e - q * K;
-            if (q < nr) o[(int64_t)(r0 + q) * K + k] = acc[e];
+        // y: line (kx, jz) -> t2[kx + p*(ky + p*jz)]
+        for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
+            const int j = e / LINES, line = e - j * LINES;
+            if (it >= s_nitem[j]) continue;
+            const int kx = line % p, jz = line / p;
+            const T *F = factors + (int64_t)item_mat[s_item0[j] + it] * 3 * p * p + p * p;
+            T v[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) v[i] = t1[j * K + kx + p * (i + p * jz)];
+#pragma unroll
+            for (int o = 0; o < p; ++o) {
+                T a = zero_v<T>();
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a, sep_factor(F, p, o, i, opA), v[i]);
+                t2[j * K + kx + p * (o + p * jz)] = a;
+            }
         }
         __syncthreads();
+        // z: line (kx, ky) -> acc[kx + p*(ky + p*kz)] +=
+        for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
+            const int j = e / LINES, line = e - j * LINES;
+            if (it >= s_nitem[j]) continue;
+            const T *F = factors + (int64_t)item_mat[s_item0[j] + it] * 3 * p * p + 2 * p * p;
+            T v[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) v[i] = t2[j * K + line + LINES * i];
+#pragma unroll
+            for (int o = 0; o < p; ++o) {
+                T a = acc[j * K + line + LINES * o];
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a, sep_factor(F, p, o, i, opA), v[i]);
+                acc[j * K + line + LINES * o] = a;
+            }
+        }
+        // the next x stage writes t1 only; the y stage after it is behind that stage's barrier
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < JB * K; e += blockDim.x) {
+        const int j = e / K, k = e - j * K;
+        const int ti = s_tgt[j];
+        if (ti >= 0) out[((int64_t)targets[ti] * R + s_r[j]) * K + k] = acc[e];
     }
 }
 }  // namespace
+
+template <class T, int P>
+static void sep_transfer_p(int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
+                           const int32_t *item_src, const int32_t *item_mat, const void *factors, const void *in,
+                           void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
+    // jobs per workgroup: three K-long LDS buffers per job, 41 KB at p = 6 for every type
+    constexpr int JB = 64 / (int)sizeof(T);
+    constexpr int K = P * P * P;
+    const int64_t njobs = (int64_t)n_targets * R;
+    const size_t sh = sizeof(T) * (size_t)(3 * JB * K);
+    sep_transfer_kernel<T, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+        opA, n_targets, targets, item_ptr, item_src, item_mat, (const T *)factors, (const T *)in, (T *)out, R,
+        acc_mode, acc_flags);
+}
 
 void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
                          const int32_t *item_src, const int32_t *item_mat, const void *factors, int p, const void *in,
                          void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
     if (n_targets == 0) return;
-    const int K = p * p * p;
     DISPATCH_T(cdtype, {
-        constexpr int SJR = sjr_of<T>();
-        const size_t sh = sizeof(T) * (size_t)(4 * SJR * K + 3 * p * p);
-        sep_transfer_kernel<T, SJR><<<n_targets, TPB, sh, s>>>(opA, targets, item_ptr, item_src, item_mat,
-                                                          (const T *)factors, p, (const T *)in, (T *)out, R,
-                                                          acc_mode, acc_flags);
+        switch (p) {
+        case 2: sep_transfer_p<T, 2>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
+        case 3: sep_transfer_p<T, 3>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
+        case 4: sep_transfer_p<T, 4>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
+        case 5: sep_transfer_p<T, 5>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
+        case 6: sep_transfer_p<T, 6>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
+        default: throw Error{H2G_EUNSUPPORTED, "separable transfers support 2 <= p <= 6"};
+        }
     });
     H2G_CHECK_LAUNCH();
 }
@@ -776,7 +830,9 @@ __global__ void near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t 
                                  const int32_t *__restrict__ nr, const int32_t *__restrict__ nc,
                                  const int64_t *__restrict__ off, const double *__restrict__ vals,
                                  const double *__restrict__ xs, double *__restrict__ ys, int R) {
-    constexpr int MAXCT = 4;  // up to 64 RHS per pass
+    // a wave owns one 16 x 16 output tile (row tile, RHS tile); k-steps are issued four at a
+    // time so that several operand loads are in flight per wave
+    constexpr int KU = 4;
     const int t = blockIdx.x;
     const int64_t b0 = ptr[t], b1 = ptr[t + 1];
     if (b0 == b1) return;
@@ -784,47 +840,38 @@ __global__ void near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t 
     const int n_out = TRANS ? nc[first] : nr[first];
     const int64_t out0 = TRANS ? cb[first] : rb[first];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const int n_rt = (n_out + 15) / 16;
-    for (int c0 = 0; c0 < R; c0 += 16 * MAXCT) {
-        const int n_ct = min(MAXCT, (R - c0 + 15) / 16);
-        for (int rt = wave; rt < n_rt; rt += nwaves) {
-            double4_t acc[MAXCT];
+    const int n_rt = (n_out + 15) / 16, n_ct = (R + 15) / 16;
+    const int arow_l = mf_a_row(lane), ak = mf_a_k(lane);
+    const int bn = lane & 15, bk = lane >> 4;
+    for (int tile = wave; tile < n_rt * n_ct; tile += nwaves) {
+        const int rt = tile / n_ct, ct = tile - rt * n_ct;
+        const int arow = rt * 16 + arow_l;
+        const int col = ct * 16 + bn;
+        double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int64_t q = b0; q < b1; ++q) {
+            const int bl = blk[q];
+            const int m = nr[bl], n = nc[bl];
+            const double *B = vals + off[bl];
+            const int kdim = TRANS ? m : n;
+            const double *xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
+            for (int k0 = 0; k0 < kdim; k0 += 4 * KU) {
+                double a[KU], bv[KU];
 #pragma unroll
-            for (int c = 0; c < MAXCT; ++c) acc[c] = double4_t{0.0, 0.0, 0.0, 0.0};
-            const int arow = rt * 16 + mf_a_row(lane);  // output row this lane feeds in A
-            const int ak = mf_a_k(lane);                // k offset within the step
-            const int bn = lane & 15, bk = lane >> 4;   // B operand: column n, k offset
-            for (int64_t q = b0; q < b1; ++q) {
-                const int bl = blk[q];
-                const int m = nr[bl], n = nc[bl];
-                const double *B = vals + off[bl];
-                const int kdim = TRANS ? m : n;
-                const double *xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
-                for (int k0 = 0; k0 < kdim; k0 += 4) {
-                    const int kk = k0 + ak;
-                    double a = 0.0;
-                    if (arow < n_out && kk < kdim) a = TRANS ? B[kk + (int64_t)m * arow] : B[arow + (int64_t)m * kk];
-                    const int kb = k0 + bk;
-#pragma unroll
-                    for (int c = 0; c < MAXCT; ++c) {
-                        if (c < n_ct) {
-                            const int col = c0 + 16 * c + bn;
-                            const double bv = (kb < kdim && col < R) ? xin[(int64_t)kb * R + col] : 0.0;
-                            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[c], 0, 0, 0);
-                        }
-                    }
+                for (int u = 0; u < KU; ++u) {
+                    const int kk = k0 + 4 * u + ak;
+                    a[u] = (arow < n_out && kk < kdim) ? (TRANS ? B[kk + (int64_t)m * arow] : B[arow + (int64_t)m * kk])
+                                                       : 0.0;
+                    const int kb = k0 + 4 * u + bk;
+                    bv[u] = (kb < kdim && col < R) ? xin[(int64_t)kb * R + col] : 0.0;
                 }
-            }
 #pragma unroll
-            for (int c = 0; c < MAXCT; ++c) {
-                if (c >= n_ct) continue;
-                const int col = c0 + 16 * c + (lane & 15);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = rt * 16 + mf_d_row(lane, v);
-                    if (row < n_out && col < R) ys[(out0 + row) * R + col] = acc[c][v];
-                }
+                for (int u = 0; u < KU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], acc, 0, 0, 0);
             }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = rt * 16 + mf_d_row(lane, v);
+            if (row < n_out && col < R) ys[(out0 + row) * R + col] = acc[v];
         }
     }
 }

@@ -210,8 +210,10 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
 // tq_op / tq_sym: [8][216] level tables (-1: no coupling).
 // --------------------------------------------------------------------------------------------
 // C: storage type of the spectra (cplx<float> / cplx<double>); accumulation is in FP64, which
-// runs at the same vector rate as unpacked FP32 on CDNA4.  D: storage type of the couplings.
-template <class C, class D>
+// runs at the same vector rate as unpacked FP32 on CDNA4.  D: storage type of the couplings;
+// the per-frequency coupling table E lives in LDS in that type (FP32 for the mixed form, so
+// each LDS read is 8 bytes).  RPL right-hand sides per lane reuse every E read RPL times.
+template <class C, class D, int RPL>
 __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ blk_src,
                                     const int32_t *__restrict__ blk_q, const int32_t *__restrict__ blk_nq,
                                     const int32_t *__restrict__ blk_tgt, const int32_t *__restrict__ tq_op,
@@ -219,7 +221,7 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
                                     const D *__restrict__ ops, int L, bool half, int n_src, int n_recv,
                                     const C *__restrict__ S, C *__restrict__ O, int R, bool conj) {
     using A = cplx<double>;
-    __shared__ A E[8 * 216];
+    __shared__ D E[8 * 216];
     const int f = blockIdx.y;
     const int L3 = L * L * L;
     const int FX = half ? (L + 1) / 2 : L;
@@ -227,25 +229,29 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
     const int ffull = fx + L * (fy + L * fz);
     for (int i = threadIdx.x; i < 8 * 216; i += blockDim.x) {
         const int op = tq_op[i];
-        A v = zero_v<A>();
+        D v = zero_v<D>();
         if (op >= 0) {
             const int sym = tq_sym ? tq_sym[i] : -1;
             const int ff = sym >= 0 ? perms[(int64_t)sym * L3 + ffull] : ffull;
-            v = maybe_conj2(cvt<A>::from(ops[(int64_t)op * L3 + ff]), conj);
+            v = maybe_conj2(ops[(int64_t)op * L3 + ff], conj);
         }
         E[i] = v;
     }
     __syncthreads();
-    const int per_wg = blockDim.x / R;  // blocks per workgroup pass (R <= blockDim.x)
-    const int bl = threadIdx.x / R, r = threadIdx.x - bl * R;
+    const int lanes_per_block = (R + RPL - 1) / RPL;
+    const int per_wg = blockDim.x / lanes_per_block;  // blocks per workgroup pass
+    const int bl = threadIdx.x / lanes_per_block;
+    const int r0 = (threadIdx.x - bl * lanes_per_block) * RPL;
     if (bl >= per_wg) return;
     const C *Sf = S + (int64_t)f * n_src * R;
     C *Of = O + (int64_t)f * n_recv * R;
     constexpr int U = 8;  // slots in flight per lane
     for (int b = blockIdx.x * per_wg + bl; b < n_blocks; b += gridDim.x * per_wg) {
-        A acc[8];
+        A acc[8][RPL];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = zero_v<A>();
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) acc[t][k] = zero_v<A>();
         // present sources are compacted to the front of the block's column of blk_src
         const int nq = blk_nq[b];
         int src[U], qs[U];
@@ -255,11 +261,14 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
             qs[u] = u < nq ? blk_q[(int64_t)u * n_blocks + b] : 0;
         }
         for (int i0 = 0; i0 < nq; i0 += U) {
-            A sv[U];
+            A sv[U][RPL];
             int qc[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                sv[u] = src[u] >= 0 ? cvt<A>::from(Sf[(int64_t)src[u] * R + r]) : zero_v<A>();
+#pragma unroll
+                for (int k = 0; k < RPL; ++k)
+                    sv[u][k] = (src[u] >= 0 && r0 + k < R) ? cvt<A>::from(Sf[(int64_t)src[u] * R + r0 + k])
+                                                           : zero_v<A>();
                 qc[u] = qs[u];
             }
 #pragma unroll
@@ -271,12 +280,19 @@ __global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ bl
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int t = 0; t < 8; ++t) fmacc(acc[t], E[t * 216 + qc[u]], sv[u]);
+                for (int t = 0; t < 8; ++t) {
+                    const A e = cvt<A>::from(E[t * 216 + qc[u]]);
+#pragma unroll
+                    for (int k = 0; k < RPL; ++k) fmacc(acc[t][k], e, sv[u][k]);
+                }
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int rv = blk_tgt[b * 8 + t];
-            if (rv >= 0) Of[(int64_t)rv * R + r] = cvt<C>::from(acc[t]);
+            if (rv < 0) continue;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k)
+                if (r0 + k < R) Of[(int64_t)rv * R + r0 + k] = cvt<C>::from(acc[t][k]);
         }
     }
 }
@@ -427,23 +443,29 @@ void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv,
 }
 
 void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const int32_t *blk_src,
-                         const int32_t *blk_q, const int32_t *blk_nq, const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
-                         const void *ops, int L, int n_src, int n_recv, const void *S, void *O, int R,
-                         hipStream_t s) {
+                         const int32_t *blk_q, const int32_t *blk_nq, const int32_t *blk_tgt, const int32_t *tq_op,
+                         const int32_t *tq_sym, const int32_t *perms, const void *ops, int L, int n_src, int n_recv,
+                         const void *S, void *O, int R, hipStream_t s) {
     if (n_blocks == 0) return;
     const int FX = half ? (L + 1) / 2 : L;
     const int F = FX * L * L;
-    const int per_wg = TPB / R;
-    const int chunks = std::max(1, std::min((n_blocks + per_wg - 1) / per_wg, 128));
-    dim3 grid(chunks, F);
-    if (cdtype == H2G_F64 || cdtype == H2G_C64)
-        hadamard_lat_kernel<cplx<double>, cplx<double>><<<grid, TPB, 0, s>>>(
-            n_blocks, blk_src, blk_q, blk_nq, blk_tgt, tq_op, tq_sym, perms, (const cplx<double> *)ops, L, half, n_src, n_recv,
-            (const cplx<double> *)S, (cplx<double> *)O, R, conj);
-    else
-        hadamard_lat_kernel<cplx<float>, cplx<float>><<<grid, TPB, 0, s>>>(
-            n_blocks, blk_src, blk_q, blk_nq, blk_tgt, tq_op, tq_sym, perms, (const cplx<float> *)ops, L, half, n_src, n_recv,
-            (const cplx<float> *)S, (cplx<float> *)O, R, conj);
+    auto go = [&](auto kern, int rpl, auto cs, auto ds) {
+        using CS = decltype(cs);
+        using DS = decltype(ds);
+        const int lanes = (R + rpl - 1) / rpl;
+        const int per_wg = std::max(1, TPB / lanes);
+        const int chunks = std::max(1, std::min((n_blocks + per_wg - 1) / per_wg, 128));
+        kern<<<dim3(chunks, F), TPB, 0, s>>>(n_blocks, blk_src, blk_q, blk_nq, blk_tgt, tq_op, tq_sym, perms,
+                                            (const DS *)ops, L, half, n_src, n_recv, (const CS *)S, (CS *)O, R, conj);
+    };
+    const bool wide = R >= 16;
+    if (cdtype == H2G_F64 || cdtype == H2G_C64) {
+        if (wide) go(hadamard_lat_kernel<cplx<double>, cplx<double>, 2>, 2, cplx<double>{}, cplx<double>{});
+        else go(hadamard_lat_kernel<cplx<double>, cplx<double>, 1>, 1, cplx<double>{}, cplx<double>{});
+    } else {
+        if (wide) go(hadamard_lat_kernel<cplx<float>, cplx<float>, 2>, 2, cplx<float>{}, cplx<float>{});
+        else go(hadamard_lat_kernel<cplx<float>, cplx<float>, 1>, 1, cplx<float>{}, cplx<float>{});
+    }
     H2G_CHECK_LAUNCH2();
 }
 
