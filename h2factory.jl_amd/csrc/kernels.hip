@@ -208,7 +208,7 @@ template <class T>
 __global__ void p2m_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                            const int64_t *__restrict__ end, const T *__restrict__ coll, int K,
                            const store_t<T> *__restrict__ xs, int R, T *__restrict__ mom, bool conj, int PCH) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *xb = reinterpret_cast<T *>(smem_raw);  // [PCH][R]
     const int b = leaves[blockIdx.x];
     const int64_t j0 = begin[b], n = end[b] - j0;
@@ -255,7 +255,7 @@ __global__ void l2p_kernel(const int32_t *__restrict__ leaves, const int64_t *__
                            const T *__restrict__ loc, int R, store_t<T> *__restrict__ ys, cplx<double> alpha,
                            bool conj, int RC) {
     using S = store_t<T>;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lb = reinterpret_cast<T *>(smem_raw);  // [K][RC]
     const int b = leaves[blockIdx.x];
     const int64_t j0 = begin[b], n = end[b] - j0;
@@ -351,7 +351,7 @@ __global__ void fft_in_kernel(int box0, const T *__restrict__ mom, int K, int p,
                               const cplx<double> *__restrict__ twiddle, bool inverse) {
     using C = typename complex_of<T>::type;
     using Rl = typename real_of<T>::type;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C *tw = reinterpret_cast<C *>(smem_raw);
     C *m = tw + L;
     C *b1 = m + p * p * p;
@@ -399,7 +399,7 @@ __global__ void fft_out_kernel(const int32_t *__restrict__ boxes, const typename
                                const cplx<double> *__restrict__ twiddle, bool inverse) {
     using C = typename complex_of<T>::type;
     using Rl = typename real_of<T>::type;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     C *tw = reinterpret_cast<C *>(smem_raw);
     const int L3 = L * L * L;
     C *s = tw + L;
@@ -675,11 +675,13 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
                                     const T *__restrict__ in, T *__restrict__ out, int R, int acc_mode,
                                     const uint8_t *__restrict__ acc_flags) {
     constexpr int p = P, K = P * P * P, LINES = P * P;
-    extern __shared__ unsigned char smem_raw[];
+    // one dynamic LDS region, element buffers first (16-byte aligned base), job tables after
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T *t1 = reinterpret_cast<T *>(smem_raw);  // [JB][K]
     T *t2 = t1 + JB * K;                      // [JB][K]
     T *acc = t2 + JB * K;                     // [JB][K]
-    __shared__ int s_item0[JB], s_nitem[JB], s_tgt[JB], s_r[JB], s_max;
+    int *s_item0 = reinterpret_cast<int *>(acc + JB * K);
+    int *s_nitem = s_item0 + JB, *s_tgt = s_nitem + JB, *s_r = s_tgt + JB, *s_maxp = s_r + JB;
     const int64_t job0 = (int64_t)blockIdx.x * JB;
     const int64_t njobs = (int64_t)n_targets * R;
     if (threadIdx.x < JB) {
@@ -699,7 +701,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
     if (threadIdx.x == 0) {
         int mx = 0;
         for (int j = 0; j < JB; ++j) mx = max(mx, s_nitem[j]);
-        s_max = mx;
+        *s_maxp = mx;
     }
     for (int e = threadIdx.x; e < JB * K; e += blockDim.x) {
         const int j = e / K, k = e - j * K;
@@ -712,7 +714,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
         acc[e] = v;
     }
     __syncthreads();
-    const int nmax = s_max;
+    const int nmax = *s_maxp;
     for (int it = 0; it < nmax; ++it) {
         // x: input line (jy, jz) -> t1[kx + p*(jy + p*jz)]
         for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
@@ -786,7 +788,7 @@ static void sep_transfer_p(int opA, int n_targets, const int32_t *targets, const
     constexpr int JB = 64 / (int)sizeof(T);
     constexpr int K = P * P * P;
     const int64_t njobs = (int64_t)n_targets * R;
-    const size_t sh = sizeof(T) * (size_t)(3 * JB * K);
+    const size_t sh = sizeof(T) * (size_t)(3 * JB * K) + sizeof(int) * (size_t)(4 * JB + 1);
     sep_transfer_kernel<T, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
         opA, n_targets, targets, item_ptr, item_src, item_mat, (const T *)factors, (const T *)in, (T *)out, R,
         acc_mode, acc_flags);

@@ -51,7 +51,7 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
                                  const cplx<double> *__restrict__ twiddle, bool inverse) {
     using C = typename complex_of<T>::type;
     using Rl = typename real_of<T>::type;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int p = P, L = 2 * P - 1;
     constexpr int FX = HALF ? (L + 1) / 2 : L;
     constexpr int n1 = FX * p * p, n2 = FX * L * p;
@@ -130,7 +130,7 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
                                   T *__restrict__ loc, const cplx<double> *__restrict__ twiddle, bool inverse) {
     using C = typename complex_of<T>::type;
     using Rl = typename real_of<T>::type;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int p = P, L = 2 * P - 1;
     constexpr int FX = HALF ? (L + 1) / 2 : L;
     constexpr int n1 = FX * L * p, n2 = FX * p * p;
