@@ -3,7 +3,10 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <array>
 #include <complex>
+#include <set>
+#include <unordered_map>
 
 #include "h2g_internal.h"
 
@@ -270,7 +273,8 @@ int level_of(const std::vector<int32_t> &off, int b) {
 
 void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<int32_t> &lists,
                          const std::vector<int64_t> &iptr_level, const std::vector<int32_t> &isrc,
-                         const std::vector<int32_t> &iop, const std::vector<int32_t> &isym, bool sym);
+                         const std::vector<int32_t> &iop, const std::vector<int32_t> &isym, bool sym,
+                         const std::vector<uint8_t> &partner);
 
 void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, const int32_t *recv_of_pair,
                   const int32_t *part_of_pair, const int32_t *pair_op, const int32_t *pair_sym, int64_t n_pairs) {
@@ -394,7 +398,7 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
         for (int l = 0; l < dst.n_levels; ++l) {
             if (P.m2l_recv[l].count == 0) continue;
             std::vector<int64_t> lvl(iptr.begin() + P.m2l_base[l], iptr.begin() + P.m2l_base[l] + P.m2l_recv[l].count + 1);
-            build_lattice_level(c, P, l, lists, lvl, isrc, iop, isym, pair_sym != nullptr);
+            build_lattice_level(c, P, l, lists, lvl, isrc, iop, isym, pair_sym != nullptr, partner);
         }
 
     // output mask: dofs of initialised dst leaves
@@ -409,7 +413,8 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
 // couplings do not follow the parent-neighbour lattice with one operator per offset.
 void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<int32_t> &lists,
                          const std::vector<int64_t> &iptr_level, const std::vector<int32_t> &isrc,
-                         const std::vector<int32_t> &iop, const std::vector<int32_t> &isym, bool sym) {
+                         const std::vector<int32_t> &iop, const std::vector<int32_t> &isym, bool sym,
+                         const std::vector<uint8_t> &partner) {
     LatticeLevel &LL = P.lattice[l];
     LL.ok = false;
     const Side &src = *P.src, &dst = *P.dst;
@@ -496,26 +501,169 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
         }
     }
     if (implied != n_items) return;
-    // compact each block's present sources to the front (the kernel loops over them only)
-    std::vector<int32_t> csrc((size_t)216 * nb, -1), cq((size_t)216 * nb, 0), nq(nb, 0);
-    for (int k = 0; k < nb; ++k) {
-        int c = 0;
+    // Tiles of parent blocks.  A region child is staged only when it is a source of some
+    // coupling (its moment is computed); the others read as zeros.
+    auto key3 = [](int64_t x, int64_t y, int64_t z) { return ((x + 1) << 42) | ((y + 1) << 21) | (z + 1); };
+    std::unordered_map<int64_t, int> src_slot, block_at;
+    src_slot.reserve(src_at.size() * 2);
+    for (const auto &kv : src_at)
+        if (partner[src.level_offsets[l] + kv.second])
+            src_slot[key3(std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first))] = kv.second;
+    for (int k = 0; k < nb; ++k)
+        block_at[key3(cd(dst, parents[k], 0), cd(dst, parents[k], 1), cd(dst, parents[k], 2))] = k;
+    auto morton = [](uint32_t x, uint32_t y, uint32_t z) {
+        uint64_t m = 0;
+        for (int bit = 0; bit < 21; ++bit)
+            m |= ((uint64_t)((x >> bit) & 1) << (3 * bit)) | ((uint64_t)((y >> bit) & 1) << (3 * bit + 1)) |
+                 ((uint64_t)((z >> bit) & 1) << (3 * bit + 2));
+        return m;
+    };
+    constexpr int QL = 208;
+    // blocks in Morton order of their parent coordinate; present sources of each block
+    std::vector<std::pair<uint64_t, int>> ord(nb);
+    for (int k = 0; k < nb; ++k)
+        ord[k] = {morton(cd(dst, parents[k], 0), cd(dst, parents[k], 1), cd(dst, parents[k], 2)), k};
+    std::sort(ord.begin(), ord.end());
+    auto sources_of = [&](int k) {  // (source slot, q) of block k's present, coupled slots
+        const int px = cd(dst, parents[k], 0), py = cd(dst, parents[k], 1), pz = cd(dst, parents[k], 2);
+        const int32_t *bt = tgt.data() + (size_t)k * 8;
+        std::vector<std::pair<int, int>> mine;
         for (int q = 0; q < 216; ++q) {
-            const int v = bsrc[(size_t)q * nb + k];
-            if (v < 0) continue;
-            csrc[(size_t)c * nb + k] = v;
-            cq[(size_t)c * nb + k] = q;
-            ++c;
+            const int dn = q / 8, sc = q % 8;
+            const int64_t cx = 2 * (int64_t)(px + dn % 3 - 1) + (sc & 1);
+            const int64_t cy = 2 * (int64_t)(py + (dn / 3) % 3 - 1) + ((sc >> 1) & 1);
+            const int64_t cz = 2 * (int64_t)(pz + dn / 9 - 1) + ((sc >> 2) & 1);
+            if (cx < 0 || cy < 0 || cz < 0) continue;
+            auto it = src_slot.find(key3(cx, cy, cz));
+            if (it == src_slot.end()) continue;
+            bool any = false;
+            for (int t = 0; t < 8; ++t) any = any || (bt[t] >= 0 && tq_op[t * 216 + q] >= 0);
+            if (any) mine.push_back({it->second, q});
         }
-        nq[k] = c;
+        return mine;
+    };
+    // Cluster tiles: up to 16 blocks, union of present sources at most 1024.
+    {
+        LatticeTiles &T = LL.clus;
+        constexpr int NB = 16, U = 1024;
+        std::vector<int32_t> slot, npb, ttgt;
+        std::vector<int16_t> qt;
+        std::unordered_map<int, int> uni;
+        std::vector<std::vector<std::pair<int, int>>> bsrc;
+        std::vector<int> bid;
+        auto close_tile = [&]() {
+            if (bid.empty()) return;
+            const size_t ti = npb.size();
+            slot.resize((ti + 1) * U, -1);
+            for (const auto &kv : uni) slot[ti * U + kv.second] = kv.first;
+            ttgt.resize((ti + 1) * NB * 8, -1);
+            qt.resize((ti + 1) * NB * 216, (int16_t)U);
+            for (size_t j = 0; j < bid.size(); ++j) {
+                for (int t = 0; t < 8; ++t) ttgt[(ti * NB + j) * 8 + t] = tgt[(size_t)bid[j] * 8 + t];
+                for (const auto &e : bsrc[j]) qt[(ti * NB + j) * 216 + e.second] = (int16_t)uni[e.first];
+            }
+            npb.push_back((int)bid.size());
+            uni.clear();
+            bsrc.clear();
+            bid.clear();
+        };
+        for (const auto &ok : ord) {
+            auto mine = sources_of(ok.second);
+            int add = 0;
+            for (const auto &e : mine) add += uni.count(e.first) ? 0 : 1;
+            if ((int)bid.size() == NB || (int)uni.size() + add > U) close_tile();
+            for (const auto &e : mine)
+                if (!uni.count(e.first)) {
+                    const int u = (int)uni.size();
+                    uni[e.first] = u;
+                }
+            bsrc.push_back(std::move(mine));
+            bid.push_back(ok.second);
+        }
+        close_tile();
+        T.n_tiles = (int)npb.size();
+        T.slot.upload_vec(slot);
+        T.npb.upload_vec(npb);
+        T.tgt.upload_vec(ttgt);
+        T.qt.upload_vec(qt);
     }
-    LL.n_blocks = nb;
-    LL.blk_src.upload_vec(csrc);
-    LL.blk_q.upload_vec(cq);
-    LL.blk_nq.upload_vec(nq);
-    LL.blk_tgt.upload_vec(tgt);
-    LL.tq_op.upload_vec(tq_op);
-    if (sym) LL.tq_sym.upload_vec(tq_sym);
+    // List tiles: up to 8 receiving parents, consecutive in Morton order, whose present sources
+    // (the children with couplings around them) number at most 512 together; the tile's slot
+    // table is that union, and each block lists (union index, q) of its own sources.  On dense
+    // levels a tile is an aligned 2 x 2 x 2 cube; on sparse levels it gathers 8 scattered
+    // parents, so the region staged per block stays small.
+    {
+        LatticeTiles &T = LL.list;
+        constexpr int NB = 8, NSLOT = 512;
+        std::vector<int32_t> slot, npb, ttgt, tlist, tnl;
+        std::unordered_map<int, int> uni;  // source slot -> union index of the open tile
+        std::vector<std::vector<std::pair<int, int>>> blk_src;  // per block of the open tile: (slot, q)
+        std::vector<int> blk_id;
+        auto close_tile = [&]() {
+            if (blk_id.empty()) return;
+            const size_t ti = npb.size();
+            slot.resize((ti + 1) * NSLOT, -1);
+            for (const auto &kv : uni) slot[ti * NSLOT + kv.second] = kv.first;
+            ttgt.resize((ti + 1) * NB * 8, -1);
+            tlist.resize((ti + 1) * NB * QL, 0);
+            tnl.resize((ti + 1) * NB, 0);
+            for (size_t j = 0; j < blk_id.size(); ++j) {
+                for (int t = 0; t < 8; ++t) ttgt[(ti * NB + j) * 8 + t] = tgt[(size_t)blk_id[j] * 8 + t];
+                // pair entries across the LDS bank halves of their region row and coupling row
+                std::vector<int32_t> cls[4];
+                for (const auto &e : blk_src[j]) {
+                    const int ri = uni[e.first], q = e.second;
+                    cls[((q & 1) << 1) | (ri & 1)].push_back(ri | (q << 16));
+                }
+                std::vector<int32_t> o;
+                for (int a = 0; a < 2; ++a) {
+                    auto &x = cls[a], &y = cls[3 - a];
+                    const size_t k = std::min(x.size(), y.size());
+                    for (size_t i = 0; i < k; ++i) {
+                        o.push_back(x[i]);
+                        o.push_back(y[i]);
+                    }
+                    x.erase(x.begin(), x.begin() + k);
+                    y.erase(y.begin(), y.begin() + k);
+                }
+                for (auto &x : cls) o.insert(o.end(), x.begin(), x.end());
+                if (o.size() & 1) o.push_back(NSLOT | (104 << 16));  // zero slot, zero coupling row
+                std::copy(o.begin(), o.end(), tlist.begin() + (ti * NB + j) * QL);
+                tnl[ti * NB + j] = (int)o.size();
+            }
+            npb.push_back((int)blk_id.size());
+            uni.clear();
+            blk_src.clear();
+            blk_id.clear();
+        };
+        for (const auto &ok : ord) {
+            const int k = ok.second;
+            auto mine = sources_of(k);
+            int add = 0;
+            for (const auto &e : mine) add += uni.count(e.first) ? 0 : 1;
+            if ((int)blk_id.size() == NB || (int)uni.size() + add > NSLOT) close_tile();
+            for (const auto &e : mine)
+                if (!uni.count(e.first)) {
+                    const int u = (int)uni.size();
+                    uni[e.first] = u;
+                }
+            blk_src.push_back(std::move(mine));
+            blk_id.push_back(k);
+        }
+        close_tile();
+        T.n_tiles = (int)npb.size();
+        T.slot.upload_vec(slot);
+        T.npb.upload_vec(npb);
+        T.tgt.upload_vec(ttgt);
+        T.list.upload_vec(tlist);
+        T.nlist.upload_vec(tnl);
+    }
+    DevBuf d_op, d_sym;
+    d_op.upload_vec(tq_op);
+    if (sym) d_sym.upload_vec(tq_sym);
+    build_lattice_ea(d_op.as<int32_t>(), sym ? d_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), c.ops.ptr, c.L,
+                     c.half_spectrum, (c.precisions & (1 << H2G_PREC_PLAIN)) != 0,
+                     (c.precisions & (1 << H2G_PREC_MIXED)) != 0, LL);
     LL.ok = true;
 }
 
@@ -632,17 +780,14 @@ void far_pass(Workspace &w, int op, int R) {
             const int box0 = src.level_offsets[l];
             const int nsrc = src.level_offsets[l + 1] - box0;
             const LatticeLevel &LL = P.lattice[l];
-            if (LL.ok && R <= 256) {
+            if (LL.ok) {
                 const bool half = c.half_spectrum;
                 w.mark_begin(4, l);
                 launch_fft_in_fm(cd, half, inv_in, box0, nsrc, w.mom.ptr, K, c.p, c.L, R, w.spec_in.ptr,
                                  c.twiddle.ptr, s);
                 w.mark_end();
                 w.mark_begin(5, l);
-                launch_hadamard_lat(cd, conj, half, LL.n_blocks, LL.blk_src.as<int32_t>(), LL.blk_q.as<int32_t>(),
-                                    LL.blk_nq.as<int32_t>(), LL.blk_tgt.as<int32_t>(),
-                                    LL.tq_op.as<int32_t>(), sym ? LL.tq_sym.as<int32_t>() : nullptr,
-                                    c.perms.as<int32_t>(), ops, c.L, nsrc, n, w.spec_in.ptr, w.spec_out.ptr, R, s);
+                launch_m2l_lattice(cd, conj, LL, nsrc, n, w.spec_in.ptr, w.spec_out.ptr, R, s);
                 w.mark_end();
                 w.mark_begin(6, l);
                 launch_fft_out_fm(cd, half, !inv_in, recv, n, w.spec_out.ptr, K, c.p, c.L, R, w.loc.ptr,

@@ -187,15 +187,26 @@ struct Side {
 };
 
 // Parent-block tables of one level for the lattice M2L kernels.
+// Tiles of receiving parents for the lattice M2L kernels: up to 16 (cluster tiles) or 8 (list
+// tiles) parents, consecutive in Morton order, and the union of their present sources.
+struct LatticeTiles {
+    int n_tiles = 0;
+    DevBuf slot;   // int32 [tile][union size 1024 / 512]: source slot (with couplings) or -1
+    DevBuf npb;    // int32 [tile]: blocks in the tile
+    DevBuf tgt;    // int32 [tile][blocks][8 children]: receiving index of the k-th block's children or -1
+    DevBuf qt;     // int16 [tile][16][216]: union index of each slot q of the k-th block, 1024 if absent
+                   // (cluster tiles)
+    DevBuf list;   // int32 [tile][8][208]: present slots of the k-th block, union index | q << 16,
+                   // padded to an even count (list tiles)
+    DevBuf nlist;  // int32 [tile][8]: list lengths (even)
+};
+
 struct LatticeLevel {
     bool ok = false;
-    int n_blocks = 0;
-    DevBuf blk_src;  // int32 [216][n_blocks]: present sources first (compacted), then -1
-    DevBuf blk_q;    // int32 [216][n_blocks]: slot id q of each compacted entry
-    DevBuf blk_nq;   // int32 [n_blocks]: number of present sources
-    DevBuf blk_tgt;  // int32 [n_blocks][8]: index into the level's receiving list or -1
-    DevBuf tq_op;    // int32 [8][216]
-    DevBuf tq_sym;   // int32 [8][216] (symmetry-reduced kinds)
+    int F = 0;                // stored frequencies
+    LatticeTiles clus;        // cluster tiles (any batch width)
+    LatticeTiles list;        // list tiles (batches of >= 16 columns, FP32 spectra)
+    DevBuf EA64, EA32;        // [F][216 slots q][16 rows m = 2t + c][2 b]: real form of the couplings
 };
 
 // Work lists of one orientation (forward: trial -> test; reverse: test -> trial).
@@ -308,10 +319,10 @@ void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, c
                       int R, void *S, const void *twiddle, hipStream_t s);
 void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,
                        int p, int L, int R, void *loc, const void *twiddle, hipStream_t s);
-void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const int32_t *blk_src,
-                         const int32_t *blk_q, const int32_t *blk_nq, const int32_t *blk_tgt, const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms,
-                         const void *ops, int L, int n_src, int n_recv, const void *S, void *O, int R,
-                         hipStream_t s);
+void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
+                        int R, hipStream_t s);
+void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
+                      bool half, bool want64, bool want32, LatticeLevel &LL);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
                     int R, void *loc, const void *twiddle, hipStream_t s);
 

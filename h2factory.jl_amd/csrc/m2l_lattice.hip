@@ -6,14 +6,10 @@
 // child targets t and 216 source slots q = (neighbour d of P, child sector s).  The coupling
 // of (t, q) depends only on the level and the offset 2d + bits(s) - bits(t).
 //
-// Spectra are frequency-major: S[f][slot][r] (slot = source box index within the level) and
-// O[f][recv][r], so that one workgroup works on one frequency f:
-//   1. E_f[t][q] = transform(D[op(t,q)][perm_sym(t,q)(f)]) is expanded into LDS (symmetry-
-//      reduced storage is read through its permutation here, once per frequency);
-//   2. each lane owns (block, r) and accumulates all 8 targets over the 216 slots; the
-//      source value is loaded once and used by up to 8 targets, E_f is an LDS broadcast.
-// For a real map with Hermitian spectra only the half spectrum fx < (L+1)/2 is stored
-// (real-to-complex modulation); complex maps keep the full L^3 spectrum.
+// This file holds the modulation stages around the coupling stage (which runs on MFMA, in
+// m2l_mfma.hip).  Spectra are frequency-major: S[f][slot][r] (slot = source box index within
+// the level) and O[f][recv][r].  For a real map with Hermitian spectra only the half spectrum
+// fx < (L+1)/2 is stored (real-to-complex modulation); complex maps keep the full L^3 spectrum.
 #include <algorithm>
 
 #include "h2g_internal.h"
@@ -23,11 +19,6 @@ namespace h2g {
 namespace {
 
 constexpr int TPB = 256;
-
-template <class T>
-__device__ inline T maybe_conj2(T v, bool c) {
-    return c ? conjv(v) : v;
-}
 
 // --------------------------------------------------------------------------------------------
 // Modulate-in: pad + 3-D DFT of JB (box, rhs) jobs per workgroup, frequency-major output.
@@ -202,101 +193,6 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
     }
 }
 
-// --------------------------------------------------------------------------------------------
-// Lattice Hadamard: one frequency per workgroup (blockIdx.y), lanes over (block, r); the
-// block chunks of one frequency are consecutive workgroups (blockIdx.x fastest), so the
-// frequency slice they share stays in L2 while they run.
-// blk_src: [216][n_blocks] source slot or -1; blk_tgt: [n_blocks][8] receiving index or -1.
-// tq_op / tq_sym: [8][216] level tables (-1: no coupling).
-// --------------------------------------------------------------------------------------------
-// C: storage type of the spectra (cplx<float> / cplx<double>); accumulation is in FP64, which
-// runs at the same vector rate as unpacked FP32 on CDNA4.  D: storage type of the couplings;
-// the per-frequency coupling table E lives in LDS in that type (FP32 for the mixed form, so
-// each LDS read is 8 bytes).  RPL right-hand sides per lane reuse every E read RPL times.
-template <class C, class D, int RPL>
-__global__ void hadamard_lat_kernel(int n_blocks, const int32_t *__restrict__ blk_src,
-                                    const int32_t *__restrict__ blk_q, const int32_t *__restrict__ blk_nq,
-                                    const int32_t *__restrict__ blk_tgt, const int32_t *__restrict__ tq_op,
-                                    const int32_t *__restrict__ tq_sym, const int32_t *__restrict__ perms,
-                                    const D *__restrict__ ops, int L, bool half, int n_src, int n_recv,
-                                    const C *__restrict__ S, C *__restrict__ O, int R, bool conj) {
-    using A = cplx<double>;
-    __shared__ D E[8 * 216];
-    const int f = blockIdx.y;
-    const int L3 = L * L * L;
-    const int FX = half ? (L + 1) / 2 : L;
-    const int fx = f % FX, fy = (f / FX) % L, fz = f / (FX * L);
-    const int ffull = fx + L * (fy + L * fz);
-    for (int i = threadIdx.x; i < 8 * 216; i += blockDim.x) {
-        const int op = tq_op[i];
-        D v = zero_v<D>();
-        if (op >= 0) {
-            const int sym = tq_sym ? tq_sym[i] : -1;
-            const int ff = sym >= 0 ? perms[(int64_t)sym * L3 + ffull] : ffull;
-            v = maybe_conj2(ops[(int64_t)op * L3 + ff], conj);
-        }
-        E[i] = v;
-    }
-    __syncthreads();
-    const int lanes_per_block = (R + RPL - 1) / RPL;
-    const int per_wg = blockDim.x / lanes_per_block;  // blocks per workgroup pass
-    const int bl = threadIdx.x / lanes_per_block;
-    const int r0 = (threadIdx.x - bl * lanes_per_block) * RPL;
-    if (bl >= per_wg) return;
-    const C *Sf = S + (int64_t)f * n_src * R;
-    C *Of = O + (int64_t)f * n_recv * R;
-    constexpr int U = 8;  // slots in flight per lane
-    for (int b = blockIdx.x * per_wg + bl; b < n_blocks; b += gridDim.x * per_wg) {
-        A acc[8][RPL];
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-#pragma unroll
-            for (int k = 0; k < RPL; ++k) acc[t][k] = zero_v<A>();
-        // present sources are compacted to the front of the block's column of blk_src
-        const int nq = blk_nq[b];
-        int src[U], qs[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            src[u] = u < nq ? blk_src[(int64_t)u * n_blocks + b] : -1;
-            qs[u] = u < nq ? blk_q[(int64_t)u * n_blocks + b] : 0;
-        }
-        for (int i0 = 0; i0 < nq; i0 += U) {
-            A sv[U][RPL];
-            int qc[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-#pragma unroll
-                for (int k = 0; k < RPL; ++k)
-                    sv[u][k] = (src[u] >= 0 && r0 + k < R) ? cvt<A>::from(Sf[(int64_t)src[u] * R + r0 + k])
-                                                           : zero_v<A>();
-                qc[u] = qs[u];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = i0 + U + u;
-                src[u] = i < nq ? blk_src[(int64_t)i * n_blocks + b] : -1;
-                qs[u] = i < nq ? blk_q[(int64_t)i * n_blocks + b] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const A e = cvt<A>::from(E[t * 216 + qc[u]]);
-#pragma unroll
-                    for (int k = 0; k < RPL; ++k) fmacc(acc[t][k], e, sv[u][k]);
-                }
-        }
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int rv = blk_tgt[b * 8 + t];
-            if (rv < 0) continue;
-#pragma unroll
-            for (int k = 0; k < RPL; ++k)
-                if (r0 + k < R) Of[(int64_t)rv * R + r0 + k] = cvt<C>::from(acc[t][k]);
-        }
-    }
-}
-
 template <int JB, class T, bool HALF>
 size_t fft_smem(int p, int L) {
     using C = typename complex_of<T>::type;
@@ -440,33 +336,6 @@ void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv,
     case H2G_C64: fft_out_fm_t<cplx<double>, false>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s); break;
     default: fft_out_fm_t<cplx<float>, false>(recv, nrecv, O, K, p, L, R, loc, twiddle, inverse, s);
     }
-}
-
-void launch_hadamard_lat(int cdtype, bool conj, bool half, int n_blocks, const int32_t *blk_src,
-                         const int32_t *blk_q, const int32_t *blk_nq, const int32_t *blk_tgt, const int32_t *tq_op,
-                         const int32_t *tq_sym, const int32_t *perms, const void *ops, int L, int n_src, int n_recv,
-                         const void *S, void *O, int R, hipStream_t s) {
-    if (n_blocks == 0) return;
-    const int FX = half ? (L + 1) / 2 : L;
-    const int F = FX * L * L;
-    auto go = [&](auto kern, int rpl, auto cs, auto ds) {
-        using CS = decltype(cs);
-        using DS = decltype(ds);
-        const int lanes = (R + rpl - 1) / rpl;
-        const int per_wg = std::max(1, TPB / lanes);
-        const int chunks = std::max(1, std::min((n_blocks + per_wg - 1) / per_wg, 128));
-        kern<<<dim3(chunks, F), TPB, 0, s>>>(n_blocks, blk_src, blk_q, blk_nq, blk_tgt, tq_op, tq_sym, perms,
-                                            (const DS *)ops, L, half, n_src, n_recv, (const CS *)S, (CS *)O, R, conj);
-    };
-    const bool wide = R >= 16;
-    if (cdtype == H2G_F64 || cdtype == H2G_C64) {
-        if (wide) go(hadamard_lat_kernel<cplx<double>, cplx<double>, 2>, 2, cplx<double>{}, cplx<double>{});
-        else go(hadamard_lat_kernel<cplx<double>, cplx<double>, 1>, 1, cplx<double>{}, cplx<double>{});
-    } else {
-        if (wide) go(hadamard_lat_kernel<cplx<float>, cplx<float>, 2>, 2, cplx<float>{}, cplx<float>{});
-        else go(hadamard_lat_kernel<cplx<float>, cplx<float>, 1>, 1, cplx<float>{}, cplx<float>{});
-    }
-    H2G_CHECK_LAUNCH2();
 }
 
 }  // namespace h2g

@@ -1,0 +1,468 @@
+// Lattice M2L coupling stage on FP64 MFMA (v_mfma_f64_16x16x4f64).
+//
+// For one level, one frequency f and one block (receiving parent P with child targets t = 0..7)
+// the coupling stage is
+//     O[t][r] = sum_q E_f[q][t] * S[src(P, q)][r],   q = (neighbour d of P, sector s), 216 slots,
+// a complex (8 x 216) x (216 x R) product whose left factor E_f is the same for every block of
+// the level.  In real form (re/im split) it is one GEMM per (f, column tile):
+//     D (16 x 16) += A (16 x 4) B (4 x 16),   m = 2t + c (c: re / im of the output),
+//                                             k = 2q + b (b: re / im of the source),
+// A[(t,c)][(q,b)] = Re E (c == b), -Im E (c = 0, b = 1), Im E (c = 1, b = 0), i.e. the 2 x 2
+// real matrix of multiplication by E, and B[(q,b)][n] = re / im part of the source of column n.
+// A is stored expanded per frequency ("EA": [f][q][m][b], 32 reals per slot q), so each lane
+// loads its A element with one LDS read; the adjoint negates the Im parts (c != b).
+//
+// Two kernels share this:
+//  * cluster kernel (any precision, narrow batches): tiles of up to 16 receiving parents
+//    (consecutive in Morton order) with the union of their present sources staged in LDS,
+//    columns = (block, r), 16 per MFMA column tile; every tile runs the 208 slots q whose
+//    neighbour is not P itself (those 8 couple only adjacent boxes), absent sources read zeros.
+//  * list kernel: up to 8 receiving parents per tile (consecutive in Morton order) whose present
+//    sources, at most 512, are staged in LDS; 16 columns = the 16 right-hand sides of one block,
+//    and each block runs only its own present slots (a packed list built on the host), which
+//    skips the absent sources of sparse levels.  FP32 spectra, R >= 16.
+// Accumulation is FP64 in both (the FP64 MFMA rate equals the FP64 / unpacked FP32 FMA rate
+// on CDNA4, and it keeps the mixed form's error at the FP32 rounding of its inputs).
+// A workgroup runs a group of consecutive tiles (Morton order, so spatially compact) for one
+// frequency: it loads the frequency's coupling table once and stages the region of tile t + 1
+// in registers while tile t runs on the MFMAs.  Workgroup i takes group (i % 8) * (gridDim.x / 8)
+// + i / 8: consecutive workgroups go to different XCDs, so each XCD gets a contiguous range of
+// groups and its L2 holds only its part of the frequency slice.
+#include <algorithm>
+
+#include "h2g_internal.h"
+
+namespace h2g {
+
+namespace {
+
+typedef double double4_m __attribute__((ext_vector_type(4)));
+
+constexpr int NT = 256;  // threads per workgroup (4 waves)
+
+template <class T> struct real_t { using type = T; };
+template <> struct real_t<cplx<float>> { using type = float; };
+template <> struct real_t<cplx<double>> { using type = double; };
+
+__device__ inline int group_of_workgroup() {
+    const int i = blockIdx.x;
+    return (i & 7) * (int)(gridDim.x >> 3) + (i >> 3);
+}
+
+template <class EA, int NTH>
+__device__ inline void fill_ea(EA *ea, const EA *__restrict__ EAf) {
+    constexpr int N = 216 * 32, PT = (N + NTH - 1) / NTH;
+    EA v[PT];
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+        const int e = u * NTH + (int)threadIdx.x;
+        v[u] = e < N ? EAf[e] : EA(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+        const int e = u * NTH + (int)threadIdx.x;
+        if (e < N) ea[e] = v[u];
+    }
+}
+
+// Region staging, software-pipelined over the tiles of a workgroup: the slot indices of tile
+// t + 2 and the spectra of tile t + 1 are loaded while tile t computes, and stored to LDS after
+// the next barrier.  The region moves in chunks of CE values (16 bytes, or one value when a
+// slot row is shorter): flat chunk e = thread + u * NTH is chunk e % CH of slot e / CH, so
+// consecutive lanes load consecutive chunks of a slot row (coalesced) and store consecutive
+// 16-byte LDS words (no bank conflicts).
+template <class C, int RC, int NSLOT, int NTH>
+struct RegionPipe {
+    static constexpr int CE = (16 / (int)sizeof(C)) < RC ? 16 / (int)sizeof(C) : RC;  // values per chunk
+    static constexpr int CH = RC / CE;                                                // chunks per slot
+    static constexpr int CPT = (NSLOT * CH + NTH - 1) / NTH;                          // chunks per thread
+    int idx[CPT];
+    C val[CPT][CE];
+    __device__ void load_idx(const int32_t *__restrict__ ts) {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int e = (int)threadIdx.x + u * NTH;
+            idx[u] = e < NSLOT * CH ? ts[e / CH] : -1;
+        }
+    }
+    // vec: 16-byte chunks are aligned and inside R (checked by the caller for this chunk of columns)
+    __device__ void load_val(const C *__restrict__ Sf, int R, int r0, bool vec) {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int c = ((int)threadIdx.x + u * NTH) % CH;
+            if (idx[u] >= 0) {
+                const C *src = Sf + (int64_t)idx[u] * R + r0 + c * CE;
+                if (CE * sizeof(C) == 16 && vec) {
+                    const float4 t = *reinterpret_cast<const float4 *>(src);
+                    __builtin_memcpy(&val[u][0], &t, 16);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < CE; ++i) val[u][i] = r0 + c * CE + i < R ? src[i] : zero_v<C>();
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < CE; ++i) val[u][i] = zero_v<C>();
+            }
+        }
+    }
+    __device__ void store(C *reg) const {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int e = (int)threadIdx.x + u * NTH;
+            if (e < NSLOT * CH)
+#pragma unroll
+                for (int i = 0; i < CE; ++i) reg[e * CE + i] = val[u][i];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Cluster kernel (any precision, narrow batches): tiles of up to 16 receiving parents,
+// consecutive in Morton order, with the union of their present sources (at most 1024 slots) and,
+// per block, the union index of each of its 216 slots (1024: absent, a zero row).  Columns =
+// (block, r), 16 per MFMA column tile, all running the same 208 slot pairs.  Each wave takes its
+// own tiles of the group with its own LDS region, so one wave's staging overlaps the MFMAs of
+// the others; the coupling table of the frequency is shared.  Workgroup = (group of tiles, f,
+// chunk of RC columns).
+// ---------------------------------------------------------------------------------------------
+constexpr int CL_NB = 16, CL_U = 1024;
+
+template <class C, class EA, int RC, bool CONJ>
+__global__ void __launch_bounds__(NT)
+    m2l_cluster_kernel(int n_tiles, int tg, const int32_t *__restrict__ tile_slot, const int32_t *__restrict__ tile_qt,
+                       const int32_t *__restrict__ tile_npb, const int32_t *__restrict__ tile_tgt,
+                       const EA *__restrict__ EAg, int n_src, int n_recv, const C *__restrict__ S, C *__restrict__ O,
+                       int R) {
+    using Rl = typename real_t<C>::type;
+    constexpr int NB = CL_NB, U = CL_U, G = RC;        // 16 * RC columns = RC column tiles
+    constexpr int QTW = NB * 216 / 2;                 // int32 words of the int16 slot table
+    constexpr size_t WAVE_BYTES = sizeof(C) * (size_t)(U + 1) * RC + sizeof(int32_t) * QTW;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    EA *ea = reinterpret_cast<EA *>(smem_raw);  // [216][32], shared
+    const int grp = group_of_workgroup();
+    const int t0 = grp * tg, t1 = min(n_tiles, t0 + tg);
+    if (t0 >= t1) return;
+    const int f = blockIdx.y;
+    const int r0 = blockIdx.z * RC;
+    const C *Sf = S + (int64_t)f * n_src * R;
+    fill_ea<EA, NT>(ea, EAg + (int64_t)f * 216 * 32);
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;  // wave index is uniform
+    unsigned char *wb = smem_raw + sizeof(EA) * 216 * 32 + (size_t)w * WAVE_BYTES;
+    C *reg = reinterpret_cast<C *>(wb);                                             // [U + 1][RC]
+    int32_t *qt32 = reinterpret_cast<int32_t *>(wb + sizeof(C) * (size_t)(U + 1) * RC);  // [NB][216] int16
+    const int16_t *qt = reinterpret_cast<const int16_t *>(qt32);
+    if (l < RC) reg[U * RC + l] = zero_v<C>();
+    const int m = l & 15, b = (l >> 4) & 1, h = l >> 5;
+    const int lane_a = h * 32 + m * 2 + b;
+    const bool neg = CONJ && ((m & 1) != b);
+    const Rl *regR = reinterpret_cast<const Rl *>(reg);
+    Rl *OfR = reinterpret_cast<Rl *>(O + (int64_t)f * n_recv * R);
+    for (int t = t0 + w; t < t1; t += NT / 64) {
+        // stage this wave's region and slot table (4 loads in flight per lane per step)
+        const int32_t *ts = tile_slot + (int64_t)t * U;
+        for (int e0 = 0; e0 < U * RC; e0 += 4 * 64) {
+            int sv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) sv[u] = ts[(e0 + u * 64 + l) / RC];
+            C v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = (e0 + u * 64 + l) % RC;
+                v[u] = (sv[u] >= 0 && r0 + r < R) ? Sf[(int64_t)sv[u] * R + r0 + r] : zero_v<C>();
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) reg[e0 + u * 64 + l] = v[u];
+        }
+        {
+            const int32_t *src = tile_qt + (int64_t)t * QTW;
+            int32_t v[(QTW + 63) / 64];
+#pragma unroll
+            for (int u = 0; u < (QTW + 63) / 64; ++u) v[u] = u * 64 + l < QTW ? src[u * 64 + l] : 0;
+#pragma unroll
+            for (int u = 0; u < (QTW + 63) / 64; ++u)
+                if (u * 64 + l < QTW) qt32[u * 64 + l] = v[u];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int npb = __builtin_amdgcn_readfirstlane(tile_npb[t]);
+        const int nct = (npb * RC + 15) / 16;
+        int cblk[G], crr[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int c = 16 * g + m;
+            cblk[g] = c / RC < npb ? c / RC : -1;
+            crr[g] = c % RC;
+        }
+        double4_m acc[G][2];
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g][0] = acc[g][1] = double4_m{0.0, 0.0, 0.0, 0.0};
+        for (int ks0 = 0; ks0 < 104; ks0 += 8) {
+            double a[8], bv[G][8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int ks = ks0 + i;
+                const int q0 = ks < 52 ? 2 * ks : 2 * ks + 8;  // slot pairs, skipping q = 104 .. 111
+                a[i] = (double)ea[q0 * 32 + lane_a];
+                if (neg) a[i] = -a[i];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const int ri = cblk[g] >= 0 ? (int)qt[cblk[g] * 216 + q0 + h] : U;
+                    bv[g][i] = (double)regR[(ri * RC + crr[g]) * 2 + b];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                    if (g < nct)  // wave-uniform
+                        acc[g][i & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[g][i], acc[g][i & 1], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (cblk[g] < 0 || r0 + crr[g] >= R) continue;
+            const int32_t *tt = tile_tgt + ((int64_t)t * NB + cblk[g]) * 8;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int mo = (l >> 4) + 4 * v;
+                const int tgi = tt[mo >> 1];
+                if (tgi >= 0) OfR[((int64_t)tgi * R + r0 + crr[g]) * 2 + (mo & 1)] = (Rl)(acc[g][0][v] + acc[g][1][v]);
+            }
+        }
+        // the next tile's staging overwrites this wave's region
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// List kernel: up to 8 parents per tile, 16 right-hand sides, FP32 spectra; 8 waves, wave w runs the
+// w-th present block of the tile over its own slot list (LDS reads of 8 slot pairs ahead of
+// their MFMAs, four accumulators); workgroup = (group of TG tiles, f, chunk of 16 columns).
+// ---------------------------------------------------------------------------------------------
+constexpr int NT_LIST = 512;
+
+template <bool CONJ>
+__global__ void __launch_bounds__(NT_LIST)
+    m2l_list_kernel(int n_tiles, int tg, const int32_t *__restrict__ tile_slot, const int32_t *__restrict__ tile_npb,
+                    const int32_t *__restrict__ tile_tgt, const int32_t *__restrict__ tile_list,
+                    const int32_t *__restrict__ tile_nlist, const float *__restrict__ EAg, int n_src, int n_recv,
+                    const cplx<float> *__restrict__ S, cplx<float> *__restrict__ O, int R, int vec_ok) {
+    using C = cplx<float>;
+    constexpr int NTL = NT_LIST, RC = 16, NB = 8, QL = 208, NSLOT = 512, LPT = (NB * QL + NTL - 1) / NTL;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C *reg = reinterpret_cast<C *>(smem_raw);                      // [NSLOT + 1][RC], last slot zero
+    float *ea = reinterpret_cast<float *>(reg + (NSLOT + 1) * RC);  // [216][32]
+    int32_t *lst = reinterpret_cast<int32_t *>(ea + 216 * 32);     // [NB][QL]
+    int32_t *s_tgt = lst + NB * QL;                                // [NB][8]
+    int32_t *s_nl = s_tgt + NB * 8;                                // [NB]
+    int32_t *s_npb = s_nl + NB;
+    const int grp = group_of_workgroup();
+    const int t0 = grp * tg, t1 = min(n_tiles, t0 + tg);
+    if (t0 >= t1) return;
+    const int f = blockIdx.y;
+    const int r0 = blockIdx.z * RC;
+    const bool vec = vec_ok && r0 + RC <= R;
+    const C *Sf = S + (int64_t)f * n_src * R;
+    if (threadIdx.x < RC) reg[NSLOT * RC + threadIdx.x] = C{0.f, 0.f};
+    fill_ea<float, NTL>(ea, EAg + (int64_t)f * 216 * 32);
+    RegionPipe<C, RC, NSLOT, NTL> pipe;
+    pipe.load_idx(tile_slot + (int64_t)t0 * NSLOT);
+    pipe.load_val(Sf, R, r0, vec);
+    int idx_n[RegionPipe<C, RC, NSLOT, NTL>::CPT];
+    int lpre[LPT], m_tg = -1, m_nl = 0, m_np = 0;
+    auto load_meta = [&](int t) {
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+            const int e = (int)threadIdx.x + u * NTL;
+            lpre[u] = e < NB * QL ? tile_list[(int64_t)t * NB * QL + e] : 0;
+        }
+        if (threadIdx.x < NB * 8) m_tg = tile_tgt[(int64_t)t * NB * 8 + threadIdx.x];
+        if (threadIdx.x < NB) m_nl = tile_nlist[(int64_t)t * NB + threadIdx.x];
+        m_np = tile_npb[t];
+    };
+    load_meta(t0);
+    if (t0 + 1 < t1) {
+        RegionPipe<C, RC, NSLOT, NTL> tmp;
+        tmp.load_idx(tile_slot + (int64_t)(t0 + 1) * NSLOT);
+#pragma unroll
+        for (int u = 0; u < RegionPipe<C, RC, NSLOT, NTL>::CPT; ++u) idx_n[u] = tmp.idx[u];
+    }
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;  // wave index is uniform
+    const int m = l & 15, b = (l >> 4) & 1, h = l >> 5;
+    const bool neg = CONJ && ((m & 1) != b);
+    const float *regR = reinterpret_cast<const float *>(reg);
+    float *OfR = reinterpret_cast<float *>(O + (int64_t)f * n_recv * R);
+    for (int t = t0; t < t1; ++t) {
+        __syncthreads();  // the previous tile's compute is done with the LDS
+        pipe.store(reg);
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+            const int e = (int)threadIdx.x + u * NTL;
+            if (e < NB * QL) lst[e] = lpre[u];
+        }
+        if (threadIdx.x < NB * 8) s_tgt[threadIdx.x] = m_tg;
+        if (threadIdx.x < NB) s_nl[threadIdx.x] = m_nl;
+        if (threadIdx.x == 0) *s_npb = m_np;
+        __syncthreads();
+        if (t + 1 < t1) {
+#pragma unroll
+            for (int u = 0; u < RegionPipe<C, RC, NSLOT, NTL>::CPT; ++u) pipe.idx[u] = idx_n[u];
+            pipe.load_val(Sf, R, r0, vec);
+            load_meta(t + 1);
+            if (t + 2 < t1) {
+                RegionPipe<C, RC, NSLOT, NTL> tmp;
+                tmp.load_idx(tile_slot + (int64_t)(t + 2) * NSLOT);
+#pragma unroll
+                for (int u = 0; u < RegionPipe<C, RC, NSLOT, NTL>::CPT; ++u) idx_n[u] = tmp.idx[u];
+            }
+        }
+        const int npb = __builtin_amdgcn_readfirstlane(*s_npb);
+        const int j = w;
+        if (j < npb) {
+            const int nk = __builtin_amdgcn_readfirstlane(s_nl[j]) / 2;  // slot pairs
+            const int32_t *lj = lst + j * QL;
+            constexpr int ZERO = (NSLOT) | (104 << 16);  // zero region slot, zero coupling row
+            double4_m acc[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] = double4_m{0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < nk; k0 += 8) {
+                int e[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) e[i] = k0 + i < nk ? lj[2 * (k0 + i) + h] : ZERO;
+                double a[8], bv[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    a[i] = (double)ea[(e[i] >> 16) * 32 + m * 2 + b];
+                    if (neg) a[i] = -a[i];
+                    bv[i] = (double)regR[((e[i] & 0xffff) * RC + m) * 2 + b];
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc[i & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[i], acc[i & 3], 0, 0, 0);
+            }
+            const double4_m sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            if (r0 + m < R) {
+                const int32_t *tt = s_tgt + j * 8;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int mo = (l >> 4) + 4 * v;
+                    const int tgi = tt[mo >> 1];
+                    if (tgi >= 0) OfR[((int64_t)tgi * R + r0 + m) * 2 + (mo & 1)] = (float)sum[v];
+                }
+            }
+        }
+    }
+}
+
+// EA[f][q][m][b] from the level's coupling table: D[op(t, q)] at frequency f, through the
+// symmetry permutation of (t, q) for the symmetry-reduced kinds; zero where (t, q) does not couple.
+template <class EA>
+__global__ void build_ea_kernel(const int32_t *__restrict__ tq_op, const int32_t *__restrict__ tq_sym,
+                                const int32_t *__restrict__ perms, const cplx<double> *__restrict__ ops, int L, int FX,
+                                int F, EA *__restrict__ out) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)F * 216 * 32) return;
+    const int b = (int)(idx & 1), m = (int)((idx >> 1) & 15);
+    const int q = (int)((idx >> 5) % 216);
+    const int f = (int)(idx / (216 * 32));
+    const int t = m >> 1, c = m & 1;
+    const int L3 = L * L * L;
+    const int fx = f % FX, fy = (f / FX) % L, fz = f / (FX * L);
+    const int ffull = fx + L * (fy + L * fz);
+    const int op = tq_op[t * 216 + q];
+    double v = 0.0;
+    if (op >= 0) {
+        const int sym = tq_sym ? tq_sym[t * 216 + q] : -1;
+        const int ff = sym >= 0 ? perms[(int64_t)sym * L3 + ffull] : ffull;
+        const cplx<double> e = ops[(int64_t)op * L3 + ff];
+        v = c == b ? e.re : (c == 0 ? -e.im : e.im);
+    }
+    out[idx] = (EA)v;
+}
+
+template <class K>
+bool allow_lds_m(K kernel, size_t bytes) {
+    int dev = 0, lim = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lim <= 0)
+        lim = 65536;
+    if (bytes > (size_t)lim) return false;
+    if (bytes > 65536)
+        return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+               hipSuccess;
+    return true;
+}
+
+template <class C, class EA, int RC>
+void cluster_go(bool conj, const LatticeLevel &LL, const EA *EAg, int n_src, int n_recv, const void *S, void *O,
+                int R, hipStream_t s) {
+    const LatticeTiles &T = LL.clus;
+    const size_t sh = sizeof(EA) * 216 * 32 +
+                      (NT / 64) * (sizeof(C) * (size_t)(CL_U + 1) * RC + sizeof(int32_t) * (CL_NB * 216 / 2));
+    auto k = conj ? m2l_cluster_kernel<C, EA, RC, true> : m2l_cluster_kernel<C, EA, RC, false>;
+    if (!allow_lds_m(k, sh)) throw Error{H2G_EUNSUPPORTED, "lattice M2L tile does not fit the LDS of this device"};
+    // each wave takes every 4th tile of its workgroup's group
+    const int tg = std::max(4, std::min(32, T.n_tiles / 32 / 4 * 4));
+    const int ng = (T.n_tiles + tg - 1) / tg;
+    const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + RC - 1) / RC));
+    k<<<grid, NT, sh, s>>>(T.n_tiles, tg, T.slot.as<int32_t>(), T.qt.as<int32_t>(), T.npb.as<int32_t>(),
+                           T.tgt.as<int32_t>(), EAg, n_src, n_recv, (const C *)S, (C *)O, R);
+}
+
+}  // namespace
+
+void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
+                        int R, hipStream_t s) {
+    const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
+    if (f64 && !LL.EA64.ptr) throw Error{H2G_EINVAL, "core was created without the plain precision"};
+    if (!f64 && !LL.EA32.ptr) throw Error{H2G_EINVAL, "core was created without the mixed precision"};
+    if (!f64 && R >= 16 && LL.list.n_tiles > 0) {
+        const LatticeTiles &T = LL.list;
+        const size_t sh = sizeof(cplx<float>) * (size_t)(8 * 64 + 1) * 16 + sizeof(float) * 216 * 32 +
+                          sizeof(int32_t) * (8 * 208 + 64 + 8 + 1);
+        auto k = conj ? m2l_list_kernel<true> : m2l_list_kernel<false>;
+        if (!allow_lds_m(k, sh)) throw Error{H2G_EUNSUPPORTED, "lattice M2L tile does not fit the LDS of this device"};
+        const int tg = std::max(1, std::min(16, T.n_tiles / 32));
+        const int ng = (T.n_tiles + tg - 1) / tg;
+        const int vec_ok = (R % 2) == 0;  // 16-byte rows of 16 complex FP32 values
+        const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + 15) / 16));
+        k<<<grid, NT_LIST, sh, s>>>(T.n_tiles, tg, T.slot.as<int32_t>(), T.npb.as<int32_t>(), T.tgt.as<int32_t>(),
+                               T.list.as<int32_t>(), T.nlist.as<int32_t>(), LL.EA32.as<float>(), n_src, n_recv,
+                               (const cplx<float> *)S, (cplx<float> *)O, R, vec_ok);
+    } else if (LL.clus.n_tiles > 0) {
+        if (f64) {
+            cluster_go<cplx<double>, double, 1>(conj, LL, LL.EA64.as<double>(), n_src, n_recv, S, O, R, s);
+        } else {
+            const float *E = LL.EA32.as<float>();
+            if (R == 1) cluster_go<cplx<float>, float, 1>(conj, LL, E, n_src, n_recv, S, O, R, s);
+            else cluster_go<cplx<float>, float, 2>(conj, LL, E, n_src, n_recv, S, O, R, s);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e)};
+}
+
+void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
+                      bool half, bool want64, bool want32, LatticeLevel &LL) {
+    const int FX = half ? (L + 1) / 2 : L;
+    LL.F = FX * L * L;
+    const int64_t n = (int64_t)LL.F * 216 * 32;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (want64) {
+        LL.EA64.alloc((size_t)n * sizeof(double));
+        build_ea_kernel<double><<<nb, 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L, FX, LL.F,
+                                             LL.EA64.as<double>());
+    }
+    if (want32) {
+        LL.EA32.alloc((size_t)n * sizeof(float));
+        build_ea_kernel<float><<<nb, 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L, FX, LL.F,
+                                            LL.EA32.as<float>());
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e)};
+    H2G_HIP(hipDeviceSynchronize());
+}
+
+}  // namespace h2g

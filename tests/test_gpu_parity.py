@@ -124,6 +124,35 @@ def test_laplace_method(gpu, kind, p, nodes):
         assert relerr(y, O.mul(core, x, op).real) < RTOL, op
 
 
+@pytest.mark.parametrize("kind,p,nodes", [("toeplitz", 6, "equi"), ("dense", 5, "cheb")])
+def test_laplace_method_high_order(gpu, kind, p, nodes):
+    """p = 5, 6 (separable transfers at their largest tile): plain == oracle to 1e-12, and the
+    batched mixed form at 32 / 33 columns within 1e-5 of the plain form."""
+    import h2g
+
+    pts = np.random.default_rng(14).random((6000, 3))
+    core = OB.build_core(pts, p=p, nodekind=nodes, m2l_kind=kind, leaf_size=64)
+    A = make_map(core, max_rhs=2)
+    x = np.random.default_rng(15).standard_normal((6000, 2))
+    for op in "NT":
+        y = np.zeros((6000, 2))
+        h2g.mul_(y, orient(A, op), x)
+        assert relerr(y, O.mul(core, x, op).real) < RTOL, op
+    from convert import to_arrays
+
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
+    plain = h2g.H2Workspace(c, max_rhs=33, precision="plain")
+    mixed = h2g.H2Workspace(c, max_rhs=33, precision="mixed", use_graph=True)
+    X = np.random.default_rng(16).standard_normal((6000, 33))
+    for nrhs in (32, 33):
+        for op in "NT":
+            y64 = np.zeros((6000, nrhs))
+            h2g.mul_(y64, orient(plain, op), X[:, :nrhs])
+            y32 = np.zeros((6000, nrhs), dtype=np.float32)
+            h2g.mul_(y32, orient(mixed, op), X[:, :nrhs].astype(np.float32))
+            assert relerr(y32, y64) < 1e-5, (op, nrhs)
+
+
 @pytest.mark.skip(reason="faulted the GPU twice (memory aperture violation) with the separable-transfer kernel "
                          "for ComplexF64; that path is switched off for complex maps and this case stays off the "
                          "GPU until the cause is shown")
