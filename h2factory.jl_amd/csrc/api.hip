@@ -605,7 +605,7 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
             slot.resize((ti + 1) * NSLOT, -1);
             for (const auto &kv : uni) slot[ti * NSLOT + kv.second] = kv.first;
             ttgt.resize((ti + 1) * NB * 8, -1);
-            tlist.resize((ti + 1) * NB * QL, 0);
+            tlist.resize((ti + 1) * NB * QL * 2, 0);
             tnl.resize((ti + 1) * NB, 0);
             for (size_t j = 0; j < blk_id.size(); ++j) {
                 for (int t = 0; t < 8; ++t) ttgt[(ti * NB + j) * 8 + t] = tgt[(size_t)blk_id[j] * 8 + t];
@@ -627,8 +627,12 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
                     y.erase(y.begin(), y.begin() + k);
                 }
                 for (auto &x : cls) o.insert(o.end(), x.begin(), x.end());
-                if (o.size() & 1) o.push_back(NSLOT | (104 << 16));  // zero slot, zero coupling row
-                std::copy(o.begin(), o.end(), tlist.begin() + (ti * NB + j) * QL);
+                // whole batches of 8 slot pairs: pad with (zero slot, zero coupling row q = 104)
+                while (o.size() % 16) o.push_back(NSLOT | (104 << 16));
+                for (size_t i = 0; i < o.size(); ++i) {
+                    tlist[((ti * NB + j) * QL + i) * 2] = o[i] & 0xffff;  // region slot
+                    tlist[((ti * NB + j) * QL + i) * 2 + 1] = o[i] >> 16;  // q
+                }
                 tnl[ti * NB + j] = (int)o.size();
             }
             npb.push_back((int)blk_id.size());
@@ -662,8 +666,7 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
     d_op.upload_vec(tq_op);
     if (sym) d_sym.upload_vec(tq_sym);
     build_lattice_ea(d_op.as<int32_t>(), sym ? d_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), c.ops.ptr, c.L,
-                     c.half_spectrum, (c.precisions & (1 << H2G_PREC_PLAIN)) != 0,
-                     (c.precisions & (1 << H2G_PREC_MIXED)) != 0, LL);
+                     c.half_spectrum, LL);
     LL.ok = true;
 }
 

@@ -196,9 +196,9 @@ struct LatticeTiles {
     DevBuf tgt;    // int32 [tile][blocks][8 children]: receiving index of the k-th block's children or -1
     DevBuf qt;     // int16 [tile][16][216]: union index of each slot q of the k-th block, 1024 if absent
                    // (cluster tiles)
-    DevBuf list;   // int32 [tile][8][208]: present slots of the k-th block, union index | q << 16,
-                   // padded to an even count (list tiles)
-    DevBuf nlist;  // int32 [tile][8]: list lengths (even)
+    DevBuf list;   // int32 [tile][8][208][2]: (union index, q) of the k-th block's present slots,
+                   // padded to a multiple of 16 entries with (512, 104) (list tiles)
+    DevBuf nlist;  // int32 [tile][8]: list lengths (multiples of 16)
 };
 
 struct LatticeLevel {
@@ -206,7 +206,7 @@ struct LatticeLevel {
     int F = 0;                // stored frequencies
     LatticeTiles clus;        // cluster tiles (any batch width)
     LatticeTiles list;        // list tiles (batches of >= 16 columns, FP32 spectra)
-    DevBuf EA64, EA32;        // [F][216 slots q][16 rows m = 2t + c][2 b]: real form of the couplings
+    DevBuf EA64;              // double [F][216 slots q][16 rows m = 2t + c][2 b]: real form of the couplings
 };
 
 // Work lists of one orientation (forward: trial -> test; reverse: test -> trial).
@@ -322,7 +322,7 @@ void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv,
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s);
 void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
-                      bool half, bool want64, bool want32, LatticeLevel &LL);
+                      bool half, LatticeLevel &LL);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
                     int R, void *loc, const void *twiddle, hipStream_t s);
 

@@ -9,8 +9,8 @@
 //                                             k = 2q + b (b: re / im of the source),
 // A[(t,c)][(q,b)] = Re E (c == b), -Im E (c = 0, b = 1), Im E (c = 1, b = 0), i.e. the 2 x 2
 // real matrix of multiplication by E, and B[(q,b)][n] = re / im part of the source of column n.
-// A is stored expanded per frequency ("EA": [f][q][m][b], 32 reals per slot q), so each lane
-// loads its A element with one LDS read; the adjoint negates the Im parts (c != b).
+// A is stored expanded per frequency ("EA": [f][q][m][b], 32 FP64 values per slot q), so each
+// lane loads its A element with one LDS read; the adjoint negates the Im parts (c != b).
 //
 // Two kernels share this:
 //  * cluster kernel (any precision, narrow batches): tiles of up to 16 receiving parents
@@ -78,18 +78,18 @@ struct RegionPipe {
     static constexpr int CPT = (NSLOT * CH + NTH - 1) / NTH;                          // chunks per thread
     int idx[CPT];
     C val[CPT][CE];
-    __device__ void load_idx(const int32_t *__restrict__ ts) {
+    __device__ void load_idx(const int32_t *__restrict__ ts, int lane) {
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
-            const int e = (int)threadIdx.x + u * NTH;
+            const int e = lane + u * NTH;
             idx[u] = e < NSLOT * CH ? ts[e / CH] : -1;
         }
     }
     // vec: 16-byte chunks are aligned and inside R (checked by the caller for this chunk of columns)
-    __device__ void load_val(const C *__restrict__ Sf, int R, int r0, bool vec) {
+    __device__ void load_val(const C *__restrict__ Sf, int R, int r0, bool vec, int lane) {
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
-            const int c = ((int)threadIdx.x + u * NTH) % CH;
+            const int c = (lane + u * NTH) % CH;
             if (idx[u] >= 0) {
                 const C *src = Sf + (int64_t)idx[u] * R + r0 + c * CE;
                 if (CE * sizeof(C) == 16 && vec) {
@@ -105,10 +105,10 @@ struct RegionPipe {
             }
         }
     }
-    __device__ void store(C *reg) const {
+    __device__ void store(C *reg, int lane) const {
 #pragma unroll
         for (int u = 0; u < CPT; ++u) {
-            const int e = (int)threadIdx.x + u * NTH;
+            const int e = lane + u * NTH;
             if (e < NSLOT * CH)
 #pragma unroll
                 for (int i = 0; i < CE; ++i) reg[e * CE + i] = val[u][i];
@@ -132,11 +132,13 @@ __global__ void __launch_bounds__(NT)
     m2l_cluster_kernel(int n_tiles, int tg, const int32_t *__restrict__ tile_slot, const int32_t *__restrict__ tile_qt,
                        const int32_t *__restrict__ tile_npb, const int32_t *__restrict__ tile_tgt,
                        const EA *__restrict__ EAg, int n_src, int n_recv, const C *__restrict__ S, C *__restrict__ O,
-                       int R) {
+                       int R, int vec_ok) {
     using Rl = typename real_t<C>::type;
-    constexpr int NB = CL_NB, U = CL_U, G = RC;        // 16 * RC columns = RC column tiles
-    constexpr int QTW = NB * 216 / 2;                 // int32 words of the int16 slot table
-    constexpr size_t WAVE_BYTES = sizeof(C) * (size_t)(U + 1) * RC + sizeof(int32_t) * QTW;
+    using Pipe = RegionPipe<C, RC, CL_U, 64>;
+    constexpr int NB = CL_NB, U = CL_U, G = RC;  // 16 * RC columns = RC column tiles
+    constexpr int QTW = NB * 216 / 2;            // int32 words of the int16 slot table
+    constexpr int QPL = (QTW + 63) / 64, TPL = NB * 8 / 64;
+    constexpr size_t WAVE_BYTES = sizeof(C) * (size_t)(U + 1) * RC + sizeof(int32_t) * (QTW + NB * 8);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     EA *ea = reinterpret_cast<EA *>(smem_raw);  // [216][32], shared
     const int grp = group_of_workgroup();
@@ -144,13 +146,15 @@ __global__ void __launch_bounds__(NT)
     if (t0 >= t1) return;
     const int f = blockIdx.y;
     const int r0 = blockIdx.z * RC;
+    const bool vec = vec_ok && r0 + RC <= R;
     const C *Sf = S + (int64_t)f * n_src * R;
     fill_ea<EA, NT>(ea, EAg + (int64_t)f * 216 * 32);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;  // wave index is uniform
     unsigned char *wb = smem_raw + sizeof(EA) * 216 * 32 + (size_t)w * WAVE_BYTES;
-    C *reg = reinterpret_cast<C *>(wb);                                             // [U + 1][RC]
+    C *reg = reinterpret_cast<C *>(wb);                                                  // [U + 1][RC]
     int32_t *qt32 = reinterpret_cast<int32_t *>(wb + sizeof(C) * (size_t)(U + 1) * RC);  // [NB][216] int16
+    int32_t *s_tgt = qt32 + QTW;                                                         // [NB][8]
     const int16_t *qt = reinterpret_cast<const int16_t *>(qt32);
     if (l < RC) reg[U * RC + l] = zero_v<C>();
     const int m = l & 15, b = (l >> 4) & 1, h = l >> 5;
@@ -158,42 +162,65 @@ __global__ void __launch_bounds__(NT)
     const bool neg = CONJ && ((m & 1) != b);
     const Rl *regR = reinterpret_cast<const Rl *>(reg);
     Rl *OfR = reinterpret_cast<Rl *>(O + (int64_t)f * n_recv * R);
-    for (int t = t0 + w; t < t1; t += NT / 64) {
-        // stage this wave's region and slot table (4 loads in flight per lane per step)
-        const int32_t *ts = tile_slot + (int64_t)t * U;
-        for (int e0 = 0; e0 < U * RC; e0 += 4 * 64) {
-            int sv[4];
+    // per-wave pipeline: the slot indices of tile t + 2 and everything else of tile t + 1 are
+    // loaded while tile t computes
+    constexpr int WS = NT / 64;  // waves; wave w takes tiles t0 + w, t0 + w + WS, ...
+    Pipe pipe;
+    int qpre[QPL], tpre[TPL], npre = 0, idx_n[Pipe::CPT];
+    auto load_meta = [&](int t) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) sv[u] = ts[(e0 + u * 64 + l) / RC];
-            C v[4];
+        for (int u = 0; u < QPL; ++u) qpre[u] = u * 64 + l < QTW ? tile_qt[(int64_t)t * QTW + u * 64 + l] : 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int r = (e0 + u * 64 + l) % RC;
-                v[u] = (sv[u] >= 0 && r0 + r < R) ? Sf[(int64_t)sv[u] * R + r0 + r] : zero_v<C>();
-            }
+        for (int u = 0; u < TPL; ++u) tpre[u] = tile_tgt[(int64_t)t * NB * 8 + u * 64 + l];
+        npre = tile_npb[t];
+    };
+    int t = t0 + w;
+    if (t < t1) {
+        pipe.load_idx(tile_slot + (int64_t)t * U, l);
+        pipe.load_val(Sf, R, r0, vec, l);
+        load_meta(t);
+        if (t + WS < t1) {
+            Pipe tmp;
+            tmp.load_idx(tile_slot + (int64_t)(t + WS) * U, l);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) reg[e0 + u * 64 + l] = v[u];
+            for (int u = 0; u < Pipe::CPT; ++u) idx_n[u] = tmp.idx[u];
         }
-        {
-            const int32_t *src = tile_qt + (int64_t)t * QTW;
-            int32_t v[(QTW + 63) / 64];
+    }
+    for (; t < t1; t += WS) {
+        // this wave's previous tile is done with its LDS region (LDS ops of a wave run in order)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        pipe.store(reg, l);
 #pragma unroll
-            for (int u = 0; u < (QTW + 63) / 64; ++u) v[u] = u * 64 + l < QTW ? src[u * 64 + l] : 0;
+        for (int u = 0; u < QPL; ++u)
+            if (u * 64 + l < QTW) qt32[u * 64 + l] = qpre[u];
 #pragma unroll
-            for (int u = 0; u < (QTW + 63) / 64; ++u)
-                if (u * 64 + l < QTW) qt32[u * 64 + l] = v[u];
-        }
+        for (int u = 0; u < TPL; ++u) s_tgt[u * 64 + l] = tpre[u];
+        const int npb = __builtin_amdgcn_readfirstlane(npre);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int npb = __builtin_amdgcn_readfirstlane(tile_npb[t]);
-        const int nct = (npb * RC + 15) / 16;
-        int cblk[G], crr[G];
+        if (t + WS < t1) {
+#pragma unroll
+            for (int u = 0; u < Pipe::CPT; ++u) pipe.idx[u] = idx_n[u];
+            pipe.load_val(Sf, R, r0, vec, l);
+            load_meta(t + WS);
+            if (t + 2 * WS < t1) {
+                Pipe tmp;
+                tmp.load_idx(tile_slot + (int64_t)(t + 2 * WS) * U, l);
+#pragma unroll
+                for (int u = 0; u < Pipe::CPT; ++u) idx_n[u] = tmp.idx[u];
+            }
+        }
+        // column (block, r) of this lane in column tile g; columns past the tile's blocks read
+        // block 0's slots and are not written
+        int cblk[G], crr[G], qbase[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int c = 16 * g + m;
             cblk[g] = c / RC < npb ? c / RC : -1;
             crr[g] = c % RC;
+            qbase[g] = (cblk[g] >= 0 ? cblk[g] : 0) * 216 + h;
         }
         double4_m acc[G][2];
 #pragma unroll
@@ -208,7 +235,7 @@ __global__ void __launch_bounds__(NT)
                 if (neg) a[i] = -a[i];
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
-                    const int ri = cblk[g] >= 0 ? (int)qt[cblk[g] * 216 + q0 + h] : U;
+                    const int ri = qt[qbase[g] + q0];
                     bv[g][i] = (double)regR[(ri * RC + crr[g]) * 2 + b];
                 }
             }
@@ -216,13 +243,12 @@ __global__ void __launch_bounds__(NT)
             for (int i = 0; i < 8; ++i)
 #pragma unroll
                 for (int g = 0; g < G; ++g)
-                    if (g < nct)  // wave-uniform
-                        acc[g][i & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[g][i], acc[g][i & 1], 0, 0, 0);
+                    acc[g][i & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[g][i], acc[g][i & 1], 0, 0, 0);
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             if (cblk[g] < 0 || r0 + crr[g] >= R) continue;
-            const int32_t *tt = tile_tgt + ((int64_t)t * NB + cblk[g]) * 8;
+            const int32_t *tt = s_tgt + cblk[g] * 8;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int mo = (l >> 4) + 4 * v;
@@ -230,10 +256,6 @@ __global__ void __launch_bounds__(NT)
                 if (tgi >= 0) OfR[((int64_t)tgi * R + r0 + crr[g]) * 2 + (mo & 1)] = (Rl)(acc[g][0][v] + acc[g][1][v]);
             }
         }
-        // the next tile's staging overwrites this wave's region
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -248,15 +270,15 @@ template <bool CONJ>
 __global__ void __launch_bounds__(NT_LIST)
     m2l_list_kernel(int n_tiles, int tg, const int32_t *__restrict__ tile_slot, const int32_t *__restrict__ tile_npb,
                     const int32_t *__restrict__ tile_tgt, const int32_t *__restrict__ tile_list,
-                    const int32_t *__restrict__ tile_nlist, const float *__restrict__ EAg, int n_src, int n_recv,
+                    const int32_t *__restrict__ tile_nlist, const double *__restrict__ EAg, int n_src, int n_recv,
                     const cplx<float> *__restrict__ S, cplx<float> *__restrict__ O, int R, int vec_ok) {
     using C = cplx<float>;
-    constexpr int NTL = NT_LIST, RC = 16, NB = 8, QL = 208, NSLOT = 512, LPT = (NB * QL + NTL - 1) / NTL;
+    constexpr int NTL = NT_LIST, RC = 16, NB = 8, QL = 208, NSLOT = 512, LW = NB * QL * 2, LPT = (LW + NTL - 1) / NTL;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *reg = reinterpret_cast<C *>(smem_raw);                      // [NSLOT + 1][RC], last slot zero
-    float *ea = reinterpret_cast<float *>(reg + (NSLOT + 1) * RC);  // [216][32]
-    int32_t *lst = reinterpret_cast<int32_t *>(ea + 216 * 32);     // [NB][QL]
-    int32_t *s_tgt = lst + NB * QL;                                // [NB][8]
+    double *ea = reinterpret_cast<double *>(reg + (NSLOT + 1) * RC);  // [216][32]
+    int32_t *lst = reinterpret_cast<int32_t *>(ea + 216 * 32);     // [NB][QL] (region slot, q) pairs
+    int32_t *s_tgt = lst + LW;                                     // [NB][8]
     int32_t *s_nl = s_tgt + NB * 8;                                // [NB]
     int32_t *s_npb = s_nl + NB;
     const int grp = group_of_workgroup();
@@ -267,17 +289,17 @@ __global__ void __launch_bounds__(NT_LIST)
     const bool vec = vec_ok && r0 + RC <= R;
     const C *Sf = S + (int64_t)f * n_src * R;
     if (threadIdx.x < RC) reg[NSLOT * RC + threadIdx.x] = C{0.f, 0.f};
-    fill_ea<float, NTL>(ea, EAg + (int64_t)f * 216 * 32);
+    fill_ea<double, NTL>(ea, EAg + (int64_t)f * 216 * 32);
     RegionPipe<C, RC, NSLOT, NTL> pipe;
-    pipe.load_idx(tile_slot + (int64_t)t0 * NSLOT);
-    pipe.load_val(Sf, R, r0, vec);
+    pipe.load_idx(tile_slot + (int64_t)t0 * NSLOT, threadIdx.x);
+    pipe.load_val(Sf, R, r0, vec, threadIdx.x);
     int idx_n[RegionPipe<C, RC, NSLOT, NTL>::CPT];
     int lpre[LPT], m_tg = -1, m_nl = 0, m_np = 0;
     auto load_meta = [&](int t) {
 #pragma unroll
         for (int u = 0; u < LPT; ++u) {
             const int e = (int)threadIdx.x + u * NTL;
-            lpre[u] = e < NB * QL ? tile_list[(int64_t)t * NB * QL + e] : 0;
+            lpre[u] = e < LW ? tile_list[(int64_t)t * LW + e] : 0;
         }
         if (threadIdx.x < NB * 8) m_tg = tile_tgt[(int64_t)t * NB * 8 + threadIdx.x];
         if (threadIdx.x < NB) m_nl = tile_nlist[(int64_t)t * NB + threadIdx.x];
@@ -286,7 +308,7 @@ __global__ void __launch_bounds__(NT_LIST)
     load_meta(t0);
     if (t0 + 1 < t1) {
         RegionPipe<C, RC, NSLOT, NTL> tmp;
-        tmp.load_idx(tile_slot + (int64_t)(t0 + 1) * NSLOT);
+        tmp.load_idx(tile_slot + (int64_t)(t0 + 1) * NSLOT, threadIdx.x);
 #pragma unroll
         for (int u = 0; u < RegionPipe<C, RC, NSLOT, NTL>::CPT; ++u) idx_n[u] = tmp.idx[u];
     }
@@ -297,11 +319,11 @@ __global__ void __launch_bounds__(NT_LIST)
     float *OfR = reinterpret_cast<float *>(O + (int64_t)f * n_recv * R);
     for (int t = t0; t < t1; ++t) {
         __syncthreads();  // the previous tile's compute is done with the LDS
-        pipe.store(reg);
+        pipe.store(reg, threadIdx.x);
 #pragma unroll
         for (int u = 0; u < LPT; ++u) {
             const int e = (int)threadIdx.x + u * NTL;
-            if (e < NB * QL) lst[e] = lpre[u];
+            if (e < LW) lst[e] = lpre[u];
         }
         if (threadIdx.x < NB * 8) s_tgt[threadIdx.x] = m_tg;
         if (threadIdx.x < NB) s_nl[threadIdx.x] = m_nl;
@@ -310,11 +332,11 @@ __global__ void __launch_bounds__(NT_LIST)
         if (t + 1 < t1) {
 #pragma unroll
             for (int u = 0; u < RegionPipe<C, RC, NSLOT, NTL>::CPT; ++u) pipe.idx[u] = idx_n[u];
-            pipe.load_val(Sf, R, r0, vec);
+            pipe.load_val(Sf, R, r0, vec, threadIdx.x);
             load_meta(t + 1);
             if (t + 2 < t1) {
                 RegionPipe<C, RC, NSLOT, NTL> tmp;
-                tmp.load_idx(tile_slot + (int64_t)(t + 2) * NSLOT);
+                tmp.load_idx(tile_slot + (int64_t)(t + 2) * NSLOT, threadIdx.x);
 #pragma unroll
                 for (int u = 0; u < RegionPipe<C, RC, NSLOT, NTL>::CPT; ++u) idx_n[u] = tmp.idx[u];
             }
@@ -322,22 +344,23 @@ __global__ void __launch_bounds__(NT_LIST)
         const int npb = __builtin_amdgcn_readfirstlane(*s_npb);
         const int j = w;
         if (j < npb) {
+            // the list holds (region slot, q) pairs, padded with zero entries to whole batches of
+            // 8 slot pairs; lane half h takes the second slot of each pair
             const int nk = __builtin_amdgcn_readfirstlane(s_nl[j]) / 2;  // slot pairs
-            const int32_t *lj = lst + j * QL;
-            constexpr int ZERO = (NSLOT) | (104 << 16);  // zero region slot, zero coupling row
+            const int2 *lj = reinterpret_cast<const int2 *>(lst) + j * QL + h;
             double4_m acc[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i] = double4_m{0.0, 0.0, 0.0, 0.0};
             for (int k0 = 0; k0 < nk; k0 += 8) {
-                int e[8];
+                int2 e[8];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) e[i] = k0 + i < nk ? lj[2 * (k0 + i) + h] : ZERO;
+                for (int i = 0; i < 8; ++i) e[i] = lj[2 * (k0 + i)];
                 double a[8], bv[8];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    a[i] = (double)ea[(e[i] >> 16) * 32 + m * 2 + b];
+                    a[i] = ea[e[i].y * 32 + m * 2 + b];
                     if (neg) a[i] = -a[i];
-                    bv[i] = (double)regR[((e[i] & 0xffff) * RC + m) * 2 + b];
+                    bv[i] = (double)regR[(e[i].x * RC + m) * 2 + b];
                 }
 #pragma unroll
                 for (int i = 0; i < 8; ++i) acc[i & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[i], acc[i & 3], 0, 0, 0);
@@ -400,15 +423,16 @@ void cluster_go(bool conj, const LatticeLevel &LL, const EA *EAg, int n_src, int
                 int R, hipStream_t s) {
     const LatticeTiles &T = LL.clus;
     const size_t sh = sizeof(EA) * 216 * 32 +
-                      (NT / 64) * (sizeof(C) * (size_t)(CL_U + 1) * RC + sizeof(int32_t) * (CL_NB * 216 / 2));
+                      (NT / 64) * (sizeof(C) * (size_t)(CL_U + 1) * RC + sizeof(int32_t) * (CL_NB * 216 / 2 + CL_NB * 8));
     auto k = conj ? m2l_cluster_kernel<C, EA, RC, true> : m2l_cluster_kernel<C, EA, RC, false>;
     if (!allow_lds_m(k, sh)) throw Error{H2G_EUNSUPPORTED, "lattice M2L tile does not fit the LDS of this device"};
     // each wave takes every 4th tile of its workgroup's group
     const int tg = std::max(4, std::min(32, T.n_tiles / 32 / 4 * 4));
     const int ng = (T.n_tiles + tg - 1) / tg;
     const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + RC - 1) / RC));
+    const int vec_ok = ((size_t)R * sizeof(C)) % 16 == 0;
     k<<<grid, NT, sh, s>>>(T.n_tiles, tg, T.slot.as<int32_t>(), T.qt.as<int32_t>(), T.npb.as<int32_t>(),
-                           T.tgt.as<int32_t>(), EAg, n_src, n_recv, (const C *)S, (C *)O, R);
+                           T.tgt.as<int32_t>(), EAg, n_src, n_recv, (const C *)S, (C *)O, R, vec_ok);
 }
 
 }  // namespace
@@ -416,12 +440,11 @@ void cluster_go(bool conj, const LatticeLevel &LL, const EA *EAg, int n_src, int
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
-    if (f64 && !LL.EA64.ptr) throw Error{H2G_EINVAL, "core was created without the plain precision"};
-    if (!f64 && !LL.EA32.ptr) throw Error{H2G_EINVAL, "core was created without the mixed precision"};
+    if (!LL.EA64.ptr) throw Error{H2G_EINVAL, "lattice coupling table missing"};
     if (!f64 && R >= 16 && LL.list.n_tiles > 0) {
         const LatticeTiles &T = LL.list;
-        const size_t sh = sizeof(cplx<float>) * (size_t)(8 * 64 + 1) * 16 + sizeof(float) * 216 * 32 +
-                          sizeof(int32_t) * (8 * 208 + 64 + 8 + 1);
+        const size_t sh = sizeof(cplx<float>) * (size_t)(8 * 64 + 1) * 16 + sizeof(double) * 216 * 32 +
+                          sizeof(int32_t) * (8 * 208 * 2 + 64 + 8 + 1);
         auto k = conj ? m2l_list_kernel<true> : m2l_list_kernel<false>;
         if (!allow_lds_m(k, sh)) throw Error{H2G_EUNSUPPORTED, "lattice M2L tile does not fit the LDS of this device"};
         const int tg = std::max(1, std::min(16, T.n_tiles / 32));
@@ -429,37 +452,26 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
         const int vec_ok = (R % 2) == 0;  // 16-byte rows of 16 complex FP32 values
         const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + 15) / 16));
         k<<<grid, NT_LIST, sh, s>>>(T.n_tiles, tg, T.slot.as<int32_t>(), T.npb.as<int32_t>(), T.tgt.as<int32_t>(),
-                               T.list.as<int32_t>(), T.nlist.as<int32_t>(), LL.EA32.as<float>(), n_src, n_recv,
+                               T.list.as<int32_t>(), T.nlist.as<int32_t>(), LL.EA64.as<double>(), n_src, n_recv,
                                (const cplx<float> *)S, (cplx<float> *)O, R, vec_ok);
     } else if (LL.clus.n_tiles > 0) {
-        if (f64) {
-            cluster_go<cplx<double>, double, 1>(conj, LL, LL.EA64.as<double>(), n_src, n_recv, S, O, R, s);
-        } else {
-            const float *E = LL.EA32.as<float>();
-            if (R == 1) cluster_go<cplx<float>, float, 1>(conj, LL, E, n_src, n_recv, S, O, R, s);
-            else cluster_go<cplx<float>, float, 2>(conj, LL, E, n_src, n_recv, S, O, R, s);
-        }
+        const double *E = LL.EA64.as<double>();
+        if (f64) cluster_go<cplx<double>, double, 1>(conj, LL, E, n_src, n_recv, S, O, R, s);
+        else if (R == 1) cluster_go<cplx<float>, double, 1>(conj, LL, E, n_src, n_recv, S, O, R, s);
+        else cluster_go<cplx<float>, double, 2>(conj, LL, E, n_src, n_recv, S, O, R, s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e)};
 }
 
 void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
-                      bool half, bool want64, bool want32, LatticeLevel &LL) {
+                      bool half, LatticeLevel &LL) {
     const int FX = half ? (L + 1) / 2 : L;
     LL.F = FX * L * L;
     const int64_t n = (int64_t)LL.F * 216 * 32;
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    if (want64) {
-        LL.EA64.alloc((size_t)n * sizeof(double));
-        build_ea_kernel<double><<<nb, 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L, FX, LL.F,
-                                             LL.EA64.as<double>());
-    }
-    if (want32) {
-        LL.EA32.alloc((size_t)n * sizeof(float));
-        build_ea_kernel<float><<<nb, 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L, FX, LL.F,
-                                            LL.EA32.as<float>());
-    }
+    LL.EA64.alloc((size_t)n * sizeof(double));
+    build_ea_kernel<double><<<(unsigned)((n + 255) / 256), 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L,
+                                                                   FX, LL.F, LL.EA64.as<double>());
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e)};
     H2G_HIP(hipDeviceSynchronize());

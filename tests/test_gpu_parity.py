@@ -246,11 +246,13 @@ def test_lattice_toeplitz_matches_generic(gpu, kind, real):
             ref = ref.real if real else ref
             assert relerr(yl, ref) < RTOL, (op, nrhs)
             assert relerr(yl, yg) < RTOL, (op, nrhs)
-    mixed = h2g.H2Workspace(lat.core, max_rhs=8, precision="mixed")
-    x = rng.standard_normal((n, 8)) + (0 if real else 1j * rng.standard_normal((n, 8)))
-    for op in "NT":
-        ref = O.mul(core, x, op)
-        assert relerr(orient(mixed, op) @ x, ref.real if real else ref) < 1e-5, op
+    # mixed form: narrow batches (1, 3, 8 columns) and a wide one (20: per-block slot lists)
+    mixed = h2g.H2Workspace(lat.core, max_rhs=20, precision="mixed")
+    for nrhs in (1, 3, 8, 20):
+        x = rng.standard_normal((n, nrhs)) + (0 if real else 1j * rng.standard_normal((n, nrhs)))
+        for op in "NTC":
+            ref = O.mul(core, x, op)
+            assert relerr(orient(mixed, op) @ x, ref.real if real else ref) < 1e-5, (op, nrhs)
 
 
 def test_mfma_f64_lane_layout(gpu):
