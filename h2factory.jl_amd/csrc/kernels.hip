@@ -562,9 +562,15 @@ void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys
     H2G_CHECK_LAUNCH();
 }
 
+bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                     const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s);
+bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                     const void *coll, int K, const void *loc, int R, void *ys, double alpha, hipStream_t s);
+
 void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                 const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
     if (n == 0) return;
+    if (launch_p2m_mfma(cdtype, leaves, n, begin, end, coll, K, xs, R, mom, s)) return;
     DISPATCH_T(cdtype, {
         const int pch = std::max(1, std::min(64, (int)(49152 / (sizeof(T) * (size_t)R))));
         p2m_kernel<T><<<n, TPB, sizeof(T) * pch * (size_t)R, s>>>(leaves, begin, end, (const T *)coll, K,
@@ -576,6 +582,7 @@ void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64
 void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                 const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, hipStream_t s) {
     if (n == 0) return;
+    if (alpha.im == 0.0 && launch_l2p_mfma(cdtype, leaves, n, begin, end, coll, K, loc, R, ys, alpha.re, s)) return;
     DISPATCH_T(cdtype, {
         // RHS per LDS pass: all of them when K x R fits in 48 KB
         const int rc = std::max(1, std::min(R, (int)(49152 / (sizeof(T) * (size_t)K))));
@@ -826,14 +833,14 @@ __device__ inline int mf_a_row(int lane) { return lane & 15; }
 __device__ inline int mf_a_k(int lane) { return lane >> 4; }
 __device__ inline int mf_d_row(int lane, int v) { return (lane >> 4) + 4 * v; }
 
-template <bool TRANS>
-__global__ void near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t *__restrict__ blk,
+template <bool TRANS, int NCT>
+__global__ void __launch_bounds__(TPB) near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t *__restrict__ blk,
                                  const int64_t *__restrict__ rb, const int64_t *__restrict__ cb,
                                  const int32_t *__restrict__ nr, const int32_t *__restrict__ nc,
                                  const int64_t *__restrict__ off, const double *__restrict__ vals,
                                  const double *__restrict__ xs, double *__restrict__ ys, int R) {
-    // a wave owns one 16 x 16 output tile (row tile, RHS tile); k-steps are issued four at a
-    // time so that several operand loads are in flight per wave
+    // a wave owns one 16-row output tile and NCT 16-column RHS tiles: every near-field value is
+    // loaded once and feeds NCT MFMAs; the operands of KU k-steps are loaded one batch ahead
     constexpr int KU = 4;
     const int t = blockIdx.x;
     const int64_t b0 = ptr[t], b1 = ptr[t + 1];
@@ -842,38 +849,191 @@ __global__ void near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t 
     const int n_out = TRANS ? nc[first] : nr[first];
     const int64_t out0 = TRANS ? cb[first] : rb[first];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const int n_rt = (n_out + 15) / 16, n_ct = (R + 15) / 16;
+    const int n_rt = (n_out + 15) / 16;
     const int arow_l = mf_a_row(lane), ak = mf_a_k(lane);
     const int bn = lane & 15, bk = lane >> 4;
-    for (int tile = wave; tile < n_rt * n_ct; tile += nwaves) {
-        const int rt = tile / n_ct, ct = tile - rt * n_ct;
+    const int c0 = blockIdx.y * NCT * 16;  // first RHS of this workgroup's column tiles
+    for (int rt = wave; rt < n_rt; rt += nwaves) {
         const int arow = rt * 16 + arow_l;
-        const int col = ct * 16 + bn;
-        double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
-        for (int64_t q = b0; q < b1; ++q) {
-            const int bl = blk[q];
-            const int m = nr[bl], n = nc[bl];
-            const double *B = vals + off[bl];
-            const int kdim = TRANS ? m : n;
-            const double *xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
-            for (int k0 = 0; k0 < kdim; k0 += 4 * KU) {
-                double a[KU], bv[KU];
+        double4_t acc[NCT];
 #pragma unroll
-                for (int u = 0; u < KU; ++u) {
-                    const int kk = k0 + 4 * u + ak;
-                    a[u] = (arow < n_out && kk < kdim) ? (TRANS ? B[kk + (int64_t)m * arow] : B[arow + (int64_t)m * kk])
-                                                       : 0.0;
-                    const int kb = k0 + 4 * u + bk;
-                    bv[u] = (kb < kdim && col < R) ? xin[(int64_t)kb * R + col] : 0.0;
+        for (int g = 0; g < NCT; ++g) acc[g] = double4_t{0.0, 0.0, 0.0, 0.0};
+        // the operands of the next KU k-steps (crossing into the next near block) are loaded
+        // before the MFMAs of the current ones
+        int64_t q = b0;
+        int bl = blk[q], m = nr[bl], n = nc[bl];
+        const double *B = vals + off[bl];
+        int kdim = TRANS ? m : n;
+        const double *xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
+        int k0 = 0;
+        double a[KU], bv[KU][NCT];
+        // unconditional loads from clamped (valid) addresses, out-of-range operands multiplied by
+        // zero: no branches around the MFMAs
+        auto load = [&]() {
+#pragma unroll
+            for (int u = 0; u < KU; ++u) {
+                const int kk = k0 + 4 * u + ak;
+                const int ar = min(arow, n_out - 1), kc = min(kk, kdim - 1);
+                a[u] = (TRANS ? B[kc + (int64_t)m * ar] : B[ar + (int64_t)m * kc]) *
+                       ((arow < n_out && kk < kdim) ? 1.0 : 0.0);
+                const int kb = k0 + 4 * u + bk;
+#pragma unroll
+                for (int g = 0; g < NCT; ++g) {
+                    const int col = c0 + 16 * g + bn;
+                    bv[u][g] = xin[(int64_t)min(kb, kdim - 1) * R + min(col, R - 1)] * ((kb < kdim && col < R) ? 1.0 : 0.0);
                 }
-#pragma unroll
-                for (int u = 0; u < KU; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], acc, 0, 0, 0);
             }
+        };
+        load();
+        while (true) {
+            double ac[KU], bc[KU][NCT];
+#pragma unroll
+            for (int u = 0; u < KU; ++u) {
+                ac[u] = a[u];
+#pragma unroll
+                for (int g = 0; g < NCT; ++g) bc[u][g] = bv[u][g];
+            }
+            k0 += 4 * KU;
+            bool more = true;
+            if (k0 >= kdim) {
+                if (++q < b1) {
+                    bl = blk[q];
+                    m = nr[bl];
+                    n = nc[bl];
+                    B = vals + off[bl];
+                    kdim = TRANS ? m : n;
+                    xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
+                    k0 = 0;
+                } else {
+                    more = false;
+                }
+            }
+            if (more) load();
+#pragma unroll
+            for (int u = 0; u < KU; ++u)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bc[u][g], acc[g], 0, 0, 0);
+            if (!more) break;
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int row = rt * 16 + mf_d_row(lane, v);
-            if (row < n_out && col < R) ys[(out0 + row) * R + col] = acc[v];
+        for (int g = 0; g < NCT; ++g) {
+            const int col = c0 + 16 * g + bn;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = rt * 16 + mf_d_row(lane, v);
+                if (row < n_out && col < R) ys[(out0 + row) * R + col] = acc[g][v];
+            }
+        }
+    }
+}
+
+// Leaf expansions on FP64 MFMA (real maps, R >= 16), FP64 accumulation.
+//   L2P: Y_leaf (n x R) += alpha C^T (n x K) L (K x R)      rows = points, contraction over k
+//   P2M: M (K x R)      = C (K x n) X (n x R)                rows = k, contraction over points
+// A k-step's 4 contraction indices are assigned so that each lane's operands over 4 consecutive
+// k-steps are 4 consecutive values (index 16 j + 4 k' + u for lane row k' and k-step 4 j + u):
+// the contraction order does not change the product, and every lane reads its operand rows
+// contiguously.  One workgroup (4 waves) per (leaf, group of NCT 16-column RHS tiles).
+template <class T, int NCT>
+__global__ void __launch_bounds__(TPB, 2)
+    l2p_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
+                    const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const T *__restrict__ loc,
+                    int R, double *__restrict__ ys, double alpha) {
+    const int b = leaves[blockIdx.x];
+    const int64_t j0 = begin[b];
+    const int n = (int)(end[b] - j0);
+    const int c0 = blockIdx.y * NCT * 16;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, kq = lane >> 4;
+    const T *lb = loc + (int64_t)b * K * R;
+    for (int rt = wave; rt * 16 < n; rt += TPB / 64) {
+        const int row = rt * 16 + m;
+        const T *crow = coll + (j0 + min(row, n - 1)) * K;
+        double4_t acc[NCT];
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) acc[g] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int jb = 0; jb < K; jb += 16) {  // uniform trip count
+            const int kb = jb + 4 * kq;
+            double a[4], bv[NCT][4];
+            // unconditional loads from clamped (valid) addresses, out-of-range operands multiplied
+            // by zero (no branches in the loop, so the accumulators stay in the MFMA registers)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                a[u] = (double)crow[min(kb + u, K - 1)] * ((row < n && kb + u < K) ? 1.0 : 0.0);
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) {
+                const int col = c0 + 16 * g + m;
+                const T *lc = lb + (int64_t)min(col, R - 1) * K;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    bv[g][u] = (double)lc[min(kb + u, K - 1)] * ((col < R && kb + u < K) ? 1.0 : 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[g][u], acc[g], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) {
+            const int col = c0 + 16 * g + m;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ro = rt * 16 + mf_d_row(lane, v);
+                if (ro < n && col < R) {
+                    double *yo = ys + (j0 + ro) * R + col;
+                    *yo = *yo + alpha * acc[g][v];
+                }
+            }
+        }
+    }
+}
+
+template <class T, int NCT>
+__global__ void __launch_bounds__(TPB, 2)
+    p2m_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
+                    const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const double *__restrict__ xs,
+                    int R, T *__restrict__ mom) {
+    const int b = leaves[blockIdx.x];
+    const int64_t j0 = begin[b];
+    const int n = (int)(end[b] - j0);
+    const int c0 = blockIdx.y * NCT * 16;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, kq = lane >> 4;
+    T *mb = mom + (int64_t)b * K * R;
+    for (int rt = wave; rt * 16 < K; rt += TPB / 64) {
+        const int krow = rt * 16 + m;
+        double4_t acc[NCT];
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) acc[g] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int jb = 0; jb < n; jb += 16) {  // uniform trip count
+            const int pb = jb + 4 * kq;
+            double a[4], bv[NCT][4];
+            // unconditional loads from clamped (valid) addresses, out-of-range operands multiplied
+            // by zero (no branches in the loop, so the accumulators stay in the MFMA registers)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                a[u] = (double)coll[(j0 + min(pb + u, n - 1)) * K + min(krow, K - 1)] *
+                       ((pb + u < n && krow < K) ? 1.0 : 0.0);
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) {
+                const int col = c0 + 16 * g + m;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    bv[g][u] = xs[(j0 + min(pb + u, n - 1)) * R + min(col, R - 1)] * ((pb + u < n && col < R) ? 1.0 : 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[g][u], acc[g], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) {
+            const int col = c0 + 16 * g + m;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ko = rt * 16 + mf_d_row(lane, v);
+                if (ko < K && col < R) mb[(int64_t)col * K + ko] = (T)acc[g][v];
+            }
         }
     }
 }
@@ -890,6 +1050,46 @@ __global__ void mfma_f64_probe_kernel(const double *__restrict__ A, const double
 }
 }  // namespace
 
+bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                     const void *coll, int K, const void *loc, int R, void *ys, double alpha, hipStream_t s) {
+    if (R < 16 || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
+    const int nct = std::min(4, (R + 15) / 16);
+    const dim3 grid((unsigned)n, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
+    auto go = [&](auto kern, auto tv) {
+        using T = decltype(tv);
+        kern<<<grid, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys, alpha);
+    };
+    const bool f64 = cdtype == H2G_F64;
+    switch (nct) {
+    case 1: f64 ? go(l2p_mfma_kernel<double, 1>, 0.0) : go(l2p_mfma_kernel<float, 1>, 0.0f); break;
+    case 2: f64 ? go(l2p_mfma_kernel<double, 2>, 0.0) : go(l2p_mfma_kernel<float, 2>, 0.0f); break;
+    case 3: f64 ? go(l2p_mfma_kernel<double, 3>, 0.0) : go(l2p_mfma_kernel<float, 3>, 0.0f); break;
+    default: f64 ? go(l2p_mfma_kernel<double, 4>, 0.0) : go(l2p_mfma_kernel<float, 4>, 0.0f); break;
+    }
+    H2G_CHECK_LAUNCH();
+    return true;
+}
+
+bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                     const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
+    if (R < 16 || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
+    const int nct = std::min(4, (R + 15) / 16);
+    const dim3 grid((unsigned)n, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
+    auto go = [&](auto kern, auto tv) {
+        using T = decltype(tv);
+        kern<<<grid, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const double *)xs, R, (T *)mom);
+    };
+    const bool f64 = cdtype == H2G_F64;
+    switch (nct) {
+    case 1: f64 ? go(p2m_mfma_kernel<double, 1>, 0.0) : go(p2m_mfma_kernel<float, 1>, 0.0f); break;
+    case 2: f64 ? go(p2m_mfma_kernel<double, 2>, 0.0) : go(p2m_mfma_kernel<float, 2>, 0.0f); break;
+    case 3: f64 ? go(p2m_mfma_kernel<double, 3>, 0.0) : go(p2m_mfma_kernel<float, 3>, 0.0f); break;
+    default: f64 ? go(p2m_mfma_kernel<double, 4>, 0.0) : go(p2m_mfma_kernel<float, 4>, 0.0f); break;
+    }
+    H2G_CHECK_LAUNCH();
+    return true;
+}
+
 bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
     if (c.dtype != H2G_F64 || R < 16) return false;
     const bool trans = op != H2G_OP_N;
@@ -897,13 +1097,24 @@ bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hi
     if (ntgt == 0) return true;
     const int64_t *ptr = trans ? c.near_rev_ptr.as<int64_t>() : c.near_fwd_ptr.as<int64_t>();
     const int32_t *blk = trans ? c.near_rev_blk.as<int32_t>() : c.near_fwd_blk.as<int32_t>();
+    // NCT column tiles per workgroup (up to 64 RHS); wider batches take several workgroups
+    const int nct = std::min(4, (R + 15) / 16);
+    const dim3 grid((unsigned)ntgt, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
     auto go = [&](auto kern) {
-        kern<<<ntgt, TPB, 0, s>>>(ptr, blk, c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(),
+        kern<<<grid, TPB, 0, s>>>(ptr, blk, c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(),
                                   c.near_blk_nr.as<int32_t>(), c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(),
                                   c.near_vals.as<double>(), (const double *)xs, (double *)ys, R);
     };
-    if (trans) go(near_mfma_kernel<true>);
-    else go(near_mfma_kernel<false>);
+    switch (nct * 2 + (trans ? 1 : 0)) {
+    case 2: go(near_mfma_kernel<false, 1>); break;
+    case 3: go(near_mfma_kernel<true, 1>); break;
+    case 4: go(near_mfma_kernel<false, 2>); break;
+    case 5: go(near_mfma_kernel<true, 2>); break;
+    case 6: go(near_mfma_kernel<false, 3>); break;
+    case 7: go(near_mfma_kernel<true, 3>); break;
+    case 8: go(near_mfma_kernel<false, 4>); break;
+    default: go(near_mfma_kernel<true, 4>); break;
+    }
     H2G_CHECK_LAUNCH();
     return true;
 }
