@@ -752,7 +752,8 @@ void far_pass(Workspace &w, int op, int R) {
             if (src.separable)
                 launch_sep_transfer(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
                                     P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
-                                    P.m2m_mat.as<int32_t>(), mixed ? src.factors32.ptr : src.factors.ptr, src.sep_p,
+                                    P.m2m_mat.as<int32_t>(), mixed ? src.factors32.ptr : src.factors.ptr, (l + 1) * 8,
+                                    src.sep_p,
                                     w.mom.ptr, w.mom.ptr, R, 0, nullptr, s);
             else
                 launch_boxgemm(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
@@ -819,7 +820,8 @@ void far_pass(Workspace &w, int op, int R) {
             if (dst.separable)
                 launch_sep_transfer(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
                                     P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),
-                                    P.l2l_mat.as<int32_t>(), mixed ? dst.factors32.ptr : dst.factors.ptr, dst.sep_p,
+                                    P.l2l_mat.as<int32_t>(), mixed ? dst.factors32.ptr : dst.factors.ptr, l * 8,
+                                    dst.sep_p,
                                     w.loc.ptr, w.loc.ptr, R, 2, P.l2l_acc.as<uint8_t>(), s);
             else
                 launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,

@@ -313,8 +313,8 @@ void launch_hadamard(int cdtype, bool conj, int n_recv, const int64_t *item_ptr,
                      int L3, int src_slot_base, const void *spec_in, void *spec_out, int R, hipStream_t s);
 void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
-                         const int32_t *item_src, const int32_t *item_mat, const void *factors, int p, const void *in,
-                         void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s);
+                         const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,
+                         const void *in, void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s);
 void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, const void *mom, int K, int p, int L,
                       int R, void *S, const void *twiddle, hipStream_t s);
 void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,

@@ -678,16 +678,18 @@ __device__ inline T sep_factor(const T *F, int p, int row, int col, int opA) {
 template <class T, int JB, int P>
 __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__restrict__ targets,
                                     const int64_t *__restrict__ item_ptr, const int32_t *__restrict__ item_src,
-                                    const int32_t *__restrict__ item_mat, const T *__restrict__ factors,
+                                    const int32_t *__restrict__ item_mat, const T *__restrict__ factors, int mat_base,
                                     const T *__restrict__ in, T *__restrict__ out, int R, int acc_mode,
                                     const uint8_t *__restrict__ acc_flags) {
     constexpr int p = P, K = P * P * P, LINES = P * P;
-    // one dynamic LDS region, element buffers first (16-byte aligned base), job tables after
+    // one dynamic LDS region, element buffers first (16-byte aligned base), then the factors of
+    // the launch's 8 sectors with opA applied, then the job tables
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *t1 = reinterpret_cast<T *>(smem_raw);  // [JB][K]
     T *t2 = t1 + JB * K;                      // [JB][K]
     T *acc = t2 + JB * K;                     // [JB][K]
-    int *s_item0 = reinterpret_cast<int *>(acc + JB * K);
+    T *fs = acc + JB * K;                     // [8 sectors][3 axes][p out][p in]
+    int *s_item0 = reinterpret_cast<int *>(fs + 8 * 3 * p * p);
     int *s_nitem = s_item0 + JB, *s_tgt = s_nitem + JB, *s_r = s_tgt + JB, *s_maxp = s_r + JB;
     const int64_t job0 = (int64_t)blockIdx.x * JB;
     const int64_t njobs = (int64_t)n_targets * R;
@@ -703,6 +705,12 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
             s_tgt[threadIdx.x] = -1;
             s_nitem[threadIdx.x] = 0;
         }
+    }
+    // the items of one launch use the factors of one level: matrices mat_base .. mat_base + 7
+    for (int e = threadIdx.x; e < 8 * 3 * p * p; e += blockDim.x) {
+        const int sa = e / (p * p), oi = e - sa * p * p;
+        const int o = oi / p, i = oi - o * p;
+        fs[e] = sep_factor(factors + ((int64_t)mat_base * 3 + sa) * p * p, p, o, i, opA);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -728,7 +736,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
             const int j = e / LINES, line = e - j * LINES;
             if (it >= s_nitem[j]) continue;
             const int64_t itg = s_item0[j] + it;
-            const T *F = factors + (int64_t)item_mat[itg] * 3 * p * p;
+            const T *F = fs + (item_mat[itg] - mat_base) * 3 * p * p;
             const T *x = in + ((int64_t)item_src[itg] * R + s_r[j]) * K + p * line;
             T v[p];
 #pragma unroll
@@ -737,7 +745,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
             for (int o = 0; o < p; ++o) {
                 T a = zero_v<T>();
 #pragma unroll
-                for (int i = 0; i < p; ++i) fmacc(a, sep_factor(F, p, o, i, opA), v[i]);
+                for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
                 t1[j * K + o + p * line] = a;
             }
         }
@@ -747,7 +755,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
             const int j = e / LINES, line = e - j * LINES;
             if (it >= s_nitem[j]) continue;
             const int kx = line % p, jz = line / p;
-            const T *F = factors + (int64_t)item_mat[s_item0[j] + it] * 3 * p * p + p * p;
+            const T *F = fs + ((item_mat[s_item0[j] + it] - mat_base) * 3 + 1) * p * p;
             T v[p];
 #pragma unroll
             for (int i = 0; i < p; ++i) v[i] = t1[j * K + kx + p * (i + p * jz)];
@@ -755,7 +763,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
             for (int o = 0; o < p; ++o) {
                 T a = zero_v<T>();
 #pragma unroll
-                for (int i = 0; i < p; ++i) fmacc(a, sep_factor(F, p, o, i, opA), v[i]);
+                for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
                 t2[j * K + kx + p * (o + p * jz)] = a;
             }
         }
@@ -764,7 +772,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
         for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
             const int j = e / LINES, line = e - j * LINES;
             if (it >= s_nitem[j]) continue;
-            const T *F = factors + (int64_t)item_mat[s_item0[j] + it] * 3 * p * p + 2 * p * p;
+            const T *F = fs + ((item_mat[s_item0[j] + it] - mat_base) * 3 + 2) * p * p;
             T v[p];
 #pragma unroll
             for (int i = 0; i < p; ++i) v[i] = t2[j * K + line + LINES * i];
@@ -772,7 +780,7 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
             for (int o = 0; o < p; ++o) {
                 T a = acc[j * K + line + LINES * o];
 #pragma unroll
-                for (int i = 0; i < p; ++i) fmacc(a, sep_factor(F, p, o, i, opA), v[i]);
+                for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
                 acc[j * K + line + LINES * o] = a;
             }
         }
@@ -789,29 +797,29 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
 
 template <class T, int P>
 static void sep_transfer_p(int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
-                           const int32_t *item_src, const int32_t *item_mat, const void *factors, const void *in,
-                           void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
+                           const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base,
+                           const void *in, void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
     // jobs per workgroup: three K-long LDS buffers per job, 41 KB at p = 6 for every type
     constexpr int JB = 64 / (int)sizeof(T);
     constexpr int K = P * P * P;
     const int64_t njobs = (int64_t)n_targets * R;
-    const size_t sh = sizeof(T) * (size_t)(3 * JB * K) + sizeof(int) * (size_t)(4 * JB + 1);
+    const size_t sh = sizeof(T) * (size_t)(3 * JB * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(4 * JB + 1);
     sep_transfer_kernel<T, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
-        opA, n_targets, targets, item_ptr, item_src, item_mat, (const T *)factors, (const T *)in, (T *)out, R,
-        acc_mode, acc_flags);
+        opA, n_targets, targets, item_ptr, item_src, item_mat, (const T *)factors, mat_base, (const T *)in,
+        (T *)out, R, acc_mode, acc_flags);
 }
 
 void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
-                         const int32_t *item_src, const int32_t *item_mat, const void *factors, int p, const void *in,
-                         void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
+                         const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,
+                         const void *in, void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s) {
     if (n_targets == 0) return;
     DISPATCH_T(cdtype, {
         switch (p) {
-        case 2: sep_transfer_p<T, 2>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
-        case 3: sep_transfer_p<T, 3>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
-        case 4: sep_transfer_p<T, 4>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
-        case 5: sep_transfer_p<T, 5>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
-        case 6: sep_transfer_p<T, 6>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, in, out, R, acc_mode, acc_flags, s); break;
+        case 2: sep_transfer_p<T, 2>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, in, out, R, acc_mode, acc_flags, s); break;
+        case 3: sep_transfer_p<T, 3>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, in, out, R, acc_mode, acc_flags, s); break;
+        case 4: sep_transfer_p<T, 4>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, in, out, R, acc_mode, acc_flags, s); break;
+        case 5: sep_transfer_p<T, 5>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, in, out, R, acc_mode, acc_flags, s); break;
+        case 6: sep_transfer_p<T, 6>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, in, out, R, acc_mode, acc_flags, s); break;
         default: throw Error{H2G_EUNSUPPORTED, "separable transfers support 2 <= p <= 6"};
         }
     });
