@@ -629,9 +629,11 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
                 for (auto &x : cls) o.insert(o.end(), x.begin(), x.end());
                 // whole batches of 8 slot pairs: pad with (zero slot, zero coupling row q = 104)
                 while (o.size() % 16) o.push_back(NSLOT | (104 << 16));
+                // byte offsets of the region row (16 complex FP32 values) and of the coupling row
+                // (32 FP64 values) in the list kernel's LDS arrays
                 for (size_t i = 0; i < o.size(); ++i) {
-                    tlist[((ti * NB + j) * QL + i) * 2] = o[i] & 0xffff;  // region slot
-                    tlist[((ti * NB + j) * QL + i) * 2 + 1] = o[i] >> 16;  // q
+                    tlist[((ti * NB + j) * QL + i) * 2] = (o[i] & 0xffff) * 16 * 8;
+                    tlist[((ti * NB + j) * QL + i) * 2 + 1] = (o[i] >> 16) * 32 * 8;
                 }
                 tnl[ti * NB + j] = (int)o.size();
             }

@@ -196,7 +196,8 @@ struct LatticeTiles {
     DevBuf tgt;    // int32 [tile][blocks][8 children]: receiving index of the k-th block's children or -1
     DevBuf qt;     // int16 [tile][16][216]: union index of each slot q of the k-th block, 1024 if absent
                    // (cluster tiles)
-    DevBuf list;   // int32 [tile][8][208][2]: (union index, q) of the k-th block's present slots,
+    DevBuf list;   // int32 [tile][8][208][2]: byte offsets (union index * 128, q * 256) of the k-th
+                   // block's present slots in the list kernel's LDS region and coupling table,
                    // padded to a multiple of 16 entries with (512, 104) (list tiles)
     DevBuf nlist;  // int32 [tile][8]: list lengths (multiples of 16)
 };

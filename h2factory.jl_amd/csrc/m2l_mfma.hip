@@ -344,10 +344,12 @@ __global__ void __launch_bounds__(NT_LIST)
         const int npb = __builtin_amdgcn_readfirstlane(*s_npb);
         const int j = w;
         if (j < npb) {
-            // the list holds (region slot, q) pairs, padded with zero entries to whole batches of
-            // 8 slot pairs; lane half h takes the second slot of each pair
+            // the list holds (region row, coupling row) byte offsets, padded with zero entries to
+            // whole batches of 8 slot pairs; lane half h takes the second slot of each pair
             const int nk = __builtin_amdgcn_readfirstlane(s_nl[j]) / 2;  // slot pairs
             const int2 *lj = reinterpret_cast<const int2 *>(lst) + j * QL + h;
+            const char *regb = reinterpret_cast<const char *>(reg) + (m * 2 + b) * (int)sizeof(float);
+            const char *eab = reinterpret_cast<const char *>(ea) + (m * 2 + b) * (int)sizeof(double);
             double4_m acc[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i] = double4_m{0.0, 0.0, 0.0, 0.0};
@@ -358,9 +360,9 @@ __global__ void __launch_bounds__(NT_LIST)
                 double a[8], bv[8];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    a[i] = ea[e[i].y * 32 + m * 2 + b];
+                    a[i] = *reinterpret_cast<const double *>(eab + e[i].y);
                     if (neg) a[i] = -a[i];
-                    bv[i] = (double)regR[(e[i].x * RC + m) * 2 + b];
+                    bv[i] = (double)*reinterpret_cast<const float *>(regb + e[i].x);
                 }
 #pragma unroll
                 for (int i = 0; i < 8; ++i) acc[i & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[i], acc[i & 3], 0, 0, 0);
