@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=60_000, help="points of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=384,
+                    help="single-RHS requests of the serving measurement (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the requests (1/s)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy five times (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -112,6 +117,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        # a plain copy kernel of known size: 2^30 bytes read and 2^30 bytes written per launch
+        a = torch.ones(2 ** 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 2 ** 30, "launches": 5}), flush=True)
+        return
     N, R = args.n, args.nrhs
 
     t0 = time.perf_counter()
@@ -219,6 +233,28 @@ def main():
     roof["kernel"] = f"{dph} level {dlv}"
     roof["launch_ms"] = round(dms, 3)
 
+    # serving: single-RHS requests with Poisson arrivals against the resident map, continuously
+    # batched (up to R per product, two workspaces in flight); latency = completion - arrival
+    serve = None
+    if args.serve_requests > 0:
+        from h2g.serve import Request, Server
+
+        del A, Tm, P
+        torch.cuda.synchronize()
+        srv = Server(budget_bytes=280 * 2 ** 30, max_batch=R, ws_per_map=2)
+        srv.add_core("bench", core)
+        rng = np.random.default_rng(5)
+        reqs = [Request("bench", "N", rng.standard_normal(N).astype(np.float32)) for _ in range(args.serve_requests)]
+        for r in reqs[: 2 * R]:  # warm the two workspaces (graphs captured per batch width)
+            srv.submit(Request("bench", "N", r.x))
+        srv.drain()
+        st = srv.run_stream(reqs, rate_hz=args.serve_rate, seed=6)
+        srv.close()
+        serve = {"requests": st["requests"], "arrival_rate_hz": args.serve_rate,
+                 "throughput_rhs_per_s": round(st["matvecs_per_s"], 1), "p50_ms": round(st["p50_ms"], 2),
+                 "p99_ms": round(st["p99_ms"], 2), "mean_batch": round(st["mean_batch"], 2)}
+        log(f"serving: {serve}")
+
     cpu = None
     if not args.no_cpu:
         dt_cpu = cpu_baseline(args.cpu_n)
@@ -244,6 +280,7 @@ def main():
         "plain_phase_ms": {k: round(v, 3) for k, v in ptot.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serve,
         "build_s": {"host_plan": round(t_build, 1), "device_core": round(t_core, 1)},
     }
     print(json.dumps(out), flush=True)

@@ -163,6 +163,21 @@ class Server:
             raise AdmissionError(f"map {name!r} exceeded the budget once built")
         return rm
 
+    def add_core(self, name, core):
+        """Admit an already built core (its workspaces are created here), or raise AdmissionError."""
+        a = core.arrays
+        es = 16 if a.dtype == "c64" else 8
+        n = max(core.shape)
+        ws_est = self.ws_per_map * (n * self.max_batch * (4 * es) + a.test.tree.n_boxes * a.K * self.max_batch * es +
+                                    2 * 726 * a.test.tree.n_boxes * self.max_batch * es)
+        if self.used_bytes() + ws_est > self.budget:
+            raise AdmissionError(f"map {name!r} needs ~{ws_est / 1e9:.1f} GB of workspaces, "
+                                 f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
+        dtype = np.complex64 if a.dtype == "c64" else np.float32
+        rm = _ResidentMap(name, core, self.ws_per_map, self.max_batch, self.precision, dtype)
+        self.maps[name] = rm
+        return rm
+
     def remove_map(self, name):
         self.maps.pop(name).close()
 
