@@ -358,6 +358,37 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
     P.l2l_mat.upload_vec(lmat);
     P.l2l_acc.upload_vec(lacc);  // indexed by item: one item per L2L target
     P.l2l_ptr.upload_vec(lptr);
+    // the same items grouped by parent (children of one parent are consecutive in level order)
+    {
+        std::vector<int32_t> pbox, cbox, cmat;
+        std::vector<uint8_t> cacc;
+        std::vector<int64_t> pptr;
+        P.l2l_par.assign(dst.n_levels, Span{});
+        P.l2lp_base.assign(dst.n_levels, 0);
+        for (int l = 0; l < dst.n_levels; ++l) {
+            P.l2l_par[l].begin = (int64_t)pbox.size();
+            P.l2lp_base[l] = (int64_t)pptr.size();
+            int last = -1;
+            for (int i = 0; i < P.l2l[l].count; ++i) {
+                const int b = lists[P.l2l[l].begin + i];
+                if (dst.parent[b] != last) {
+                    last = dst.parent[b];
+                    pbox.push_back(last);
+                    pptr.push_back((int64_t)cbox.size());
+                }
+                cbox.push_back(b);
+                cmat.push_back(l * 8 + dst.sector[b]);
+                cacc.push_back(has_m2l[b]);
+            }
+            pptr.push_back((int64_t)cbox.size());
+            P.l2l_par[l].count = (int32_t)((int64_t)pbox.size() - P.l2l_par[l].begin);
+        }
+        P.l2lp_box.upload_vec(pbox);
+        P.l2lp_ptr.upload_vec(pptr);
+        P.l2lp_child.upload_vec(cbox);
+        P.l2lp_mat.upload_vec(cmat);
+        P.l2lp_acc.upload_vec(cacc);
+    }
 
     // M2L items
     std::vector<int64_t> iptr;
@@ -820,11 +851,10 @@ void far_pass(Workspace &w, int op, int R) {
         if (P.l2l[l].count) {
             w.mark_begin(7, l);
             if (dst.separable)
-                launch_sep_transfer(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
-                                    P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),
-                                    P.l2l_mat.as<int32_t>(), mixed ? dst.factors32.ptr : dst.factors.ptr, l * 8,
-                                    dst.sep_p,
-                                    w.loc.ptr, w.loc.ptr, R, 2, P.l2l_acc.as<uint8_t>(), s);
+                launch_sep_l2l(cd, conj ? 2 : 1, P.l2l_par[l].count, P.l2lp_box.as<int32_t>() + P.l2l_par[l].begin,
+                               P.l2lp_ptr.as<int64_t>() + P.l2lp_base[l], P.l2lp_child.as<int32_t>(),
+                               P.l2lp_mat.as<int32_t>(), P.l2lp_acc.as<uint8_t>(),
+                               mixed ? dst.factors32.ptr : dst.factors.ptr, l * 8, dst.sep_p, w.loc.ptr, R, s);
             else
                 launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
                                P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),

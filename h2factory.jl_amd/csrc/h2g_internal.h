@@ -222,6 +222,11 @@ struct OrientPlan {
     // L2L items: one per target: (parent box, matrix index, accumulate flag)
     DevBuf l2l_ptr, l2l_src, l2l_mat, l2l_acc;
     std::vector<int64_t> l2l_base;
+    // L2L grouped by parent (separable transfers): per level the parents, and per parent its
+    // children (box, matrix index, accumulate flag) as a CSR
+    std::vector<Span> l2l_par;            // per level: span of the parent list in l2lp_box
+    std::vector<int64_t> l2lp_base;       // per level: offset of the level's parents into l2lp_ptr
+    DevBuf l2lp_box, l2lp_ptr, l2lp_child, l2lp_mat, l2lp_acc;
     // M2L items: CSR over receiving boxes (concatenated by level)
     DevBuf m2l_ptr, m2l_src, m2l_op, m2l_sym;
     std::vector<int64_t> m2l_base;
@@ -313,6 +318,9 @@ void launch_hadamard(int cdtype, bool conj, int n_recv, const int64_t *item_ptr,
                      const int32_t *item_op, const int32_t *item_sym, const int32_t *perms, const void *ops,
                      int L3, int src_slot_base, const void *spec_in, void *spec_out, int R, hipStream_t s);
 void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+void launch_sep_l2l(int cdtype, int opA, int n_parents, const int32_t *parents, const int64_t *child_ptr,
+                    const int32_t *child_box, const int32_t *child_mat, const uint8_t *child_acc, const void *factors,
+                    int mat_base, int p, void *loc, int R, hipStream_t s);
 void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
                          const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,
                          const void *in, void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s);

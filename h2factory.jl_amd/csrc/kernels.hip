@@ -795,6 +795,144 @@ __global__ void sep_transfer_kernel(int opA, int n_targets, const int32_t *__res
 }
 }  // namespace
 
+// L2L grouped by parent: a job is (parent, RHS); the parent's local expansion is read once into
+// LDS and every child c of the parent gets loc[c] = (acc_c ? loc[c] : 0) + T_{sector(c)}^op loc[p],
+// the three line stages per child as in sep_transfer_kernel.
+template <class T, int JB, int P>
+__global__ void sep_l2l_kernel(int opA, int n_par, const int32_t *__restrict__ parents,
+                               const int64_t *__restrict__ child_ptr, const int32_t *__restrict__ child_box,
+                               const int32_t *__restrict__ child_mat, const uint8_t *__restrict__ child_acc,
+                               const T *__restrict__ factors, int mat_base, T *__restrict__ loc, int R) {
+    constexpr int p = P, K = P * P * P, LINES = P * P;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *pin = reinterpret_cast<T *>(smem_raw);  // [JB][K] parent
+    T *t1 = pin + JB * K;                      // [JB][K]
+    T *t2 = t1 + JB * K;                       // [JB][K]
+    T *fs = t2 + JB * K;                       // [8 sectors][3 axes][p out][p in]
+    int *s_c0 = reinterpret_cast<int *>(fs + 8 * 3 * p * p);
+    int *s_nc = s_c0 + JB, *s_r = s_nc + JB, *s_maxc = s_r + JB;
+    const int64_t job0 = (int64_t)blockIdx.x * JB;
+    const int64_t njobs = (int64_t)n_par * R;
+    if (threadIdx.x < JB) {
+        const int64_t job = job0 + threadIdx.x;
+        if (job < njobs) {
+            const int pi = (int)(job / R);
+            s_r[threadIdx.x] = (int)(job - (int64_t)pi * R);
+            s_c0[threadIdx.x] = (int)child_ptr[pi];
+            s_nc[threadIdx.x] = (int)(child_ptr[pi + 1] - child_ptr[pi]);
+        } else {
+            s_r[threadIdx.x] = 0;
+            s_c0[threadIdx.x] = 0;
+            s_nc[threadIdx.x] = 0;
+        }
+    }
+    for (int e = threadIdx.x; e < 8 * 3 * p * p; e += blockDim.x) {
+        const int sa = e / (p * p), oi = e - sa * p * p;
+        const int o = oi / p, i = oi - o * p;
+        fs[e] = sep_factor(factors + ((int64_t)mat_base * 3 + sa) * p * p, p, o, i, opA);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < JB * K; e += blockDim.x) {
+        const int j = e / K, k = e - j * K;
+        const int64_t job = job0 + j;
+        pin[e] = job < njobs ? loc[((int64_t)parents[job / R] * R + s_r[j]) * K + k] : zero_v<T>();
+    }
+    if (threadIdx.x == 0) {
+        int mx = 0;
+        for (int j = 0; j < JB; ++j) mx = max(mx, s_nc[j]);
+        *s_maxc = mx;
+    }
+    __syncthreads();
+    const int nmax = *s_maxc;
+    for (int it = 0; it < nmax; ++it) {
+        // x: parent line (jy, jz) -> t1[kx + p*(jy + p*jz)]
+        for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
+            const int j = e / LINES, line = e - j * LINES;
+            if (it >= s_nc[j]) continue;
+            const T *F = fs + (child_mat[s_c0[j] + it] - mat_base) * 3 * p * p;
+            T v[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) v[i] = pin[j * K + p * line + i];
+#pragma unroll
+            for (int o = 0; o < p; ++o) {
+                T a = zero_v<T>();
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
+                t1[j * K + o + p * line] = a;
+            }
+        }
+        __syncthreads();
+        // y: line (kx, jz) -> t2[kx + p*(ky + p*jz)]
+        for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
+            const int j = e / LINES, line = e - j * LINES;
+            if (it >= s_nc[j]) continue;
+            const int kx = line % p, jz = line / p;
+            const T *F = fs + ((child_mat[s_c0[j] + it] - mat_base) * 3 + 1) * p * p;
+            T v[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) v[i] = t1[j * K + kx + p * (i + p * jz)];
+#pragma unroll
+            for (int o = 0; o < p; ++o) {
+                T a = zero_v<T>();
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
+                t2[j * K + kx + p * (o + p * jz)] = a;
+            }
+        }
+        __syncthreads();
+        // z: line (kx, ky) -> the child's expansion, accumulated when the child already holds
+        // couplings
+        for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
+            const int j = e / LINES, line = e - j * LINES;
+            if (it >= s_nc[j]) continue;
+            const int item = s_c0[j] + it;
+            const T *F = fs + ((child_mat[item] - mat_base) * 3 + 2) * p * p;
+            const bool acc = child_acc[item] != 0;
+            T *out = loc + ((int64_t)child_box[item] * R + s_r[j]) * K + line;
+            T v[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) v[i] = t2[j * K + line + LINES * i];
+#pragma unroll
+            for (int o = 0; o < p; ++o) {
+                T a = acc ? out[LINES * o] : zero_v<T>();
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
+                out[LINES * o] = a;
+            }
+        }
+        // the next x stage writes t1 only; the y stage after it is behind that stage's barrier
+    }
+}
+
+template <class T, int P>
+static void sep_l2l_p(int opA, int n_par, const int32_t *parents, const int64_t *child_ptr, const int32_t *child_box,
+                      const int32_t *child_mat, const uint8_t *child_acc, const void *factors, int mat_base, void *loc,
+                      int R, hipStream_t s) {
+    constexpr int JB = 64 / (int)sizeof(T);
+    constexpr int K = P * P * P;
+    const int64_t njobs = (int64_t)n_par * R;
+    const size_t sh = sizeof(T) * (size_t)(3 * JB * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(3 * JB + 1);
+    sep_l2l_kernel<T, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+        opA, n_par, parents, child_ptr, child_box, child_mat, child_acc, (const T *)factors, mat_base, (T *)loc, R);
+}
+
+void launch_sep_l2l(int cdtype, int opA, int n_parents, const int32_t *parents, const int64_t *child_ptr,
+                    const int32_t *child_box, const int32_t *child_mat, const uint8_t *child_acc, const void *factors,
+                    int mat_base, int p, void *loc, int R, hipStream_t s) {
+    if (n_parents == 0) return;
+    DISPATCH_T(cdtype, {
+        switch (p) {
+        case 2: sep_l2l_p<T, 2>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
+        case 3: sep_l2l_p<T, 3>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
+        case 4: sep_l2l_p<T, 4>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
+        case 5: sep_l2l_p<T, 5>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
+        case 6: sep_l2l_p<T, 6>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
+        default: throw Error{H2G_EUNSUPPORTED, "separable transfers support 2 <= p <= 6"};
+        }
+    });
+    H2G_CHECK_LAUNCH();
+}
+
 template <class T, int P>
 static void sep_transfer_p(int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
                            const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base,
