@@ -661,10 +661,10 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
                 // whole batches of 8 slot pairs: pad with (zero slot, zero coupling row q = 104)
                 while (o.size() % 16) o.push_back(NSLOT | (104 << 16));
                 // byte offsets of the region row (16 complex FP32 values) and of the coupling row
-                // (32 FP64 values) in the list kernel's LDS arrays
+                // (32 FP32 values) in the list kernel's LDS arrays
                 for (size_t i = 0; i < o.size(); ++i) {
                     tlist[((ti * NB + j) * QL + i) * 2] = (o[i] & 0xffff) * 16 * 8;
-                    tlist[((ti * NB + j) * QL + i) * 2 + 1] = (o[i] >> 16) * 32 * 8;
+                    tlist[((ti * NB + j) * QL + i) * 2 + 1] = (o[i] >> 16) * 32 * 4;
                 }
                 tnl[ti * NB + j] = (int)o.size();
             }
@@ -699,7 +699,7 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
     d_op.upload_vec(tq_op);
     if (sym) d_sym.upload_vec(tq_sym);
     build_lattice_ea(d_op.as<int32_t>(), sym ? d_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), c.ops.ptr, c.L,
-                     c.half_spectrum, LL);
+                     c.half_spectrum, (c.precisions & (1 << H2G_PREC_MIXED)) != 0, LL);
     LL.ok = true;
 }
 

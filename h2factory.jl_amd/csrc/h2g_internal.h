@@ -196,7 +196,7 @@ struct LatticeTiles {
     DevBuf tgt;    // int32 [tile][blocks][8 children]: receiving index of the k-th block's children or -1
     DevBuf qt;     // int16 [tile][16][216]: union index of each slot q of the k-th block, 1024 if absent
                    // (cluster tiles)
-    DevBuf list;   // int32 [tile][8][208][2]: byte offsets (union index * 128, q * 256) of the k-th
+    DevBuf list;   // int32 [tile][8][208][2]: byte offsets (union index * 128, q * 128) of the k-th
                    // block's present slots in the list kernel's LDS region and coupling table,
                    // padded to a multiple of 16 entries with (512, 104) (list tiles)
     DevBuf nlist;  // int32 [tile][8]: list lengths (multiples of 16)
@@ -208,6 +208,7 @@ struct LatticeLevel {
     LatticeTiles clus;        // cluster tiles (any batch width)
     LatticeTiles list;        // list tiles (batches of >= 16 columns, FP32 spectra)
     DevBuf EA64;              // double [F][216 slots q][16 rows m = 2t + c][2 b]: real form of the couplings
+    DevBuf EA32;              // the same in FP32 (cores with the mixed precision)
 };
 
 // Work lists of one orientation (forward: trial -> test; reverse: test -> trial).
@@ -331,7 +332,7 @@ void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv,
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s);
 void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
-                      bool half, LatticeLevel &LL);
+                      bool half, bool want32, LatticeLevel &LL);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
                     int R, void *loc, const void *twiddle, hipStream_t s);
 

@@ -1194,6 +1194,17 @@ __global__ void mfma_f64_probe_kernel(const double *__restrict__ A, const double
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     for (int v = 0; v < 4; ++v) D[mf_d_row(lane, v) * 16 + (lane & 15)] = acc[v];
 }
+// probe of the FP32 16x16x4 layout the list M2L kernel uses: operands as for FP64, result
+// D[m][n] in lane n + 16 (m / 4), element m % 4
+__global__ void mfma_f32_probe_kernel(const float *__restrict__ A, const float *__restrict__ B, float *__restrict__ D) {
+    typedef float float4_t __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const float a = A[mf_a_row(lane) * 4 + mf_a_k(lane)];
+    const float b = B[(lane >> 4) * 16 + (lane & 15)];
+    float4_t acc = float4_t{0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = acc[v];
+}
 }  // namespace
 
 bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
@@ -1268,6 +1279,24 @@ bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hi
 }  // namespace h2g
 
 // Test hook: runs the MFMA layout probe on host arrays (A 16x4, B 4x16 row-major; D 16x16).
+// Test hook: the FP32 layout probe (A 16x4, B 4x16 row-major; D 16x16).
+extern "C" int h2g_probe_mfma_f32(const float *A, const float *B, float *D) {
+    using namespace h2g;
+    try {
+        DevBuf a, b, d;
+        a.upload(A, 64 * sizeof(float));
+        b.upload(B, 64 * sizeof(float));
+        d.alloc(256 * sizeof(float));
+        mfma_f32_probe_kernel<<<1, 64>>>(a.as<float>(), b.as<float>(), d.as<float>());
+        H2G_HIP(hipGetLastError());
+        H2G_HIP(hipMemcpy(D, d.ptr, 256 * sizeof(float), hipMemcpyDeviceToHost));
+        return H2G_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+}
+
 extern "C" int h2g_probe_mfma_f64(const double *A, const double *B, double *D) {
     using namespace h2g;
     try {

@@ -21,8 +21,11 @@
 //    sources, at most 512, are staged in LDS; 16 columns = the 16 right-hand sides of one block,
 //    and each block runs only its own present slots (a packed list built on the host), which
 //    skips the absent sources of sparse levels.  FP32 spectra, R >= 16.
-// Accumulation is FP64 in both (the FP64 MFMA rate equals the FP64 / unpacked FP32 FMA rate
-// on CDNA4, and it keeps the mixed form's error at the FP32 rounding of its inputs).
+// The cluster kernel runs on FP64 MFMA (v_mfma_f64_16x16x4f64; the FP64 MFMA rate equals the
+// FP64 / unpacked FP32 FMA rate on CDNA4).  The list kernel, whose operands are FP32 (spectra as
+// stored, FP32 coupling table), runs on FP32 MFMA (v_mfma_f32_16x16x4f32, twice the FP64 rate)
+// and adds each partial sum of 4 slot pairs (16 products) into an FP64 accumulator, which keeps
+// the mixed form's error at the FP32 rounding of its inputs.
 // A workgroup runs a group of consecutive tiles (Morton order, so spatially compact) for one
 // frequency: it loads the frequency's coupling table once and stages the region of tile t + 1
 // in registers while tile t runs on the MFMAs.  Workgroup i takes group (i % 8) * (gridDim.x / 8)
@@ -37,6 +40,7 @@ namespace h2g {
 namespace {
 
 typedef double double4_m __attribute__((ext_vector_type(4)));
+typedef float float4_m __attribute__((ext_vector_type(4)));
 
 constexpr int NT = 256;  // threads per workgroup (4 waves)
 
@@ -270,13 +274,13 @@ template <bool CONJ>
 __global__ void __launch_bounds__(NT_LIST)
     m2l_list_kernel(int n_tiles, int tg, const int32_t *__restrict__ tile_slot, const int32_t *__restrict__ tile_npb,
                     const int32_t *__restrict__ tile_tgt, const int32_t *__restrict__ tile_list,
-                    const int32_t *__restrict__ tile_nlist, const double *__restrict__ EAg, int n_src, int n_recv,
+                    const int32_t *__restrict__ tile_nlist, const float *__restrict__ EAg, int n_src, int n_recv,
                     const cplx<float> *__restrict__ S, cplx<float> *__restrict__ O, int R, int vec_ok) {
     using C = cplx<float>;
     constexpr int NTL = NT_LIST, RC = 16, NB = 8, QL = 208, NSLOT = 512, LW = NB * QL * 2, LPT = (LW + NTL - 1) / NTL;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *reg = reinterpret_cast<C *>(smem_raw);                      // [NSLOT + 1][RC], last slot zero
-    double *ea = reinterpret_cast<double *>(reg + (NSLOT + 1) * RC);  // [216][32]
+    float *ea = reinterpret_cast<float *>(reg + (NSLOT + 1) * RC);  // [216][32]
     int32_t *lst = reinterpret_cast<int32_t *>(ea + 216 * 32);     // [NB][QL] (region slot, q) pairs
     int32_t *s_tgt = lst + LW;                                     // [NB][8]
     int32_t *s_nl = s_tgt + NB * 8;                                // [NB]
@@ -289,7 +293,7 @@ __global__ void __launch_bounds__(NT_LIST)
     const bool vec = vec_ok && r0 + RC <= R;
     const C *Sf = S + (int64_t)f * n_src * R;
     if (threadIdx.x < RC) reg[NSLOT * RC + threadIdx.x] = C{0.f, 0.f};
-    fill_ea<double, NTL>(ea, EAg + (int64_t)f * 216 * 32);
+    fill_ea<float, NTL>(ea, EAg + (int64_t)f * 216 * 32);
     RegionPipe<C, RC, NSLOT, NTL> pipe;
     pipe.load_idx(tile_slot + (int64_t)t0 * NSLOT, threadIdx.x);
     pipe.load_val(Sf, R, r0, vec, threadIdx.x);
@@ -349,30 +353,36 @@ __global__ void __launch_bounds__(NT_LIST)
             const int nk = __builtin_amdgcn_readfirstlane(s_nl[j]) / 2;  // slot pairs
             const int2 *lj = reinterpret_cast<const int2 *>(lst) + j * QL + h;
             const char *regb = reinterpret_cast<const char *>(reg) + (m * 2 + b) * (int)sizeof(float);
-            const char *eab = reinterpret_cast<const char *>(ea) + (m * 2 + b) * (int)sizeof(double);
-            double4_m acc[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i] = double4_m{0.0, 0.0, 0.0, 0.0};
+            const char *eab = reinterpret_cast<const char *>(ea) + (m * 2 + b) * (int)sizeof(float);
+            // FP32 MFMA (FP32 operands, as stored) over 4 slot pairs per partial sum, partial sums
+            // added in FP64: 16 rounded products per FP32 accumulation
+            double4_m acc = double4_m{0.0, 0.0, 0.0, 0.0};
             for (int k0 = 0; k0 < nk; k0 += 8) {
                 int2 e[8];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) e[i] = lj[2 * (k0 + i)];
-                double a[8], bv[8];
+                float a[8], bv[8];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    a[i] = *reinterpret_cast<const double *>(eab + e[i].y);
+                    a[i] = *reinterpret_cast<const float *>(eab + e[i].y);
                     if (neg) a[i] = -a[i];
-                    bv[i] = (double)*reinterpret_cast<const float *>(regb + e[i].x);
+                    bv[i] = *reinterpret_cast<const float *>(regb + e[i].x);
+                }
+                float4_m f0 = float4_m{0.f, 0.f, 0.f, 0.f}, f1 = f0;
+#pragma unroll
+                for (int i = 0; i < 8; i += 2) {
+                    f0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], bv[i], f0, 0, 0, 0);
+                    f1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i + 1], bv[i + 1], f1, 0, 0, 0);
                 }
 #pragma unroll
-                for (int i = 0; i < 8; ++i) acc[i & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[i], acc[i & 3], 0, 0, 0);
+                for (int v = 0; v < 4; ++v) acc[v] += (double)f0[v] + (double)f1[v];
             }
-            const double4_m sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            const double4_m sum = acc;
             if (r0 + m < R) {
                 const int32_t *tt = s_tgt + j * 8;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int mo = (l >> 4) + 4 * v;
+                    const int mo = 4 * (l >> 4) + v;  // FP32 16x16x4 result layout: rows 4 (l / 16) + v
                     const int tgi = tt[mo >> 1];
                     if (tgi >= 0) OfR[((int64_t)tgi * R + r0 + m) * 2 + (mo & 1)] = (float)sum[v];
                 }
@@ -443,9 +453,9 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
                         int R, hipStream_t s) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
     if (!LL.EA64.ptr) throw Error{H2G_EINVAL, "lattice coupling table missing"};
-    if (!f64 && R >= 16 && LL.list.n_tiles > 0) {
+    if (!f64 && R >= 16 && LL.list.n_tiles > 0 && LL.EA32.ptr) {
         const LatticeTiles &T = LL.list;
-        const size_t sh = sizeof(cplx<float>) * (size_t)(8 * 64 + 1) * 16 + sizeof(double) * 216 * 32 +
+        const size_t sh = sizeof(cplx<float>) * (size_t)(8 * 64 + 1) * 16 + sizeof(float) * 216 * 32 +
                           sizeof(int32_t) * (8 * 208 * 2 + 64 + 8 + 1);
         auto k = conj ? m2l_list_kernel<true> : m2l_list_kernel<false>;
         if (!allow_lds_m(k, sh)) throw Error{H2G_EUNSUPPORTED, "lattice M2L tile does not fit the LDS of this device"};
@@ -454,7 +464,7 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
         const int vec_ok = (R % 2) == 0;  // 16-byte rows of 16 complex FP32 values
         const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + 15) / 16));
         k<<<grid, NT_LIST, sh, s>>>(T.n_tiles, tg, T.slot.as<int32_t>(), T.npb.as<int32_t>(), T.tgt.as<int32_t>(),
-                               T.list.as<int32_t>(), T.nlist.as<int32_t>(), LL.EA64.as<double>(), n_src, n_recv,
+                               T.list.as<int32_t>(), T.nlist.as<int32_t>(), LL.EA32.as<float>(), n_src, n_recv,
                                (const cplx<float> *)S, (cplx<float> *)O, R, vec_ok);
     } else if (LL.clus.n_tiles > 0) {
         const double *E = LL.EA64.as<double>();
@@ -467,13 +477,19 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
 }
 
 void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
-                      bool half, LatticeLevel &LL) {
+                      bool half, bool want32, LatticeLevel &LL) {
     const int FX = half ? (L + 1) / 2 : L;
     LL.F = FX * L * L;
     const int64_t n = (int64_t)LL.F * 216 * 32;
+    const unsigned nb = (unsigned)((n + 255) / 256);
     LL.EA64.alloc((size_t)n * sizeof(double));
-    build_ea_kernel<double><<<(unsigned)((n + 255) / 256), 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L,
-                                                                   FX, LL.F, LL.EA64.as<double>());
+    build_ea_kernel<double><<<nb, 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L, FX, LL.F,
+                                         LL.EA64.as<double>());
+    if (want32) {
+        LL.EA32.alloc((size_t)n * sizeof(float));
+        build_ea_kernel<float><<<nb, 256>>>(tq_op, tq_sym, perms, (const cplx<double> *)ops, L, FX, LL.F,
+                                            LL.EA32.as<float>());
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e)};
     H2G_HIP(hipDeviceSynchronize());

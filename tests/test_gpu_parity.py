@@ -271,6 +271,23 @@ def test_mfma_f64_lane_layout(gpu):
     assert np.allclose(D, A @ B, rtol=1e-14, atol=1e-14)
 
 
+def test_mfma_f32_lane_layout(gpu):
+    """The lane layout the list M2L kernel assumes for v_mfma_f32_16x16x4f32 (result rows
+    4 (lane / 16) + element); small integers, so every product and sum is exact."""
+    import ctypes as C
+
+    from h2g import _lib as H
+
+    rng = np.random.default_rng(1)
+    A = rng.integers(-4, 5, (16, 4)).astype(np.float32)
+    B = rng.integers(-4, 5, (4, 16)).astype(np.float32)
+    D = np.zeros((16, 16), dtype=np.float32)
+    lib = H.lib()
+    lib.h2g_probe_mfma_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    assert lib.h2g_probe_mfma_f32(H.ptr(A), H.ptr(B), H.ptr(D)) == 0
+    assert np.array_equal(D, A @ B)
+
+
 def test_wide_near_field_all_orientations(gpu):
     """Near field alone at 16..40 RHS (FP64 MFMA path for real maps) against the oracle."""
     import h2g
