@@ -84,6 +84,17 @@ def algorithmic_work(arrays, nrhs, phase, level, mixed):
     return "hbm", 2.0 * n * nrhs * 8
 
 
+def traffic_of(kernel, n, p, nrhs):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement of this workload (scaled by the copy-kernel calibration), else None."""
+    path = os.path.join(ROOT, "profiles", "m2l_list_traffic_n2m_p6_r32.json")
+    if (n, p, nrhs) != (2_000_000, 6, 32) or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d["traffic_bytes_per_launch"] if d.get("kernel") == kernel else None
+
+
 def cpu_baseline(n, seed_pts=0, seed_rhs=1, p=6):
     """The oracle (NumPy port of the reference traversal, FP64, one thread) on a bounded sample."""
     import numpy as np
@@ -226,10 +237,11 @@ def main():
                 "frac": round(achieved / PEAK["hbm"], 4), "traffic": None}
     else:
         achieved = work / (dms * 1e-3) / 1e12
-        # the coupling stage multiplies and accumulates in FP64 (FP64 FMA, 78.6 TFLOP/s on MI355X)
-        pk = PEAK["fp64"] if dph == "m2l_apply" else PEAK["fp32"]
+        # the batched coupling stage runs on FP32 MFMA (FP32 operands, FP64 partial-sum accumulation):
+        # priced against the dense FP32 MFMA peak
+        pk = PEAK["fp32"]
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": pk, "unit": "TFLOP/s",
-                "frac": round(achieved / pk, 4), "traffic": None}
+                "frac": round(achieved / pk, 4), "traffic": traffic_of(f"{dph} level {dlv}", N, args.p, R)}
     roof["kernel"] = f"{dph} level {dlv}"
     roof["launch_ms"] = round(dms, 3)
 
