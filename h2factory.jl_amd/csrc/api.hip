@@ -618,14 +618,13 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
         T.tgt.upload_vec(ttgt);
         T.qt.upload_vec(qt);
     }
-    // List tiles: up to 8 receiving parents, consecutive in Morton order, whose present sources
-    // (the children with couplings around them) number at most 512 together; the tile's slot
-    // table is that union, and each block lists (union index, q) of its own sources.  On dense
-    // levels a tile is an aligned 2 x 2 x 2 cube; on sparse levels it gathers 8 scattered
-    // parents, so the region staged per block stays small.
+    // List tiles: up to LIST_NB receiving parents, consecutive in Morton order, whose present
+    // sources (the children with couplings around them) number at most LIST_NSLOT together; the
+    // tile's slot table is that union, and each block lists (union index, q) of its own sources.
+    // On sparse levels a tile gathers scattered parents, so the region staged per block stays small.
     {
         LatticeTiles &T = LL.list;
-        constexpr int NB = 8, NSLOT = 512;
+        constexpr int NB = LIST_NB, NSLOT = LIST_NSLOT;
         std::vector<int32_t> slot, npb, ttgt, tlist, tnl;
         std::unordered_map<int, int> uni;  // source slot -> union index of the open tile
         std::vector<std::vector<std::pair<int, int>>> blk_src;  // per block of the open tile: (slot, q)

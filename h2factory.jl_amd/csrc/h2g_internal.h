@@ -187,19 +187,22 @@ struct Side {
 };
 
 // Parent-block tables of one level for the lattice M2L kernels.
-// Tiles of receiving parents for the lattice M2L kernels: up to 16 (cluster tiles) or 8 (list
-// tiles) parents, consecutive in Morton order, and the union of their present sources.
+// Tiles of receiving parents for the lattice M2L kernels: up to 16 (cluster tiles) or LIST_NB
+// (list tiles) parents, consecutive in Morton order, and the union of their present sources.
+// List tiles: blocks per tile (one wave each) and the cap on the union of their present sources.
+constexpr int LIST_NB = 12, LIST_NSLOT = 768, LIST_QL = 208;
+
 struct LatticeTiles {
     int n_tiles = 0;
-    DevBuf slot;   // int32 [tile][union size 1024 / 512]: source slot (with couplings) or -1
+    DevBuf slot;   // int32 [tile][union size 1024 / LIST_NSLOT]: source slot (with couplings) or -1
     DevBuf npb;    // int32 [tile]: blocks in the tile
     DevBuf tgt;    // int32 [tile][blocks][8 children]: receiving index of the k-th block's children or -1
     DevBuf qt;     // int16 [tile][16][216]: union index of each slot q of the k-th block, 1024 if absent
                    // (cluster tiles)
-    DevBuf list;   // int32 [tile][8][208][2]: byte offsets (union index * 128, q * 128) of the k-th
-                   // block's present slots in the list kernel's LDS region and coupling table,
-                   // padded to a multiple of 16 entries with (512, 104) (list tiles)
-    DevBuf nlist;  // int32 [tile][8]: list lengths (multiples of 16)
+    DevBuf list;   // int32 [tile][LIST_NB][208][2]: byte offsets (union index * 128, q * 128) of the
+                   // k-th block's present slots in the list kernel's LDS region and coupling table,
+                   // padded to a multiple of 16 entries with (LIST_NSLOT, 104) (list tiles)
+    DevBuf nlist;  // int32 [tile][LIST_NB]: list lengths (multiples of 16)
 };
 
 struct LatticeLevel {

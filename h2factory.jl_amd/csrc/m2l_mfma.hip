@@ -17,8 +17,8 @@
 //    (consecutive in Morton order) with the union of their present sources staged in LDS,
 //    columns = (block, r), 16 per MFMA column tile; every tile runs the 208 slots q whose
 //    neighbour is not P itself (those 8 couple only adjacent boxes), absent sources read zeros.
-//  * list kernel: up to 8 receiving parents per tile (consecutive in Morton order) whose present
-//    sources, at most 512, are staged in LDS; 16 columns = the 16 right-hand sides of one block,
+//  * list kernel: up to 12 receiving parents per tile (consecutive in Morton order) whose present
+//    sources, at most 768, are staged in LDS; 16 columns = the 16 right-hand sides of one block,
 //    and each block runs only its own present slots (a packed list built on the host), which
 //    skips the absent sources of sparse levels.  FP32 spectra, R >= 16.
 // The cluster kernel runs on FP64 MFMA (v_mfma_f64_16x16x4f64; the FP64 MFMA rate equals the
@@ -264,11 +264,11 @@ __global__ void __launch_bounds__(NT)
 }
 
 // ---------------------------------------------------------------------------------------------
-// List kernel: up to 8 parents per tile, 16 right-hand sides, FP32 spectra; 8 waves, wave w runs the
+// List kernel: up to LIST_NB parents per tile, 16 right-hand sides, FP32 spectra; one wave per block,
 // w-th present block of the tile over its own slot list (LDS reads of 8 slot pairs ahead of
 // their MFMAs, four accumulators); workgroup = (group of TG tiles, f, chunk of 16 columns).
 // ---------------------------------------------------------------------------------------------
-constexpr int NT_LIST = 512;
+constexpr int NT_LIST = 64 * LIST_NB;  // one wave per block
 
 template <bool CONJ>
 __global__ void __launch_bounds__(NT_LIST)
@@ -277,7 +277,8 @@ __global__ void __launch_bounds__(NT_LIST)
                     const int32_t *__restrict__ tile_nlist, const float *__restrict__ EAg, int n_src, int n_recv,
                     const cplx<float> *__restrict__ S, cplx<float> *__restrict__ O, int R, int vec_ok) {
     using C = cplx<float>;
-    constexpr int NTL = NT_LIST, RC = 16, NB = 8, QL = 208, NSLOT = 512, LW = NB * QL * 2, LPT = (LW + NTL - 1) / NTL;
+    constexpr int NTL = NT_LIST, RC = 16, NB = LIST_NB, QL = LIST_QL, NSLOT = LIST_NSLOT, LW = NB * QL * 2,
+                  LPT = (LW + NTL - 1) / NTL;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *reg = reinterpret_cast<C *>(smem_raw);                      // [NSLOT + 1][RC], last slot zero
     float *ea = reinterpret_cast<float *>(reg + (NSLOT + 1) * RC);  // [216][32]
@@ -455,8 +456,8 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
     if (!LL.EA64.ptr) throw Error{H2G_EINVAL, "lattice coupling table missing"};
     if (!f64 && R >= 16 && LL.list.n_tiles > 0 && LL.EA32.ptr) {
         const LatticeTiles &T = LL.list;
-        const size_t sh = sizeof(cplx<float>) * (size_t)(8 * 64 + 1) * 16 + sizeof(float) * 216 * 32 +
-                          sizeof(int32_t) * (8 * 208 * 2 + 64 + 8 + 1);
+        const size_t sh = sizeof(cplx<float>) * (size_t)(LIST_NSLOT + 1) * 16 + sizeof(float) * 216 * 32 +
+                          sizeof(int32_t) * (LIST_NB * LIST_QL * 2 + LIST_NB * 8 + LIST_NB + 1);
         auto k = conj ? m2l_list_kernel<true> : m2l_list_kernel<false>;
         if (!allow_lds_m(k, sh)) throw Error{H2G_EUNSUPPORTED, "lattice M2L tile does not fit the LDS of this device"};
         const int tg = std::max(1, std::min(16, T.n_tiles / 32));
