@@ -1,6 +1,7 @@
 // C ABI of libh2g: core / workspace lifetime, validation, traversal plans and the matvec.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <numeric>
 #include <array>
@@ -702,6 +703,62 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
     LL.ok = true;
 }
 
+// Row-block tasks of one orientation: for each output leaf (CSR row t = blocks order[p[t]] ..
+// order[p[t+1]-1], all with the same output points), the source columns of all its blocks in
+// order, and one task per 64 output rows.  Tasks are grouped by row-tile count, largest first.
+// Source indices must fit int32; otherwise the plan stays empty and the block kernels run.
+void build_near_plan(NearPlan &plan, bool trans, const std::vector<int64_t> &p, const std::vector<int32_t> &order,
+                     const std::vector<int64_t> &rb, const std::vector<int64_t> &cb, const std::vector<int32_t> &nr,
+                     const std::vector<int32_t> &nc, const std::vector<int64_t> &off) {
+    plan.task.release();
+    plan.col.release();
+    plan.ok = false;
+    std::vector<NearCol> cols;
+    std::vector<NearTask> byrt[4];
+    const int64_t ntgt = (int64_t)p.size() - 1;
+    for (int64_t t = 0; t < ntgt; ++t) {
+        const int first = order[p[t]];
+        const int n_out = trans ? nc[first] : nr[first];
+        const int64_t out0 = trans ? cb[first] : rb[first];
+        const int64_t col0 = (int64_t)cols.size();
+        for (int64_t q = p[t]; q < p[t + 1]; ++q) {
+            const int bl = order[q];
+            const int kdim = trans ? nr[bl] : nc[bl];
+            const int64_t src0 = trans ? rb[bl] : cb[bl];
+            if (src0 + kdim > INT32_MAX) return;
+            for (int kc = 0; kc < kdim; ++kc) {
+                NearCol e;
+                e.aoff = trans ? off[bl] + kc : off[bl] + (int64_t)nr[bl] * kc;
+                e.xrow = (int32_t)(src0 + kc);
+                e.rstride = trans ? nr[bl] : 1;
+                cols.push_back(e);
+            }
+        }
+        const int64_t ncol = (int64_t)cols.size() - col0;
+        if (ncol == 0) continue;  // nothing to add: the output rows stay zero
+        if (ncol > INT32_MAX) return;
+        for (int r0 = 0; r0 < n_out; r0 += 64) {
+            NearTask k;
+            k.out0 = out0;
+            k.col0 = col0;
+            k.nrow = std::min(64, n_out - r0);
+            k.ncol = (int32_t)ncol;
+            k.row0 = r0;
+            k.pad = 0;
+            byrt[(k.nrow + 15) / 16 - 1].push_back(k);
+        }
+    }
+    std::vector<NearTask> tasks;
+    for (int i = 0; i < 4; ++i) {
+        plan.cls[i] = (int64_t)tasks.size();
+        tasks.insert(tasks.end(), byrt[3 - i].begin(), byrt[3 - i].end());
+    }
+    plan.cls[4] = (int64_t)tasks.size();
+    plan.task.upload_vec(tasks);
+    plan.col.upload_vec(cols);
+    plan.ok = true;
+}
+
 void build_near(Core &c, const h2g_core_desc &d) {
     const h2g_side_desc &tr = d.galerkin ? d.test : d.trial;
     const int64_t nb = d.n_near_blocks;
@@ -726,7 +783,8 @@ void build_near(Core &c, const h2g_core_desc &d) {
     c.near_blk_nr.upload_vec(nr);
     c.near_blk_nc.upload_vec(nc);
     c.near_blk_off.upload_vec(off);
-    auto csr = [&](const int32_t *key, int nkeys, DevBuf &tgt, DevBuf &ptr, DevBuf &blk, int32_t &ntgt) {
+    auto csr = [&](const int32_t *key, int nkeys, DevBuf &tgt, DevBuf &ptr, DevBuf &blk, int32_t &ntgt,
+                   NearPlan &plan, bool trans) {
         std::vector<int64_t> cnt(nkeys + 1, 0);
         for (int64_t q = 0; q < nb; ++q) cnt[key[q] + 1]++;
         std::partial_sum(cnt.begin(), cnt.end(), cnt.begin());
@@ -745,9 +803,12 @@ void build_near(Core &c, const h2g_core_desc &d) {
         tgt.upload_vec(targets);
         ptr.upload_vec(p);
         blk.upload_vec(order);
+        if (c.dtype == H2G_F64) build_near_plan(plan, trans, p, order, rb, cb, nr, nc, off);
     };
-    csr(d.near_test, d.test.n_boxes, c.near_fwd_tgt, c.near_fwd_ptr, c.near_fwd_blk, c.n_near_fwd_tgt);
-    csr(d.near_trial, tr.n_boxes, c.near_rev_tgt, c.near_rev_ptr, c.near_rev_blk, c.n_near_rev_tgt);
+    csr(d.near_test, d.test.n_boxes, c.near_fwd_tgt, c.near_fwd_ptr, c.near_fwd_blk, c.n_near_fwd_tgt,
+        c.near_fwd_plan, false);
+    csr(d.near_trial, tr.n_boxes, c.near_rev_tgt, c.near_rev_ptr, c.near_rev_blk, c.n_near_rev_tgt,
+        c.near_rev_plan, true);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -901,6 +962,7 @@ int64_t Core::device_bytes() const {
     side(test);
     if (!galerkin) side(trial_own);
     b += ops.bytes + ops32.bytes + perms.bytes + perms_inv.bytes + near_vals.bytes;
+    b += near_fwd_plan.task.bytes + near_fwd_plan.col.bytes + near_rev_plan.task.bytes + near_rev_plan.col.bytes;
     return b;
 }
 

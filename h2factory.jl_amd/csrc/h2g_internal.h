@@ -214,6 +214,26 @@ struct LatticeLevel {
     DevBuf EA32;              // the same in FP32 (cores with the mixed precision)
 };
 
+// Near field of one output leaf (or of its rows row0 .. row0 + nrow - 1, at most 64) as one
+// product Y (nrow x R) = A (nrow x ncol) X (ncol x R) whose ncol columns are the source points of
+// all the leaf's near blocks, concatenated.  Column j of A starts at value aoff (row 0) with row
+// stride rstride (1 for a block applied as stored, the block's row count for its transpose), and
+// multiplies source point xrow.
+struct NearTask {
+    int64_t out0;  // first output point of the leaf
+    int64_t col0;  // first entry in the column table
+    int32_t nrow, ncol, row0, pad;
+};
+struct NearCol {
+    int64_t aoff;
+    int32_t xrow, rstride;
+};
+struct NearPlan {
+    DevBuf task, col;
+    int64_t cls[5] = {0, 0, 0, 0, 0};  // tasks of 4, 3, 2, 1 row tiles: [cls[i], cls[i+1])
+    bool ok = false;
+};
+
 // Work lists of one orientation (forward: trial -> test; reverse: test -> trial).
 struct OrientPlan {
     std::vector<LatticeLevel> lattice;  // per dst level
@@ -260,6 +280,9 @@ struct Core {
     DevBuf near_fwd_tgt, near_rev_tgt;   // target leaf lists
     int32_t n_near_fwd_tgt = 0, n_near_rev_tgt = 0;
     DevBuf near_blk_rb, near_blk_cb, near_blk_nr, near_blk_nc, near_blk_off;  // per block
+    // near field of real maps as one row-block product per output leaf (wide batches): per
+    // orientation, tasks sorted by row-tile count and the concatenated source-column table
+    NearPlan near_fwd_plan, near_rev_plan;
     cplx<double> alpha{1.0, 0.0};
     int precisions = 1;
     OrientPlan fwd, rev;

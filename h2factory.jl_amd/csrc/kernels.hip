@@ -966,9 +966,11 @@ void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targ
 
 
 // ------------------------------------------------------------------------------------------
-// Near field on FP64 MFMA (real maps, R >= 16): Y_t (n_out x R) += B (n_out x n_in) X (n_in x R)
-// with v_mfma_f64_16x16x4f64.  One workgroup (4 waves) per target leaf; a wave owns 16-row
-// output tiles across all 16-column RHS tiles.  Operand lane layouts of the 16x16x4 FP64
+// Near field on FP64 MFMA (real maps, R >= 16), one row-block product per task:
+//   Y (nrow x R) = A (nrow x ncol) X (ncol x R)
+// with v_mfma_f64_16x16x4f64.  A's columns are the source points of all near blocks of the
+// output leaf, concatenated (one NearCol entry each), so the contraction is padded to a multiple
+// of 4 KU once per leaf instead of once per block.  Operand lane layouts of the 16x16x4 FP64
 // instruction (checked by the layout probe below on gfx950): A[m][k] in lane m + 16k,
 // B[k][n] in lane n + 16k, D[m][n] in lane n + 16(m % 4), element m / 4.
 // ------------------------------------------------------------------------------------------
@@ -979,98 +981,113 @@ __device__ inline int mf_a_row(int lane) { return lane & 15; }
 __device__ inline int mf_a_k(int lane) { return lane >> 4; }
 __device__ inline int mf_d_row(int lane, int v) { return (lane >> 4) + 4 * v; }
 
-template <bool TRANS, int NCT>
-__global__ void __launch_bounds__(TPB) near_mfma_kernel(const int64_t *__restrict__ ptr, const int32_t *__restrict__ blk,
-                                 const int64_t *__restrict__ rb, const int64_t *__restrict__ cb,
-                                 const int32_t *__restrict__ nr, const int32_t *__restrict__ nc,
-                                 const int64_t *__restrict__ off, const double *__restrict__ vals,
-                                 const double *__restrict__ xs, double *__restrict__ ys, int R) {
-    // a wave owns one 16-row output tile and NCT 16-column RHS tiles: every near-field value is
-    // loaded once and feeds NCT MFMAs; the operands of KU k-steps are loaded one batch ahead
-    constexpr int KU = 4;
-    const int t = blockIdx.x;
-    const int64_t b0 = ptr[t], b1 = ptr[t + 1];
-    if (b0 == b1) return;
-    const int first = blk[b0];
-    const int n_out = TRANS ? nc[first] : nr[first];
-    const int64_t out0 = TRANS ? cb[first] : rb[first];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const int n_rt = (n_out + 15) / 16;
-    const int arow_l = mf_a_row(lane), ak = mf_a_k(lane);
-    const int bn = lane & 15, bk = lane >> 4;
-    const int c0 = blockIdx.y * NCT * 16;  // first RHS of this workgroup's column tiles
-    for (int rt = wave; rt < n_rt; rt += nwaves) {
-        const int arow = rt * 16 + arow_l;
-        double4_t acc[NCT];
+// One wave per task, 4 independent waves per workgroup; tasks [t0, t1) all have RT row tiles.
+// The wave holds RT x NCT accumulator tiles, so every source value x[j][c] it loads feeds RT
+// MFMAs and every matrix value NCT.  A batch is KU k-steps (KB = 4 KU columns).  Lane l holds the
+// column-table entry l % KB of a batch, loaded two batches ahead of its use; a lane takes the
+// entries of its own columns from the owning lanes (ds_bpermute), so the table costs 2 x 4
+// registers rather than 4 per column in flight.  Operands are loaded one batch ahead of the
+// MFMAs.  Every load reads a clamped (valid) address and out-of-range rows, columns and RHS are
+// multiplied by zero, so the loop has no branches and the accumulators stay in the MFMA
+// registers.  Workgroups are spread so that each XCD (dispatched round robin) works on a
+// contiguous range of tasks, whose source points overlap in its L2.
+template <bool TRANS, int RT, int KU, int NCT>
+__global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__restrict__ tasks, int64_t t0, int64_t t1,
+                                                        const NearCol *__restrict__ cols, const double *__restrict__ vals,
+                                                        const double *__restrict__ xs, double *__restrict__ ys, int R) {
+    constexpr int KB = 4 * KU;
+    static_assert(KB <= 64, "a batch's table entries fit one wave");
+    int wg = blockIdx.x;
+    const int n8 = gridDim.x & ~7;
+    if (wg < n8) wg = (wg & 7) * (n8 >> 3) + (wg >> 3);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t ti = t0 + (int64_t)wg * (TPB / 64) + wave;
+    if (ti >= t1) return;
+    const NearTask tk = tasks[ti];
+    const int nrow = tk.nrow, ncol = tk.ncol;
+    const NearCol *__restrict__ cl = cols + tk.col0;
+    const int lane = threadIdx.x & 63;
+    const int ar = mf_a_row(lane), ak = mf_a_k(lane);
+    const int c0 = blockIdx.y * NCT * 16;
+    const int tl = lane & (KB - 1);
+    int arow[RT];
+    double rmask[RT];
 #pragma unroll
-        for (int g = 0; g < NCT; ++g) acc[g] = double4_t{0.0, 0.0, 0.0, 0.0};
-        // the operands of the next KU k-steps (crossing into the next near block) are loaded
-        // before the MFMAs of the current ones
-        int64_t q = b0;
-        int bl = blk[q], m = nr[bl], n = nc[bl];
-        const double *B = vals + off[bl];
-        int kdim = TRANS ? m : n;
-        const double *xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
-        int k0 = 0;
-        double a[KU], bv[KU][NCT];
-        // unconditional loads from clamped (valid) addresses, out-of-range operands multiplied by
-        // zero: no branches around the MFMAs
-        auto load = [&]() {
+    for (int rt = 0; rt < RT; ++rt) {
+        const int r = 16 * rt + ar;
+        arow[rt] = tk.row0 + min(r, nrow - 1);
+        rmask[rt] = r < nrow ? 1.0 : 0.0;
+    }
+    int xcol[NCT];
+    double cmask[NCT];
 #pragma unroll
-            for (int u = 0; u < KU; ++u) {
-                const int kk = k0 + 4 * u + ak;
-                const int ar = min(arow, n_out - 1), kc = min(kk, kdim - 1);
-                a[u] = (TRANS ? B[kc + (int64_t)m * ar] : B[ar + (int64_t)m * kc]) *
-                       ((arow < n_out && kk < kdim) ? 1.0 : 0.0);
-                const int kb = k0 + 4 * u + bk;
+    for (int g = 0; g < NCT; ++g) {
+        const int col = c0 + 16 * g + ar;
+        xcol[g] = min(col, R - 1);
+        cmask[g] = col < R ? 1.0 : 0.0;
+    }
+    // raw operands of the batch in flight and their column masks; the masks are applied when the
+    // batch is used, so that nothing waits for the loads before the next batch's MFMAs
+    double a[KU][RT], b[KU][NCT], km[KU];
+    auto load_ops = [&](int k0, const NearCol &ev) {
 #pragma unroll
-                for (int g = 0; g < NCT; ++g) {
-                    const int col = c0 + 16 * g + bn;
-                    bv[u][g] = xin[(int64_t)min(kb, kdim - 1) * R + min(col, R - 1)] * ((kb < kdim && col < R) ? 1.0 : 0.0);
-                }
-            }
-        };
-        load();
-        while (true) {
-            double ac[KU], bc[KU][NCT];
+        for (int u = 0; u < KU; ++u) {
+            const int src = (4 * u + ak) << 2;  // byte address of the lane holding this column
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)ev.aoff);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(ev.aoff >> 32));
+            const int xr = __builtin_amdgcn_ds_bpermute(src, ev.xrow);
+            const int64_t rs = TRANS ? (int64_t)__builtin_amdgcn_ds_bpermute(src, ev.rstride) : 1;
+            km[u] = (k0 + 4 * u + ak < ncol) ? 1.0 : 0.0;
+            const double *__restrict__ ap = vals + (int64_t)(((uint64_t)hi << 32) | lo);
 #pragma unroll
-            for (int u = 0; u < KU; ++u) {
-                ac[u] = a[u];
+            for (int rt = 0; rt < RT; ++rt) a[u][rt] = ap[arow[rt] * rs];
+            const double *__restrict__ xp = xs + (int64_t)xr * R;
 #pragma unroll
-                for (int g = 0; g < NCT; ++g) bc[u][g] = bv[u][g];
-            }
-            k0 += 4 * KU;
-            bool more = true;
-            if (k0 >= kdim) {
-                if (++q < b1) {
-                    bl = blk[q];
-                    m = nr[bl];
-                    n = nc[bl];
-                    B = vals + off[bl];
-                    kdim = TRANS ? m : n;
-                    xin = xs + (TRANS ? rb[bl] : cb[bl]) * R;
-                    k0 = 0;
-                } else {
-                    more = false;
-                }
-            }
-            if (more) load();
-#pragma unroll
-            for (int u = 0; u < KU; ++u)
-#pragma unroll
-                for (int g = 0; g < NCT; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bc[u][g], acc[g], 0, 0, 0);
-            if (!more) break;
+            for (int g = 0; g < NCT; ++g) b[u][g] = xp[xcol[g]];
         }
+    };
+    double4_t acc[RT][NCT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) acc[rt][g] = double4_t{0.0, 0.0, 0.0, 0.0};
+    const int nb = (ncol + KB - 1) / KB;
+    NearCol tb0 = cl[min(tl, ncol - 1)], tb1 = cl[min(KB + tl, ncol - 1)];
+    load_ops(0, tb0);
+    for (int ib = 0; ib < nb; ++ib) {
+        double ac[KU][RT], bc[KU][NCT];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) ac[u][rt] = a[u][rt] * (rmask[rt] * km[u]);
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) bc[u][g] = b[u][g] * (cmask[g] * km[u]);
+        }
+        const NearCol tb2 = cl[min((ib + 2) * KB + tl, ncol - 1)];
+        load_ops((ib + 1) * KB, tb1);
+        // keep the next batch's loads ahead of this batch's MFMAs
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < KU; ++u)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g)
+                    acc[rt][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u][rt], bc[u][g], acc[rt][g], 0, 0, 0);
+        tb1 = tb2;
+    }
+    double *__restrict__ yo = ys + (tk.out0 + tk.row0) * R;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int g = 0; g < NCT; ++g) {
-            const int col = c0 + 16 * g + bn;
+            const int col = c0 + 16 * g + ar;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = rt * 16 + mf_d_row(lane, v);
-                if (row < n_out && col < R) ys[(out0 + row) * R + col] = acc[g][v];
+                const int row = 16 * rt + mf_d_row(lane, v);
+                if (row < nrow && col < R) yo[(int64_t)row * R + col] = acc[rt][g][v];
             }
         }
-    }
 }
 
 // Leaf expansions on FP64 MFMA (real maps, R >= 16), FP64 accumulation.
@@ -1250,29 +1267,39 @@ bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
 bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
     if (c.dtype != H2G_F64 || R < 16) return false;
     const bool trans = op != H2G_OP_N;
-    const int ntgt = trans ? c.n_near_rev_tgt : c.n_near_fwd_tgt;
-    if (ntgt == 0) return true;
-    const int64_t *ptr = trans ? c.near_rev_ptr.as<int64_t>() : c.near_fwd_ptr.as<int64_t>();
-    const int32_t *blk = trans ? c.near_rev_blk.as<int32_t>() : c.near_fwd_blk.as<int32_t>();
+    const NearPlan &P = trans ? c.near_rev_plan : c.near_fwd_plan;
+    if (!P.ok) return false;
     // NCT column tiles per workgroup (up to 64 RHS); wider batches take several workgroups
     const int nct = std::min(4, (R + 15) / 16);
-    const dim3 grid((unsigned)ntgt, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
-    auto go = [&](auto kern) {
-        kern<<<grid, TPB, 0, s>>>(ptr, blk, c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(),
-                                  c.near_blk_nr.as<int32_t>(), c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(),
-                                  c.near_vals.as<double>(), (const double *)xs, (double *)ys, R);
-    };
-    switch (nct * 2 + (trans ? 1 : 0)) {
-    case 2: go(near_mfma_kernel<false, 1>); break;
-    case 3: go(near_mfma_kernel<true, 1>); break;
-    case 4: go(near_mfma_kernel<false, 2>); break;
-    case 5: go(near_mfma_kernel<true, 2>); break;
-    case 6: go(near_mfma_kernel<false, 3>); break;
-    case 7: go(near_mfma_kernel<true, 3>); break;
-    case 8: go(near_mfma_kernel<false, 4>); break;
-    default: go(near_mfma_kernel<true, 4>); break;
+    const unsigned gy = (unsigned)((R + 16 * nct - 1) / (16 * nct));
+    for (int i = 0; i < 4; ++i) {  // tasks of 4, 3, 2, 1 row tiles
+        const int64_t t0 = P.cls[i], t1 = P.cls[i + 1];
+        if (t1 == t0) continue;
+        const dim3 grid((unsigned)((t1 - t0 + TPB / 64 - 1) / (TPB / 64)), gy);
+        auto go = [&](auto kern) {
+            kern<<<grid, TPB, 0, s>>>(P.task.as<NearTask>(), t0, t1, P.col.as<NearCol>(), c.near_vals.as<double>(),
+                                      (const double *)xs, (double *)ys, R);
+        };
+        switch ((4 - i) * 8 + nct) {
+        case 8 + 1: trans ? go(near_task_kernel<true, 1, 4, 1>) : go(near_task_kernel<false, 1, 4, 1>); break;
+        case 8 + 2: trans ? go(near_task_kernel<true, 1, 4, 2>) : go(near_task_kernel<false, 1, 4, 2>); break;
+        case 8 + 3: trans ? go(near_task_kernel<true, 1, 4, 3>) : go(near_task_kernel<false, 1, 4, 3>); break;
+        case 8 + 4: trans ? go(near_task_kernel<true, 1, 4, 4>) : go(near_task_kernel<false, 1, 4, 4>); break;
+        case 16 + 1: trans ? go(near_task_kernel<true, 2, 2, 1>) : go(near_task_kernel<false, 2, 2, 1>); break;
+        case 16 + 2: trans ? go(near_task_kernel<true, 2, 2, 2>) : go(near_task_kernel<false, 2, 2, 2>); break;
+        case 16 + 3: trans ? go(near_task_kernel<true, 2, 2, 3>) : go(near_task_kernel<false, 2, 2, 3>); break;
+        case 16 + 4: trans ? go(near_task_kernel<true, 2, 2, 4>) : go(near_task_kernel<false, 2, 2, 4>); break;
+        case 24 + 1: trans ? go(near_task_kernel<true, 3, 2, 1>) : go(near_task_kernel<false, 3, 2, 1>); break;
+        case 24 + 2: trans ? go(near_task_kernel<true, 3, 2, 2>) : go(near_task_kernel<false, 3, 2, 2>); break;
+        case 24 + 3: trans ? go(near_task_kernel<true, 3, 2, 3>) : go(near_task_kernel<false, 3, 2, 3>); break;
+        case 24 + 4: trans ? go(near_task_kernel<true, 3, 2, 4>) : go(near_task_kernel<false, 3, 2, 4>); break;
+        case 32 + 1: trans ? go(near_task_kernel<true, 4, 2, 1>) : go(near_task_kernel<false, 4, 2, 1>); break;
+        case 32 + 2: trans ? go(near_task_kernel<true, 4, 2, 2>) : go(near_task_kernel<false, 4, 2, 2>); break;
+        case 32 + 3: trans ? go(near_task_kernel<true, 4, 2, 3>) : go(near_task_kernel<false, 4, 2, 3>); break;
+        default: trans ? go(near_task_kernel<true, 4, 2, 4>) : go(near_task_kernel<false, 4, 2, 4>); break;
+        }
+        H2G_CHECK_LAUNCH();
     }
-    H2G_CHECK_LAUNCH();
     return true;
 }
 

@@ -288,17 +288,25 @@ def test_mfma_f32_lane_layout(gpu):
     assert np.array_equal(D, A @ B)
 
 
-def test_wide_near_field_all_orientations(gpu):
-    """Near field alone at 16..40 RHS (FP64 MFMA path for real maps) against the oracle."""
+@pytest.mark.parametrize("petrov,leaf", [(False, 40), (True, 70)])
+def test_wide_near_field_all_orientations(gpu, petrov, leaf):
+    """Near field alone at 16..70 RHS (FP64 MFMA path for real maps) against the oracle.
+
+    Leaves of up to 70 points give output leaves of 1..4 row tiles and leaves split into two
+    row-block tasks; distinct test and trial points give rectangular blocks, whose transposes
+    read rows with a stride other than the column count.  70 RHS take two column groups."""
     import h2g
 
-    pts = np.random.default_rng(21).random((2500, 3))
-    core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=40)
-    A = make_map(core, max_rhs=40)
-    rng = np.random.default_rng(1)
-    for nrhs in (16, 21, 40):
-        x = rng.standard_normal((2500, nrhs))
+    rng = np.random.default_rng(21)
+    pts = rng.random((2500, 3))
+    pts_b = rng.random((2100, 3)) if petrov else None
+    core = OB.build_core(pts, pts_b, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=leaf)
+    A = make_map(core, max_rhs=70)
+    for nrhs in (16, 21, 40, 70):
         for op in "NT":
-            y = np.zeros((2500, nrhs))
-            h2g.nearmul_(y, orient(A, op), x)
+            M = orient(A, op)
+            n_out, n_in = M.shape
+            x = rng.standard_normal((n_in, nrhs))
+            y = np.zeros((n_out, nrhs))
+            h2g.nearmul_(y, M, x)
             assert relerr(y, O.nearmul(core, x, op)) < RTOL, (op, nrhs)
