@@ -695,6 +695,51 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
         T.list.upload_vec(tlist);
         T.nlist.upload_vec(tnl);
     }
+    // Direct lists: per block (Morton order) its present, coupled slots as pairs of one even and one
+    // odd q where possible (the two halves of a pair read coupling rows in opposite LDS bank
+    // halves), in batches of 8 pairs padded with (slot 0, zero coupling row 104).
+    {
+        LatticeDirect &D = LL.direct;
+        D.max_batches = 0;
+        std::vector<int32_t> dptr, dlist, dtgt;
+        dptr.push_back(0);
+        for (const auto &ok : ord) {
+            const int k = ok.second;
+            const auto mine = sources_of(k);
+            std::vector<std::pair<int, int>> cls[2];
+            for (const auto &e : mine) cls[e.second & 1].push_back(e);
+            std::vector<std::pair<int, int>> h0, h1;
+            const size_t both = std::min(cls[0].size(), cls[1].size());
+            for (size_t i = 0; i < both; ++i) {
+                h0.push_back(cls[0][i]);
+                h1.push_back(cls[1][i]);
+            }
+            std::vector<std::pair<int, int>> rest;
+            for (int a = 0; a < 2; ++a) rest.insert(rest.end(), cls[a].begin() + both, cls[a].end());
+            for (size_t i = 0; i < rest.size(); i += 2) {
+                h0.push_back(rest[i]);
+                h1.push_back(i + 1 < rest.size() ? rest[i + 1] : std::make_pair(0, 104));
+            }
+            // at least one batch, so that every block writes its children (zeros without couplings)
+            while (h0.empty() || h0.size() % 8) {
+                h0.push_back({0, 104});
+                h1.push_back({0, 104});
+            }
+            for (size_t b0 = 0; b0 < h0.size(); b0 += 8)
+                for (const auto *h : {&h0, &h1})
+                    for (size_t i = b0; i < b0 + 8; ++i) {
+                        dlist.push_back((*h)[i].first);
+                        dlist.push_back((*h)[i].second * 32 * 4);
+                    }
+            dptr.push_back(dptr.back() + (int32_t)(h0.size() / 8));
+            D.max_batches = std::max(D.max_batches, (int)(h0.size() / 8));
+            for (int t = 0; t < 8; ++t) dtgt.push_back(tgt[(size_t)k * 8 + t]);
+        }
+        D.n_blocks = nb;
+        D.ptr.upload_vec(dptr);
+        D.list.upload_vec(dlist);
+        D.tgt.upload_vec(dtgt);
+    }
     DevBuf d_op, d_sym;
     d_op.upload_vec(tq_op);
     if (sym) d_sym.upload_vec(tq_sym);

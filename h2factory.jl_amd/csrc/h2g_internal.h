@@ -205,11 +205,23 @@ struct LatticeTiles {
     DevBuf nlist;  // int32 [tile][LIST_NB]: list lengths (multiples of 16)
 };
 
+// Direct lists (batches of >= 16 columns, FP32 spectra): blocks in Morton order, each with its
+// present, coupled slots as slot pairs in batches of 8 pairs.  Batch k holds 16 entries (slot,
+// q * 128) laid out [pair half h][pair i]: pair i couples slots of the two halves (one even and
+// one odd q where possible); padding entries are (0, 104 * 128), a zero coupling row.
+struct LatticeDirect {
+    int n_blocks = 0, max_batches = 0;
+    DevBuf ptr;   // int32 [n_blocks + 1]: first batch of each block
+    DevBuf list;  // int32 [batches][2][8][2]
+    DevBuf tgt;   // int32 [n_blocks][8]: receiving index of the block's children or -1
+};
+
 struct LatticeLevel {
     bool ok = false;
     int F = 0;                // stored frequencies
     LatticeTiles clus;        // cluster tiles (any batch width)
     LatticeTiles list;        // list tiles (batches of >= 16 columns, FP32 spectra)
+    LatticeDirect direct;     // direct lists (batches of >= 16 columns, FP32 spectra)
     DevBuf EA64;              // double [F][216 slots q][16 rows m = 2t + c][2 b]: real form of the couplings
     DevBuf EA32;              // the same in FP32 (cores with the mixed precision)
 };

@@ -32,6 +32,7 @@
 // + i / 8: consecutive workgroups go to different XCDs, so each XCD gets a contiguous range of
 // groups and its L2 holds only its part of the frequency slice.
 #include <algorithm>
+#include <cstdint>
 
 #include "h2g_internal.h"
 
@@ -392,6 +393,185 @@ __global__ void __launch_bounds__(NT_LIST)
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Direct kernel (FP32 spectra, R >= 16): no staging of source spectra.  Workgroup = (group of
+// consecutive blocks in Morton order, f, chunk of 32 columns); the frequency's coupling table is
+// loaded into LDS once, then every wave takes its own blocks of the group and, per slot pair of
+// the block's list, reads the two source rows straight from global memory (L2: the group's blocks
+// and their neighbours on one XCD share them) and the two coupling rows from LDS.  A wave covers
+// NCH 16-column tiles of the chunk, so every coupling value it reads feeds NCH MFMAs.  A block's
+// list is copied into the wave's part of LDS (one coalesced load per lane, issued while the
+// previous block computes), so the per-pair entries, which all lanes of a half share, are read
+// from LDS and not through the vector memory return path.  List entries are read two batches
+// (of 8 pairs) ahead of their MFMAs and source values one batch ahead; FP32 MFMA over 4 slot
+// pairs per partial sum, partial sums added in FP64 one batch later, when the MFMAs that
+// produced them have finished.
+// ---------------------------------------------------------------------------------------------
+constexpr int DR_NT = 256, DR_MAXB = 16;  // threads; batches of a block's list held in LDS
+
+template <bool CONJ, int NCH>
+__global__ void __launch_bounds__(DR_NT, 2)
+    m2l_direct_kernel(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr, const int4 *__restrict__ list,
+                      int n_list, const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg, int n_src,
+                      int n_recv, const cplx<float> *__restrict__ S, cplx<float> *__restrict__ O, int R) {
+    __shared__ __align__(16) float ea[216 * 32];
+    __shared__ int4 lsd[DR_NT / 64][2][DR_MAXB * 8];  // per wave: lists of the current / next block
+    const int grp = group_of_workgroup();
+    const int bb0 = grp * gsz, bb1 = min(n_blocks, bb0 + gsz);
+    if (bb0 >= bb1) return;
+    const int f = blockIdx.y, r0 = blockIdx.z * 32;
+    fill_ea<float, DR_NT>(ea, EAg + (int64_t)f * 216 * 32);
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int m = l & 15, bq = (l >> 4) & 1, h = l >> 5;
+    // adjoint: the Im parts of the couplings (rows c != b) change sign
+    const float sgn = (CONJ && ((m & 1) != bq)) ? -1.f : 1.f;
+    // source rows of frequency f through a buffer resource: byte offsets, reads past the slice
+    // return zero.  The lane reads column r0 + 16 g + m, re / im part bq, of a row; a column >= R
+    // reads the next row (or zero), which only reaches result columns that are not stored.
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<cplx<float> *>(S + (int64_t)f * n_src * R), (short)0, n_src * R * (int)sizeof(cplx<float>),
+        0x00020000);
+    const uint32_t lo = (uint32_t)(((r0 + m) * 2 + bq) * 4);
+    // 8-byte loads (both column tiles, NCH = 2): column r0 + 16 bq + m, clamped into the row
+    const char *__restrict__ Sfb = reinterpret_cast<const char *>(S + (int64_t)f * n_src * R);
+    const uint32_t lo2 = (uint32_t)(min(r0 + 16 * bq + m, R - 1) * (int)sizeof(cplx<float>));
+    const uint32_t rowb = (uint32_t)R * 8;  // bytes per source row
+    const char *eab = reinterpret_cast<const char *>(ea) + (m * 2 + bq) * (int)sizeof(float);
+    float *__restrict__ Of = reinterpret_cast<float *>(O + (int64_t)f * n_recv * R);
+    // list copies: lane l moves entries l and 64 + l (int4, two pairs each) of a block's list
+    int4 nx[2];
+    auto fetch = [&](int blk) {
+        const int base = blk_ptr[blk] * 8;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) nx[u] = list[min(base + 64 * u + l, n_list - 1)];
+    };
+    fetch(bb0 + w);
+    int buf = 0;
+    for (int blk = bb0 + w; blk < bb1; blk += DR_NT / 64) {
+        const int k0 = blk_ptr[blk], k1 = blk_ptr[blk + 1];
+        int4 *lw = lsd[w][buf];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) lw[64 * u + l] = nx[u];
+        if (blk + DR_NT / 64 < bb1) fetch(blk + DR_NT / 64);
+        // batch k, half h: 8 entries (slot, coupling row byte offset) = 4 int4 at (k - k0) * 8 + 4 h
+        const int4 *lh = lw + h * 4;
+        int4 L1[4], L2[4];
+        auto ldl = [&](int k, int4(&L)[4]) {
+            const int4 *p = lh + (min(k, k1 - 1) - k0) * 8;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) L[j] = p[j];
+        };
+        float an[8], bn[8][NCH];
+        auto issue = [&](const int4(&L)[4]) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int i = 2 * j + e;
+                    const uint32_t slot = (uint32_t)(e ? L[j].z : L[j].x);
+                    const int qo = e ? L[j].w : L[j].y;
+                    an[i] = *reinterpret_cast<const float *>(eab + qo);
+                    if constexpr (NCH == 2) {
+                        // one 8-byte load per lane: the complex value (re, im) of column
+                        // r0 + 16 bq + m, rearranged when the batch is used
+                        const float2 v = *reinterpret_cast<const float2 *>(Sfb + (slot * rowb + lo2));
+                        bn[i][0] = v.x;
+                        bn[i][NCH - 1] = v.y;
+                    } else {
+                        bn[i][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, slot * rowb + lo, 0, 0));
+                    }
+                }
+        };
+        ldl(k0, L2);
+        issue(L2);  // operands of batch k0
+        ldl(k0 + 1, L1);
+        double4_m acc[NCH];
+        float4_m p0[NCH], p1[NCH];
+#pragma unroll
+        for (int g = 0; g < NCH; ++g) {
+            acc[g] = double4_m{0.0, 0.0, 0.0, 0.0};
+            p0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
+            p1[g] = p0[g];
+        }
+        // batch k: operands (an, bn) loaded; Lc = list of batch k + 1; Ln receives batch k + 2.
+        // A batch past the block's last (k = k1 on an odd count) runs with zero couplings.
+        auto step = [&](int k, const int4(&Lc)[4], int4(&Ln)[4]) {
+            const float sk = k < k1 ? sgn : 0.f;
+            float ac[8], bc[8][NCH];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                ac[i] = an[i] * sk;
+                if constexpr (NCH == 2) {
+                    // swapping the odd 16-lane rows of the re parts with the even rows of the im
+                    // parts leaves lane (m, bq) with part bq of columns r0 + m and r0 + 16 + m
+                    // (elements are copied out of the returned pair before reinterpreting them:
+                    // __builtin_bit_cast of a vector element reads element 0 with this compiler)
+                    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(bn[i][0]), __float_as_uint(bn[i][1]),
+                                                                     false, false);
+                    const uint32_t s0 = sw[0], s1 = sw[1];
+                    bc[i][0] = __uint_as_float(s0);
+                    bc[i][1] = __uint_as_float(s1);
+                } else {
+                    bc[i][0] = bn[i][0];
+                }
+            }
+            ldl(k + 2, Ln);
+            issue(Lc);
+            // the next batches' loads stay ahead of this batch's MFMAs: not sunk to their uses
+            // (memory clobber) and not scheduled after the MFMAs (scheduling barrier)
+            __asm__ volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+            float4_m f0[NCH], f1[NCH];
+#pragma unroll
+            for (int g = 0; g < NCH; ++g) {
+                f0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
+                f1[g] = f0[g];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i += 2)
+#pragma unroll
+                for (int g = 0; g < NCH; ++g) {
+                    f0[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i], bc[i][g], f0[g], 0, 0, 0);
+                    f1[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i + 1], bc[i + 1][g], f1[g], 0, 0, 0);
+                }
+#pragma unroll
+            for (int g = 0; g < NCH; ++g) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[g][v] += (double)p0[g][v] + (double)p1[g][v];
+                p0[g] = f0[g];
+                p1[g] = f1[g];
+            }
+        };
+        // two batches per trip, the list registers swapping roles (no copies that would wait for
+        // the loads in flight, no branch around the MFMAs)
+        for (int k = k0; k < k1; k += 2) {
+            step(k, L1, L2);
+            step(k + 1, L2, L1);
+        }
+#pragma unroll
+        for (int g = 0; g < NCH; ++g) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[g][v] += (double)p0[g][v] + (double)p1[g][v];
+            // FP32 16x16x4 result layout: rows 4 (l / 16) + v = (target 2 (l / 16) + v / 2, re / im v % 2)
+            const int col = r0 + 16 * g + m;
+            if (col < R) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int tgi = blk_tgt[blk * 8 + 2 * (l >> 4) + u];
+                    if (tgi >= 0) {
+                        float2 o;
+                        o.x = (float)acc[g][2 * u];
+                        o.y = (float)acc[g][2 * u + 1];
+                        *reinterpret_cast<float2 *>(Of + ((int64_t)tgi * R + col) * 2) = o;
+                    }
+                }
+            }
+        }
+        buf ^= 1;
+    }
+}
+
 // EA[f][q][m][b] from the level's coupling table: D[op(t, q)] at frequency f, through the
 // symmetry permutation of (t, q) for the symmetry-reduced kinds; zero where (t, q) does not couple.
 template <class EA>
@@ -454,7 +634,22 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
                         int R, hipStream_t s) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
     if (!LL.EA64.ptr) throw Error{H2G_EINVAL, "lattice coupling table missing"};
-    if (!f64 && R >= 16 && LL.list.n_tiles > 0 && LL.EA32.ptr) {
+    if (!f64 && R >= 16 && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB && LL.EA32.ptr &&
+        (int64_t)n_src * R * (int64_t)sizeof(cplx<float>) < INT32_MAX) {
+        const LatticeDirect &D = LL.direct;
+        // groups of consecutive blocks: at least one block per wave, and enough groups per
+        // frequency that the workgroups resident on an XCD share frequencies (L2 reuse)
+        const int gsz = std::max(DR_NT / 64, D.n_blocks / 1024);
+        const int ng = (D.n_blocks + gsz - 1) / gsz;
+        const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + 31) / 32));
+        auto go = [&](auto kern) {
+            kern<<<grid, DR_NT, 0, s>>>(D.n_blocks, gsz, D.ptr.as<int32_t>(), D.list.as<int4>(),
+                                        (int)(D.list.bytes / sizeof(int4)), D.tgt.as<int32_t>(), LL.EA32.as<float>(),
+                                        n_src, n_recv, (const cplx<float> *)S, (cplx<float> *)O, R);
+        };
+        if (R <= 16) conj ? go(m2l_direct_kernel<true, 1>) : go(m2l_direct_kernel<false, 1>);
+        else conj ? go(m2l_direct_kernel<true, 2>) : go(m2l_direct_kernel<false, 2>);
+    } else if (!f64 && R >= 16 && LL.list.n_tiles > 0 && LL.EA32.ptr) {
         const LatticeTiles &T = LL.list;
         const size_t sh = sizeof(cplx<float>) * (size_t)(LIST_NSLOT + 1) * 16 + sizeof(float) * 216 * 32 +
                           sizeof(int32_t) * (LIST_NB * LIST_QL * 2 + LIST_NB * 8 + LIST_NB + 1);

@@ -246,9 +246,10 @@ def test_lattice_toeplitz_matches_generic(gpu, kind, real):
             ref = ref.real if real else ref
             assert relerr(yl, ref) < RTOL, (op, nrhs)
             assert relerr(yl, yg) < RTOL, (op, nrhs)
-    # mixed form: narrow batches (1, 3, 8 columns) and a wide one (20: per-block slot lists)
-    mixed = h2g.H2Workspace(lat.core, max_rhs=20, precision="mixed")
-    for nrhs in (1, 3, 8, 20):
+    # mixed form: narrow batches (1, 3, 8 columns) and wide ones (16, 20, 40: per-block slot
+    # lists; 40 takes a second, partly filled chunk of 32 columns)
+    mixed = h2g.H2Workspace(lat.core, max_rhs=40, precision="mixed")
+    for nrhs in (1, 3, 8, 16, 20, 40):
         x = rng.standard_normal((n, nrhs)) + (0 if real else 1j * rng.standard_normal((n, nrhs)))
         for op in "NTC":
             ref = O.mul(core, x, op)
