@@ -861,7 +861,9 @@ void build_near(Core &c, const h2g_core_desc &d) {
 // ------------------------------------------------------------------------------------------
 bool is_toeplitz(const Core &c) { return c.m2l_kind == H2G_M2L_TOEPLITZ || c.m2l_kind == H2G_M2L_SYMTOEPLITZ; }
 
-void far_pass(Workspace &w, int op, int R) {
+// join: an event the first write of the result (L2P) waits for (the near field on the side
+// stream), or null
+void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     Core &c = *w.core;
     const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
     const Side &src = *P.src, &dst = *P.dst;
@@ -968,12 +970,17 @@ void far_pass(Workspace &w, int op, int R) {
             w.mark_end();
         }
         if (P.l2p[l].count) {
+            if (join) {
+                H2G_HIP(hipStreamWaitEvent(s, join, 0));
+                join = nullptr;
+            }
             w.mark_begin(8, l);
             launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(),
                        dst.d_end.as<int64_t>(), dcoll, K, w.loc.ptr, R, w.ys.ptr, a, s);
             w.mark_end();
         }
     }
+    if (join) H2G_HIP(hipStreamWaitEvent(s, join, 0));
 }
 
 }  // namespace
@@ -1164,6 +1171,9 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             w->opts = o;
             w->R_cap = o.max_rhs;
             H2G_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+            H2G_HIP(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
+            H2G_HIP(hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming));
+            H2G_HIP(hipEventCreateWithFlags(&w->ev_near, hipEventDisableTiming));
             const Core &c = *core;
             const int64_t nmax = std::max(c.test.n_dofs, c.trial->n_dofs);
             const int64_t bmax = std::max(c.test.n_boxes, c.trial->n_boxes);
@@ -1182,6 +1192,9 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
                 w->spec_out.alloc((size_t)nr * w->R_cap * c.L3 * cc);
             }
         } catch (...) {
+            if (w->ev_near) (void)hipEventDestroy(w->ev_near);
+            if (w->ev_fork) (void)hipEventDestroy(w->ev_fork);
+            if (w->side) (void)hipStreamDestroy(w->side);
             if (w->stream) (void)hipStreamDestroy(w->stream);
             delete w;
             throw;
@@ -1195,11 +1208,15 @@ int h2g_ws_destroy(h2g_ws *ws) {
     if (!ws) return H2G_EINVAL;
     (void)hipSetDevice(ws->core->device);
     if (ws->stream) (void)hipStreamSynchronize(ws->stream);
+    if (ws->side) (void)hipStreamSynchronize(ws->side);
     for (auto &kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto &e : ws->events) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    if (ws->ev_near) (void)hipEventDestroy(ws->ev_near);
+    if (ws->ev_fork) (void)hipEventDestroy(ws->ev_fork);
+    if (ws->side) (void)hipStreamDestroy(ws->side);
     if (ws->stream) (void)hipStreamDestroy(ws->stream);
     h2g_core *c = static_cast<h2g_core *>(ws->core);
     delete ws;
@@ -1328,12 +1345,22 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
             ws->mark_end();
             auto body = [&]() {
                 launch_fill_zero(ws->ys.ptr, (size_t)dst.n_dofs * R * se, s);
+                if ((which & H2G_NEARMUL) && (which & H2G_FARMUL) && !ws->opts.timing) {
+                    // the near field (which sets its rows of the result) on the side stream,
+                    // concurrent with the far field up to its first write of the result
+                    H2G_HIP(hipEventRecord(ws->ev_fork, s));
+                    H2G_HIP(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
+                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, ws->side);
+                    H2G_HIP(hipEventRecord(ws->ev_near, ws->side));
+                    far_pass(*ws, op, R, ws->ev_near);
+                    return;
+                }
                 if (which & H2G_NEARMUL) {
                     ws->mark_begin(1, -1);
                     launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, s);
                     ws->mark_end();
                 }
-                if (which & H2G_FARMUL) far_pass(*ws, op, R);
+                if (which & H2G_FARMUL) far_pass(*ws, op, R, nullptr);
             };
             if (use_graph) {
                 GraphKey key{op, which, R};

@@ -315,6 +315,10 @@ struct Workspace {
     h2g_ws_opts opts{};
     std::atomic<bool> busy{false};
     hipStream_t stream = nullptr;
+    // the near field runs on `side` (default priority) while the far field's upward pass and
+    // M2L run on `stream`; the first far-field write of the result waits for it (ev_near)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_near = nullptr;
     int R_cap = 1;
     DevBuf xs, ys;        // tree-ordered FP64/C64, [n][R]
     DevBuf mom, loc;      // [box][R][K] compute type

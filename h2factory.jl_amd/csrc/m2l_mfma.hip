@@ -637,9 +637,11 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
     if (!f64 && R >= 16 && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB && LL.EA32.ptr &&
         (int64_t)n_src * R * (int64_t)sizeof(cplx<float>) < INT32_MAX) {
         const LatticeDirect &D = LL.direct;
-        // groups of consecutive blocks: at least one block per wave, and enough groups per
-        // frequency that the workgroups resident on an XCD share frequencies (L2 reuse)
-        const int gsz = std::max(DR_NT / 64, D.n_blocks / 1024);
+        // groups of consecutive blocks: at least two blocks per wave (the coupling table is
+        // loaded once per group), and about 256 groups per frequency, so that the workgroups
+        // resident on an XCD work on few frequencies at a time (L2 reuse of source rows); the
+        // group size was chosen by measurement on the N = 2M, p = 6 map (1024 .. 64 groups)
+        const int gsz = std::max(2 * DR_NT / 64, D.n_blocks / 256);
         const int ng = (D.n_blocks + gsz - 1) / gsz;
         const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + 31) / 32));
         auto go = [&](auto kern) {
