@@ -87,7 +87,7 @@ def algorithmic_work(arrays, nrhs, phase, level, mixed):
 def traffic_of(kernel, n, p, nrhs):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE measurement of this workload (scaled by the copy-kernel calibration), else None."""
-    path = os.path.join(ROOT, "profiles", "m2l_list_traffic_n2m_p6_r32.json")
+    path = os.path.join(ROOT, "profiles", "m2l_direct_traffic_n2m_p6_r32.json")
     if (n, p, nrhs) != (2_000_000, 6, 32) or not os.path.exists(path):
         return None
     with open(path) as fh:
