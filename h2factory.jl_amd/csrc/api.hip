@@ -192,6 +192,58 @@ bool extract_factors(const V *tr, int n_levels, int K, int p, std::vector<V> &fa
     return true;
 }
 
+// Shared factors: when each sector's transfer is C_z[bit z] (x) C_y[bit y] (x) C_x[bit x] with one
+// p x p factor per axis and child bit (interpolation transfers), rewrite the per-sector factors of
+// every level as those shared ones (same values in every sector with the same bit), so that
+// kernels may share line stages between sectors.  The per-sector factors fix each sector up to
+// a scale per axis; C_a[1] takes the scales of the other two axes relative to sector 0.  Returns
+// false (factors unchanged) unless every sector is reproduced to 1e-12.
+template <class V>
+bool share_factors(const V *tr, int n_levels, int K, int p, std::vector<V> &fac) {
+    auto mag = [](const V &v) { return std::abs(v); };
+    const int pp = p * p;
+    std::vector<V> out(fac);
+    for (int l = 1; l < n_levels; ++l) {
+        auto Fs = [&](int sec, int ax) { return fac.data() + (((size_t)l * 8 + sec) * 3 + ax) * pp; };
+        auto ratio = [&](const V *X, const V *Y) {  // X = ratio * Y
+            int best = 0;
+            for (int i = 1; i < pp; ++i)
+                if (mag(Y[i]) > mag(Y[best])) best = i;
+            return X[best] / Y[best];
+        };
+        std::vector<V> C(3 * 2 * pp);  // [axis][bit][p x p]
+        for (int ax = 0; ax < 3; ++ax) {
+            std::copy(Fs(0, ax), Fs(0, ax) + pp, C.begin() + (ax * 2) * pp);
+            V sc = V(1);
+            for (int o = 0; o < 3; ++o)
+                if (o != ax) sc *= ratio(Fs(1 << ax, o), Fs(0, o));
+            for (int i = 0; i < pp; ++i) C[(ax * 2 + 1) * pp + i] = Fs(1 << ax, ax)[i] * sc;
+        }
+        for (int sec = 0; sec < 8; ++sec) {
+            const V *T = tr + ((size_t)l * 8 + sec) * K * K;
+            double mx = 0.0;
+            for (size_t i = 0; i < (size_t)K * K; ++i) mx = std::max(mx, (double)mag(T[i]));
+            const V *Cx = C.data() + (0 * 2 + (sec & 1)) * pp, *Cy = C.data() + (1 * 2 + ((sec >> 1) & 1)) * pp,
+                    *Cz = C.data() + (2 * 2 + ((sec >> 2) & 1)) * pp;
+            for (int c = 0; c < p; ++c)
+                for (int f = 0; f < p; ++f)
+                    for (int b = 0; b < p; ++b)
+                        for (int e = 0; e < p; ++e)
+                            for (int a = 0; a < p; ++a)
+                                for (int d = 0; d < p; ++d) {
+                                    const V v = Cx[a + p * d] * Cy[b + p * e] * Cz[c + p * f];
+                                    const V t = T[(a + p * (b + p * c)) + (size_t)K * (d + p * (e + p * f))];
+                                    if (mag(v - t) > 1e-12 * mx) return false;
+                                }
+            std::copy(Cx, Cx + pp, out.begin() + (((size_t)l * 8 + sec) * 3 + 0) * pp);
+            std::copy(Cy, Cy + pp, out.begin() + (((size_t)l * 8 + sec) * 3 + 1) * pp);
+            std::copy(Cz, Cz + pp, out.begin() + (((size_t)l * 8 + sec) * 3 + 2) * pp);
+        }
+    }
+    fac.swap(out);
+    return true;
+}
+
 void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
     S.n_dofs = d.n_dofs;
     S.n_levels = d.n_levels;
@@ -226,7 +278,10 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
             if (dtype == H2G_F64) {
                 std::vector<double> fac;
                 ok = extract_factors((const double *)d.transfers, d.n_levels, K, p, fac);
-                if (ok) S.factors.upload_vec(fac);
+                if (ok) {
+                    S.sep_shared = share_factors((const double *)d.transfers, d.n_levels, K, p, fac);
+                    S.factors.upload_vec(fac);
+                }
                 nf = fac.size();
             } else {
                 std::vector<std::complex<double>> fac;
@@ -961,7 +1016,7 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
                 launch_sep_l2l(cd, conj ? 2 : 1, P.l2l_par[l].count, P.l2lp_box.as<int32_t>() + P.l2l_par[l].begin,
                                P.l2lp_ptr.as<int64_t>() + P.l2lp_base[l], P.l2lp_child.as<int32_t>(),
                                P.l2lp_mat.as<int32_t>(), P.l2lp_acc.as<uint8_t>(),
-                               mixed ? dst.factors32.ptr : dst.factors.ptr, l * 8, dst.sep_p, w.loc.ptr, R, s);
+                               mixed ? dst.factors32.ptr : dst.factors.ptr, l * 8, dst.sep_p, dst.sep_shared, w.loc.ptr, R, s);
             else
                 launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
                                P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),

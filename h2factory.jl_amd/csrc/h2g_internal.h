@@ -177,6 +177,7 @@ struct Side {
     DevBuf coll, coll32;            // K x n_dofs core type / reduced type
     DevBuf transfers, transfers32;  // [n_levels][8] K x K
     bool separable = false;         // every transfer is Az (x) Ay (x) Ax
+    bool sep_shared = false;        // ... with one factor per axis and child bit (stored per sector)
     int sep_p = 0;
     DevBuf factors, factors32;      // [n_levels*8][3][p][p]
     int level_of(int b) const {
@@ -363,7 +364,7 @@ void launch_hadamard(int cdtype, bool conj, int n_recv, const int64_t *item_ptr,
 void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 void launch_sep_l2l(int cdtype, int opA, int n_parents, const int32_t *parents, const int64_t *child_ptr,
                     const int32_t *child_box, const int32_t *child_mat, const uint8_t *child_acc, const void *factors,
-                    int mat_base, int p, void *loc, int R, hipStream_t s);
+                    int mat_base, int p, bool shared, void *loc, int R, hipStream_t s);
 void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
                          const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,
                          const void *in, void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s);

@@ -904,13 +904,149 @@ __global__ void sep_l2l_kernel(int opA, int n_par, const int32_t *__restrict__ p
     }
 }
 
+// L2L grouped by parent when the Kronecker factors are shared between sectors (the x factor
+// depends on the child's x bit only, and so on: interpolation transfers).  The children's
+// transforms then share stages: 2 x-line variants (child x bit), 4 y variants (x, y bits) and 8
+// z variants, 14 line batches instead of 24, all in one pass with two barriers.  The children's
+// current expansions (accumulated when they already hold couplings) are loaded at the start and
+// used by the last stage, so their latency overlaps the first two stages.
+template <class T, int JB, int P>
+__global__ void __launch_bounds__(TPB) sep_l2l_shared_kernel(int opA, int n_par, const int32_t *__restrict__ parents,
+                                                            const int64_t *__restrict__ child_ptr,
+                                                            const int32_t *__restrict__ child_box,
+                                                            const int32_t *__restrict__ child_mat,
+                                                            const uint8_t *__restrict__ child_acc,
+                                                            const T *__restrict__ factors, int mat_base,
+                                                            T *__restrict__ loc, int R) {
+    constexpr int p = P, K = P * P * P, LINES = P * P;
+    constexpr int ZL = JB * 8 * LINES, ZPT = (ZL + TPB - 1) / TPB;  // z lines, per thread
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *pin = reinterpret_cast<T *>(smem_raw);  // [JB][K] parent
+    T *t1 = pin + JB * K;                      // [JB][2][K] x stage per child x bit
+    T *t2 = t1 + JB * 2 * K;                   // [JB][4][K] y stage per child (x, y) bits
+    T *fs = t2 + JB * 4 * K;                   // [8 sectors][3 axes][p out][p in]
+    int *s_box = reinterpret_cast<int *>(fs + 8 * 3 * p * p);  // [JB][8] child box of a sector, or -1
+    int *s_acc = s_box + JB * 8;                                // [JB][8]
+    int *s_r = s_acc + JB * 8;                                  // [JB]
+    const int64_t job0 = (int64_t)blockIdx.x * JB;
+    const int64_t njobs = (int64_t)n_par * R;
+    for (int e = threadIdx.x; e < JB * 8; e += blockDim.x) {
+        s_box[e] = -1;
+        s_acc[e] = 0;
+    }
+    for (int e = threadIdx.x; e < 8 * 3 * p * p; e += blockDim.x) {
+        const int sa = e / (p * p), oi = e - sa * p * p;
+        const int o = oi / p, i = oi - o * p;
+        fs[e] = sep_factor(factors + ((int64_t)mat_base * 3 + sa) * p * p, p, o, i, opA);
+    }
+    __syncthreads();
+    if (threadIdx.x < JB) {
+        const int64_t job = job0 + threadIdx.x;
+        s_r[threadIdx.x] = 0;
+        if (job < njobs) {
+            const int pi = (int)(job / R);
+            s_r[threadIdx.x] = (int)(job - (int64_t)pi * R);
+            for (int64_t c = child_ptr[pi]; c < child_ptr[pi + 1]; ++c) {
+                const int sec = child_mat[c] - mat_base;
+                s_box[threadIdx.x * 8 + sec] = child_box[c];
+                s_acc[threadIdx.x * 8 + sec] = child_acc[c];
+            }
+        }
+    }
+    // (the job tables above are written by threads 0 .. JB-1 in this same phase: recompute r here)
+    for (int e = threadIdx.x; e < JB * K; e += blockDim.x) {
+        const int j = e / K, k = e - j * K;
+        const int64_t job = job0 + j;
+        pin[e] = job < njobs ? loc[(int64_t)parents[job / R] * R * K + (job % R) * K + k] : zero_v<T>();
+    }
+    __syncthreads();
+    // the children's current values for the last stage, loaded now
+    T pre[ZPT][p];
+#pragma unroll
+    for (int u = 0; u < ZPT; ++u) {
+        const int e = threadIdx.x + u * TPB;
+        const int js = e / LINES, line = e - js * LINES;  // js = j * 8 + sector
+        const int box = e < ZL ? s_box[js] : -1;
+        const bool acc = box >= 0 && s_acc[js] != 0;
+        const T *src = loc + ((int64_t)(box < 0 ? 0 : box) * R + s_r[js >> 3]) * K + line;
+#pragma unroll
+        for (int o = 0; o < p; ++o) pre[u][o] = acc ? src[LINES * o] : zero_v<T>();
+    }
+    // x: parent line (jy, jz) -> t1[j][bx][kx + p*(jy + p*jz)]
+    for (int e = threadIdx.x; e < JB * 2 * LINES; e += blockDim.x) {
+        const int jb = e / LINES, line = e - jb * LINES;  // jb = j * 2 + bx
+        const int j = jb >> 1, bx = jb & 1;
+        const T *F = fs + (bx * 3 + 0) * p * p;
+        T v[p];
+#pragma unroll
+        for (int i = 0; i < p; ++i) v[i] = pin[j * K + p * line + i];
+#pragma unroll
+        for (int o = 0; o < p; ++o) {
+            T a = zero_v<T>();
+#pragma unroll
+            for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
+            t1[jb * K + o + p * line] = a;
+        }
+    }
+    __syncthreads();
+    // y: line (kx, jz) of t1[j][bx] -> t2[j][bxy][kx + p*(ky + p*jz)], bxy = bx + 2 by
+    for (int e = threadIdx.x; e < JB * 4 * LINES; e += blockDim.x) {
+        const int jb = e / LINES, line = e - jb * LINES;  // jb = j * 4 + bxy
+        const int j = jb >> 2, bxy = jb & 3;
+        const int kx = line % p, jz = line / p;
+        const T *F = fs + (bxy * 3 + 1) * p * p;
+        const T *src = t1 + (j * 2 + (bxy & 1)) * K;
+        T v[p];
+#pragma unroll
+        for (int i = 0; i < p; ++i) v[i] = src[kx + p * (i + p * jz)];
+#pragma unroll
+        for (int o = 0; o < p; ++o) {
+            T a = zero_v<T>();
+#pragma unroll
+            for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
+            t2[jb * K + kx + p * (o + p * jz)] = a;
+        }
+    }
+    __syncthreads();
+    // z: line (kx, ky) of t2[j][s % 4] -> child expansion of sector s
+#pragma unroll
+    for (int u = 0; u < ZPT; ++u) {
+        const int e = threadIdx.x + u * TPB;
+        if (e >= ZL) break;
+        const int js = e / LINES, line = e - js * LINES;
+        const int box = s_box[js];
+        if (box < 0) continue;
+        const int j = js >> 3, sec = js & 7;
+        const T *F = fs + (sec * 3 + 2) * p * p;
+        const T *src = t2 + (j * 4 + (sec & 3)) * K + line;
+        T v[p];
+#pragma unroll
+        for (int i = 0; i < p; ++i) v[i] = src[LINES * i];
+        T *out = loc + ((int64_t)box * R + s_r[j]) * K + line;
+#pragma unroll
+        for (int o = 0; o < p; ++o) {
+            T a = pre[u][o];
+#pragma unroll
+            for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], v[i]);
+            out[LINES * o] = a;
+        }
+    }
+}
+
 template <class T, int P>
 static void sep_l2l_p(int opA, int n_par, const int32_t *parents, const int64_t *child_ptr, const int32_t *child_box,
                       const int32_t *child_mat, const uint8_t *child_acc, const void *factors, int mat_base, void *loc,
-                      int R, hipStream_t s) {
+                      int R, bool shared, hipStream_t s) {
     constexpr int JB = 64 / (int)sizeof(T);
     constexpr int K = P * P * P;
     const int64_t njobs = (int64_t)n_par * R;
+    if (shared) {
+        constexpr int JS = 32 / (int)sizeof(T);  // jobs per workgroup: 8 (FP32), 4 (FP64)
+        const size_t shs = sizeof(T) * (size_t)(7 * JS * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(17 * JS);
+        sep_l2l_shared_kernel<T, JS, P><<<(unsigned)((njobs + JS - 1) / JS), TPB, shs, s>>>(
+            opA, n_par, parents, child_ptr, child_box, child_mat, child_acc, (const T *)factors, mat_base, (T *)loc, R);
+        return;
+    }
     const size_t sh = sizeof(T) * (size_t)(3 * JB * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(3 * JB + 1);
     sep_l2l_kernel<T, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
         opA, n_par, parents, child_ptr, child_box, child_mat, child_acc, (const T *)factors, mat_base, (T *)loc, R);
@@ -918,15 +1054,15 @@ static void sep_l2l_p(int opA, int n_par, const int32_t *parents, const int64_t 
 
 void launch_sep_l2l(int cdtype, int opA, int n_parents, const int32_t *parents, const int64_t *child_ptr,
                     const int32_t *child_box, const int32_t *child_mat, const uint8_t *child_acc, const void *factors,
-                    int mat_base, int p, void *loc, int R, hipStream_t s) {
+                    int mat_base, int p, bool shared, void *loc, int R, hipStream_t s) {
     if (n_parents == 0) return;
     DISPATCH_T(cdtype, {
         switch (p) {
-        case 2: sep_l2l_p<T, 2>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
-        case 3: sep_l2l_p<T, 3>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
-        case 4: sep_l2l_p<T, 4>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
-        case 5: sep_l2l_p<T, 5>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
-        case 6: sep_l2l_p<T, 6>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, s); break;
+        case 2: sep_l2l_p<T, 2>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, shared, s); break;
+        case 3: sep_l2l_p<T, 3>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, shared, s); break;
+        case 4: sep_l2l_p<T, 4>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, shared, s); break;
+        case 5: sep_l2l_p<T, 5>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, shared, s); break;
+        case 6: sep_l2l_p<T, 6>(opA, n_parents, parents, child_ptr, child_box, child_mat, child_acc, factors, mat_base, loc, R, shared, s); break;
         default: throw Error{H2G_EUNSUPPORTED, "separable transfers support 2 <= p <= 6"};
         }
     });
