@@ -944,7 +944,12 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
         }
         if (P.m2m[l].count) {
             w.mark_begin(3, l);
-            if (src.separable)
+            if (src.separable && src.sep_shared)
+                launch_sep_m2m(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
+                               P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
+                               P.m2m_mat.as<int32_t>(), mixed ? src.factors32.ptr : src.factors.ptr, (l + 1) * 8,
+                               src.sep_p, w.mom.ptr, R, s);
+            else if (src.separable)
                 launch_sep_transfer(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
                                     P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
                                     P.m2m_mat.as<int32_t>(), mixed ? src.factors32.ptr : src.factors.ptr, (l + 1) * 8,

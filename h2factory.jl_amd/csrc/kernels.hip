@@ -1033,6 +1033,139 @@ __global__ void __launch_bounds__(TPB) sep_l2l_shared_kernel(int opA, int n_par,
     }
 }
 
+// M2M of all children of a parent in one pass, for shared factors: the parent's moment is
+// sum_s Cz[sz] (x) Cy[sy] (x) Cx[sx] m_s = sum_sz Cz[sz] (sum_sy Cy[sy] (sum_sx Cx[sx] m_s)), so the
+// x stage reads the two children of each (y, z) bit pair and sums them (4 results per parent),
+// the y stage sums over y bits (2 results), the z stage over z bits: 14 line batches, two
+// barriers.  The children's lines are loaded at the start, all at once.  A missing child (empty
+// sector) contributes zero.  The parent's moment is overwritten.
+template <class T, int JB, int P>
+__global__ void __launch_bounds__(TPB) sep_m2m_shared_kernel(int opA, int n_targets, const int32_t *__restrict__ targets,
+                                                            const int64_t *__restrict__ item_ptr,
+                                                            const int32_t *__restrict__ item_src,
+                                                            const int32_t *__restrict__ item_mat,
+                                                            const T *__restrict__ factors, int mat_base,
+                                                            T *__restrict__ mom, int R) {
+    constexpr int p = P, K = P * P * P, LINES = P * P;
+    constexpr int XL = JB * 4 * LINES, XPT = (XL + TPB - 1) / TPB;  // x items, per thread
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *w = reinterpret_cast<T *>(smem_raw);  // [JB][4 (y, z bits)][K]
+    T *v = w + JB * 4 * K;                   // [JB][2 (z bit)][K]
+    T *fs = v + JB * 2 * K;                  // [8 sectors][3 axes][p out][p in]
+    int *s_box = reinterpret_cast<int *>(fs + 8 * 3 * p * p);  // [JB][8] child box of a sector, or -1
+    const int64_t job0 = (int64_t)blockIdx.x * JB;
+    const int64_t njobs = (int64_t)n_targets * R;
+    for (int e = threadIdx.x; e < JB * 8; e += blockDim.x) s_box[e] = -1;
+    for (int e = threadIdx.x; e < 8 * 3 * p * p; e += blockDim.x) {
+        const int sa = e / (p * p), oi = e - sa * p * p;
+        const int o = oi / p, i = oi - o * p;
+        fs[e] = sep_factor(factors + ((int64_t)mat_base * 3 + sa) * p * p, p, o, i, opA);
+    }
+    __syncthreads();
+    if (threadIdx.x < JB) {
+        const int64_t job = job0 + threadIdx.x;
+        if (job < njobs) {
+            const int ti = (int)(job / R);
+            for (int64_t it = item_ptr[ti]; it < item_ptr[ti + 1]; ++it)
+                s_box[threadIdx.x * 8 + (item_mat[it] - mat_base)] = item_src[it];
+        }
+    }
+    __syncthreads();
+    // x: child lines (jy, jz) of sectors (0, syz) and (1, syz) -> w[j][syz][kx + p*(jy + p*jz)]
+    T in[XPT][2][p];
+#pragma unroll
+    for (int u = 0; u < XPT; ++u) {
+        const int e = threadIdx.x + u * TPB;
+        const int jq = e / LINES, line = e - jq * LINES;  // jq = j * 4 + syz
+        const int j = jq >> 2, syz = jq & 3;
+        const int r = (int)((job0 + j) % R);
+#pragma unroll
+        for (int sx = 0; sx < 2; ++sx) {
+            const int box = e < XL ? s_box[j * 8 + sx + 2 * syz] : -1;
+            const T *src = mom + ((int64_t)(box < 0 ? 0 : box) * R + r) * K + p * line;
+#pragma unroll
+            for (int i = 0; i < p; ++i) in[u][sx][i] = box >= 0 ? src[i] : zero_v<T>();
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < XPT; ++u) {
+        const int e = threadIdx.x + u * TPB;
+        if (e >= XL) break;
+        const int jq = e / LINES, line = e - jq * LINES;
+#pragma unroll
+        for (int o = 0; o < p; ++o) {
+            T a = zero_v<T>();
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx) {
+                const T *F = fs + (sx * 3 + 0) * p * p;
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a, F[o * p + i], in[u][sx][i]);
+            }
+            w[jq * K + o + p * line] = a;
+        }
+    }
+    __syncthreads();
+    // y: lines (kx, jz) of w[j][sy + 2 sz] -> v[j][sz][kx + p*(ky + p*jz)], summed over sy
+    for (int e = threadIdx.x; e < JB * 2 * LINES; e += blockDim.x) {
+        const int jz2 = e / LINES, line = e - jz2 * LINES;  // jz2 = j * 2 + sz
+        const int j = jz2 >> 1, sz = jz2 & 1;
+        const int kx = line % p, jz = line / p;
+        T a[p];
+#pragma unroll
+        for (int o = 0; o < p; ++o) a[o] = zero_v<T>();
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy) {
+            const T *F = fs + ((2 * sy) * 3 + 1) * p * p;
+            const T *src = w + (j * 4 + sy + 2 * sz) * K;
+            T x[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) x[i] = src[kx + p * (i + p * jz)];
+#pragma unroll
+            for (int o = 0; o < p; ++o)
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a[o], F[o * p + i], x[i]);
+        }
+#pragma unroll
+        for (int o = 0; o < p; ++o) v[jz2 * K + kx + p * (o + p * jz)] = a[o];
+    }
+    __syncthreads();
+    // z: lines (kx, ky) of v[j][sz] -> the parent's moment, summed over sz
+    for (int e = threadIdx.x; e < JB * LINES; e += blockDim.x) {
+        const int j = e / LINES, line = e - j * LINES;
+        const int64_t job = job0 + j;
+        if (job >= njobs) continue;
+        T a[p];
+#pragma unroll
+        for (int o = 0; o < p; ++o) a[o] = zero_v<T>();
+#pragma unroll
+        for (int sz = 0; sz < 2; ++sz) {
+            const T *F = fs + ((4 * sz) * 3 + 2) * p * p;
+            const T *src = v + (j * 2 + sz) * K + line;
+            T x[p];
+#pragma unroll
+            for (int i = 0; i < p; ++i) x[i] = src[LINES * i];
+#pragma unroll
+            for (int o = 0; o < p; ++o)
+#pragma unroll
+                for (int i = 0; i < p; ++i) fmacc(a[o], F[o * p + i], x[i]);
+        }
+        T *out = mom + ((int64_t)targets[job / R] * R + job % R) * K + line;
+#pragma unroll
+        for (int o = 0; o < p; ++o) out[LINES * o] = a[o];
+    }
+}
+
+template <class T, int P>
+static void sep_m2m_p(int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr, const int32_t *item_src,
+                      const int32_t *item_mat, const void *factors, int mat_base, void *mom, int R, hipStream_t s) {
+    constexpr int JS = 32 / (int)sizeof(T);  // jobs per workgroup: 8 (FP32), 4 (FP64)
+    constexpr int K = P * P * P;
+    const int64_t njobs = (int64_t)n_targets * R;
+    const size_t sh = sizeof(T) * (size_t)(6 * JS * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(8 * JS);
+    sep_m2m_shared_kernel<T, JS, P><<<(unsigned)((njobs + JS - 1) / JS), TPB, sh, s>>>(
+        opA, n_targets, targets, item_ptr, item_src, item_mat, (const T *)factors, mat_base, (T *)mom, R);
+}
+
 template <class T, int P>
 static void sep_l2l_p(int opA, int n_par, const int32_t *parents, const int64_t *child_ptr, const int32_t *child_box,
                       const int32_t *child_mat, const uint8_t *child_acc, const void *factors, int mat_base, void *loc,
@@ -1050,6 +1183,23 @@ static void sep_l2l_p(int opA, int n_par, const int32_t *parents, const int64_t 
     const size_t sh = sizeof(T) * (size_t)(3 * JB * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(3 * JB + 1);
     sep_l2l_kernel<T, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
         opA, n_par, parents, child_ptr, child_box, child_mat, child_acc, (const T *)factors, mat_base, (T *)loc, R);
+}
+
+void launch_sep_m2m(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
+                    const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,
+                    void *mom, int R, hipStream_t s) {
+    if (n_targets == 0) return;
+    DISPATCH_T(cdtype, {
+        switch (p) {
+        case 2: sep_m2m_p<T, 2>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, mom, R, s); break;
+        case 3: sep_m2m_p<T, 3>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, mom, R, s); break;
+        case 4: sep_m2m_p<T, 4>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, mom, R, s); break;
+        case 5: sep_m2m_p<T, 5>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, mom, R, s); break;
+        case 6: sep_m2m_p<T, 6>(opA, n_targets, targets, item_ptr, item_src, item_mat, factors, mat_base, mom, R, s); break;
+        default: throw Error{H2G_EUNSUPPORTED, "separable transfers support 2 <= p <= 6"};
+        }
+    });
+    H2G_CHECK_LAUNCH();
 }
 
 void launch_sep_l2l(int cdtype, int opA, int n_parents, const int32_t *parents, const int64_t *child_ptr,
