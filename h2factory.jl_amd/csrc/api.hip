@@ -369,6 +369,15 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
     span_of(P.m2l_recv, dst.n_levels, [&](int b) { return (bool)has_m2l[b]; }, dst);
     span_of(P.l2l, dst.n_levels, [&](int b) { return dst.parent[b] >= 0 && init[dst.parent[b]]; }, dst);
     span_of(P.l2p, dst.n_levels, [&](int b) { return init[b] && dst.is_leaf[b]; }, dst);
+    P.l2p_avg.assign(dst.n_levels, 0);
+    for (int l = 0; l < dst.n_levels; ++l) {
+        int64_t pts = 0;
+        for (int i = 0; i < P.l2p[l].count; ++i) {
+            const int b = lists[P.l2p[l].begin + i];
+            pts += dst.end[b] - dst.begin[b];
+        }
+        if (P.l2p[l].count) P.l2p_avg[l] = (int32_t)(pts / P.l2p[l].count);
+    }
     P.lists.upload_vec(lists);
 
     // M2M items: children of a box are contiguous on the next level
@@ -1036,7 +1045,7 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
             }
             w.mark_begin(8, l);
             launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(),
-                       dst.d_end.as<int64_t>(), dcoll, K, w.loc.ptr, R, w.ys.ptr, a, s);
+                       dst.d_end.as<int64_t>(), dcoll, K, w.loc.ptr, R, w.ys.ptr, a, P.l2p_avg[l], s);
             w.mark_end();
         }
     }

@@ -253,6 +253,7 @@ struct OrientPlan {
     const Side *src = nullptr, *dst = nullptr;
     DevBuf lists;                       // concatenated int32 box lists
     std::vector<Span> p2m, m2m, l2l, l2p, m2l_recv;  // per level
+    std::vector<int32_t> l2p_avg;                    // per level: mean points per L2P leaf
     // M2M items: CSR over m2m targets (all levels concatenated); item = (src child box, matrix index)
     DevBuf m2m_ptr, m2m_src, m2m_mat;
     std::vector<int64_t> m2m_ptr_base;  // per level: offset of that level's targets into m2m_ptr
@@ -348,7 +349,7 @@ void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys
 void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                 const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s);
 void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
-                const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, hipStream_t s);
+                const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, int avg_points, hipStream_t s);
 // out[t] = (acc ? out[t] : 0) + sum_items opA(mats[item_mat]) * in[item_src]  (optionally permuted)
 // acc_mode: 0 overwrite, 1 accumulate, 2 accumulate when acc_flags[item_ptr[t]] (first item of t)
 // opA: 0 = A, 1 = A^T, 2 = A^H, 3 = conj(A)

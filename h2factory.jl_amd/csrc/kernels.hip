@@ -565,7 +565,8 @@ void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys
 bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                      const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s);
 bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
-                     const void *coll, int K, const void *loc, int R, void *ys, double alpha, hipStream_t s);
+                     const void *coll, int K, const void *loc, int R, void *ys, double alpha, bool stage,
+                     hipStream_t s);
 
 void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                 const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
@@ -580,9 +581,12 @@ void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64
 }
 
 void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
-                const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, hipStream_t s) {
+                const void *coll, int K, const void *loc, int R, void *ys, cplx<double> alpha, int avg_points,
+                hipStream_t s) {
     if (n == 0) return;
-    if (alpha.im == 0.0 && launch_l2p_mfma(cdtype, leaves, n, begin, end, coll, K, loc, R, ys, alpha.re, s)) return;
+    if (alpha.im == 0.0 &&
+        launch_l2p_mfma(cdtype, leaves, n, begin, end, coll, K, loc, R, ys, alpha.re, avg_points > 16, s))
+        return;
     DISPATCH_T(cdtype, {
         // RHS per LDS pass: all of them when K x R fits in 48 KB
         const int rc = std::max(1, std::min(R, (int)(49152 / (sizeof(T) * (size_t)K))));
@@ -1383,18 +1387,35 @@ __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__res
 // k-steps are 4 consecutive values (index 16 j + 4 k' + u for lane row k' and k-step 4 j + u):
 // the contraction order does not change the product, and every lane reads its operand rows
 // contiguously.  One workgroup (4 waves) per (leaf, group of NCT 16-column RHS tiles).
-template <class T, int NCT>
+template <class T, int NCT, bool STAGE>
 __global__ void __launch_bounds__(TPB, 2)
     l2p_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                     const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const T *__restrict__ loc,
                     int R, double *__restrict__ ys, double alpha) {
+    // STAGE (leaves of many points): the leaf's expansions of this workgroup's columns, [col][k]
+    // with a row pitch of K + 1, read from the [box][col][k] layout with coalesced loads (the
+    // MFMA operand order reads 16 columns K values apart); small leaves read them directly, where
+    // the staging would cost more than the operand loads it saves
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lt = reinterpret_cast<T *>(smem_raw);
+    const int KP = K + 1;
     const int b = leaves[blockIdx.x];
     const int64_t j0 = begin[b];
     const int n = (int)(end[b] - j0);
     const int c0 = blockIdx.y * NCT * 16;
+    const int ncol = min(NCT * 16, R - c0);
+    const T *lb = loc + ((int64_t)b * R + c0) * K;
+    if (STAGE) {
+        for (int e = threadIdx.x; e < ncol * K; e += TPB) {
+            const int c = e / K, k = e - c * K;
+            lt[c * KP + k] = lb[e];
+        }
+        __syncthreads();
+    }
+    const T *lsrc = STAGE ? lt : lb;
+    const int pitch = STAGE ? KP : K;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
-    const T *lb = loc + (int64_t)b * K * R;
     for (int rt = wave; rt * 16 < n; rt += TPB / 64) {
         const int row = rt * 16 + m;
         const T *crow = coll + (j0 + min(row, n - 1)) * K;
@@ -1411,11 +1432,11 @@ __global__ void __launch_bounds__(TPB, 2)
                 a[u] = (double)crow[min(kb + u, K - 1)] * ((row < n && kb + u < K) ? 1.0 : 0.0);
 #pragma unroll
             for (int g = 0; g < NCT; ++g) {
-                const int col = c0 + 16 * g + m;
-                const T *lc = lb + (int64_t)min(col, R - 1) * K;
+                const int cl = min(16 * g + m, ncol - 1);
+                const T *lc = lsrc + cl * pitch;
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    bv[g][u] = (double)lc[min(kb + u, K - 1)] * ((col < R && kb + u < K) ? 1.0 : 0.0);
+                    bv[g][u] = (double)lc[min(kb + u, K - 1)] * ((16 * g + m < ncol && kb + u < K) ? 1.0 : 0.0);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -1442,13 +1463,18 @@ __global__ void __launch_bounds__(TPB, 2)
     p2m_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                     const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const double *__restrict__ xs,
                     int R, T *__restrict__ mom) {
+    // results go through LDS ([col][k], row pitch K + 1) and leave with coalesced stores in the
+    // [box][col][k] layout (the MFMA result order would write 16 columns K values apart)
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *mt = reinterpret_cast<T *>(smem_raw);
+    const int KP = K + 1;
     const int b = leaves[blockIdx.x];
     const int64_t j0 = begin[b];
     const int n = (int)(end[b] - j0);
     const int c0 = blockIdx.y * NCT * 16;
+    const int ncol = min(NCT * 16, R - c0);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
-    T *mb = mom + (int64_t)b * K * R;
     for (int rt = wave; rt * 16 < K; rt += TPB / 64) {
         const int krow = rt * 16 + m;
         double4_t acc[NCT];
@@ -1477,13 +1503,19 @@ __global__ void __launch_bounds__(TPB, 2)
         }
 #pragma unroll
         for (int g = 0; g < NCT; ++g) {
-            const int col = c0 + 16 * g + m;
+            const int cl = 16 * g + m;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int ko = rt * 16 + mf_d_row(lane, v);
-                if (ko < K && col < R) mb[(int64_t)col * K + ko] = (T)acc[g][v];
+                if (ko < K && cl < ncol) mt[cl * KP + ko] = (T)acc[g][v];
             }
         }
+    }
+    __syncthreads();
+    T *mb = mom + ((int64_t)b * R + c0) * K;
+    for (int e = threadIdx.x; e < ncol * K; e += TPB) {
+        const int c = e / K, k = e - c * K;
+        mb[e] = mt[c * KP + k];
     }
 }
 
@@ -1511,20 +1543,25 @@ __global__ void mfma_f32_probe_kernel(const float *__restrict__ A, const float *
 }  // namespace
 
 bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
-                     const void *coll, int K, const void *loc, int R, void *ys, double alpha, hipStream_t s) {
+                     const void *coll, int K, const void *loc, int R, void *ys, double alpha, bool stage,
+                     hipStream_t s) {
     if (R < 16 || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
-    const int nct = std::min(4, (R + 15) / 16);
+    // column tiles per workgroup: up to 4, as many as the LDS block of 64 KB allows
+    const size_t es = cdtype == H2G_F64 ? 8 : 4;
+    const int nct = std::min({4, (R + 15) / 16, stage ? (int)(65536 / (es * 16 * (size_t)(K + 1))) : 4});
+    if (nct < 1) return false;
     const dim3 grid((unsigned)n, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
     auto go = [&](auto kern, auto tv) {
         using T = decltype(tv);
-        kern<<<grid, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys, alpha);
+        const size_t sh = stage ? sizeof(T) * (size_t)nct * 16 * (K + 1) : 0;
+        kern<<<grid, TPB, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys, alpha);
     };
     const bool f64 = cdtype == H2G_F64;
     switch (nct) {
-    case 1: f64 ? go(l2p_mfma_kernel<double, 1>, 0.0) : go(l2p_mfma_kernel<float, 1>, 0.0f); break;
-    case 2: f64 ? go(l2p_mfma_kernel<double, 2>, 0.0) : go(l2p_mfma_kernel<float, 2>, 0.0f); break;
-    case 3: f64 ? go(l2p_mfma_kernel<double, 3>, 0.0) : go(l2p_mfma_kernel<float, 3>, 0.0f); break;
-    default: f64 ? go(l2p_mfma_kernel<double, 4>, 0.0) : go(l2p_mfma_kernel<float, 4>, 0.0f); break;
+    case 1: f64 ? go((stage ? l2p_mfma_kernel<double, 1, true> : l2p_mfma_kernel<double, 1, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 1, true> : l2p_mfma_kernel<float, 1, false>), 0.0f); break;
+    case 2: f64 ? go((stage ? l2p_mfma_kernel<double, 2, true> : l2p_mfma_kernel<double, 2, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 2, true> : l2p_mfma_kernel<float, 2, false>), 0.0f); break;
+    case 3: f64 ? go((stage ? l2p_mfma_kernel<double, 3, true> : l2p_mfma_kernel<double, 3, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 3, true> : l2p_mfma_kernel<float, 3, false>), 0.0f); break;
+    default: f64 ? go((stage ? l2p_mfma_kernel<double, 4, true> : l2p_mfma_kernel<double, 4, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 4, true> : l2p_mfma_kernel<float, 4, false>), 0.0f); break;
     }
     H2G_CHECK_LAUNCH();
     return true;
@@ -1533,11 +1570,15 @@ bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
 bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                      const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
     if (R < 16 || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
-    const int nct = std::min(4, (R + 15) / 16);
+    // column tiles per workgroup: up to 4, as many as the LDS block of 64 KB allows
+    const size_t es = cdtype == H2G_F64 ? 8 : 4;
+    const int nct = std::min({4, (R + 15) / 16, (int)(65536 / (es * 16 * (size_t)(K + 1)))});
+    if (nct < 1) return false;
     const dim3 grid((unsigned)n, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
     auto go = [&](auto kern, auto tv) {
         using T = decltype(tv);
-        kern<<<grid, TPB, 0, s>>>(leaves, begin, end, (const T *)coll, K, (const double *)xs, R, (T *)mom);
+        const size_t sh = sizeof(T) * (size_t)nct * 16 * (K + 1);
+        kern<<<grid, TPB, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const double *)xs, R, (T *)mom);
     };
     const bool f64 = cdtype == H2G_F64;
     switch (nct) {
