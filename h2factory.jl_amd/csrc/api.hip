@@ -996,7 +996,8 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
             if (LL.ok) {
                 const bool half = c.half_spectrum;
                 w.mark_begin(4, l);
-                launch_fft_in_fm(cd, half, inv_in, box0, nsrc, w.mom.ptr, K, c.p, c.L, R, w.spec_in.ptr,
+                const bool perm = m2l_direct_ok(cd, LL, nsrc, R);  // the direct kernel's column layout
+                launch_fft_in_fm(cd, half, inv_in, box0, nsrc, w.mom.ptr, K, c.p, c.L, R, perm, w.spec_in.ptr,
                                  c.twiddle.ptr, s);
                 w.mark_end();
                 w.mark_begin(5, l);
@@ -1257,7 +1258,9 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
                 const size_t cc = mixed ? 8 : 16;
                 const int64_t ns = std::max(c.fwd.max_src_level_boxes, c.rev.max_src_level_boxes);
                 const int64_t nr = std::max(c.fwd.max_recv_level_boxes, c.rev.max_recv_level_boxes);
-                w->spec_in.alloc((size_t)ns * w->R_cap * c.L3 * cc);
+                // source spectra of wide FP32 batches may be stored with rows padded to 32 columns
+                const int64_t rin = mixed && w->R_cap >= 16 ? (w->R_cap + 31) / 32 * 32 : w->R_cap;
+                w->spec_in.alloc((size_t)ns * rin * c.L3 * cc);
                 w->spec_out.alloc((size_t)nr * w->R_cap * c.L3 * cc);
             }
         } catch (...) {

@@ -36,8 +36,11 @@ __device__ inline void load_twiddles(C (&tw)[L], const cplx<double> *__restrict_
     }
 }
 
+// Rrow: stored columns per source row (R, or R rounded up to 32 when perm); perm: within each
+// chunk of 32 columns, column c is stored at 2 (c % 16) + c / 16, so that columns c and c + 16
+// are adjacent (one 16-byte load holds both); padding columns (>= R) are zero.
 template <class T, bool HALF, int JB, int P>
-__global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, int K, int R,
+__global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, int K, int R, int Rrow, bool perm,
                                  typename complex_of<T>::type *__restrict__ S,
                                  const cplx<double> *__restrict__ twiddle, bool inverse) {
     using C = typename complex_of<T>::type;
@@ -50,16 +53,17 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
     C *b2 = b1 + JB * n1;                     // [JB][FX*L*p]
     C tw[L];
     load_twiddles(tw, twiddle, inverse);
-    const int64_t njobs = (int64_t)nbox * R;
+    const int64_t njobs = (int64_t)nbox * Rrow;
     const int64_t job0 = (int64_t)blockIdx.x * JB;
     // x: line (iy, iz) of a job -> b1[fx + FX*(iy + p*iz)], fx < FX
     for (int i = threadIdx.x; i < JB * p * p; i += blockDim.x) {
         const int j = i / (p * p), rest = i - j * (p * p);
         const int64_t job = job0 + j;
         C in[p];
-        if (job < njobs) {
-            const int64_t slot = job / R;
-            const int r = (int)(job - slot * R);
+        const int64_t slot = job / Rrow;
+        const int q = (int)(job - slot * Rrow);
+        const int r = perm ? (q & ~31) + ((q & 31) >> 1) + 16 * (q & 1) : q;
+        if (job < njobs && r < R) {
             const T *mj = mom + ((int64_t)(box0 + slot) * R + r) * K + p * rest;
 #pragma unroll
             for (int ix = 0; ix < p; ++ix) in[ix] = cvt<C>::from(mj[ix]);
@@ -239,20 +243,21 @@ bool allow_lds(K kernel, size_t bytes) {
     } while (0)
 
 template <class T, bool HALF, int P>
-static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, void *S, const void *tw, bool inverse,
-                        hipStream_t s) {
+static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, bool perm, void *S, const void *tw,
+                        bool inverse, hipStream_t s) {
     constexpr int JB = fft_jobs<T, HALF>();
     constexpr int JH = JB / 2;
     constexpr int L = 2 * P - 1;
-    const int64_t njobs = (int64_t)nbox * R;
+    const int Rrow = perm ? (R + 31) / 32 * 32 : R;
+    const int64_t njobs = (int64_t)nbox * Rrow;
     const size_t sh = fft_smem<JB, T, HALF>(P, L);
     using C = typename complex_of<T>::type;
     if (allow_lds(fft_in_fm_kernel<T, HALF, JB, P>, sh)) {
         fft_in_fm_kernel<T, HALF, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
-            box0, nbox, (const T *)mom, K, R, (C *)S, (const cplx<double> *)tw, inverse);
+            box0, nbox, (const T *)mom, K, R, Rrow, perm, (C *)S, (const cplx<double> *)tw, inverse);
     } else {
         fft_in_fm_kernel<T, HALF, JH, P><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(P, L), s>>>(
-            box0, nbox, (const T *)mom, K, R, (C *)S, (const cplx<double> *)tw, inverse);
+            box0, nbox, (const T *)mom, K, R, Rrow, perm, (C *)S, (const cplx<double> *)tw, inverse);
     }
     H2G_CHECK_LAUNCH2();
 }
@@ -277,14 +282,14 @@ static void fft_out_fm_p(const int32_t *recv, int nrecv, const void *O, int K, i
 }
 
 template <class T, bool HALF>
-static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, void *S, const void *tw,
-                        bool inverse, hipStream_t s) {
+static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, bool perm, void *S,
+                        const void *tw, bool inverse, hipStream_t s) {
     switch (p) {
-    case 2: fft_in_fm_p<T, HALF, 2>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
-    case 3: fft_in_fm_p<T, HALF, 3>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
-    case 4: fft_in_fm_p<T, HALF, 4>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
-    case 5: fft_in_fm_p<T, HALF, 5>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
-    case 6: fft_in_fm_p<T, HALF, 6>(box0, nbox, mom, K, R, S, tw, inverse, s); break;
+    case 2: fft_in_fm_p<T, HALF, 2>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
+    case 3: fft_in_fm_p<T, HALF, 3>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
+    case 4: fft_in_fm_p<T, HALF, 4>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
+    case 5: fft_in_fm_p<T, HALF, 5>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
+    case 6: fft_in_fm_p<T, HALF, 6>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
     default: throw Error{H2G_EUNSUPPORTED, "Toeplitz modulation supports 2 <= p <= 6"};
     }
     (void)L;
@@ -305,19 +310,19 @@ static void fft_out_fm_t(const int32_t *recv, int nrecv, const void *O, int K, i
 }
 
 void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, const void *mom, int K, int p, int L,
-                      int R, void *S, const void *twiddle, hipStream_t s) {
+                      int R, bool perm, void *S, const void *twiddle, hipStream_t s) {
     if (nbox == 0) return;
     switch (cdtype) {
     case H2G_F64:
-        if (half) fft_in_fm_t<double, true>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
-        else fft_in_fm_t<double, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+        if (half) fft_in_fm_t<double, true>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
+        else fft_in_fm_t<double, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
         break;
     case H2G_F32:
-        if (half) fft_in_fm_t<float, true>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
-        else fft_in_fm_t<float, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+        if (half) fft_in_fm_t<float, true>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
+        else fft_in_fm_t<float, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
         break;
-    case H2G_C64: fft_in_fm_t<cplx<double>, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s); break;
-    default: fft_in_fm_t<cplx<float>, false>(box0, nbox, mom, K, p, L, R, S, twiddle, inverse, s);
+    case H2G_C64: fft_in_fm_t<cplx<double>, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s); break;
+    default: fft_in_fm_t<cplx<float>, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
     }
 }
 

@@ -409,11 +409,11 @@ __global__ void __launch_bounds__(NT_LIST)
 // ---------------------------------------------------------------------------------------------
 constexpr int DR_NT = 256, DR_MAXB = 16;  // threads; batches of a block's list held in LDS
 
-template <bool CONJ, int NCH>
+template <bool CONJ>
 __global__ void __launch_bounds__(DR_NT, 2)
     m2l_direct_kernel(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr, const int4 *__restrict__ list,
                       int n_list, const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg, int n_src,
-                      int n_recv, const cplx<float> *__restrict__ S, cplx<float> *__restrict__ O, int R) {
+                      int n_recv, const cplx<float> *__restrict__ S, int Rrow, cplx<float> *__restrict__ O, int R) {
     __shared__ __align__(16) float ea[216 * 32];
     __shared__ int4 lsd[DR_NT / 64][2][DR_MAXB * 8];  // per wave: lists of the current / next block
     const int grp = group_of_workgroup();
@@ -426,17 +426,11 @@ __global__ void __launch_bounds__(DR_NT, 2)
     const int m = l & 15, bq = (l >> 4) & 1, h = l >> 5;
     // adjoint: the Im parts of the couplings (rows c != b) change sign
     const float sgn = (CONJ && ((m & 1) != bq)) ? -1.f : 1.f;
-    // source rows of frequency f through a buffer resource: byte offsets, reads past the slice
-    // return zero.  The lane reads column r0 + 16 g + m, re / im part bq, of a row; a column >= R
-    // reads the next row (or zero), which only reaches result columns that are not stored.
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<cplx<float> *>(S + (int64_t)f * n_src * R), (short)0, n_src * R * (int)sizeof(cplx<float>),
-        0x00020000);
-    const uint32_t lo = (uint32_t)(((r0 + m) * 2 + bq) * 4);
-    // 8-byte loads (both column tiles, NCH = 2): column r0 + 16 bq + m, clamped into the row
-    const char *__restrict__ Sfb = reinterpret_cast<const char *>(S + (int64_t)f * n_src * R);
-    const uint32_t lo2 = (uint32_t)(min(r0 + 16 * bq + m, R - 1) * (int)sizeof(cplx<float>));
-    const uint32_t rowb = (uint32_t)R * 8;  // bytes per source row
+    // source rows of frequency f in the permuted column layout (columns c and c + 16 of a chunk
+    // of 32 adjacent): 16-byte unit m of chunk r0 / 32 holds columns r0 + m and r0 + 16 + m
+    const char *__restrict__ Sfb = reinterpret_cast<const char *>(S + (int64_t)f * n_src * Rrow);
+    const uint32_t rowb = (uint32_t)Rrow * (uint32_t)sizeof(cplx<float>);
+    const uint32_t lo4 = (uint32_t)(r0 * (int)sizeof(cplx<float>) + m * 16);
     const char *eab = reinterpret_cast<const char *>(ea) + (m * 2 + bq) * (int)sizeof(float);
     float *__restrict__ Of = reinterpret_cast<float *>(O + (int64_t)f * n_recv * R);
     // list copies: lane l moves entries l and 64 + l (int4, two pairs each) of a block's list
@@ -462,34 +456,26 @@ __global__ void __launch_bounds__(DR_NT, 2)
 #pragma unroll
             for (int j = 0; j < 4; ++j) L[j] = p[j];
         };
-        float an[8], bn[8][NCH];
+        // per entry pair j (slot pairs 2j, 2j + 1): the lane of 16-lane row bq loads its half's
+        // source row of pair 2j + bq (one 16-byte unit); the swap at use time sorts the values
+        float an[8];
+        float4 bn[4];
         auto issue = [&](const int4(&L)[4]) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const int i = 2 * j + e;
-                    const uint32_t slot = (uint32_t)(e ? L[j].z : L[j].x);
-                    const int qo = e ? L[j].w : L[j].y;
-                    an[i] = *reinterpret_cast<const float *>(eab + qo);
-                    if constexpr (NCH == 2) {
-                        // one 8-byte load per lane: the complex value (re, im) of column
-                        // r0 + 16 bq + m, rearranged when the batch is used
-                        const float2 v = *reinterpret_cast<const float2 *>(Sfb + (slot * rowb + lo2));
-                        bn[i][0] = v.x;
-                        bn[i][NCH - 1] = v.y;
-                    } else {
-                        bn[i][0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, slot * rowb + lo, 0, 0));
-                    }
-                }
+            for (int j = 0; j < 4; ++j) {
+                an[2 * j] = *reinterpret_cast<const float *>(eab + L[j].y);
+                an[2 * j + 1] = *reinterpret_cast<const float *>(eab + L[j].w);
+                const uint32_t slot = (uint32_t)(bq ? L[j].z : L[j].x);
+                bn[j] = *reinterpret_cast<const float4 *>(Sfb + (slot * rowb + lo4));
+            }
         };
         ldl(k0, L2);
         issue(L2);  // operands of batch k0
         ldl(k0 + 1, L1);
-        double4_m acc[NCH];
-        float4_m p0[NCH], p1[NCH];
+        double4_m acc[2];
+        float4_m p0[2], p1[2];
 #pragma unroll
-        for (int g = 0; g < NCH; ++g) {
+        for (int g = 0; g < 2; ++g) {
             acc[g] = double4_m{0.0, 0.0, 0.0, 0.0};
             p0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
             p1[g] = p0[g];
@@ -498,23 +484,25 @@ __global__ void __launch_bounds__(DR_NT, 2)
         // A batch past the block's last (k = k1 on an odd count) runs with zero couplings.
         auto step = [&](int k, const int4(&Lc)[4], int4(&Ln)[4]) {
             const float sk = k < k1 ? sgn : 0.f;
-            float ac[8], bc[8][NCH];
+            float ac[8], bc[8][2];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                ac[i] = an[i] * sk;
-                if constexpr (NCH == 2) {
-                    // swapping the odd 16-lane rows of the re parts with the even rows of the im
-                    // parts leaves lane (m, bq) with part bq of columns r0 + m and r0 + 16 + m
-                    // (elements are copied out of the returned pair before reinterpreting them:
-                    // __builtin_bit_cast of a vector element reads element 0 with this compiler)
-                    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(bn[i][0]), __float_as_uint(bn[i][1]),
-                                                                     false, false);
-                    const uint32_t s0 = sw[0], s1 = sw[1];
-                    bc[i][0] = __uint_as_float(s0);
-                    bc[i][1] = __uint_as_float(s1);
-                } else {
-                    bc[i][0] = bn[i][0];
-                }
+            for (int j = 0; j < 4; ++j) {
+                ac[2 * j] = an[2 * j] * sk;
+                ac[2 * j + 1] = an[2 * j + 1] * sk;
+                // the lane holds (re, im) of columns m and 16 + m of pair 2j + bq; swapping the
+                // odd 16-lane rows of the re parts with the even rows of the im parts leaves it
+                // with part bq of those columns for pairs 2j (x, z) and 2j + 1 (y, w).  (Elements
+                // are copied out of the returned pairs before reinterpreting them:
+                // __builtin_bit_cast of a vector element reads element 0 with this compiler.)
+                const auto s0 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bn[j].x), __float_as_uint(bn[j].y),
+                                                                 false, false);
+                const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bn[j].z), __float_as_uint(bn[j].w),
+                                                                 false, false);
+                const uint32_t x = s0[0], y = s0[1], z = s1[0], ww = s1[1];
+                bc[2 * j][0] = __uint_as_float(x);
+                bc[2 * j][1] = __uint_as_float(z);
+                bc[2 * j + 1][0] = __uint_as_float(y);
+                bc[2 * j + 1][1] = __uint_as_float(ww);
             }
             ldl(k + 2, Ln);
             issue(Lc);
@@ -522,21 +510,21 @@ __global__ void __launch_bounds__(DR_NT, 2)
             // (memory clobber) and not scheduled after the MFMAs (scheduling barrier)
             __asm__ volatile("" ::: "memory");
             __builtin_amdgcn_sched_barrier(0);
-            float4_m f0[NCH], f1[NCH];
+            float4_m f0[2], f1[2];
 #pragma unroll
-            for (int g = 0; g < NCH; ++g) {
+            for (int g = 0; g < 2; ++g) {
                 f0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
                 f1[g] = f0[g];
             }
 #pragma unroll
             for (int i = 0; i < 8; i += 2)
 #pragma unroll
-                for (int g = 0; g < NCH; ++g) {
+                for (int g = 0; g < 2; ++g) {
                     f0[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i], bc[i][g], f0[g], 0, 0, 0);
                     f1[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i + 1], bc[i + 1][g], f1[g], 0, 0, 0);
                 }
 #pragma unroll
-            for (int g = 0; g < NCH; ++g) {
+            for (int g = 0; g < 2; ++g) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[g][v] += (double)p0[g][v] + (double)p1[g][v];
                 p0[g] = f0[g];
@@ -550,7 +538,7 @@ __global__ void __launch_bounds__(DR_NT, 2)
             step(k + 1, L2, L1);
         }
 #pragma unroll
-        for (int g = 0; g < NCH; ++g) {
+        for (int g = 0; g < 2; ++g) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[g][v] += (double)p0[g][v] + (double)p1[g][v];
             // FP32 16x16x4 result layout: rows 4 (l / 16) + v = (target 2 (l / 16) + v / 2, re / im v % 2)
@@ -630,12 +618,18 @@ void cluster_go(bool conj, const LatticeLevel &LL, const EA *EAg, int n_src, int
 
 }  // namespace
 
+bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R) {
+    const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
+    const int64_t Rrow = (R + 31) / 32 * 32;
+    return !f64 && R >= 16 && LL.ok && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB && LL.EA32.ptr &&
+           (int64_t)n_src * Rrow * (int64_t)sizeof(cplx<float>) < INT32_MAX;
+}
+
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
     if (!LL.EA64.ptr) throw Error{H2G_EINVAL, "lattice coupling table missing"};
-    if (!f64 && R >= 16 && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB && LL.EA32.ptr &&
-        (int64_t)n_src * R * (int64_t)sizeof(cplx<float>) < INT32_MAX) {
+    if (m2l_direct_ok(cdtype, LL, n_src, R)) {
         const LatticeDirect &D = LL.direct;
         // groups of consecutive blocks: at least two blocks per wave (the coupling table is
         // loaded once per group), and about 256 groups per frequency, so that the workgroups
@@ -643,14 +637,14 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
         // group size was chosen by measurement on the N = 2M, p = 6 map (1024 .. 64 groups)
         const int gsz = std::max(2 * DR_NT / 64, D.n_blocks / 256);
         const int ng = (D.n_blocks + gsz - 1) / gsz;
-        const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)((R + 31) / 32));
+        const int Rrow = (R + 31) / 32 * 32;
+        const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)(Rrow / 32));
         auto go = [&](auto kern) {
             kern<<<grid, DR_NT, 0, s>>>(D.n_blocks, gsz, D.ptr.as<int32_t>(), D.list.as<int4>(),
                                         (int)(D.list.bytes / sizeof(int4)), D.tgt.as<int32_t>(), LL.EA32.as<float>(),
-                                        n_src, n_recv, (const cplx<float> *)S, (cplx<float> *)O, R);
+                                        n_src, n_recv, (const cplx<float> *)S, Rrow, (cplx<float> *)O, R);
         };
-        if (R <= 16) conj ? go(m2l_direct_kernel<true, 1>) : go(m2l_direct_kernel<false, 1>);
-        else conj ? go(m2l_direct_kernel<true, 2>) : go(m2l_direct_kernel<false, 2>);
+        conj ? go(m2l_direct_kernel<true>) : go(m2l_direct_kernel<false>);
     } else if (!f64 && R >= 16 && LL.list.n_tiles > 0 && LL.EA32.ptr) {
         const LatticeTiles &T = LL.list;
         const size_t sh = sizeof(cplx<float>) * (size_t)(LIST_NSLOT + 1) * 16 + sizeof(float) * 216 * 32 +
