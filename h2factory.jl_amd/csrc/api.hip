@@ -1258,8 +1258,9 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
                 const size_t cc = mixed ? 8 : 16;
                 const int64_t ns = std::max(c.fwd.max_src_level_boxes, c.rev.max_src_level_boxes);
                 const int64_t nr = std::max(c.fwd.max_recv_level_boxes, c.rev.max_recv_level_boxes);
-                // source spectra of wide FP32 batches may be stored with rows padded to 32 columns
-                const int64_t rin = mixed && w->R_cap >= 16 ? (w->R_cap + 31) / 32 * 32 : w->R_cap;
+                // source spectra of FP32 batches of >= 4 columns may be stored with rows padded to
+                // 32 columns (the direct M2L kernel's layout)
+                const int64_t rin = mixed && w->R_cap >= 4 ? (w->R_cap + 31) / 32 * 32 : w->R_cap;
                 w->spec_in.alloc((size_t)ns * rin * c.L3 * cc);
                 w->spec_out.alloc((size_t)nr * w->R_cap * c.L3 * cc);
             }

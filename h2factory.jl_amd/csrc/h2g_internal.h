@@ -377,7 +377,7 @@ void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, c
                       int R, bool perm, void *S, const void *twiddle, hipStream_t s);
 void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,
                        int p, int L, int R, void *loc, const void *twiddle, hipStream_t s);
-// true when the lattice M2L of this level runs the direct kernel (FP32 spectra, R >= 16); its
+// true when the lattice M2L of this level runs the direct kernel (FP32 spectra, R >= 4); its
 // source spectra are then stored in the permuted column layout of launch_fft_in_fm (perm)
 bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R);
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,

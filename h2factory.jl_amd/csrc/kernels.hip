@@ -1264,6 +1264,11 @@ void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targ
 // instruction (checked by the layout probe below on gfx950): A[m][k] in lane m + 16k,
 // B[k][n] in lane n + 16k, D[m][n] in lane n + 16(m % 4), element m / 4.
 // ------------------------------------------------------------------------------------------
+// narrowest batch for the MFMA near-field and leaf kernels (16-column tiles, unused columns
+// masked): from 4 columns on they beat the vector kernels (N = 2M, p = 6, FP32: 15 columns took
+// 32 ms more than 16 on the vector kernels)
+constexpr int MFMA_MIN_R = 4;
+
 namespace {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
@@ -1545,7 +1550,7 @@ __global__ void mfma_f32_probe_kernel(const float *__restrict__ A, const float *
 bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                      const void *coll, int K, const void *loc, int R, void *ys, double alpha, bool stage,
                      hipStream_t s) {
-    if (R < 16 || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
+    if (R < MFMA_MIN_R || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
     // column tiles per workgroup: up to 4, as many as the LDS block of 64 KB allows
     const size_t es = cdtype == H2G_F64 ? 8 : 4;
     const int nct = std::min({4, (R + 15) / 16, stage ? (int)(65536 / (es * 16 * (size_t)(K + 1))) : 4});
@@ -1569,7 +1574,7 @@ bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
 
 bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                      const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
-    if (R < 16 || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
+    if (R < MFMA_MIN_R || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
     // column tiles per workgroup: up to 4, as many as the LDS block of 64 KB allows
     const size_t es = cdtype == H2G_F64 ? 8 : 4;
     const int nct = std::min({4, (R + 15) / 16, (int)(65536 / (es * 16 * (size_t)(K + 1)))});
@@ -1592,7 +1597,7 @@ bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
 }
 
 bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
-    if (c.dtype != H2G_F64 || R < 16) return false;
+    if (c.dtype != H2G_F64 || R < MFMA_MIN_R) return false;
     const bool trans = op != H2G_OP_N;
     const NearPlan &P = trans ? c.near_rev_plan : c.near_fwd_plan;
     if (!P.ok) return false;

@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(NT_LIST)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Direct kernel (FP32 spectra, R >= 16): no staging of source spectra.  Workgroup = (group of
+// Direct kernel (FP32 spectra, R >= DR_MIN_R): no staging of source spectra.  Workgroup = (group of
 // consecutive blocks in Morton order, f, chunk of 32 columns); the frequency's coupling table is
 // loaded into LDS once, then every wave takes its own blocks of the group and, per slot pair of
 // the block's list, reads the two source rows straight from global memory (L2: the group's blocks
@@ -408,6 +408,10 @@ __global__ void __launch_bounds__(NT_LIST)
 // produced them have finished.
 // ---------------------------------------------------------------------------------------------
 constexpr int DR_NT = 256, DR_MAXB = 16;  // threads; batches of a block's list held in LDS
+// narrowest batch for the direct kernel: it computes whole chunks of 32 columns (padding columns
+// are zero), which costs less than the cluster kernel's per-column work from about 4 columns on
+// (N = 2M, p = 6: 4 columns 44 ms either way, 8 columns 74 ms on the cluster kernel)
+constexpr int DR_MIN_R = 4;
 
 template <bool CONJ>
 __global__ void __launch_bounds__(DR_NT, 2)
@@ -621,7 +625,7 @@ void cluster_go(bool conj, const LatticeLevel &LL, const EA *EAg, int n_src, int
 bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
     const int64_t Rrow = (R + 31) / 32 * 32;
-    return !f64 && R >= 16 && LL.ok && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB && LL.EA32.ptr &&
+    return !f64 && R >= DR_MIN_R && LL.ok && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB && LL.EA32.ptr &&
            (int64_t)n_src * Rrow * (int64_t)sizeof(cplx<float>) < INT32_MAX;
 }
 

@@ -291,7 +291,7 @@ def test_mfma_f32_lane_layout(gpu):
 
 @pytest.mark.parametrize("petrov,leaf", [(False, 40), (True, 70)])
 def test_wide_near_field_all_orientations(gpu, petrov, leaf):
-    """Near field alone at 16..70 RHS (FP64 MFMA path for real maps) against the oracle.
+    """Near field alone at 4..70 RHS (FP64 MFMA path for real maps) against the oracle.
 
     Leaves of up to 70 points give output leaves of 1..4 row tiles and leaves split into two
     row-block tasks; distinct test and trial points give rectangular blocks, whose transposes
@@ -303,7 +303,7 @@ def test_wide_near_field_all_orientations(gpu, petrov, leaf):
     pts_b = rng.random((2100, 3)) if petrov else None
     core = OB.build_core(pts, pts_b, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=leaf)
     A = make_map(core, max_rhs=70)
-    for nrhs in (16, 21, 40, 70):
+    for nrhs in (4, 7, 16, 21, 40, 70):
         for op in "NT":
             M = orient(A, op)
             n_out, n_in = M.shape
