@@ -257,6 +257,17 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
     S.perm.assign(d.perm, d.perm + d.n_dofs);
     if (d.coord) S.coord.assign(d.coord, d.coord + 3 * (size_t)d.n_boxes);
     S.d_perm.upload_vec(S.perm);
+    {
+        // inverse permutation when perm is one (every input index exactly once)
+        std::vector<int64_t> inv(S.n_dofs, -1);
+        bool ok = true;
+        for (int64_t i = 0; i < S.n_dofs && ok; ++i) {
+            const int64_t j = S.perm[i];
+            if (j < 0 || j >= S.n_dofs || inv[j] >= 0) ok = false;
+            else inv[j] = i;
+        }
+        if (ok) S.d_iperm.upload_vec(inv);
+    }
     S.d_begin.upload_vec(S.begin);
     S.d_end.upload_vec(S.end);
     const size_t es = elem_size(dtype);
@@ -1078,7 +1089,7 @@ void Workspace::mark_end() {
 int64_t Core::device_bytes() const {
     int64_t b = 0;
     auto side = [&](const Side &S) {
-        b += S.d_perm.bytes + S.d_begin.bytes + S.d_end.bytes + S.coll.bytes + S.coll32.bytes + S.transfers.bytes +
+        b += S.d_perm.bytes + S.d_iperm.bytes + S.d_begin.bytes + S.d_end.bytes + S.coll.bytes + S.coll32.bytes + S.transfers.bytes +
              S.transfers32.bytes;
     };
     side(test);
@@ -1414,6 +1425,7 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
             const int R = (int)std::min<int64_t>(ws->R_cap, nrhs - c0);
             ws->mark_begin(0, -1);
             launch_gather(c.dtype, dtype, (const char *)dx + (size_t)c0 * lx * xe, lx, src.d_perm.as<int64_t>(),
+                          src.d_iperm.as<int64_t>(),
                           src.n_dofs, R, ws->xs.ptr, s);
             ws->mark_end();
             auto body = [&]() {
@@ -1459,7 +1471,7 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
             }
             const uint8_t *mask = (which == H2G_FARMUL && five) ? P.out_mask.as<uint8_t>() : nullptr;
             ws->mark_begin(9, -1);
-            launch_scatter(c.dtype, dtype, ws->ys.ptr, dst.d_perm.as<int64_t>(), dst.n_dofs, R,
+            launch_scatter(c.dtype, dtype, ws->ys.ptr, dst.d_perm.as<int64_t>(), dst.d_iperm.as<int64_t>(), dst.n_dofs, R,
                            (char *)dy + (size_t)c0 * ly * xe, ly, a, five ? b : cplx<double>{0.0, 0.0}, mask, s);
             ws->mark_end();
         }

@@ -173,7 +173,7 @@ struct Side {
     std::vector<uint8_t> is_leaf;
     std::vector<int64_t> perm;
     std::vector<int32_t> coord;  // [n_boxes][3] or empty
-    DevBuf d_perm, d_begin, d_end;
+    DevBuf d_perm, d_iperm, d_begin, d_end;  // d_iperm: input index -> tree index
     DevBuf coll, coll32;            // K x n_dofs core type / reduced type
     DevBuf transfers, transfers32;  // [n_levels][8] K x K
     bool separable = false;         // every transfer is Az (x) Ay (x) Ax
@@ -341,10 +341,11 @@ struct Workspace {
 // Kernel launchers (kernels.hip), all on `s`.
 // T = compute scalar, S = storage scalar of xs/ys (double or cplx<double>).
 // ------------------------------------------------------------------------------------------
-void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, const int64_t *perm, int64_t n,
-                   int R, void *xs, hipStream_t s);
-void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, int64_t n, int R, void *y,
-                    int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s);
+// perm: tree index -> input index; iperm: its inverse (for the tiled batch forms)
+void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, const int64_t *perm,
+                   const int64_t *iperm, int64_t n, int R, void *xs, hipStream_t s);
+void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, const int64_t *iperm, int64_t n,
+                    int R, void *y, int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s);
 void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s);
 void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                 const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s);

@@ -57,6 +57,59 @@ __global__ void scatter_kernel(const S *__restrict__ ys, const int64_t *__restri
 }
 
 // ------------------------------------------------------------------------------------------
+// Tiled forms for batches (R >= 8): a workgroup takes GT consecutive points in input order and a
+// chunk of up to GR columns, with the inverse permutation (input index -> tree index).  The
+// column-major x / y side is read / written coalesced along the points of each column, the
+// tree-ordered xs / ys side row by row (R contiguous values per point); LDS holds the tile.
+// (The plain forms above touch x / y at random rows for every (point, column).)
+// ------------------------------------------------------------------------------------------
+constexpr int GT = 64, GR = 32;
+
+template <class S, class X>
+__global__ void __launch_bounds__(TPB) gather_tiled_kernel(const X *__restrict__ x, int64_t ldx,
+                                                          const int64_t *__restrict__ iperm, int64_t n, int R,
+                                                          S *__restrict__ xs) {
+    __shared__ S t[GT][GR + 1];
+    const int64_t j0 = (int64_t)blockIdx.x * GT;
+    const int r0 = blockIdx.y * GR;
+    const int nr = min(GR, R - r0);
+    for (int e = threadIdx.x; e < GT * nr; e += TPB) {
+        const int c = e / GT, j = e - c * GT;
+        if (j0 + j < n) t[j][c] = cvt<S>::from(x[j0 + j + (int64_t)(r0 + c) * ldx]);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < GT * nr; e += TPB) {
+        const int j = e / nr, c = e - j * nr;
+        if (j0 + j < n) xs[iperm[j0 + j] * R + r0 + c] = t[j][c];
+    }
+}
+
+template <class S, class Y>
+__global__ void __launch_bounds__(TPB) scatter_tiled_kernel(const S *__restrict__ ys, const int64_t *__restrict__ iperm,
+                                                           int64_t n, int R, Y *__restrict__ y, int64_t ldy,
+                                                           cplx<double> a, cplx<double> b,
+                                                           const uint8_t *__restrict__ mask) {
+    __shared__ S t[GT][GR + 1];
+    const int64_t j0 = (int64_t)blockIdx.x * GT;
+    const int r0 = blockIdx.y * GR;
+    const int nr = min(GR, R - r0);
+    for (int e = threadIdx.x; e < GT * nr; e += TPB) {
+        const int j = e / nr, c = e - j * nr;
+        if (j0 + j < n) t[j][c] = ys[iperm[j0 + j] * R + r0 + c];
+    }
+    __syncthreads();
+    const bool beta = b.re != 0.0 || b.im != 0.0;
+    for (int e = threadIdx.x; e < GT * nr; e += TPB) {
+        const int c = e / GT, j = e - c * GT;
+        if (j0 + j >= n || (mask && !mask[iperm[j0 + j]])) continue;
+        const int64_t o = j0 + j + (int64_t)(r0 + c) * ldy;
+        S v = scale_by<S>(a, t[j][c]);
+        if (beta) v = v + scale_by<S>(b, cvt<S>::from(y[o]));
+        y[o] = cvt<Y>::from(v);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // near field: one workgroup per target leaf; blocks are column-major nr x nc.
 //   forward:   ys[rb + i] += B[i, j] * xs[cb + j]
 //   transpose: ys[cb + j] += op(B[i, j]) * xs[rb + i]
@@ -515,20 +568,28 @@ void launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
     H2G_CHECK_LAUNCH();
 }
 
-void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, const int64_t *perm, int64_t n, int R,
-                   void *xs, hipStream_t s) {
+void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, const int64_t *perm,
+                   const int64_t *iperm, int64_t n, int R, void *xs, hipStream_t s) {
     if (n * R == 0) return;
+    const dim3 tg((unsigned)((n + GT - 1) / GT), (unsigned)((R + GR - 1) / GR));
     DISPATCH_S(dtype_core, DISPATCH_X(dtype_x, {
-        gather_kernel<S, X><<<grid1(n * R), TPB, 0, s>>>((const X *)x, ldx, perm, n, R, (S *)xs);
+        if (R >= 8 && iperm)
+            gather_tiled_kernel<S, X><<<tg, TPB, 0, s>>>((const X *)x, ldx, iperm, n, R, (S *)xs);
+        else
+            gather_kernel<S, X><<<grid1(n * R), TPB, 0, s>>>((const X *)x, ldx, perm, n, R, (S *)xs);
     }));
     H2G_CHECK_LAUNCH();
 }
 
-void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, int64_t n, int R, void *y,
-                    int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s) {
+void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, const int64_t *iperm, int64_t n,
+                    int R, void *y, int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s) {
     if (n * R == 0) return;
+    const dim3 tg((unsigned)((n + GT - 1) / GT), (unsigned)((R + GR - 1) / GR));
     DISPATCH_S(dtype_core, DISPATCH_X(dtype_y, {
-        scatter_kernel<S, X><<<grid1(n * R), TPB, 0, s>>>((const S *)ys, perm, n, R, (X *)y, ldy, a, b, mask);
+        if (R >= 8 && iperm)
+            scatter_tiled_kernel<S, X><<<tg, TPB, 0, s>>>((const S *)ys, iperm, n, R, (X *)y, ldy, a, b, mask);
+        else
+            scatter_kernel<S, X><<<grid1(n * R), TPB, 0, s>>>((const S *)ys, perm, n, R, (X *)y, ldy, a, b, mask);
     }));
     H2G_CHECK_LAUNCH();
 }
