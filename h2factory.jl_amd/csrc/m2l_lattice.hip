@@ -74,8 +74,14 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
 #pragma unroll
         for (int fx = 0; fx < FX; ++fx) {
             C acc = zero_v<C>();
+            if constexpr (is_complex<T>::value) {
 #pragma unroll
-            for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], in[ix]);
+                for (int ix = 0; ix < p; ++ix) fmacc(acc, tw[(fx * ix) % L], in[ix]);
+            } else {
+                // real moments: real x complex twiddle (2 FMA per term instead of 4)
+#pragma unroll
+                for (int ix = 0; ix < p; ++ix) fmacc(acc, in[ix].re, tw[(fx * ix) % L]);
+            }
             b1[j * n1 + fx + FX * rest] = acc;
         }
     }
