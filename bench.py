@@ -4,7 +4,8 @@ equispaced p = 6 interpolation (K = 216), Toeplitz M2L (11^3 spectra), leaf size
 32 right-hand sides of N(0,1) FP32 entries (seed 1).
 
 One step = one multi-RHS product Y = A X with all 32 columns (near field + far field) in the
-batched mixed-precision form (FP32 far field, FP64 near field and final sum, hipGraph replay).
+batched mixed-precision form (FP32 far field; near values stored in FP32, near products and the
+final sum in FP64; hipGraph replay).
 The same run times the plain FP64 form one right-hand side at a time, checks the batched
 result against it, reports the error against the O(N^2) direct sum on 1,000 sampled targets,
 measures the dominant kernel's per-launch time with HIP events for the roofline figure and
@@ -290,7 +291,7 @@ def main():
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"laplace equispaced p={args.p} {args.m2l} M2L, N={N} uniform cube, leaf<=64, "
-                               f"{R} RHS FP32 batched (FP64 near + final sum)",
+                               f"{R} RHS FP32 batched (FP32-stored near values, FP64 near arithmetic + final sum)",
                    "n": N, "nrhs": R, "p": args.p, "m2l": args.m2l, "leaf_size": 64},
         "plain_fp64_rhs_per_s": round(plain_rhs_per_s, 2),
         "speedup_vs_plain": round(value / plain_rhs_per_s, 2),

@@ -888,6 +888,15 @@ void build_near(Core &c, const h2g_core_desc &d) {
         c.near_vals.upload(d.near_values, (size_t)d.n_near_values * elem_size(d.dtype));
     else
         c.near_vals.alloc((size_t)d.n_near_values * elem_size(d.dtype));
+    if (d.dtype == H2G_F64 && (c.precisions & (1 << H2G_PREC_MIXED)) && d.n_near_values > 0) {
+        // the mixed form reads the near values rounded to FP32 (half the HBM traffic of the
+        // near field) and still multiplies and accumulates in FP64
+        c.near_vals32.alloc((size_t)d.n_near_values * sizeof(float));
+        if (d.near_values) {
+            launch_narrow(c.near_vals.as<double>(), c.near_vals32.as<float>(), d.n_near_values, nullptr);
+            H2G_HIP(hipDeviceSynchronize());
+        }
+    }
     std::vector<int64_t> rb(nb), cb(nb), off(nb);
     std::vector<int32_t> nr(nb), nc(nb);
     for (int64_t q = 0; q < nb; ++q) {
@@ -1094,7 +1103,7 @@ int64_t Core::device_bytes() const {
     };
     side(test);
     if (!galerkin) side(trial_own);
-    b += ops.bytes + ops32.bytes + perms.bytes + perms_inv.bytes + near_vals.bytes;
+    b += ops.bytes + ops32.bytes + perms.bytes + perms_inv.bytes + near_vals.bytes + near_vals32.bytes;
     b += near_fwd_plan.task.bytes + near_fwd_plan.col.bytes + near_rev_plan.task.bytes + near_rev_plan.col.bytes;
     return b;
 }
@@ -1419,6 +1428,7 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
         const Side &src = *P.src, &dst = *P.dst;
         const size_t se = elem_size(c.dtype);
         const bool use_graph = ws->opts.use_graph && !ws->opts.timing;
+        const bool mixed_ws = ws->opts.precision == H2G_PREC_MIXED;
         ws->timings.clear();
         ws->n_events_used = 0;
         for (int64_t c0 = 0; c0 < nrhs; c0 += ws->R_cap) {
@@ -1435,14 +1445,14 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
                     // concurrent with the far field up to its first write of the result
                     H2G_HIP(hipEventRecord(ws->ev_fork, s));
                     H2G_HIP(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
-                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, ws->side);
+                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, mixed_ws, ws->side);
                     H2G_HIP(hipEventRecord(ws->ev_near, ws->side));
                     far_pass(*ws, op, R, ws->ev_near);
                     return;
                 }
                 if (which & H2G_NEARMUL) {
                     ws->mark_begin(1, -1);
-                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, s);
+                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, mixed_ws, s);
                     ws->mark_end();
                 }
                 if (which & H2G_FARMUL) far_pass(*ws, op, R, nullptr);

@@ -289,6 +289,7 @@ struct Core {
     // near: blocks
     int64_t n_near_blocks = 0, n_near_values = 0;
     DevBuf near_vals;
+    DevBuf near_vals32;  // FP32 copy of real near values (cores with the mixed precision)
     DevBuf near_fwd_ptr, near_fwd_blk;   // CSR over test leaves (by list index)
     DevBuf near_rev_ptr, near_rev_blk;   // CSR over trial leaves
     DevBuf near_fwd_tgt, near_rev_tgt;   // target leaf lists
@@ -346,7 +347,9 @@ void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, cons
                    const int64_t *iperm, int64_t n, int R, void *xs, hipStream_t s);
 void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, const int64_t *iperm, int64_t n,
                     int R, void *y, int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s);
-void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s);
+// vals32: read the FP32 copy of the near values where the core has one (mixed precision)
+void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, bool vals32, hipStream_t s);
+void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s);
 void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                 const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s);
 void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,

@@ -594,10 +594,10 @@ void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *
     H2G_CHECK_LAUNCH();
 }
 
-bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s);
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, bool vals32, hipStream_t s);
 
-void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
-    if (launch_near_mfma(op, c, xs, ys, R, s)) return;
+void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, bool vals32, hipStream_t s) {
+    if (launch_near_mfma(op, c, xs, ys, R, vals32, s)) return;
     const bool trans = op != H2G_OP_N;
     const int ntgt = trans ? c.n_near_rev_tgt : c.n_near_fwd_tgt;
     if (ntgt == 0) return;
@@ -1347,9 +1347,9 @@ __device__ inline int mf_d_row(int lane, int v) { return (lane >> 4) + 4 * v; }
 // multiplied by zero, so the loop has no branches and the accumulators stay in the MFMA
 // registers.  Workgroups are spread so that each XCD (dispatched round robin) works on a
 // contiguous range of tasks, whose source points overlap in its L2.
-template <bool TRANS, int RT, int KU, int NCT>
+template <bool TRANS, int RT, int KU, int NCT, class V>
 __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__restrict__ tasks, int64_t t0, int64_t t1,
-                                                        const NearCol *__restrict__ cols, const double *__restrict__ vals,
+                                                        const NearCol *__restrict__ cols, const V *__restrict__ vals,
                                                         const double *__restrict__ xs, double *__restrict__ ys, int R) {
     constexpr int KB = 4 * KU;
     static_assert(KB <= 64, "a batch's table entries fit one wave");
@@ -1394,9 +1394,9 @@ __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__res
             const int xr = __builtin_amdgcn_ds_bpermute(src, ev.xrow);
             const int64_t rs = TRANS ? (int64_t)__builtin_amdgcn_ds_bpermute(src, ev.rstride) : 1;
             km[u] = (k0 + 4 * u + ak < ncol) ? 1.0 : 0.0;
-            const double *__restrict__ ap = vals + (int64_t)(((uint64_t)hi << 32) | lo);
+            const V *__restrict__ ap = vals + (int64_t)(((uint64_t)hi << 32) | lo);
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) a[u][rt] = ap[arow[rt] * rs];
+            for (int rt = 0; rt < RT; ++rt) a[u][rt] = (double)ap[arow[rt] * rs];
             const double *__restrict__ xp = xs + (int64_t)xr * R;
 #pragma unroll
             for (int g = 0; g < NCT; ++g) b[u][g] = xp[xcol[g]];
@@ -1585,6 +1585,12 @@ __global__ void __launch_bounds__(TPB, 2)
     }
 }
 
+// FP32 copy of FP64 values (grid-stride)
+__global__ void narrow_values_kernel(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
 // probe of the operand / result lane layout used above: D = A (16x4) * B (4x16)
 __global__ void mfma_f64_probe_kernel(const double *__restrict__ A, const double *__restrict__ B,
                                       double *__restrict__ D) {
@@ -1657,7 +1663,7 @@ bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
     return true;
 }
 
-bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hipStream_t s) {
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, bool vals32, hipStream_t s) {
     if (c.dtype != H2G_F64 || R < MFMA_MIN_R) return false;
     const bool trans = op != H2G_OP_N;
     const NearPlan &P = trans ? c.near_rev_plan : c.near_fwd_plan;
@@ -1665,35 +1671,51 @@ bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, hi
     // NCT column tiles per workgroup (up to 64 RHS); wider batches take several workgroups
     const int nct = std::min(4, (R + 15) / 16);
     const unsigned gy = (unsigned)((R + 16 * nct - 1) / (16 * nct));
+    const bool f32 = vals32 && c.near_vals32.ptr;
     for (int i = 0; i < 4; ++i) {  // tasks of 4, 3, 2, 1 row tiles
         const int64_t t0 = P.cls[i], t1 = P.cls[i + 1];
         if (t1 == t0) continue;
         const dim3 grid((unsigned)((t1 - t0 + TPB / 64 - 1) / (TPB / 64)), gy);
-        auto go = [&](auto kern) {
-            kern<<<grid, TPB, 0, s>>>(P.task.as<NearTask>(), t0, t1, P.col.as<NearCol>(), c.near_vals.as<double>(),
+        auto go = [&](auto kern, auto vv) {
+            using V = decltype(vv);
+            kern<<<grid, TPB, 0, s>>>(P.task.as<NearTask>(), t0, t1, P.col.as<NearCol>(),
+                                      f32 ? (const V *)c.near_vals32.ptr : (const V *)c.near_vals.ptr,
                                       (const double *)xs, (double *)ys, R);
         };
-        switch ((4 - i) * 8 + nct) {
-        case 8 + 1: trans ? go(near_task_kernel<true, 1, 4, 1>) : go(near_task_kernel<false, 1, 4, 1>); break;
-        case 8 + 2: trans ? go(near_task_kernel<true, 1, 4, 2>) : go(near_task_kernel<false, 1, 4, 2>); break;
-        case 8 + 3: trans ? go(near_task_kernel<true, 1, 4, 3>) : go(near_task_kernel<false, 1, 4, 3>); break;
-        case 8 + 4: trans ? go(near_task_kernel<true, 1, 4, 4>) : go(near_task_kernel<false, 1, 4, 4>); break;
-        case 16 + 1: trans ? go(near_task_kernel<true, 2, 2, 1>) : go(near_task_kernel<false, 2, 2, 1>); break;
-        case 16 + 2: trans ? go(near_task_kernel<true, 2, 2, 2>) : go(near_task_kernel<false, 2, 2, 2>); break;
-        case 16 + 3: trans ? go(near_task_kernel<true, 2, 2, 3>) : go(near_task_kernel<false, 2, 2, 3>); break;
-        case 16 + 4: trans ? go(near_task_kernel<true, 2, 2, 4>) : go(near_task_kernel<false, 2, 2, 4>); break;
-        case 24 + 1: trans ? go(near_task_kernel<true, 3, 2, 1>) : go(near_task_kernel<false, 3, 2, 1>); break;
-        case 24 + 2: trans ? go(near_task_kernel<true, 3, 2, 2>) : go(near_task_kernel<false, 3, 2, 2>); break;
-        case 24 + 3: trans ? go(near_task_kernel<true, 3, 2, 3>) : go(near_task_kernel<false, 3, 2, 3>); break;
-        case 24 + 4: trans ? go(near_task_kernel<true, 3, 2, 4>) : go(near_task_kernel<false, 3, 2, 4>); break;
-        case 32 + 1: trans ? go(near_task_kernel<true, 4, 2, 1>) : go(near_task_kernel<false, 4, 2, 1>); break;
-        case 32 + 2: trans ? go(near_task_kernel<true, 4, 2, 2>) : go(near_task_kernel<false, 4, 2, 2>); break;
-        case 32 + 3: trans ? go(near_task_kernel<true, 4, 2, 3>) : go(near_task_kernel<false, 4, 2, 3>); break;
-        default: trans ? go(near_task_kernel<true, 4, 2, 4>) : go(near_task_kernel<false, 4, 2, 4>); break;
+        // the value type (FP32 copy or FP64) and the orientation select the instantiation
+        auto pick = [&](auto rt_c, auto ku_c, auto nct_c) {
+            constexpr int RT = decltype(rt_c)::value, KU = decltype(ku_c)::value, NC = decltype(nct_c)::value;
+            if (f32)
+                trans ? go(near_task_kernel<true, RT, KU, NC, float>, 0.f)
+                      : go(near_task_kernel<false, RT, KU, NC, float>, 0.f);
+            else
+                trans ? go(near_task_kernel<true, RT, KU, NC, double>, 0.0)
+                      : go(near_task_kernel<false, RT, KU, NC, double>, 0.0);
+        };
+        using std::integral_constant;
+        auto by_nct = [&](auto rt_c, auto ku_c) {
+            switch (nct) {
+            case 1: pick(rt_c, ku_c, integral_constant<int, 1>{}); break;
+            case 2: pick(rt_c, ku_c, integral_constant<int, 2>{}); break;
+            case 3: pick(rt_c, ku_c, integral_constant<int, 3>{}); break;
+            default: pick(rt_c, ku_c, integral_constant<int, 4>{}); break;
+            }
+        };
+        switch (4 - i) {
+        case 1: by_nct(integral_constant<int, 1>{}, integral_constant<int, 4>{}); break;
+        case 2: by_nct(integral_constant<int, 2>{}, integral_constant<int, 2>{}); break;
+        case 3: by_nct(integral_constant<int, 3>{}, integral_constant<int, 2>{}); break;
+        default: by_nct(integral_constant<int, 4>{}, integral_constant<int, 2>{}); break;
         }
         H2G_CHECK_LAUNCH();
     }
     return true;
+}
+
+void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    narrow_values_kernel<<<(unsigned)std::min<int64_t>((n + TPB - 1) / TPB, 1 << 20), TPB, 0, s>>>(src, dst, n);
+    H2G_CHECK_LAUNCH();
 }
 
 }  // namespace h2g

@@ -485,14 +485,14 @@ __global__ void __launch_bounds__(DR_NT, 2)
             p1[g] = p0[g];
         }
         // batch k: operands (an, bn) loaded; Lc = list of batch k + 1; Ln receives batch k + 2.
-        // A batch past the block's last (k = k1 on an odd count) runs with zero couplings.
+        // A batch past the block's last (k = k1 on an odd count) loads clamped (valid) operands
+        // and skips its MFMAs.
         auto step = [&](int k, const int4(&Lc)[4], int4(&Ln)[4]) {
-            const float sk = k < k1 ? sgn : 0.f;
             float ac[8], bc[8][2];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                ac[2 * j] = an[2 * j] * sk;
-                ac[2 * j + 1] = an[2 * j + 1] * sk;
+                ac[2 * j] = an[2 * j] * sgn;
+                ac[2 * j + 1] = an[2 * j + 1] * sgn;
                 // the lane holds (re, im) of columns m and 16 + m of pair 2j + bq; swapping the
                 // odd 16-lane rows of the re parts with the even rows of the im parts leaves it
                 // with part bq of those columns for pairs 2j (x, z) and 2j + 1 (y, w).  (Elements
@@ -520,13 +520,17 @@ __global__ void __launch_bounds__(DR_NT, 2)
                 f0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
                 f1[g] = f0[g];
             }
+            // a batch past the block's last (odd batch count) skips its MFMAs (wave-uniform
+            // branch after the loads, which stay unconditional)
+            if (k < k1) {
 #pragma unroll
-            for (int i = 0; i < 8; i += 2)
+                for (int i = 0; i < 8; i += 2)
 #pragma unroll
-                for (int g = 0; g < 2; ++g) {
-                    f0[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i], bc[i][g], f0[g], 0, 0, 0);
-                    f1[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i + 1], bc[i + 1][g], f1[g], 0, 0, 0);
-                }
+                    for (int g = 0; g < 2; ++g) {
+                        f0[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i], bc[i][g], f0[g], 0, 0, 0);
+                        f1[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i + 1], bc[i + 1][g], f1[g], 0, 0, 0);
+                    }
+            }
 #pragma unroll
             for (int g = 0; g < 2; ++g) {
 #pragma unroll

@@ -154,6 +154,9 @@ extern "C" int h2g_method_fill(h2g_core *core, const h2g_method_desc *desc) {
                     c.near_vals.as<cplx<double>>());
             H2G_HIP(hipGetLastError());
             H2G_HIP(hipDeviceSynchronize());
+            if (c.near_vals32.ptr)
+                launch_narrow(c.near_vals.as<double>(), c.near_vals32.as<float>(), c.n_near_values, nullptr);
+            H2G_HIP(hipDeviceSynchronize());
         }
         if (d.fill_collections) {
             if (d.p < 1 || d.p * d.p * d.p != c.K || !d.nodes) throw Error{H2G_EINVAL, "collections need K == p^3 and nodes"};
