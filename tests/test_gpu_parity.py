@@ -311,3 +311,26 @@ def test_wide_near_field_all_orientations(gpu, petrov, leaf):
             y = np.zeros((n_out, nrhs))
             h2g.nearmul_(y, M, x)
             assert relerr(y, O.nearmul(core, x, op)) < RTOL, (op, nrhs)
+
+
+@pytest.mark.parametrize("petrov,leaf", [(False, 40), (True, 70)])
+def test_wide_near_field_mixed(gpu, petrov, leaf):
+    """Near field alone in the mixed form at 4..70 RHS: the FP32 copy of the values (rounded on
+    the device after the upload) on FP64 MFMA, against the oracle on the same FP32 inputs.
+    Rounding the values to FP32 bounds the error by about 1e-7; reading the FP64 array as FP32
+    (or the reverse) or a missing copy would give errors of order one."""
+    import h2g
+
+    rng = np.random.default_rng(22)
+    pts = rng.random((2500, 3))
+    pts_b = rng.random((2100, 3)) if petrov else None
+    core = OB.build_core(pts, pts_b, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=leaf)
+    A = make_map(core, max_rhs=70, precision="mixed")
+    for nrhs in (4, 7, 16, 21, 40, 70):
+        for op in "NT":
+            M = orient(A, op)
+            n_out, n_in = M.shape
+            x = rng.standard_normal((n_in, nrhs)).astype(np.float32)
+            y = np.zeros((n_out, nrhs), dtype=np.float32)
+            h2g.nearmul_(y, M, x)
+            assert relerr(y, O.nearmul(core, x.astype(np.float64), op)) < 1e-6, (op, nrhs)
