@@ -402,24 +402,30 @@ __global__ void __launch_bounds__(NT_LIST)
 // NCH 16-column tiles of the chunk, so every coupling value it reads feeds NCH MFMAs.  A block's
 // list is copied into the wave's part of LDS (one coalesced load per lane, issued while the
 // previous block computes), so the per-pair entries, which all lanes of a half share, are read
-// from LDS and not through the vector memory return path.  List entries are read two batches
-// (of 8 pairs) ahead of their MFMAs and source values one batch ahead; FP32 MFMA over 4 slot
-// pairs per partial sum, partial sums added in FP64 one batch later, when the MFMAs that
-// produced them have finished.
+// from LDS and not through the vector memory return path.  Source values load one batch (of 8
+// pairs) ahead into the other of two operand sets, list entries two batches ahead; FP32 MFMA over
+// 4 slot pairs per partial sum, each batch's partial sums added in FP64.  116-120 registers and
+// 35 KB of LDS per workgroup leave room for 4 workgroups (16 waves) per CU.
 // ---------------------------------------------------------------------------------------------
 constexpr int DR_NT = 256, DR_MAXB = 16;  // threads; batches of a block's list held in LDS
+// operand sets in flight: batch k's MFMAs read set k % DR_SETS while the loads of batch k + 1 fill
+// the other set (three sets, two batches ahead, need more registers than 4 waves per SIMD allow
+// and measured slower)
+constexpr int DR_SETS = 2;
 // narrowest batch for the direct kernel: it computes whole chunks of 32 columns (padding columns
 // are zero), which costs less than the cluster kernel's per-column work from about 4 columns on
 // (N = 2M, p = 6: 4 columns 44 ms either way, 8 columns 74 ms on the cluster kernel)
 constexpr int DR_MIN_R = 4;
 
 template <bool CONJ>
-__global__ void __launch_bounds__(DR_NT, 2)
+__global__ void __launch_bounds__(DR_NT, 4)
     m2l_direct_kernel(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr, const int4 *__restrict__ list,
                       int n_list, const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg, int n_src,
-                      int n_recv, const cplx<float> *__restrict__ S, int Rrow, cplx<float> *__restrict__ O, int R) {
+                      int n_recv, const cplx<float> *S, int Rrow, cplx<float> *__restrict__ O, int R) {
     __shared__ __align__(16) float ea[216 * 32];
-    __shared__ int4 lsd[DR_NT / 64][2][DR_MAXB * 8];  // per wave: lists of the current / next block
+    // per wave: the list of its current block (the next block's list waits in registers and is
+    // written after the wave has finished reading the current one)
+    __shared__ int4 lsd[DR_NT / 64][DR_MAXB * 8];
     const int grp = group_of_workgroup();
     const int bb0 = grp * gsz, bb1 = min(n_blocks, bb0 + gsz);
     if (bb0 >= bb1) return;
@@ -432,29 +438,28 @@ __global__ void __launch_bounds__(DR_NT, 2)
     const float sgn = (CONJ && ((m & 1) != bq)) ? -1.f : 1.f;
     // source rows of frequency f in the permuted column layout (columns c and c + 16 of a chunk
     // of 32 adjacent): 16-byte unit m of chunk r0 / 32 holds columns r0 + m and r0 + 16 + m
-    const char *__restrict__ Sfb = reinterpret_cast<const char *>(S + (int64_t)f * n_src * Rrow);
+    const char *Sfb = reinterpret_cast<const char *>(S + (int64_t)f * n_src * Rrow);
     const uint32_t rowb = (uint32_t)Rrow * (uint32_t)sizeof(cplx<float>);
     const uint32_t lo4 = (uint32_t)(r0 * (int)sizeof(cplx<float>) + m * 16);
     const char *eab = reinterpret_cast<const char *>(ea) + (m * 2 + bq) * (int)sizeof(float);
     float *__restrict__ Of = reinterpret_cast<float *>(O + (int64_t)f * n_recv * R);
     // list copies: lane l moves entries l and 64 + l (int4, two pairs each) of a block's list
-    int4 nx[2];
+    // (two named registers: an indexed array would be placed in LDS by the compiler)
+    int4 nx0, nx1;
     auto fetch = [&](int blk) {
         const int base = blk_ptr[blk] * 8;
-#pragma unroll
-        for (int u = 0; u < 2; ++u) nx[u] = list[min(base + 64 * u + l, n_list - 1)];
+        nx0 = list[min(base + l, n_list - 1)];
+        nx1 = list[min(base + 64 + l, n_list - 1)];
     };
     fetch(bb0 + w);
-    int buf = 0;
+    int4 *lw = lsd[w];
     for (int blk = bb0 + w; blk < bb1; blk += DR_NT / 64) {
         const int k0 = blk_ptr[blk], k1 = blk_ptr[blk + 1];
-        int4 *lw = lsd[w][buf];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) lw[64 * u + l] = nx[u];
+        lw[l] = nx0;
+        lw[64 + l] = nx1;
         if (blk + DR_NT / 64 < bb1) fetch(blk + DR_NT / 64);
         // batch k, half h: 8 entries (slot, coupling row byte offset) = 4 int4 at (k - k0) * 8 + 4 h
         const int4 *lh = lw + h * 4;
-        int4 L1[4], L2[4];
         auto ldl = [&](int k, int4(&L)[4]) {
             const int4 *p = lh + (min(k, k1 - 1) - k0) * 8;
 #pragma unroll
@@ -462,9 +467,7 @@ __global__ void __launch_bounds__(DR_NT, 2)
         };
         // per entry pair j (slot pairs 2j, 2j + 1): the lane of 16-lane row bq loads its half's
         // source row of pair 2j + bq (one 16-byte unit); the swap at use time sorts the values
-        float an[8];
-        float4 bn[4];
-        auto issue = [&](const int4(&L)[4]) {
+        auto issue = [&](const int4(&L)[4], float(&an)[8], float4(&bn)[4]) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 an[2 * j] = *reinterpret_cast<const float *>(eab + L[j].y);
@@ -473,34 +476,37 @@ __global__ void __launch_bounds__(DR_NT, 2)
                 bn[j] = *reinterpret_cast<const float4 *>(Sfb + (slot * rowb + lo4));
             }
         };
-        ldl(k0, L2);
-        issue(L2);  // operands of batch k0
-        ldl(k0 + 1, L1);
-        double4_m acc[2];
-        float4_m p0[2], p1[2];
+        // DR_SETS operand sets rotate: batch k's MFMA operands come from set k % DR_SETS while
+        // batch k + DR_SETS - 1 loads into set (k - 1) % DR_SETS, whose operands have been taken;
+        // Lx is the list of the next batch to issue
+        int4 Lx[4];
+        float an[DR_SETS][8];
+        float4 bn[DR_SETS][4];
 #pragma unroll
-        for (int g = 0; g < 2; ++g) {
-            acc[g] = double4_m{0.0, 0.0, 0.0, 0.0};
-            p0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
-            p1[g] = p0[g];
+        for (int u = 0; u < DR_SETS - 1; ++u) {
+            ldl(k0 + u, Lx);
+            issue(Lx, an[u], bn[u]);
         }
-        // batch k: operands (an, bn) loaded; Lc = list of batch k + 1; Ln receives batch k + 2.
-        // A batch past the block's last (k = k1 on an odd count) loads clamped (valid) operands
-        // and skips its MFMAs.
-        auto step = [&](int k, const int4(&Lc)[4], int4(&Ln)[4]) {
+        ldl(k0 + DR_SETS - 1, Lx);
+        double4_m acc[2];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) acc[g] = double4_m{0.0, 0.0, 0.0, 0.0};
+        // a batch past the block's last (the trip's tail) loads clamped (valid) operands and
+        // skips its MFMAs
+        auto step = [&](int k, const float(&anc)[8], const float4(&bnc)[4], float(&ann)[8], float4(&bnn)[4]) {
             float ac[8], bc[8][2];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                ac[2 * j] = an[2 * j] * sgn;
-                ac[2 * j + 1] = an[2 * j + 1] * sgn;
+                ac[2 * j] = anc[2 * j] * sgn;
+                ac[2 * j + 1] = anc[2 * j + 1] * sgn;
                 // the lane holds (re, im) of columns m and 16 + m of pair 2j + bq; swapping the
                 // odd 16-lane rows of the re parts with the even rows of the im parts leaves it
                 // with part bq of those columns for pairs 2j (x, z) and 2j + 1 (y, w).  (Elements
                 // are copied out of the returned pairs before reinterpreting them:
                 // __builtin_bit_cast of a vector element reads element 0 with this compiler.)
-                const auto s0 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bn[j].x), __float_as_uint(bn[j].y),
+                const auto s0 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bnc[j].x), __float_as_uint(bnc[j].y),
                                                                  false, false);
-                const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bn[j].z), __float_as_uint(bn[j].w),
+                const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bnc[j].z), __float_as_uint(bnc[j].w),
                                                                  false, false);
                 const uint32_t x = s0[0], y = s0[1], z = s1[0], ww = s1[1];
                 bc[2 * j][0] = __uint_as_float(x);
@@ -508,21 +514,19 @@ __global__ void __launch_bounds__(DR_NT, 2)
                 bc[2 * j + 1][0] = __uint_as_float(y);
                 bc[2 * j + 1][1] = __uint_as_float(ww);
             }
-            ldl(k + 2, Ln);
-            issue(Lc);
+            issue(Lx, ann, bnn);
+            ldl(k + DR_SETS, Lx);
             // the next batches' loads stay ahead of this batch's MFMAs: not sunk to their uses
             // (memory clobber) and not scheduled after the MFMAs (scheduling barrier)
             __asm__ volatile("" ::: "memory");
             __builtin_amdgcn_sched_barrier(0);
-            float4_m f0[2], f1[2];
-#pragma unroll
-            for (int g = 0; g < 2; ++g) {
-                f0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
-                f1[g] = f0[g];
-            }
-            // a batch past the block's last (odd batch count) skips its MFMAs (wave-uniform
-            // branch after the loads, which stay unconditional)
             if (k < k1) {
+                float4_m f0[2], f1[2];
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    f0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
+                    f1[g] = f0[g];
+                }
 #pragma unroll
                 for (int i = 0; i < 8; i += 2)
 #pragma unroll
@@ -530,25 +534,19 @@ __global__ void __launch_bounds__(DR_NT, 2)
                         f0[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i], bc[i][g], f0[g], 0, 0, 0);
                         f1[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i + 1], bc[i + 1][g], f1[g], 0, 0, 0);
                     }
-            }
 #pragma unroll
-            for (int g = 0; g < 2; ++g) {
+                for (int g = 0; g < 2; ++g)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[g][v] += (double)p0[g][v] + (double)p1[g][v];
-                p0[g] = f0[g];
-                p1[g] = f1[g];
+                    for (int v = 0; v < 4; ++v) acc[g][v] += (double)f0[g][v] + (double)f1[g][v];
             }
         };
-        // two batches per trip, the list registers swapping roles (no copies that would wait for
-        // the loads in flight, no branch around the MFMAs)
-        for (int k = k0; k < k1; k += 2) {
-            step(k, L1, L2);
-            step(k + 1, L2, L1);
+        for (int k = k0; k < k1; k += DR_SETS) {
+#pragma unroll
+            for (int u = 0; u < DR_SETS; ++u)
+                step(k + u, an[u], bn[u], an[(u + DR_SETS - 1) % DR_SETS], bn[(u + DR_SETS - 1) % DR_SETS]);
         }
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[g][v] += (double)p0[g][v] + (double)p1[g][v];
             // FP32 16x16x4 result layout: rows 4 (l / 16) + v = (target 2 (l / 16) + v / 2, re / im v % 2)
             const int col = r0 + 16 * g + m;
             if (col < R) {
@@ -564,7 +562,6 @@ __global__ void __launch_bounds__(DR_NT, 2)
                 }
             }
         }
-        buf ^= 1;
     }
 }
 
