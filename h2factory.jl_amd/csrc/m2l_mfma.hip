@@ -408,6 +408,9 @@ __global__ void __launch_bounds__(NT_LIST)
 // 35 KB of LDS per workgroup leave room for 4 workgroups (16 waves) per CU.
 // ---------------------------------------------------------------------------------------------
 constexpr int DR_NT = 256, DR_MAXB = 16;  // threads; batches of a block's list held in LDS
+// workgroups per frequency (groups of consecutive blocks); measured on the N = 2M, p = 6 map with
+// 4 workgroups per CU: 128 groups 33.7 ms, 256 32.5 ms, 512 to 1024 32.0 to 32.3 ms (levels 4-6)
+constexpr int DR_GROUPS = 512;
 // operand sets in flight: batch k's MFMAs read set k % DR_SETS while the loads of batch k + 1 fill
 // the other set (three sets, two batches ahead, need more registers than 4 waves per SIMD allow
 // and measured slower)
@@ -637,10 +640,10 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
     if (m2l_direct_ok(cdtype, LL, n_src, R)) {
         const LatticeDirect &D = LL.direct;
         // groups of consecutive blocks: at least two blocks per wave (the coupling table is
-        // loaded once per group), and about 256 groups per frequency, so that the workgroups
-        // resident on an XCD work on few frequencies at a time (L2 reuse of source rows); the
-        // group size was chosen by measurement on the N = 2M, p = 6 map (1024 .. 64 groups)
-        const int gsz = std::max(2 * DR_NT / 64, D.n_blocks / 256);
+        // loaded once per group), and about DR_GROUPS groups per frequency, so that the
+        // workgroups resident on an XCD work on few frequencies at a time (L2 reuse of source
+        // rows)
+        const int gsz = std::max(2 * DR_NT / 64, D.n_blocks / DR_GROUPS);
         const int ng = (D.n_blocks + gsz - 1) / gsz;
         const int Rrow = (R + 31) / 32 * 32;
         const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)(Rrow / 32));
