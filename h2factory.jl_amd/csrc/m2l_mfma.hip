@@ -560,7 +560,10 @@ __global__ void __launch_bounds__(DR_NT, 4)
                         float2 o;
                         o.x = (float)acc[g][2 * u];
                         o.y = (float)acc[g][2 * u + 1];
-                        *reinterpret_cast<float2 *>(Of + ((int64_t)tgi * R + col) * 2) = o;
+                        // streaming store: the receiving spectra are read once, by modulate-out
+                        uint64_t u;
+                        __builtin_memcpy(&u, &o, 8);
+                        __builtin_nontemporal_store(u, reinterpret_cast<uint64_t *>(Of + ((int64_t)tgi * R + col) * 2));
                     }
                 }
             }
