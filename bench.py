@@ -69,8 +69,10 @@ def algorithmic_work(arrays, nrhs, phase, level, mixed):
         lv = tr.level[arrays.pairs[:, 0]]
         npairs = int(np.sum(lv == level))
         if arrays.m2l_kind in ("toeplitz", "symtoeplitz"):
-            # one complex multiply-add per stored frequency: half spectrum (p L^2) for real maps
-            F = p * (2 * p - 1) ** 2 if arrays.dtype == "f64" else L3
+            # one complex multiply-add per stored frequency: for real maps the half spectrum
+            # without the conjugate half of its plane fx = 0, ((L^2 + 1) / 2 + (p - 1) L^2)
+            L = 2 * p - 1
+            F = (L * L + 1) // 2 + (p - 1) * L * L if arrays.dtype == "f64" else L3
             return "flop", npairs * F * 8.0 * nrhs
         return "flop", npairs * 2.0 * K * K * nrhs
     if phase in ("p2m", "l2p"):

@@ -217,9 +217,42 @@ struct LatticeDirect {
     DevBuf tgt;   // int32 [n_blocks][8]: receiving index of the block's children or -1
 };
 
+// Stored frequencies of the lattice levels' frequency-major spectra.  Full spectra: f = fx + L (fy
+// + L fz).  Half spectra (real input, Hermitian couplings): a value at (fx, fy, fz) is the
+// conjugate of the one at (-fx, -fy, -fz) (indices mod L), so only fx < (L + 1) / 2 is kept and,
+// in the self-conjugate plane fx = 0, only one of each pair: fz = 0 with fy <= (L - 1) / 2, and
+// 1 <= fz <= (L - 1) / 2 with any fy; (L^2 + 1) / 2 values, stored first, then the planes fx >= 1.
+__host__ __device__ inline int half_freq_count(int L) { return (L * L + 1) / 2 + (L - 1) / 2 * L * L; }
+__host__ __device__ inline bool half_freq_kept(int L, int fx, int fy, int fz) {
+    return fx > 0 || (fz == 0 ? fy <= (L - 1) / 2 : fz <= (L - 1) / 2);
+}
+// index of a kept frequency
+__host__ __device__ inline int half_freq_index(int L, int fx, int fy, int fz) {
+    if (fx == 0) return fz == 0 ? fy : (L + 1) / 2 + (fz - 1) * L + fy;
+    return (L * L + 1) / 2 + (fx - 1) + (L - 1) / 2 * (fy + L * fz);
+}
+__host__ __device__ inline void half_freq_of(int L, int f, int &fx, int &fy, int &fz) {
+    const int h = (L * L + 1) / 2;
+    if (f < h) {
+        fx = 0;
+        if (f < (L + 1) / 2) {
+            fz = 0;
+            fy = f;
+        } else {
+            fz = 1 + (f - (L + 1) / 2) / L;
+            fy = (f - (L + 1) / 2) % L;
+        }
+    } else {
+        const int FX1 = (L - 1) / 2, t = f - h;
+        fx = 1 + t % FX1;
+        fy = (t / FX1) % L;
+        fz = t / FX1 / L;
+    }
+}
+
 struct LatticeLevel {
     bool ok = false;
-    int F = 0;                // stored frequencies
+    int F = 0;                // stored frequencies (L^3, or half_freq_count(L) for half spectra)
     LatticeTiles clus;        // cluster tiles (any batch width)
     LatticeTiles list;        // list tiles (batches of >= 16 columns, FP32 spectra)
     LatticeDirect direct;     // direct lists (batches of >= 16 columns, FP32 spectra)

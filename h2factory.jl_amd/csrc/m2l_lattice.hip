@@ -138,10 +138,13 @@ __global__ void fft_in_fm_kernel(int box0, int nbox, const T *__restrict__ mom, 
         for (int iz = 0; iz < p; ++iz) in[iz] = b2[j * n2 + fxy + FX * L * iz];
 #pragma unroll
         for (int fz = 0; fz < L; ++fz) {
+            // half spectra: the conjugate half of the plane fx = 0 is not stored
+            if (HALF && !half_freq_kept(L, fxy % FX, fxy / FX, fz)) continue;
             C acc = zero_v<C>();
 #pragma unroll
             for (int iz = 0; iz < p; ++iz) fmacc(acc, tw[(fz * iz) % L], in[iz]);
-            store_stream(&S[(int64_t)(fxy + FX * L * fz) * njobs + job], acc * scale);
+            const int f = HALF ? half_freq_index(L, fxy % FX, fxy / FX, fz) : fxy + FX * L * fz;
+            store_stream(&S[(int64_t)f * njobs + job], acc * scale);
         }
     }
 }
@@ -171,9 +174,23 @@ __global__ void fft_out_fm_kernel(const int32_t *__restrict__ recv_boxes, int nr
         const int j = i % JB, fxy = i / JB;
         const int64_t job = job0 + j;
         C in[L];
+        // every load is issued (clamped job, valid address) and selected afterwards, so the L loads
+        // of a line go out together; half spectra: a value of the plane fx = 0 that is not stored
+        // is the conjugate of the stored one at (0, -fy, -fz)
+        const int64_t jc = job < njobs ? job : njobs - 1;
+        const int fx = fxy % FX, fy = fxy / FX;
 #pragma unroll
-        for (int fz = 0; fz < L; ++fz)
-            in[fz] = job < njobs ? load_stream(&O[(int64_t)(fxy + FX * L * fz) * njobs + job]) : zero_v<C>();
+        for (int fz = 0; fz < L; ++fz) {
+            int f = fxy + FX * L * fz;
+            bool conj = false;
+            if (HALF) {
+                conj = !half_freq_kept(L, fx, fy, fz);
+                f = conj ? half_freq_index(L, 0, (L - fy) % L, (L - fz) % L) : half_freq_index(L, fx, fy, fz);
+            }
+            const C v = load_stream(&O[(int64_t)f * njobs + jc]);
+            const Rl z = job < njobs ? (Rl)1 : (Rl)0;
+            in[fz] = C{v.re * z, (conj ? -v.im : v.im) * z};
+        }
 #pragma unroll
         for (int iz = 0; iz < p; ++iz) {
             C acc = zero_v<C>();

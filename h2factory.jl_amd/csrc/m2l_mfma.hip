@@ -584,7 +584,14 @@ __global__ void build_ea_kernel(const int32_t *__restrict__ tq_op, const int32_t
     const int f = (int)(idx / (216 * 32));
     const int t = m >> 1, c = m & 1;
     const int L3 = L * L * L;
-    const int fx = f % FX, fy = (f / FX) % L, fz = f / (FX * L);
+    int fx, fy, fz;
+    if (FX == L) {
+        fx = f % L;
+        fy = (f / L) % L;
+        fz = f / (L * L);
+    } else {
+        half_freq_of(L, f, fx, fy, fz);
+    }
     const int ffull = fx + L * (fy + L * fz);
     const int op = tq_op[t * 216 + q];
     double v = 0.0;
@@ -682,7 +689,7 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
 void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
                       bool half, bool want32, LatticeLevel &LL) {
     const int FX = half ? (L + 1) / 2 : L;
-    LL.F = FX * L * L;
+    LL.F = half ? half_freq_count(L) : L * L * L;
     const int64_t n = (int64_t)LL.F * 216 * 32;
     const unsigned nb = (unsigned)((n + 255) / 256);
     LL.EA64.alloc((size_t)n * sizeof(double));
