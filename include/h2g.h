@@ -63,8 +63,10 @@ enum { H2G_M2L_DENSE = 0, H2G_M2L_TOEPLITZ = 1, H2G_M2L_SYMDENSE = 2, H2G_M2L_SY
 
 /* Arithmetic of the far field in a workspace.
  *   PLAIN  everything in the core's type (Float64 / ComplexF64): the drop-in.
- *   MIXED  P2M, M2M, M2L, L2L, L2P in FP32 (ComplexF32); near field, the near+far sum and all
- *          precomputed spectra stay FP64. */
+ *   MIXED  P2M, M2M, M2L, L2L, L2P in FP32 (ComplexF32); the near field of real cores reads an
+ *          FP32 copy of the near values (made at core creation, or by h2g_method_fill) and
+ *          multiplies and accumulates in FP64; the near+far sum and all precomputed spectra stay
+ *          FP64. */
 enum { H2G_PREC_PLAIN = 0, H2G_PREC_MIXED = 1 };
 
 /* Flags of h2g_matvec. */
