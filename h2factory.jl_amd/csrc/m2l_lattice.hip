@@ -19,6 +19,14 @@ namespace h2g {
 namespace {
 
 constexpr int TPB = 256;
+// FP32 half spectra (the mixed form of real maps): jobs and threads per workgroup of the two
+// modulation kernels, measured on the N = 2M, p = 6 map (levels 5 + 6, ms; 256 threads and 16 jobs
+// each: 6.83 in, 6.15 out): modulate-in 32 jobs (157 KB of LDS, one workgroup per CU) on 1024
+// threads 5.81, on 512 threads 5.92; modulate-out 16 jobs on 1024 threads 5.12, on 512 threads
+// 5.38, 32 jobs on 1024 threads 5.37.  More threads per job shorten each stage's rounds over the
+// workgroup's lines; more jobs per workgroup lengthen modulate-in's contiguous stores per frequency
+// (32 jobs x 8 bytes).
+constexpr int FFT_IN_JB = 32, FFT_IN_TPB = 1024, FFT_OUT_JB = 16, FFT_OUT_TPB = 1024;
 
 // streaming (nontemporal) access to the spectra, which are written once and read once from HBM
 // (measured on the N = 2M, p = 6 map: modulate-in, modulate-out and the coupling stage together
@@ -253,7 +261,7 @@ size_t fft_smem(int p, int L) {
     return sizeof(C) * (size_t)(JB * (FX * p * p + FX * L * p));
 }
 
-// jobs per workgroup: 16 for FP32 half spectra at p = 6 (78 KB of LDS), fewer for wider types
+// jobs per workgroup (other than FP32 half spectra): 16 FP32, 8 FP64 (78 KB of LDS at p = 6)
 template <class T, bool HALF>
 constexpr int fft_jobs() {
     return (sizeof(typename real_of<T>::type) == 4 ? 16 : 8) / (HALF ? 1 : 2) / (is_complex<T>::value ? 1 : 1);
@@ -291,10 +299,18 @@ bool allow_lds(K kernel, size_t bytes) {
         if (e_ != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e_)}; \
     } while (0)
 
+// FP32 half spectra (the mixed form of real maps): jobs and threads per workgroup of each
+// modulation kernel
+template <class T, bool HALF>
+constexpr bool fft_f32_half() {
+    return HALF && !is_complex<T>::value && sizeof(T) == 4;
+}
+
 template <class T, bool HALF, int P>
 static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, bool perm, void *S, const void *tw,
                         bool inverse, hipStream_t s) {
-    constexpr int JB = fft_jobs<T, HALF>();
+    constexpr int JB = fft_f32_half<T, HALF>() ? FFT_IN_JB : fft_jobs<T, HALF>();
+    constexpr int NT = fft_f32_half<T, HALF>() ? FFT_IN_TPB : TPB;
     constexpr int JH = JB / 2;
     constexpr int L = 2 * P - 1;
     const int Rrow = perm ? (R + 31) / 32 * 32 : R;
@@ -302,10 +318,10 @@ static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, bool 
     const size_t sh = fft_smem<JB, T, HALF>(P, L);
     using C = typename complex_of<T>::type;
     if (allow_lds(fft_in_fm_kernel<T, HALF, JB, P>, sh)) {
-        fft_in_fm_kernel<T, HALF, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+        fft_in_fm_kernel<T, HALF, JB, P><<<(unsigned)((njobs + JB - 1) / JB), NT, sh, s>>>(
             box0, nbox, (const T *)mom, K, R, Rrow, perm, (C *)S, (const cplx<double> *)tw, inverse);
     } else {
-        fft_in_fm_kernel<T, HALF, JH, P><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(P, L), s>>>(
+        fft_in_fm_kernel<T, HALF, JH, P><<<(unsigned)((njobs + JH - 1) / JH), NT, fft_smem<JH, T, HALF>(P, L), s>>>(
             box0, nbox, (const T *)mom, K, R, Rrow, perm, (C *)S, (const cplx<double> *)tw, inverse);
     }
     H2G_CHECK_LAUNCH2();
@@ -314,17 +330,18 @@ static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, bool 
 template <class T, bool HALF, int P>
 static void fft_out_fm_p(const int32_t *recv, int nrecv, const void *O, int K, int R, void *loc, const void *tw,
                          bool inverse, hipStream_t s) {
-    constexpr int JB = fft_jobs<T, HALF>();
+    constexpr int JB = fft_f32_half<T, HALF>() ? FFT_OUT_JB : fft_jobs<T, HALF>();
+    constexpr int NT = fft_f32_half<T, HALF>() ? FFT_OUT_TPB : TPB;
     constexpr int JH = JB / 2;
     constexpr int L = 2 * P - 1;
     const int64_t njobs = (int64_t)nrecv * R;
     const size_t sh = fft_smem<JB, T, HALF>(P, L);
     using C = typename complex_of<T>::type;
     if (allow_lds(fft_out_fm_kernel<T, HALF, JB, P>, sh)) {
-        fft_out_fm_kernel<T, HALF, JB, P><<<(unsigned)((njobs + JB - 1) / JB), TPB, sh, s>>>(
+        fft_out_fm_kernel<T, HALF, JB, P><<<(unsigned)((njobs + JB - 1) / JB), NT, sh, s>>>(
             recv, nrecv, (const C *)O, K, R, (T *)loc, (const cplx<double> *)tw, inverse);
     } else {
-        fft_out_fm_kernel<T, HALF, JH, P><<<(unsigned)((njobs + JH - 1) / JH), TPB, fft_smem<JH, T, HALF>(P, L), s>>>(
+        fft_out_fm_kernel<T, HALF, JH, P><<<(unsigned)((njobs + JH - 1) / JH), NT, fft_smem<JH, T, HALF>(P, L), s>>>(
             recv, nrecv, (const C *)O, K, R, (T *)loc, (const cplx<double> *)tw, inverse);
     }
     H2G_CHECK_LAUNCH2();
