@@ -969,14 +969,21 @@ __global__ void sep_l2l_kernel(int opA, int n_par, const int32_t *__restrict__ p
     }
 }
 
+// threads per workgroup of the shared-factor L2L for FP32 (8 jobs per workgroup), measured on the
+// N = 2M, p = 6 map (levels 5 + 6): 256 threads 1.97 ms, 512 threads 1.57 ms; 16 jobs on 512 or
+// 1024 threads and 4 jobs on 256 threads were slower.  The shared-factor M2M is fastest at 256.
+constexpr int SEP_L2L_NT4 = 512;
+// threads per workgroup of the MFMA leaf kernels (the waves of a workgroup take its row tiles):
+// 512 measured 0.05-0.1 ms faster than 256 per level on the N = 2M, p = 6 map, 1024 slower for P2M
+constexpr int L2P_NT = 512, P2M_NT = 512;
 // L2L grouped by parent when the Kronecker factors are shared between sectors (the x factor
 // depends on the child's x bit only, and so on: interpolation transfers).  The children's
 // transforms then share stages: 2 x-line variants (child x bit), 4 y variants (x, y bits) and 8
 // z variants, 14 line batches instead of 24, all in one pass with two barriers.  The children's
 // current expansions (accumulated when they already hold couplings) are loaded at the start and
 // used by the last stage, so their latency overlaps the first two stages.
-template <class T, int JB, int P>
-__global__ void __launch_bounds__(TPB) sep_l2l_shared_kernel(int opA, int n_par, const int32_t *__restrict__ parents,
+template <class T, int JB, int P, int NT>
+__global__ void __launch_bounds__(NT) sep_l2l_shared_kernel(int opA, int n_par, const int32_t *__restrict__ parents,
                                                             const int64_t *__restrict__ child_ptr,
                                                             const int32_t *__restrict__ child_box,
                                                             const int32_t *__restrict__ child_mat,
@@ -984,7 +991,7 @@ __global__ void __launch_bounds__(TPB) sep_l2l_shared_kernel(int opA, int n_par,
                                                             const T *__restrict__ factors, int mat_base,
                                                             T *__restrict__ loc, int R) {
     constexpr int p = P, K = P * P * P, LINES = P * P;
-    constexpr int ZL = JB * 8 * LINES, ZPT = (ZL + TPB - 1) / TPB;  // z lines, per thread
+    constexpr int ZL = JB * 8 * LINES, ZPT = (ZL + NT - 1) / NT;  // z lines, per thread
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *pin = reinterpret_cast<T *>(smem_raw);  // [JB][K] parent
     T *t1 = pin + JB * K;                      // [JB][2][K] x stage per child x bit
@@ -1029,7 +1036,7 @@ __global__ void __launch_bounds__(TPB) sep_l2l_shared_kernel(int opA, int n_par,
     T pre[ZPT][p];
 #pragma unroll
     for (int u = 0; u < ZPT; ++u) {
-        const int e = threadIdx.x + u * TPB;
+        const int e = threadIdx.x + u * NT;
         const int js = e / LINES, line = e - js * LINES;  // js = j * 8 + sector
         const int box = e < ZL ? s_box[js] : -1;
         const bool acc = box >= 0 && s_acc[js] != 0;
@@ -1076,7 +1083,7 @@ __global__ void __launch_bounds__(TPB) sep_l2l_shared_kernel(int opA, int n_par,
     // z: line (kx, ky) of t2[j][s % 4] -> child expansion of sector s
 #pragma unroll
     for (int u = 0; u < ZPT; ++u) {
-        const int e = threadIdx.x + u * TPB;
+        const int e = threadIdx.x + u * NT;
         if (e >= ZL) break;
         const int js = e / LINES, line = e - js * LINES;
         const int box = s_box[js];
@@ -1104,15 +1111,15 @@ __global__ void __launch_bounds__(TPB) sep_l2l_shared_kernel(int opA, int n_par,
 // the y stage sums over y bits (2 results), the z stage over z bits: 14 line batches, two
 // barriers.  The children's lines are loaded at the start, all at once.  A missing child (empty
 // sector) contributes zero.  The parent's moment is overwritten.
-template <class T, int JB, int P>
-__global__ void __launch_bounds__(TPB) sep_m2m_shared_kernel(int opA, int n_targets, const int32_t *__restrict__ targets,
+template <class T, int JB, int P, int NT>
+__global__ void __launch_bounds__(NT) sep_m2m_shared_kernel(int opA, int n_targets, const int32_t *__restrict__ targets,
                                                             const int64_t *__restrict__ item_ptr,
                                                             const int32_t *__restrict__ item_src,
                                                             const int32_t *__restrict__ item_mat,
                                                             const T *__restrict__ factors, int mat_base,
                                                             T *__restrict__ mom, int R) {
     constexpr int p = P, K = P * P * P, LINES = P * P;
-    constexpr int XL = JB * 4 * LINES, XPT = (XL + TPB - 1) / TPB;  // x items, per thread
+    constexpr int XL = JB * 4 * LINES, XPT = (XL + NT - 1) / NT;  // x items, per thread
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *w = reinterpret_cast<T *>(smem_raw);  // [JB][4 (y, z bits)][K]
     T *v = w + JB * 4 * K;                   // [JB][2 (z bit)][K]
@@ -1140,7 +1147,7 @@ __global__ void __launch_bounds__(TPB) sep_m2m_shared_kernel(int opA, int n_targ
     T in[XPT][2][p];
 #pragma unroll
     for (int u = 0; u < XPT; ++u) {
-        const int e = threadIdx.x + u * TPB;
+        const int e = threadIdx.x + u * NT;
         const int jq = e / LINES, line = e - jq * LINES;  // jq = j * 4 + syz
         const int j = jq >> 2, syz = jq & 3;
         const int r = (int)((job0 + j) % R);
@@ -1154,7 +1161,7 @@ __global__ void __launch_bounds__(TPB) sep_m2m_shared_kernel(int opA, int n_targ
     }
 #pragma unroll
     for (int u = 0; u < XPT; ++u) {
-        const int e = threadIdx.x + u * TPB;
+        const int e = threadIdx.x + u * NT;
         if (e >= XL) break;
         const int jq = e / LINES, line = e - jq * LINES;
 #pragma unroll
@@ -1224,10 +1231,11 @@ template <class T, int P>
 static void sep_m2m_p(int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr, const int32_t *item_src,
                       const int32_t *item_mat, const void *factors, int mat_base, void *mom, int R, hipStream_t s) {
     constexpr int JS = 32 / (int)sizeof(T);  // jobs per workgroup: 8 (FP32), 4 (FP64)
+    constexpr int NT = TPB;
     constexpr int K = P * P * P;
     const int64_t njobs = (int64_t)n_targets * R;
     const size_t sh = sizeof(T) * (size_t)(6 * JS * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(8 * JS);
-    sep_m2m_shared_kernel<T, JS, P><<<(unsigned)((njobs + JS - 1) / JS), TPB, sh, s>>>(
+    sep_m2m_shared_kernel<T, JS, P, NT><<<(unsigned)((njobs + JS - 1) / JS), NT, sh, s>>>(
         opA, n_targets, targets, item_ptr, item_src, item_mat, (const T *)factors, mat_base, (T *)mom, R);
 }
 
@@ -1240,8 +1248,9 @@ static void sep_l2l_p(int opA, int n_par, const int32_t *parents, const int64_t 
     const int64_t njobs = (int64_t)n_par * R;
     if (shared) {
         constexpr int JS = 32 / (int)sizeof(T);  // jobs per workgroup: 8 (FP32), 4 (FP64)
+        constexpr int NT = sizeof(T) == 4 ? SEP_L2L_NT4 : TPB;
         const size_t shs = sizeof(T) * (size_t)(7 * JS * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(17 * JS);
-        sep_l2l_shared_kernel<T, JS, P><<<(unsigned)((njobs + JS - 1) / JS), TPB, shs, s>>>(
+        sep_l2l_shared_kernel<T, JS, P, NT><<<(unsigned)((njobs + JS - 1) / JS), NT, shs, s>>>(
             opA, n_par, parents, child_ptr, child_box, child_mat, child_acc, (const T *)factors, mat_base, (T *)loc, R);
         return;
     }
@@ -1454,7 +1463,7 @@ __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__res
 // the contraction order does not change the product, and every lane reads its operand rows
 // contiguously.  One workgroup (4 waves) per (leaf, group of NCT 16-column RHS tiles).
 template <class T, int NCT, bool STAGE>
-__global__ void __launch_bounds__(TPB, 2)
+__global__ void __launch_bounds__(L2P_NT)
     l2p_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                     const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const T *__restrict__ loc,
                     int R, double *__restrict__ ys, double alpha) {
@@ -1472,7 +1481,7 @@ __global__ void __launch_bounds__(TPB, 2)
     const int ncol = min(NCT * 16, R - c0);
     const T *lb = loc + ((int64_t)b * R + c0) * K;
     if (STAGE) {
-        for (int e = threadIdx.x; e < ncol * K; e += TPB) {
+        for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
             const int c = e / K, k = e - c * K;
             lt[c * KP + k] = lb[e];
         }
@@ -1482,7 +1491,7 @@ __global__ void __launch_bounds__(TPB, 2)
     const int pitch = STAGE ? KP : K;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
-    for (int rt = wave; rt * 16 < n; rt += TPB / 64) {
+    for (int rt = wave; rt * 16 < n; rt += (int)(blockDim.x >> 6)) {
         const int row = rt * 16 + m;
         const T *crow = coll + (j0 + min(row, n - 1)) * K;
         double4_t acc[NCT];
@@ -1525,7 +1534,7 @@ __global__ void __launch_bounds__(TPB, 2)
 }
 
 template <class T, int NCT>
-__global__ void __launch_bounds__(TPB, 2)
+__global__ void __launch_bounds__(P2M_NT)
     p2m_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
                     const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const double *__restrict__ xs,
                     int R, T *__restrict__ mom) {
@@ -1541,7 +1550,7 @@ __global__ void __launch_bounds__(TPB, 2)
     const int ncol = min(NCT * 16, R - c0);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = lane & 15, kq = lane >> 4;
-    for (int rt = wave; rt * 16 < K; rt += TPB / 64) {
+    for (int rt = wave; rt * 16 < K; rt += (int)(blockDim.x >> 6)) {
         const int krow = rt * 16 + m;
         double4_t acc[NCT];
 #pragma unroll
@@ -1579,7 +1588,7 @@ __global__ void __launch_bounds__(TPB, 2)
     }
     __syncthreads();
     T *mb = mom + ((int64_t)b * R + c0) * K;
-    for (int e = threadIdx.x; e < ncol * K; e += TPB) {
+    for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
         const int c = e / K, k = e - c * K;
         mb[e] = mt[c * KP + k];
     }
@@ -1626,7 +1635,7 @@ bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
     auto go = [&](auto kern, auto tv) {
         using T = decltype(tv);
         const size_t sh = stage ? sizeof(T) * (size_t)nct * 16 * (K + 1) : 0;
-        kern<<<grid, TPB, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys, alpha);
+        kern<<<grid, L2P_NT, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys, alpha);
     };
     const bool f64 = cdtype == H2G_F64;
     switch (nct) {
@@ -1650,7 +1659,7 @@ bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
     auto go = [&](auto kern, auto tv) {
         using T = decltype(tv);
         const size_t sh = sizeof(T) * (size_t)nct * 16 * (K + 1);
-        kern<<<grid, TPB, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const double *)xs, R, (T *)mom);
+        kern<<<grid, P2M_NT, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const double *)xs, R, (T *)mom);
     };
     const bool f64 = cdtype == H2G_F64;
     switch (nct) {
