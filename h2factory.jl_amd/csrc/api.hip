@@ -334,6 +334,19 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
     }
 }
 
+// Spectrum elements per column one orientation needs: the largest level's source boxes (in) and
+// receiving boxes (out) times the frequencies the level stores (lattice levels: LL.F, half spectra
+// for real maps; the per-pair Hadamard levels: L^3).
+void spectrum_sizes(const Core &c, const OrientPlan &P, const Side &src, int64_t &fin, int64_t &fout) {
+    for (int l = 0; l < (int)P.m2l_recv.size(); ++l) {
+        if (P.m2l_recv[l].count == 0) continue;
+        const int64_t F = l < (int)P.lattice.size() && P.lattice[l].ok ? P.lattice[l].F : c.L3;
+        if (l < src.n_levels)
+            fin = std::max<int64_t>(fin, (int64_t)(src.level_offsets[l + 1] - src.level_offsets[l]) * F);
+        fout = std::max<int64_t>(fout, (int64_t)P.m2l_recv[l].count * F);
+    }
+}
+
 int level_of(const std::vector<int32_t> &off, int b) {
     return (int)(std::upper_bound(off.begin(), off.end(), b) - off.begin()) - 1;
 }
@@ -1276,13 +1289,15 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             w->loc.alloc((size_t)bmax * c.K * w->R_cap * ce);
             if (is_toeplitz(c)) {
                 const size_t cc = mixed ? 8 : 16;
-                const int64_t ns = std::max(c.fwd.max_src_level_boxes, c.rev.max_src_level_boxes);
-                const int64_t nr = std::max(c.fwd.max_recv_level_boxes, c.rev.max_recv_level_boxes);
+                // per column, the largest level's (boxes x stored frequencies) of either orientation
+                int64_t fin = 0, fout = 0;
+                spectrum_sizes(c, c.fwd, *c.trial, fin, fout);
+                spectrum_sizes(c, c.rev, c.test, fin, fout);
                 // source spectra of FP32 batches of >= 4 columns may be stored with rows padded to
                 // 32 columns (the direct M2L kernel's layout)
                 const int64_t rin = mixed && w->R_cap >= 4 ? (w->R_cap + 31) / 32 * 32 : w->R_cap;
-                w->spec_in.alloc((size_t)ns * rin * c.L3 * cc);
-                w->spec_out.alloc((size_t)nr * w->R_cap * c.L3 * cc);
+                w->spec_in.alloc((size_t)fin * rin * cc);
+                w->spec_out.alloc((size_t)fout * w->R_cap * cc);
             }
         } catch (...) {
             if (w->ev_near) (void)hipEventDestroy(w->ev_near);

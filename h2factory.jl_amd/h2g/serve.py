@@ -39,6 +39,15 @@ from .maps import GpuH2Core, H2Workspace, _raise
 OPS = {"N": H.H2G_OP_N, "T": H.H2G_OP_T, "C": H.H2G_OP_C}
 
 
+def _spectrum_values(arrays):
+    """Stored frequencies per box of a Toeplitz map (L^3; real maps about half: one value of each
+    conjugate pair), the estimate of the workspaces' spectrum buffers; 0 for dense couplings."""
+    if arrays.m2l_kind not in ("toeplitz", "symtoeplitz"):
+        return 0
+    L = 2 * arrays.p - 1
+    return L ** 3 if arrays.dtype == "c64" else (L * L + 1) // 2 + (arrays.p - 1) * L * L
+
+
 class AdmissionError(RuntimeError):
     """A map does not fit the server's HBM budget."""
 
@@ -149,7 +158,8 @@ class Server:
         n = arrays.test.tree.points.shape[0]
         es = 16 if arrays.dtype == "c64" else 8
         ws_est = self.ws_per_map * (n * self.max_batch * (4 * es) + arrays.test.tree.n_boxes * arrays.K *
-                                    self.max_batch * es + 2 * 726 * arrays.test.tree.n_boxes * self.max_batch * es)
+                                    self.max_batch * es + 2 * _spectrum_values(arrays) *
+                                    arrays.test.tree.n_boxes * self.max_batch * es)
         if self.used_bytes() + est + ws_est > self.budget:
             raise AdmissionError(f"map {name!r} needs ~{(est + ws_est) / 1e9:.1f} GB, "
                                  f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
@@ -169,7 +179,7 @@ class Server:
         es = 16 if a.dtype == "c64" else 8
         n = max(core.shape)
         ws_est = self.ws_per_map * (n * self.max_batch * (4 * es) + a.test.tree.n_boxes * a.K * self.max_batch * es +
-                                    2 * 726 * a.test.tree.n_boxes * self.max_batch * es)
+                                    2 * _spectrum_values(a) * a.test.tree.n_boxes * self.max_batch * es)
         if self.used_bytes() + ws_est > self.budget:
             raise AdmissionError(f"map {name!r} needs ~{ws_est / 1e9:.1f} GB of workspaces, "
                                  f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
