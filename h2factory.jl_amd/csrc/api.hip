@@ -1029,8 +1029,10 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
             if (LL.ok) {
                 const bool half = c.half_spectrum;
                 w.mark_begin(4, l);
-                const bool perm = m2l_direct_ok(cd, LL, nsrc, R);  // the direct kernel's column layout
-                launch_fft_in_fm(cd, half, inv_in, box0, nsrc, w.mom.ptr, K, c.p, c.L, R, perm, w.spec_in.ptr,
+                // the direct kernel's row layout: 16 columns, or a multiple of 32 permuted
+                const int drows = m2l_direct_rows(cd, LL, nsrc, R);
+                launch_fft_in_fm(cd, half, inv_in, box0, nsrc, w.mom.ptr, K, c.p, c.L, R, drows ? drows : R, drows > 16,
+                                 w.spec_in.ptr,
                                  c.twiddle.ptr, s);
                 w.mark_end();
                 w.mark_begin(5, l);

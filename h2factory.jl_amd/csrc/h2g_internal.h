@@ -410,13 +410,17 @@ void launch_sep_l2l(int cdtype, int opA, int n_parents, const int32_t *parents, 
 void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
                          const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,
                          const void *in, void *out, int R, int acc_mode, const uint8_t *acc_flags, hipStream_t s);
+// Rrow >= R: stored columns per row (padding columns zero); perm: the 32-column permuted layout
 void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, const void *mom, int K, int p, int L,
-                      int R, bool perm, void *S, const void *twiddle, hipStream_t s);
+                      int R, int Rrow, bool perm, void *S, const void *twiddle, hipStream_t s);
 void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,
                        int p, int L, int R, void *loc, const void *twiddle, hipStream_t s);
 // true when the lattice M2L of this level runs the direct kernel (FP32 spectra, R >= 4); its
 // source spectra are then stored in the permuted column layout of launch_fft_in_fm (perm)
 bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R);
+// stored columns per source row for the direct kernel (16: natural order, batches of <= 16
+// columns; a multiple of 32: permuted layout), 0 when the level does not run it
+int m2l_direct_rows(int cdtype, const LatticeLevel &LL, int n_src, int R);
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s);
 void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,

@@ -69,7 +69,8 @@ __device__ inline void load_twiddles(C (&tw)[L], const cplx<double> *__restrict_
     }
 }
 
-// Rrow: stored columns per source row (R, or R rounded up to 32 when perm); perm: within each
+// Rrow: stored columns per source row (R; 16 for the direct kernel's narrow batches; R rounded up
+// to 32 when perm); perm: within each
 // chunk of 32 columns, column c is stored at 2 (c % 16) + c / 16, so that columns c and c + 16
 // are adjacent (one 16-byte load holds both); padding columns (>= R) are zero.
 template <class T, bool HALF, int JB, int P>
@@ -307,13 +308,12 @@ constexpr bool fft_f32_half() {
 }
 
 template <class T, bool HALF, int P>
-static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, bool perm, void *S, const void *tw,
+static void fft_in_fm_p(int box0, int nbox, const void *mom, int K, int R, int Rrow, bool perm, void *S, const void *tw,
                         bool inverse, hipStream_t s) {
     constexpr int JB = fft_f32_half<T, HALF>() ? FFT_IN_JB : fft_jobs<T, HALF>();
     constexpr int NT = fft_f32_half<T, HALF>() ? FFT_IN_TPB : TPB;
     constexpr int JH = JB / 2;
     constexpr int L = 2 * P - 1;
-    const int Rrow = perm ? (R + 31) / 32 * 32 : R;
     const int64_t njobs = (int64_t)nbox * Rrow;
     const size_t sh = fft_smem<JB, T, HALF>(P, L);
     using C = typename complex_of<T>::type;
@@ -348,14 +348,14 @@ static void fft_out_fm_p(const int32_t *recv, int nrecv, const void *O, int K, i
 }
 
 template <class T, bool HALF>
-static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, bool perm, void *S,
+static void fft_in_fm_t(int box0, int nbox, const void *mom, int K, int p, int L, int R, int Rrow, bool perm, void *S,
                         const void *tw, bool inverse, hipStream_t s) {
     switch (p) {
-    case 2: fft_in_fm_p<T, HALF, 2>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
-    case 3: fft_in_fm_p<T, HALF, 3>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
-    case 4: fft_in_fm_p<T, HALF, 4>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
-    case 5: fft_in_fm_p<T, HALF, 5>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
-    case 6: fft_in_fm_p<T, HALF, 6>(box0, nbox, mom, K, R, perm, S, tw, inverse, s); break;
+    case 2: fft_in_fm_p<T, HALF, 2>(box0, nbox, mom, K, R, Rrow, perm, S, tw, inverse, s); break;
+    case 3: fft_in_fm_p<T, HALF, 3>(box0, nbox, mom, K, R, Rrow, perm, S, tw, inverse, s); break;
+    case 4: fft_in_fm_p<T, HALF, 4>(box0, nbox, mom, K, R, Rrow, perm, S, tw, inverse, s); break;
+    case 5: fft_in_fm_p<T, HALF, 5>(box0, nbox, mom, K, R, Rrow, perm, S, tw, inverse, s); break;
+    case 6: fft_in_fm_p<T, HALF, 6>(box0, nbox, mom, K, R, Rrow, perm, S, tw, inverse, s); break;
     default: throw Error{H2G_EUNSUPPORTED, "Toeplitz modulation supports 2 <= p <= 6"};
     }
     (void)L;
@@ -376,19 +376,19 @@ static void fft_out_fm_t(const int32_t *recv, int nrecv, const void *O, int K, i
 }
 
 void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, const void *mom, int K, int p, int L,
-                      int R, bool perm, void *S, const void *twiddle, hipStream_t s) {
+                      int R, int Rrow, bool perm, void *S, const void *twiddle, hipStream_t s) {
     if (nbox == 0) return;
     switch (cdtype) {
     case H2G_F64:
-        if (half) fft_in_fm_t<double, true>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
-        else fft_in_fm_t<double, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
+        if (half) fft_in_fm_t<double, true>(box0, nbox, mom, K, p, L, R, Rrow, perm, S, twiddle, inverse, s);
+        else fft_in_fm_t<double, false>(box0, nbox, mom, K, p, L, R, Rrow, perm, S, twiddle, inverse, s);
         break;
     case H2G_F32:
-        if (half) fft_in_fm_t<float, true>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
-        else fft_in_fm_t<float, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
+        if (half) fft_in_fm_t<float, true>(box0, nbox, mom, K, p, L, R, Rrow, perm, S, twiddle, inverse, s);
+        else fft_in_fm_t<float, false>(box0, nbox, mom, K, p, L, R, Rrow, perm, S, twiddle, inverse, s);
         break;
-    case H2G_C64: fft_in_fm_t<cplx<double>, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s); break;
-    default: fft_in_fm_t<cplx<float>, false>(box0, nbox, mom, K, p, L, R, perm, S, twiddle, inverse, s);
+    case H2G_C64: fft_in_fm_t<cplx<double>, false>(box0, nbox, mom, K, p, L, R, Rrow, perm, S, twiddle, inverse, s); break;
+    default: fft_in_fm_t<cplx<float>, false>(box0, nbox, mom, K, p, L, R, Rrow, perm, S, twiddle, inverse, s);
     }
 }
 

@@ -33,6 +33,7 @@
 // groups and its L2 holds only its part of the frequency slice.
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "h2g_internal.h"
 
@@ -419,8 +420,11 @@ constexpr int DR_SETS = 2;
 // are zero), which costs less than the cluster kernel's per-column work from about 4 columns on
 // (N = 2M, p = 6: 4 columns 44 ms either way, 8 columns 74 ms on the cluster kernel)
 constexpr int DR_MIN_R = 4;
+#ifndef DR_NCH1
+#define DR_NCH1 1
+#endif
 
-template <bool CONJ>
+template <bool CONJ, int NCH>
 __global__ void __launch_bounds__(DR_NT, 4)
     m2l_direct_kernel(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr, const int4 *__restrict__ list,
                       int n_list, const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg, int n_src,
@@ -432,18 +436,23 @@ __global__ void __launch_bounds__(DR_NT, 4)
     const int grp = group_of_workgroup();
     const int bb0 = grp * gsz, bb1 = min(n_blocks, bb0 + gsz);
     if (bb0 >= bb1) return;
-    const int f = blockIdx.y, r0 = blockIdx.z * 32;
+    // NCH 16-column tiles per chunk of columns: 2 (rows padded to 32 columns in the permuted
+    // layout) or 1 (rows of 16 columns in natural order, batches of at most 16 columns)
+    constexpr int CH = 16 * NCH;
+    using BV = typename std::conditional<NCH == 2, float4, float2>::type;
+    const int f = blockIdx.y, r0 = blockIdx.z * CH;
     fill_ea<float, DR_NT>(ea, EAg + (int64_t)f * 216 * 32);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
     const int m = l & 15, bq = (l >> 4) & 1, h = l >> 5;
     // adjoint: the Im parts of the couplings (rows c != b) change sign
     const float sgn = (CONJ && ((m & 1) != bq)) ? -1.f : 1.f;
-    // source rows of frequency f in the permuted column layout (columns c and c + 16 of a chunk
-    // of 32 adjacent): 16-byte unit m of chunk r0 / 32 holds columns r0 + m and r0 + 16 + m
+    // source rows of frequency f.  NCH = 2: permuted column layout (columns c and c + 16 of a
+    // chunk of 32 adjacent), 16-byte unit m of chunk r0 / 32 holds columns r0 + m and r0 + 16 + m;
+    // NCH = 1: the 8 bytes of column r0 + m
     const char *Sfb = reinterpret_cast<const char *>(S + (int64_t)f * n_src * Rrow);
     const uint32_t rowb = (uint32_t)Rrow * (uint32_t)sizeof(cplx<float>);
-    const uint32_t lo4 = (uint32_t)(r0 * (int)sizeof(cplx<float>) + m * 16);
+    const uint32_t lo4 = (uint32_t)(r0 * (int)sizeof(cplx<float>) + m * (int)sizeof(BV));
     const char *eab = reinterpret_cast<const char *>(ea) + (m * 2 + bq) * (int)sizeof(float);
     float *__restrict__ Of = reinterpret_cast<float *>(O + (int64_t)f * n_recv * R);
     // list copies: lane l moves entries l and 64 + l (int4, two pairs each) of a block's list
@@ -470,13 +479,13 @@ __global__ void __launch_bounds__(DR_NT, 4)
         };
         // per entry pair j (slot pairs 2j, 2j + 1): the lane of 16-lane row bq loads its half's
         // source row of pair 2j + bq (one 16-byte unit); the swap at use time sorts the values
-        auto issue = [&](const int4(&L)[4], float(&an)[8], float4(&bn)[4]) {
+        auto issue = [&](const int4(&L)[4], float(&an)[8], BV(&bn)[4]) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 an[2 * j] = *reinterpret_cast<const float *>(eab + L[j].y);
                 an[2 * j + 1] = *reinterpret_cast<const float *>(eab + L[j].w);
                 const uint32_t slot = (uint32_t)(bq ? L[j].z : L[j].x);
-                bn[j] = *reinterpret_cast<const float4 *>(Sfb + (slot * rowb + lo4));
+                bn[j] = *reinterpret_cast<const BV *>(Sfb + (slot * rowb + lo4));
             }
         };
         // DR_SETS operand sets rotate: batch k's MFMA operands come from set k % DR_SETS while
@@ -484,20 +493,20 @@ __global__ void __launch_bounds__(DR_NT, 4)
         // Lx is the list of the next batch to issue
         int4 Lx[4];
         float an[DR_SETS][8];
-        float4 bn[DR_SETS][4];
+        BV bn[DR_SETS][4];
 #pragma unroll
         for (int u = 0; u < DR_SETS - 1; ++u) {
             ldl(k0 + u, Lx);
             issue(Lx, an[u], bn[u]);
         }
         ldl(k0 + DR_SETS - 1, Lx);
-        double4_m acc[2];
+        double4_m acc[NCH];
 #pragma unroll
-        for (int g = 0; g < 2; ++g) acc[g] = double4_m{0.0, 0.0, 0.0, 0.0};
+        for (int g = 0; g < NCH; ++g) acc[g] = double4_m{0.0, 0.0, 0.0, 0.0};
         // a batch past the block's last (the trip's tail) loads clamped (valid) operands and
         // skips its MFMAs
-        auto step = [&](int k, const float(&anc)[8], const float4(&bnc)[4], float(&ann)[8], float4(&bnn)[4]) {
-            float ac[8], bc[8][2];
+        auto step = [&](int k, const float(&anc)[8], const BV(&bnc)[4], float(&ann)[8], BV(&bnn)[4]) {
+            float ac[8], bc[8][NCH];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 ac[2 * j] = anc[2 * j] * sgn;
@@ -509,13 +518,16 @@ __global__ void __launch_bounds__(DR_NT, 4)
                 // __builtin_bit_cast of a vector element reads element 0 with this compiler.)
                 const auto s0 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bnc[j].x), __float_as_uint(bnc[j].y),
                                                                  false, false);
-                const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bnc[j].z), __float_as_uint(bnc[j].w),
-                                                                 false, false);
-                const uint32_t x = s0[0], y = s0[1], z = s1[0], ww = s1[1];
+                const uint32_t x = s0[0], y = s0[1];
                 bc[2 * j][0] = __uint_as_float(x);
-                bc[2 * j][1] = __uint_as_float(z);
                 bc[2 * j + 1][0] = __uint_as_float(y);
-                bc[2 * j + 1][1] = __uint_as_float(ww);
+                if constexpr (NCH == 2) {
+                    const auto s1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(bnc[j].z),
+                                                                     __float_as_uint(bnc[j].w), false, false);
+                    const uint32_t z = s1[0], ww = s1[1];
+                    bc[2 * j][NCH - 1] = __uint_as_float(z);
+                    bc[2 * j + 1][NCH - 1] = __uint_as_float(ww);
+                }
             }
             issue(Lx, ann, bnn);
             ldl(k + DR_SETS, Lx);
@@ -524,21 +536,21 @@ __global__ void __launch_bounds__(DR_NT, 4)
             __asm__ volatile("" ::: "memory");
             __builtin_amdgcn_sched_barrier(0);
             if (k < k1) {
-                float4_m f0[2], f1[2];
+                float4_m f0[NCH], f1[NCH];
 #pragma unroll
-                for (int g = 0; g < 2; ++g) {
+                for (int g = 0; g < NCH; ++g) {
                     f0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
                     f1[g] = f0[g];
                 }
 #pragma unroll
                 for (int i = 0; i < 8; i += 2)
 #pragma unroll
-                    for (int g = 0; g < 2; ++g) {
+                    for (int g = 0; g < NCH; ++g) {
                         f0[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i], bc[i][g], f0[g], 0, 0, 0);
                         f1[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i + 1], bc[i + 1][g], f1[g], 0, 0, 0);
                     }
 #pragma unroll
-                for (int g = 0; g < 2; ++g)
+                for (int g = 0; g < NCH; ++g)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) acc[g][v] += (double)f0[g][v] + (double)f1[g][v];
             }
@@ -549,7 +561,7 @@ __global__ void __launch_bounds__(DR_NT, 4)
                 step(k + u, an[u], bn[u], an[(u + DR_SETS - 1) % DR_SETS], bn[(u + DR_SETS - 1) % DR_SETS]);
         }
 #pragma unroll
-        for (int g = 0; g < 2; ++g) {
+        for (int g = 0; g < NCH; ++g) {
             // FP32 16x16x4 result layout: rows 4 (l / 16) + v = (target 2 (l / 16) + v / 2, re / im v % 2)
             const int col = r0 + 16 * g + m;
             if (col < R) {
@@ -561,9 +573,9 @@ __global__ void __launch_bounds__(DR_NT, 4)
                         o.x = (float)acc[g][2 * u];
                         o.y = (float)acc[g][2 * u + 1];
                         // streaming store: the receiving spectra are read once, by modulate-out
-                        uint64_t u;
-                        __builtin_memcpy(&u, &o, 8);
-                        __builtin_nontemporal_store(u, reinterpret_cast<uint64_t *>(Of + ((int64_t)tgi * R + col) * 2));
+                        uint64_t bits;
+                        __builtin_memcpy(&bits, &o, 8);
+                        __builtin_nontemporal_store(bits, reinterpret_cast<uint64_t *>(Of + ((int64_t)tgi * R + col) * 2));
                     }
                 }
             }
@@ -636,18 +648,25 @@ void cluster_go(bool conj, const LatticeLevel &LL, const EA *EAg, int n_src, int
 
 }  // namespace
 
-bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R) {
+int m2l_direct_rows(int cdtype, const LatticeLevel &LL, int n_src, int R) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
-    const int64_t Rrow = (R + 31) / 32 * 32;
-    return !f64 && R >= DR_MIN_R && LL.ok && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB && LL.EA32.ptr &&
-           (int64_t)n_src * Rrow * (int64_t)sizeof(cplx<float>) < INT32_MAX;
+    // batches of at most 16 columns: one 16-column tile per chunk, rows of 16 columns; wider
+    // ones: chunks of two tiles, rows padded to 32 columns
+    const int64_t Rrow = R <= 16 && DR_NCH1 ? 16 : (R + 31) / 32 * 32;
+    const bool ok = !f64 && R >= DR_MIN_R && LL.ok && LL.direct.n_blocks > 0 && LL.direct.max_batches <= DR_MAXB &&
+                    LL.EA32.ptr && (int64_t)n_src * Rrow * (int64_t)sizeof(cplx<float>) < INT32_MAX;
+    return ok ? (int)Rrow : 0;
+}
+
+bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R) {
+    return m2l_direct_rows(cdtype, LL, n_src, R) > 0;
 }
 
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
     if (!LL.EA64.ptr) throw Error{H2G_EINVAL, "lattice coupling table missing"};
-    if (m2l_direct_ok(cdtype, LL, n_src, R)) {
+    if (const int Rrow = m2l_direct_rows(cdtype, LL, n_src, R)) {
         const LatticeDirect &D = LL.direct;
         // groups of consecutive blocks: at least two blocks per wave (the coupling table is
         // loaded once per group), and about DR_GROUPS groups per frequency, so that the
@@ -655,14 +674,17 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
         // rows)
         const int gsz = std::max(2 * DR_NT / 64, D.n_blocks / DR_GROUPS);
         const int ng = (D.n_blocks + gsz - 1) / gsz;
-        const int Rrow = (R + 31) / 32 * 32;
-        const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)(Rrow / 32));
+        const int ch = Rrow == 16 ? 16 : 32;  // columns per chunk (NCH 16-column tiles)
+        const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)(Rrow / ch));
         auto go = [&](auto kern) {
             kern<<<grid, DR_NT, 0, s>>>(D.n_blocks, gsz, D.ptr.as<int32_t>(), D.list.as<int4>(),
                                         (int)(D.list.bytes / sizeof(int4)), D.tgt.as<int32_t>(), LL.EA32.as<float>(),
                                         n_src, n_recv, (const cplx<float> *)S, Rrow, (cplx<float> *)O, R);
         };
-        conj ? go(m2l_direct_kernel<true>) : go(m2l_direct_kernel<false>);
+        if (ch == 16)
+            conj ? go(m2l_direct_kernel<true, 1>) : go(m2l_direct_kernel<false, 1>);
+        else
+            conj ? go(m2l_direct_kernel<true, 2>) : go(m2l_direct_kernel<false, 2>);
     } else if (!f64 && R >= 16 && LL.list.n_tiles > 0 && LL.EA32.ptr) {
         const LatticeTiles &T = LL.list;
         const size_t sh = sizeof(cplx<float>) * (size_t)(LIST_NSLOT + 1) * 16 + sizeof(float) * 216 * 32 +
