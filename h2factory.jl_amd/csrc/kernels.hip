@@ -1326,7 +1326,7 @@ void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targ
 
 
 // ------------------------------------------------------------------------------------------
-// Near field on FP64 MFMA (real maps, R >= 16), one row-block product per task:
+// Near field on FP64 MFMA (real maps, R >= MFMA_MIN_R), one row-block product per task:
 //   Y (nrow x R) = A (nrow x ncol) X (ncol x R)
 // with v_mfma_f64_16x16x4f64.  A's columns are the source points of all near blocks of the
 // output leaf, concatenated (one NearCol entry each), so the contraction is padded to a multiple
@@ -1335,9 +1335,9 @@ void launch_sep_transfer(int cdtype, int opA, int n_targets, const int32_t *targ
 // B[k][n] in lane n + 16k, D[m][n] in lane n + 16(m % 4), element m / 4.
 // ------------------------------------------------------------------------------------------
 // narrowest batch for the MFMA near-field and leaf kernels (16-column tiles, unused columns
-// masked): from 4 columns on they beat the vector kernels (N = 2M, p = 6, FP32: 15 columns took
-// 32 ms more than 16 on the vector kernels)
-constexpr int MFMA_MIN_R = 4;
+// masked): they beat the vector kernels at every width (N = 2M, p = 6, mixed products: 1 column
+// 14.5 -> 11.6 ms, 2 columns 22.9 -> 15.1 ms, 3 columns 37.2 -> 24.4 ms)
+constexpr int MFMA_MIN_R = 1;
 
 namespace {
 typedef double double4_t __attribute__((ext_vector_type(4)));
@@ -1455,7 +1455,7 @@ __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__res
         }
 }
 
-// Leaf expansions on FP64 MFMA (real maps, R >= 16), FP64 accumulation.
+// Leaf expansions on FP64 MFMA (real maps, R >= MFMA_MIN_R), FP64 accumulation.
 //   L2P: Y_leaf (n x R) += alpha C^T (n x K) L (K x R)      rows = points, contraction over k
 //   P2M: M (K x R)      = C (K x n) X (n x R)                rows = k, contraction over points
 // A k-step's 4 contraction indices are assigned so that each lane's operands over 4 consecutive
