@@ -1439,7 +1439,6 @@ __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__res
 #pragma unroll
                 for (int g = 0; g < NCT; ++g)
                     acc[rt][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u][rt], bc[u][g], acc[rt][g], 0, 0, 0);
-
         tb1 = tb2;
     }
     double *__restrict__ yo = ys + (tk.out0 + tk.row0) * R;
