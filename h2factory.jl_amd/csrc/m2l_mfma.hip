@@ -423,9 +423,8 @@ constexpr int DR_SETS = 2;
 // are zero), which costs less than the cluster kernel's per-column work from about 4 columns on
 // (N = 2M, p = 6: 4 columns 44 ms either way, 8 columns 74 ms on the cluster kernel)
 constexpr int DR_MIN_R = 4;
-#ifndef DR_NCH1
-#define DR_NCH1 1
-#endif
+// batches of at most 16 columns run one 16-column tile per chunk (rows of 16 columns)
+constexpr bool DR_NCH1 = true;
 
 template <bool CONJ, int NCH>
 __global__ void __launch_bounds__(DR_NT, 4)
