@@ -415,11 +415,10 @@ void launch_fft_in_fm(int cdtype, bool half, bool inverse, int box0, int nbox, c
                       int R, int Rrow, bool perm, void *S, const void *twiddle, hipStream_t s);
 void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv, int nrecv, const void *O, int K,
                        int p, int L, int R, void *loc, const void *twiddle, hipStream_t s);
-// true when the lattice M2L of this level runs the direct kernel (FP32 spectra, R >= 4); its
-// source spectra are then stored in the permuted column layout of launch_fft_in_fm (perm)
-bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R);
-// stored columns per source row for the direct kernel (16: natural order, batches of <= 16
-// columns; a multiple of 32: permuted layout), 0 when the level does not run it
+// The lattice M2L of a level runs the direct kernel for FP32 spectra and R >= 4 columns; its
+// source spectra are then stored in rows of this many columns (16: natural order, batches of at
+// most 16 columns; a multiple of 32: the permuted layout of launch_fft_in_fm, perm).  0 when the
+// level does not run the direct kernel.
 int m2l_direct_rows(int cdtype, const LatticeLevel &LL, int n_src, int R);
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s);

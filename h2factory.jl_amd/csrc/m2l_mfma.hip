@@ -665,10 +665,6 @@ int m2l_direct_rows(int cdtype, const LatticeLevel &LL, int n_src, int R) {
     return ok ? (int)Rrow : 0;
 }
 
-bool m2l_direct_ok(int cdtype, const LatticeLevel &LL, int n_src, int R) {
-    return m2l_direct_rows(cdtype, LL, n_src, R) > 0;
-}
-
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, hipStream_t s) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
