@@ -115,6 +115,13 @@ struct Error {
             throw ::h2g::Error{H2G_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)};  \
     } while (0)
 
+// after a kernel launch: turns a launch error into an Error
+#define H2G_CHECK_LAUNCH()                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = hipGetLastError();                                                   \
+        if (e_ != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e_)}; \
+    } while (0)
+
 inline size_t elem_size(int dtype) {
     switch (dtype) {
     case H2G_F64: return 8;

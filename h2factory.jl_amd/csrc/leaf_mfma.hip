@@ -1,0 +1,453 @@
+// MFMA kernels of libh2g for real maps: the near field as row-block products on FP64 MFMA, the
+// leaf expansions (P2M, L2P) on FP64 MFMA, and the lane-layout probes of the MFMA instructions.
+#include <algorithm>
+#include <type_traits>
+
+#include "h2g_internal.h"
+
+namespace h2g {
+
+constexpr int TPB = 256;
+// threads per workgroup of the MFMA leaf kernels (the waves of a workgroup take its row tiles):
+// 512 measured 0.05-0.1 ms faster than 256 per level on the N = 2M, p = 6 map, 1024 slower for P2M
+constexpr int L2P_NT = 512, P2M_NT = 512;
+
+// ------------------------------------------------------------------------------------------
+// Near field on FP64 MFMA (real maps, R >= MFMA_MIN_R), one row-block product per task:
+//   Y (nrow x R) = A (nrow x ncol) X (ncol x R)
+// with v_mfma_f64_16x16x4f64.  A's columns are the source points of all near blocks of the
+// output leaf, concatenated (one NearCol entry each), so the contraction is padded to a multiple
+// of 4 KU once per leaf instead of once per block.  Operand lane layouts of the 16x16x4 FP64
+// instruction (checked by the layout probe below on gfx950): A[m][k] in lane m + 16k,
+// B[k][n] in lane n + 16k, D[m][n] in lane n + 16(m % 4), element m / 4.
+// ------------------------------------------------------------------------------------------
+// narrowest batch for the MFMA near-field and leaf kernels (16-column tiles, unused columns
+// masked): they beat the vector kernels at every width (N = 2M, p = 6, mixed products: 1 column
+// 14.5 -> 11.6 ms, 2 columns 22.9 -> 15.1 ms, 3 columns 37.2 -> 24.4 ms)
+constexpr int MFMA_MIN_R = 1;
+
+namespace {
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline int mf_a_row(int lane) { return lane & 15; }
+__device__ inline int mf_a_k(int lane) { return lane >> 4; }
+__device__ inline int mf_d_row(int lane, int v) { return (lane >> 4) + 4 * v; }
+
+// One wave per task, 4 independent waves per workgroup; tasks [t0, t1) all have RT row tiles.
+// The wave holds RT x NCT accumulator tiles, so every source value x[j][c] it loads feeds RT
+// MFMAs and every matrix value NCT.  A batch is KU k-steps (KB = 4 KU columns).  Lane l holds the
+// column-table entry l % KB of a batch, loaded two batches ahead of its use; a lane takes the
+// entries of its own columns from the owning lanes (ds_bpermute), so the table costs 2 x 4
+// registers rather than 4 per column in flight.  Operands are loaded one batch ahead of the
+// MFMAs.  Every load reads a clamped (valid) address and out-of-range rows, columns and RHS are
+// multiplied by zero, so the loop has no branches and the accumulators stay in the MFMA
+// registers.  Workgroups are spread so that each XCD (dispatched round robin) works on a
+// contiguous range of tasks, whose source points overlap in its L2.
+template <bool TRANS, int RT, int KU, int NCT, class V>
+__global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__restrict__ tasks, int64_t t0, int64_t t1,
+                                                        const NearCol *__restrict__ cols, const V *__restrict__ vals,
+                                                        const double *__restrict__ xs, double *__restrict__ ys, int R) {
+    constexpr int KB = 4 * KU;
+    static_assert(KB <= 64, "a batch's table entries fit one wave");
+    int wg = blockIdx.x;
+    const int n8 = gridDim.x & ~7;
+    if (wg < n8) wg = (wg & 7) * (n8 >> 3) + (wg >> 3);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t ti = t0 + (int64_t)wg * (TPB / 64) + wave;
+    if (ti >= t1) return;
+    const NearTask tk = tasks[ti];
+    const int nrow = tk.nrow, ncol = tk.ncol;
+    const NearCol *__restrict__ cl = cols + tk.col0;
+    const int lane = threadIdx.x & 63;
+    const int ar = mf_a_row(lane), ak = mf_a_k(lane);
+    const int c0 = blockIdx.y * NCT * 16;
+    const int tl = lane & (KB - 1);
+    int arow[RT];
+    double rmask[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int r = 16 * rt + ar;
+        arow[rt] = tk.row0 + min(r, nrow - 1);
+        rmask[rt] = r < nrow ? 1.0 : 0.0;
+    }
+    int xcol[NCT];
+    double cmask[NCT];
+#pragma unroll
+    for (int g = 0; g < NCT; ++g) {
+        const int col = c0 + 16 * g + ar;
+        xcol[g] = min(col, R - 1);
+        cmask[g] = col < R ? 1.0 : 0.0;
+    }
+    // raw operands of the batch in flight and their column masks; the masks are applied when the
+    // batch is used, so that nothing waits for the loads before the next batch's MFMAs
+    double a[KU][RT], b[KU][NCT], km[KU];
+    auto load_ops = [&](int k0, const NearCol &ev) {
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int src = (4 * u + ak) << 2;  // byte address of the lane holding this column
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)ev.aoff);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(ev.aoff >> 32));
+            const int xr = __builtin_amdgcn_ds_bpermute(src, ev.xrow);
+            const int64_t rs = TRANS ? (int64_t)__builtin_amdgcn_ds_bpermute(src, ev.rstride) : 1;
+            km[u] = (k0 + 4 * u + ak < ncol) ? 1.0 : 0.0;
+            const V *__restrict__ ap = vals + (int64_t)(((uint64_t)hi << 32) | lo);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) a[u][rt] = (double)ap[arow[rt] * rs];
+            const double *__restrict__ xp = xs + (int64_t)xr * R;
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) b[u][g] = xp[xcol[g]];
+        }
+    };
+    double4_t acc[RT][NCT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) acc[rt][g] = double4_t{0.0, 0.0, 0.0, 0.0};
+    const int nb = (ncol + KB - 1) / KB;
+    NearCol tb0 = cl[min(tl, ncol - 1)], tb1 = cl[min(KB + tl, ncol - 1)];
+    load_ops(0, tb0);
+    for (int ib = 0; ib < nb; ++ib) {
+        double ac[KU][RT], bc[KU][NCT];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) ac[u][rt] = a[u][rt] * (rmask[rt] * km[u]);
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) bc[u][g] = b[u][g] * (cmask[g] * km[u]);
+        }
+        const NearCol tb2 = cl[min((ib + 2) * KB + tl, ncol - 1)];
+        load_ops((ib + 1) * KB, tb1);
+        // keep the next batch's loads ahead of this batch's MFMAs
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < KU; ++u)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g)
+                    acc[rt][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u][rt], bc[u][g], acc[rt][g], 0, 0, 0);
+        tb1 = tb2;
+    }
+    double *__restrict__ yo = ys + (tk.out0 + tk.row0) * R;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) {
+            const int col = c0 + 16 * g + ar;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * rt + mf_d_row(lane, v);
+                if (row < nrow && col < R) yo[(int64_t)row * R + col] = acc[rt][g][v];
+            }
+        }
+}
+
+// Leaf expansions on FP64 MFMA (real maps, R >= MFMA_MIN_R), FP64 accumulation.
+//   L2P: Y_leaf (n x R) += alpha C^T (n x K) L (K x R)      rows = points, contraction over k
+//   P2M: M (K x R)      = C (K x n) X (n x R)                rows = k, contraction over points
+// A k-step's 4 contraction indices are assigned so that each lane's operands over 4 consecutive
+// k-steps are 4 consecutive values (index 16 j + 4 k' + u for lane row k' and k-step 4 j + u):
+// the contraction order does not change the product, and every lane reads its operand rows
+// contiguously.  One workgroup (4 waves) per (leaf, group of NCT 16-column RHS tiles).
+template <class T, int NCT, bool STAGE>
+__global__ void __launch_bounds__(L2P_NT)
+    l2p_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
+                    const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const T *__restrict__ loc,
+                    int R, double *__restrict__ ys, double alpha) {
+    // STAGE (leaves of many points): the leaf's expansions of this workgroup's columns, [col][k]
+    // with a row pitch of K + 1, read from the [box][col][k] layout with coalesced loads (the
+    // MFMA operand order reads 16 columns K values apart); small leaves read them directly, where
+    // the staging would cost more than the operand loads it saves
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lt = reinterpret_cast<T *>(smem_raw);
+    const int KP = K + 1;
+    const int b = leaves[blockIdx.x];
+    const int64_t j0 = begin[b];
+    const int n = (int)(end[b] - j0);
+    const int c0 = blockIdx.y * NCT * 16;
+    const int ncol = min(NCT * 16, R - c0);
+    const T *lb = loc + ((int64_t)b * R + c0) * K;
+    if (STAGE) {
+        for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
+            const int c = e / K, k = e - c * K;
+            lt[c * KP + k] = lb[e];
+        }
+        __syncthreads();
+    }
+    const T *lsrc = STAGE ? lt : lb;
+    const int pitch = STAGE ? KP : K;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, kq = lane >> 4;
+    for (int rt = wave; rt * 16 < n; rt += (int)(blockDim.x >> 6)) {
+        const int row = rt * 16 + m;
+        const T *crow = coll + (j0 + min(row, n - 1)) * K;
+        double4_t acc[NCT];
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) acc[g] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int jb = 0; jb < K; jb += 16) {  // uniform trip count
+            const int kb = jb + 4 * kq;
+            double a[4], bv[NCT][4];
+            // unconditional loads from clamped (valid) addresses, out-of-range operands multiplied
+            // by zero (no branches in the loop, so the accumulators stay in the MFMA registers)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                a[u] = (double)crow[min(kb + u, K - 1)] * ((row < n && kb + u < K) ? 1.0 : 0.0);
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) {
+                const int cl = min(16 * g + m, ncol - 1);
+                const T *lc = lsrc + cl * pitch;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    bv[g][u] = (double)lc[min(kb + u, K - 1)] * ((16 * g + m < ncol && kb + u < K) ? 1.0 : 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[g][u], acc[g], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) {
+            const int col = c0 + 16 * g + m;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ro = rt * 16 + mf_d_row(lane, v);
+                if (ro < n && col < R) {
+                    double *yo = ys + (j0 + ro) * R + col;
+                    *yo = *yo + alpha * acc[g][v];
+                }
+            }
+        }
+    }
+}
+
+template <class T, int NCT>
+__global__ void __launch_bounds__(P2M_NT)
+    p2m_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
+                    const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const double *__restrict__ xs,
+                    int R, T *__restrict__ mom) {
+    // results go through LDS ([col][k], row pitch K + 1) and leave with coalesced stores in the
+    // [box][col][k] layout (the MFMA result order would write 16 columns K values apart)
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *mt = reinterpret_cast<T *>(smem_raw);
+    const int KP = K + 1;
+    const int b = leaves[blockIdx.x];
+    const int64_t j0 = begin[b];
+    const int n = (int)(end[b] - j0);
+    const int c0 = blockIdx.y * NCT * 16;
+    const int ncol = min(NCT * 16, R - c0);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, kq = lane >> 4;
+    for (int rt = wave; rt * 16 < K; rt += (int)(blockDim.x >> 6)) {
+        const int krow = rt * 16 + m;
+        double4_t acc[NCT];
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) acc[g] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int jb = 0; jb < n; jb += 16) {  // uniform trip count
+            const int pb = jb + 4 * kq;
+            double a[4], bv[NCT][4];
+            // unconditional loads from clamped (valid) addresses, out-of-range operands multiplied
+            // by zero (no branches in the loop, so the accumulators stay in the MFMA registers)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                a[u] = (double)coll[(j0 + min(pb + u, n - 1)) * K + min(krow, K - 1)] *
+                       ((pb + u < n && krow < K) ? 1.0 : 0.0);
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) {
+                const int col = c0 + 16 * g + m;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    bv[g][u] = xs[(j0 + min(pb + u, n - 1)) * R + min(col, R - 1)] * ((pb + u < n && col < R) ? 1.0 : 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[g][u], acc[g], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) {
+            const int cl = 16 * g + m;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ko = rt * 16 + mf_d_row(lane, v);
+                if (ko < K && cl < ncol) mt[cl * KP + ko] = (T)acc[g][v];
+            }
+        }
+    }
+    __syncthreads();
+    T *mb = mom + ((int64_t)b * R + c0) * K;
+    for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
+        const int c = e / K, k = e - c * K;
+        mb[e] = mt[c * KP + k];
+    }
+}
+
+// FP32 copy of FP64 values (grid-stride)
+__global__ void narrow_values_kernel(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+// probe of the operand / result lane layout used above: D = A (16x4) * B (4x16)
+__global__ void mfma_f64_probe_kernel(const double *__restrict__ A, const double *__restrict__ B,
+                                      double *__restrict__ D) {
+    const int lane = threadIdx.x & 63;
+    const double a = A[mf_a_row(lane) * 4 + mf_a_k(lane)];
+    const double b = B[(lane >> 4) * 16 + (lane & 15)];
+    double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) D[mf_d_row(lane, v) * 16 + (lane & 15)] = acc[v];
+}
+// probe of the FP32 16x16x4 layout the list M2L kernel uses: operands as for FP64, result
+// D[m][n] in lane n + 16 (m / 4), element m % 4
+__global__ void mfma_f32_probe_kernel(const float *__restrict__ A, const float *__restrict__ B, float *__restrict__ D) {
+    typedef float float4_t __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const float a = A[mf_a_row(lane) * 4 + mf_a_k(lane)];
+    const float b = B[(lane >> 4) * 16 + (lane & 15)];
+    float4_t acc = float4_t{0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = acc[v];
+}
+}  // namespace
+
+bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                     const void *coll, int K, const void *loc, int R, void *ys, double alpha, bool stage,
+                     hipStream_t s) {
+    if (R < MFMA_MIN_R || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
+    // column tiles per workgroup: up to 4, as many as the LDS block of 64 KB allows
+    const size_t es = cdtype == H2G_F64 ? 8 : 4;
+    const int nct = std::min({4, (R + 15) / 16, stage ? (int)(65536 / (es * 16 * (size_t)(K + 1))) : 4});
+    if (nct < 1) return false;
+    const dim3 grid((unsigned)n, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
+    auto go = [&](auto kern, auto tv) {
+        using T = decltype(tv);
+        const size_t sh = stage ? sizeof(T) * (size_t)nct * 16 * (K + 1) : 0;
+        kern<<<grid, L2P_NT, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys, alpha);
+    };
+    const bool f64 = cdtype == H2G_F64;
+    switch (nct) {
+    case 1: f64 ? go((stage ? l2p_mfma_kernel<double, 1, true> : l2p_mfma_kernel<double, 1, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 1, true> : l2p_mfma_kernel<float, 1, false>), 0.0f); break;
+    case 2: f64 ? go((stage ? l2p_mfma_kernel<double, 2, true> : l2p_mfma_kernel<double, 2, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 2, true> : l2p_mfma_kernel<float, 2, false>), 0.0f); break;
+    case 3: f64 ? go((stage ? l2p_mfma_kernel<double, 3, true> : l2p_mfma_kernel<double, 3, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 3, true> : l2p_mfma_kernel<float, 3, false>), 0.0f); break;
+    default: f64 ? go((stage ? l2p_mfma_kernel<double, 4, true> : l2p_mfma_kernel<double, 4, false>), 0.0) : go((stage ? l2p_mfma_kernel<float, 4, true> : l2p_mfma_kernel<float, 4, false>), 0.0f); break;
+    }
+    H2G_CHECK_LAUNCH();
+    return true;
+}
+
+bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
+                     const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s) {
+    if (R < MFMA_MIN_R || (cdtype != H2G_F64 && cdtype != H2G_F32)) return false;
+    // column tiles per workgroup: up to 4, as many as the LDS block of 64 KB allows
+    const size_t es = cdtype == H2G_F64 ? 8 : 4;
+    const int nct = std::min({4, (R + 15) / 16, (int)(65536 / (es * 16 * (size_t)(K + 1)))});
+    if (nct < 1) return false;
+    const dim3 grid((unsigned)n, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
+    auto go = [&](auto kern, auto tv) {
+        using T = decltype(tv);
+        const size_t sh = sizeof(T) * (size_t)nct * 16 * (K + 1);
+        kern<<<grid, P2M_NT, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const double *)xs, R, (T *)mom);
+    };
+    const bool f64 = cdtype == H2G_F64;
+    switch (nct) {
+    case 1: f64 ? go(p2m_mfma_kernel<double, 1>, 0.0) : go(p2m_mfma_kernel<float, 1>, 0.0f); break;
+    case 2: f64 ? go(p2m_mfma_kernel<double, 2>, 0.0) : go(p2m_mfma_kernel<float, 2>, 0.0f); break;
+    case 3: f64 ? go(p2m_mfma_kernel<double, 3>, 0.0) : go(p2m_mfma_kernel<float, 3>, 0.0f); break;
+    default: f64 ? go(p2m_mfma_kernel<double, 4>, 0.0) : go(p2m_mfma_kernel<float, 4>, 0.0f); break;
+    }
+    H2G_CHECK_LAUNCH();
+    return true;
+}
+
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, bool vals32, hipStream_t s) {
+    if (c.dtype != H2G_F64 || R < MFMA_MIN_R) return false;
+    const bool trans = op != H2G_OP_N;
+    const NearPlan &P = trans ? c.near_rev_plan : c.near_fwd_plan;
+    if (!P.ok) return false;
+    // NCT column tiles per workgroup (up to 64 RHS); wider batches take several workgroups
+    const int nct = std::min(4, (R + 15) / 16);
+    const unsigned gy = (unsigned)((R + 16 * nct - 1) / (16 * nct));
+    const bool f32 = vals32 && c.near_vals32.ptr;
+    for (int i = 0; i < 4; ++i) {  // tasks of 4, 3, 2, 1 row tiles
+        const int64_t t0 = P.cls[i], t1 = P.cls[i + 1];
+        if (t1 == t0) continue;
+        const dim3 grid((unsigned)((t1 - t0 + TPB / 64 - 1) / (TPB / 64)), gy);
+        auto go = [&](auto kern, auto vv) {
+            using V = decltype(vv);
+            kern<<<grid, TPB, 0, s>>>(P.task.as<NearTask>(), t0, t1, P.col.as<NearCol>(),
+                                      f32 ? (const V *)c.near_vals32.ptr : (const V *)c.near_vals.ptr,
+                                      (const double *)xs, (double *)ys, R);
+        };
+        // the value type (FP32 copy or FP64) and the orientation select the instantiation
+        auto pick = [&](auto rt_c, auto ku_c, auto nct_c) {
+            constexpr int RT = decltype(rt_c)::value, KU = decltype(ku_c)::value, NC = decltype(nct_c)::value;
+            if (f32)
+                trans ? go(near_task_kernel<true, RT, KU, NC, float>, 0.f)
+                      : go(near_task_kernel<false, RT, KU, NC, float>, 0.f);
+            else
+                trans ? go(near_task_kernel<true, RT, KU, NC, double>, 0.0)
+                      : go(near_task_kernel<false, RT, KU, NC, double>, 0.0);
+        };
+        using std::integral_constant;
+        auto by_nct = [&](auto rt_c, auto ku_c) {
+            switch (nct) {
+            case 1: pick(rt_c, ku_c, integral_constant<int, 1>{}); break;
+            case 2: pick(rt_c, ku_c, integral_constant<int, 2>{}); break;
+            case 3: pick(rt_c, ku_c, integral_constant<int, 3>{}); break;
+            default: pick(rt_c, ku_c, integral_constant<int, 4>{}); break;
+            }
+        };
+        switch (4 - i) {
+        case 1: by_nct(integral_constant<int, 1>{}, integral_constant<int, 4>{}); break;
+        case 2: by_nct(integral_constant<int, 2>{}, integral_constant<int, 2>{}); break;
+        case 3: by_nct(integral_constant<int, 3>{}, integral_constant<int, 2>{}); break;
+        default: by_nct(integral_constant<int, 4>{}, integral_constant<int, 2>{}); break;
+        }
+        H2G_CHECK_LAUNCH();
+    }
+    return true;
+}
+
+void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    narrow_values_kernel<<<(unsigned)std::min<int64_t>((n + TPB - 1) / TPB, 1 << 20), TPB, 0, s>>>(src, dst, n);
+    H2G_CHECK_LAUNCH();
+}
+
+}  // namespace h2g
+
+// Test hook: runs the MFMA layout probe on host arrays (A 16x4, B 4x16 row-major; D 16x16).
+// Test hook: the FP32 layout probe (A 16x4, B 4x16 row-major; D 16x16).
+extern "C" int h2g_probe_mfma_f32(const float *A, const float *B, float *D) {
+    using namespace h2g;
+    try {
+        DevBuf a, b, d;
+        a.upload(A, 64 * sizeof(float));
+        b.upload(B, 64 * sizeof(float));
+        d.alloc(256 * sizeof(float));
+        mfma_f32_probe_kernel<<<1, 64>>>(a.as<float>(), b.as<float>(), d.as<float>());
+        H2G_HIP(hipGetLastError());
+        H2G_HIP(hipMemcpy(D, d.ptr, 256 * sizeof(float), hipMemcpyDeviceToHost));
+        return H2G_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+}
+
+extern "C" int h2g_probe_mfma_f64(const double *A, const double *B, double *D) {
+    using namespace h2g;
+    try {
+        DevBuf a, b, d;
+        a.upload(A, 64 * sizeof(double));
+        b.upload(B, 64 * sizeof(double));
+        d.alloc(256 * sizeof(double));
+        mfma_f64_probe_kernel<<<1, 64>>>(a.as<double>(), b.as<double>(), d.as<double>());
+        H2G_HIP(hipGetLastError());
+        H2G_HIP(hipMemcpy(D, d.ptr, 256 * sizeof(double), hipMemcpyDeviceToHost));
+        return H2G_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+}

@@ -98,19 +98,19 @@ void fill_side_collection(const Side &S, const h2g_method_side_desc &m, int dtyp
     leaves.upload_vec(lv);
     if (lv.empty()) return;
     if (dtype == H2G_F64)
-        coll_fill_kernel<double><<<(unsigned)lv.size(), 256>>>(leaves.as<int32_t>(), S.d_begin.as<int64_t>(),
+        hipLaunchKernelGGL((coll_fill_kernel<double>), (unsigned)lv.size(), 256, 0, 0, leaves.as<int32_t>(), S.d_begin.as<int64_t>(),
                                                                S.d_end.as<int64_t>(), S.d_perm.as<int64_t>(),
                                                                pts.as<double>(), cen.as<double>(), hw.as<double>(),
                                                                d_nodes, p, S.coll.as<double>());
     else
-        coll_fill_kernel<cplx<double>><<<(unsigned)lv.size(), 256>>>(
+        hipLaunchKernelGGL((coll_fill_kernel<cplx<double>>), (unsigned)lv.size(), 256, 0, 0,
             leaves.as<int32_t>(), S.d_begin.as<int64_t>(), S.d_end.as<int64_t>(), S.d_perm.as<int64_t>(),
             pts.as<double>(), cen.as<double>(), hw.as<double>(), d_nodes, p, S.coll.as<cplx<double>>());
     H2G_HIP(hipGetLastError());
     H2G_HIP(hipDeviceSynchronize());
     if (S.coll32.ptr) {
         const int64_t n = (int64_t)(S.coll.bytes / sizeof(double));
-        narrow_kernel<<<(unsigned)((n + 255) / 256), 256>>>(S.coll.as<double>(), S.coll32.as<float>(), n);
+        hipLaunchKernelGGL((narrow_kernel), (unsigned)((n + 255) / 256), 256, 0, 0, S.coll.as<double>(), S.coll32.as<float>(), n);
         H2G_HIP(hipGetLastError());
         H2G_HIP(hipDeviceSynchronize());
     }
@@ -141,13 +141,13 @@ extern "C" int h2g_method_fill(h2g_core *core, const h2g_method_desc *desc) {
             }
             const unsigned nb = (unsigned)c.n_near_blocks;
             if (c.dtype == H2G_F64)
-                near_fill_kernel<double><<<nb, 256>>>(c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(),
+                hipLaunchKernelGGL((near_fill_kernel<double>), nb, 256, 0, 0, c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(),
                                                       c.near_blk_nr.as<int32_t>(), c.near_blk_nc.as<int32_t>(),
                                                       c.near_blk_off.as<int64_t>(), c.test.d_perm.as<int64_t>(),
                                                       tr.d_perm.as<int64_t>(), pt.as<double>(), dps, d.kernel,
                                                       d.wavenumber, c.near_vals.as<double>());
             else
-                near_fill_kernel<cplx<double>><<<nb, 256>>>(
+                hipLaunchKernelGGL((near_fill_kernel<cplx<double>>), nb, 256, 0, 0,
                     c.near_blk_rb.as<int64_t>(), c.near_blk_cb.as<int64_t>(), c.near_blk_nr.as<int32_t>(),
                     c.near_blk_nc.as<int32_t>(), c.near_blk_off.as<int64_t>(), c.test.d_perm.as<int64_t>(),
                     tr.d_perm.as<int64_t>(), pt.as<double>(), dps, d.kernel, d.wavenumber,
