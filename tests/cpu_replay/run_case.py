@@ -1,0 +1,104 @@
+"""Runs one map through the host replay of libh2g under AddressSanitizer -- TEST INFRASTRUCTURE
+ONLY.  Started by tests/test_cpu_replay.py in a child process with the ASan runtime preloaded;
+prints one line per product (relative error against the oracle) and exits non-zero on a mismatch.
+
+    python run_case.py CASE
+"""
+
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "h2factory.jl_amd"), os.path.join(ROOT, "tests")]
+
+import numpy as np  # noqa: E402
+
+import h2g._lib as H  # noqa: E402
+
+H.LIB_PATH = os.path.join(HERE, "_build", "libh2g_replay.so")
+
+import h2g  # noqa: E402
+from convert import to_arrays  # noqa: E402
+from oracle import build as OB  # noqa: E402
+from oracle import h2 as O  # noqa: E402
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def orient(A, op):
+    return {"N": A, "T": h2g.transpose(A), "C": h2g.adjoint(A)}[op]
+
+
+def helmholtz_sphere():
+    """The map of the Helmholtz parity test: 1,500 points on a sphere, k = 10, Chebyshev p = 3,
+    dense couplings, leaves of at most 32 points."""
+    pts = np.random.default_rng(8).standard_normal((1500, 3))
+    pts = 0.5 + 0.45 * pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=10.0,
+                         leaf_size=32)
+    x = np.random.default_rng(2).standard_normal(1500) + 1j * np.random.default_rng(3).standard_normal(1500)
+    return core, x
+
+
+def random_petrov_symdense():
+    """Random complex operators, separate test and trial trees, symmetry-reduced dense couplings
+    (the box GEMM with grid permutations), two right-hand sides."""
+    rng = np.random.default_rng(7)
+    core = OB.random_core(rng.random((260, 3)), rng.random((190, 3)), p=2, m2l_kind="symdense", leaf_size=8,
+                          seed=11, alpha=0.8 - 0.3j)
+    x = rng.standard_normal((190, 2)) + 1j * rng.standard_normal((190, 2))
+    return core, x
+
+
+def _products(name, A, core, x, ops, fns=("mul",)):
+    ok = True
+    for op in ops:
+        M = orient(A, op)
+        n_out = M.shape[0]
+        for fn in fns:
+            xx = x if op == "N" else _in_for(core, x, op)
+            y = np.zeros((n_out,) + xx.shape[1:], dtype=np.result_type(xx, np.complex128 if core.dtype == np.complex128 else np.float64))
+            getattr(h2g, fn + "_")(y, M, xx)
+            ref = getattr(O, fn)(core, xx, op)
+            ref = ref if np.iscomplexobj(y) else ref.real
+            e = relerr(y, ref)
+            print(f"{name} {fn} op={op}: rel err vs oracle {e:.2e}", flush=True)
+            ok &= e < 1e-12
+    return ok
+
+
+def _in_for(core, x, op):
+    n_in = core.test.points.shape[0]
+    rng = np.random.default_rng(31)
+    shp = (n_in,) + x.shape[1:]
+    return rng.standard_normal(shp) + (1j * rng.standard_normal(shp) if np.iscomplexobj(x) else 0)
+
+
+def main(name):
+    if name == "laplace_device_fill":
+        # near values and leaf collections filled by the library (h2g_method_fill) against the
+        # oracle's host assembly on the same points
+        from h2g import method as PM
+
+        pts = np.random.default_rng(4).random((1200, 3))
+        core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=40)
+        arrays = PM.build_map(pts, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=40, host_values=False)
+        c = h2g.GpuH2Core(arrays, precisions=("plain", "mixed"))
+        A = h2g.H2Workspace(c, max_rhs=3)
+        x = np.random.default_rng(9).standard_normal((1200, 3))
+        ok = _products(name, A, core, x, "NT", ("mul", "nearmul", "farmul"))
+    else:
+        core, x = {"helmholtz_sphere": helmholtz_sphere, "random_petrov_symdense": random_petrov_symdense}[name]()
+        c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
+        A = h2g.H2Workspace(c, max_rhs=2)
+        ok = _products(name, A, core, x, "NTC", ("mul", "farmul", "nearmul"))
+    A.close()
+    c.close()
+    sys.exit(0 if ok else 1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1])

@@ -1,0 +1,46 @@
+// Host replay of libh2g -- TEST INFRASTRUCTURE ONLY.
+//
+// The MFMA near-field and leaf kernels (leaf_mfma.hip) and the lattice Toeplitz kernels
+// (m2l_lattice.hip, m2l_mfma.hip) use gfx950 instructions and cannot run on the host.  These
+// stand-ins let the replay link: the MFMA launchers decline (as they do on the GPU for complex
+// element types, so a complex map takes the same generic kernels there and here), the FP32 value
+// copy runs as a host loop, and the lattice entry points refuse -- the replay covers maps whose
+// couplings run on the generic kernels (dense, symmetry-reduced dense, and Toeplitz maps built
+// without lattice coordinates).
+#include "h2g_internal.h"
+
+namespace h2g {
+
+bool launch_near_mfma(int, const Core &, const void *, void *, int, bool, hipStream_t) { return false; }
+bool launch_p2m_mfma(int, const int32_t *, int, const int64_t *, const int64_t *, const void *, int, const void *,
+                     int, void *, hipStream_t) {
+    return false;
+}
+bool launch_l2p_mfma(int, const int32_t *, int, const int64_t *, const int64_t *, const void *, int, const void *,
+                     int, void *, double, bool, hipStream_t) {
+    return false;
+}
+void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t) {
+    for (int64_t i = 0; i < n; ++i) dst[i] = (float)src[i];
+}
+
+[[noreturn]] static void lattice_refused() {
+    throw Error{H2G_EUNSUPPORTED, "host replay: the lattice Toeplitz kernels run on the GPU only"};
+}
+int m2l_direct_rows(int, const LatticeLevel &, int, int) { lattice_refused(); }
+void launch_m2l_lattice(int, bool, const LatticeLevel &, int, int, const void *, void *, int, hipStream_t) {
+    lattice_refused();
+}
+void build_lattice_ea(const int32_t *, const int32_t *, const int32_t *, const void *, int, bool, bool, LatticeLevel &) {
+    lattice_refused();
+}
+void launch_fft_in_fm(int, bool, bool, int, int, const void *, int, int, int, int, int, bool, void *, const void *,
+                      hipStream_t) {
+    lattice_refused();
+}
+void launch_fft_out_fm(int, bool, bool, const int32_t *, int, const void *, int, int, int, int, void *, const void *,
+                       hipStream_t) {
+    lattice_refused();
+}
+
+}  // namespace h2g
