@@ -298,6 +298,7 @@ def main():
         "plain_fp64_rhs_per_s": round(plain_rhs_per_s, 2),
         "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
         "rel_err_vs_plain_fp64": err_vs_plain,
+        "gate_1e-6_vs_plain": bool(err_vs_plain <= 1e-6),
         "rel_err_vs_direct_sum_1000": {"plain": err_direct_plain, "batched": err_direct_batched},
         "phase_ms": {k: round(v, 3) for k, v in totals.items()},
         "plain_phase_ms": {k: round(v, 3) for k, v in ptot.items()},
@@ -307,6 +308,9 @@ def main():
         "build_s": {"host_plan": round(t_build, 1), "device_core": round(t_core, 1)},
     }
     print(json.dumps(out), flush=True)
+    if not err_vs_plain <= 1e-6:
+        log(f"INVALID: batched result {err_vs_plain:.2e} from the plain FP64 product (gate 1e-6)")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -211,3 +211,46 @@ def worked_example_core(leaf_size=2):
     )
     F = kern(s[:, None], s[None, :])
     return core, s, F
+
+
+def farmul_fixture_core():
+    """The reference's hand-derived far-field fixture (``test/farinteractions/test_farmul.jl:11-97``):
+    two clusters of four points on a line (x = 0, 0.1, 5, 5.1 and 40, 40.1, 45, 45.1, scaled here
+    by 1/64 into the unit cube), each split into two sub-clusters.  Moments are the plain charge
+    sum (K = 1, unit collections), transfers are the identity and every coupling is the scalar
+    ``c = 2 - i``, so the far matrix is ``F[left, right] = F[right, left] = c`` and 0 elsewhere.
+    On this build's octree (leaves of two points) the two clusters translate at level 2 and the
+    sub-clusters of one cluster are adjacent leaves, as in the reference's tree; near values are
+    zero (the fixture checks the far field only)."""
+    s = np.array([0.0, 0.1, 5.0, 5.1, 40.0, 40.1, 45.0, 45.1]) / 64.0
+    pts = np.zeros((8, 3))
+    pts[:, 0] = s
+    pts[:, 1] = 0.3
+    pts[:, 2] = 0.6
+    tree = build_tree(pts, 2)
+    c = 2.0 - 1.0j
+    one = np.ones((1, 8), dtype=np.complex128)
+    I1 = np.eye(1, dtype=np.complex128)
+    transfers = [[I1] * 8 for _ in range(tree.n_levels)]
+    pairs = m2l_pairs(tree, tree)
+    npairs = near_pairs(tree, tree)
+    vals = [np.zeros((tree.end[b] - tree.begin[b], tree.end[q] - tree.begin[q]), dtype=np.complex128)
+            for b, q in npairs]
+    core = Core(
+        test=tree, trial=tree, K=1, p=0, dtype=np.complex128,
+        coll_test=one, coll_trial=one, transfers_test=transfers, transfers_trial=transfers,
+        pairs=pairs, pair_op=np.zeros(len(pairs), dtype=np.int64), m2l_kind="dense",
+        ops=np.array([[[c]]]), near_pairs=npairs, near_values=vals, alpha=1.0 + 0j,
+    )
+    F = np.zeros((8, 8), dtype=np.complex128)
+    F[:4, 4:] = c
+    F[4:, :4] = c
+    return core, pts, F
+
+
+def no_far_core():
+    """Five points inside adjacent leaves with no far pair at all, random complex operators: every
+    receiving leaf is uninitialised, so the 5-argument farmul! writes no row
+    (``src/farinteractions/traversalhelpers.jl:103-112``)."""
+    pts = np.array([[0.49, 0.5, 0.5], [0.48, 0.5, 0.5], [0.51, 0.5, 0.5], [0.52, 0.5, 0.5], [0.5, 0.51, 0.5]])
+    return random_core(pts, p=2, m2l_kind="dense", leaf_size=2, seed=3)

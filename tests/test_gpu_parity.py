@@ -127,7 +127,7 @@ def test_laplace_method(gpu, kind, p, nodes):
 @pytest.mark.parametrize("kind,p,nodes", [("toeplitz", 6, "equi"), ("dense", 5, "cheb")])
 def test_laplace_method_high_order(gpu, kind, p, nodes):
     """p = 5, 6 (separable transfers at their largest tile): plain == oracle to 1e-12, and the
-    batched mixed form at 32 / 33 columns within 1e-5 of the plain form."""
+    batched mixed form at 32 / 33 columns within the 1e-6 gate of the plain form."""
     import h2g
 
     pts = np.random.default_rng(14).random((6000, 3))
@@ -150,7 +150,7 @@ def test_laplace_method_high_order(gpu, kind, p, nodes):
             h2g.mul_(y64, orient(plain, op), X[:, :nrhs])
             y32 = np.zeros((6000, nrhs), dtype=np.float32)
             h2g.mul_(y32, orient(mixed, op), X[:, :nrhs].astype(np.float32))
-            assert relerr(y32, y64) < 1e-5, (op, nrhs)
+            assert relerr(y32, y64) < 1e-6, (op, nrhs)
 
 
 @pytest.mark.skip(reason="faulted the GPU twice (memory aperture violation) with the separable-transfer kernel "
@@ -253,7 +253,7 @@ def test_lattice_toeplitz_matches_generic(gpu, kind, real):
         x = rng.standard_normal((n, nrhs)) + (0 if real else 1j * rng.standard_normal((n, nrhs)))
         for op in "NTC":
             ref = O.mul(core, x, op)
-            assert relerr(orient(mixed, op) @ x, ref.real if real else ref) < 1e-5, (op, nrhs)
+            assert relerr(orient(mixed, op) @ x, ref.real if real else ref) < 1e-6, (op, nrhs)
 
 
 def test_mfma_f64_lane_layout(gpu):
@@ -334,3 +334,90 @@ def test_wide_near_field_mixed(gpu, petrov, leaf):
             y = np.zeros((n_out, nrhs), dtype=np.float32)
             h2g.nearmul_(y, M, x)
             assert relerr(y, O.nearmul(core, x.astype(np.float64), op)) < 1e-6, (op, nrhs)
+
+
+def test_reference_farmul_fixture(gpu):
+    """test/farinteractions/test_farmul.jl:11-97 through the C ABI: the hand-derived far matrix
+    (c = 2 - i between the two clusters) in every orientation, and the 5-argument form."""
+    import h2g
+
+    core, _, F = OB.farmul_fixture_core()
+    A = make_map(core, max_rhs=2)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(8) + 1j * rng.standard_normal(8)
+    for M, R in ((A, F), (h2g.transpose(A), F.T), (h2g.adjoint(A), F.conj().T)):
+        y = np.zeros(8, dtype=np.complex128)
+        h2g.farmul_(y, M, x)
+        assert relerr(y, R @ x) < RTOL
+    y0 = rng.standard_normal(8) + 1j * rng.standard_normal(8)
+    a, b = 0.7 + 0.3j, -0.4 + 0.2j
+    y = y0.copy()
+    h2g.farmul_(y, A, x, a, b)
+    assert relerr(y, a * F @ x + b * y0) < RTOL
+
+
+def test_farmul5_without_far_pairs_writes_no_row(gpu):
+    """No far pair: the 5-argument farmul! leaves y bit for bit (traversalhelpers.jl:103-112)."""
+    import h2g
+
+    core = OB.no_far_core()
+    A = make_map(core)
+    y0 = np.arange(5) + 1j
+    y = y0.copy()
+    h2g.farmul_(y, A, np.ones(5) + 0j, 2.0, 3.0)
+    assert np.array_equal(y, y0)
+
+
+@pytest.mark.parametrize("kind,p,nodes,petrov", [("dense", 4, "cheb", False), ("toeplitz", 6, "equi", False),
+                                                 ("toeplitz", 4, "equi", True)])
+def test_device_filled_map_matches_oracle(gpu, kind, p, nodes, petrov):
+    """The build path the benchmark takes: near values and leaf collections filled on the device
+    (h2g_method_fill, FP64 and their FP32 copies) against the oracle's host assembly on the same
+    points -- plain form to 1e-12 (near field alone and the full product, forward and transpose),
+    mixed form within the 1e-6 gate."""
+    import h2g
+    from h2g import method as PM
+
+    rng = np.random.default_rng(31)
+    pts = rng.random((6000, 3))
+    pts_b = rng.random((4500, 3)) if petrov else None
+    core = OB.build_core(pts, pts_b, p=p, nodekind=nodes, m2l_kind=kind, leaf_size=64)
+    arrays = PM.build_map(pts, pts_b, p=p, nodekind=nodes, m2l_kind=kind, leaf_size=64, host_values=False)
+    assert arrays.near_values is None and arrays.test.collection is None
+    c = h2g.GpuH2Core(arrays, precisions=("plain", "mixed"))
+    A = h2g.H2Workspace(c, max_rhs=4)
+    Am = h2g.H2Workspace(c, max_rhs=32, precision="mixed")
+    for op in "NT":
+        n_out, n_in = orient(A, op).shape
+        x = rng.standard_normal((n_in, 4))
+        y = np.zeros((n_out, 4))
+        h2g.nearmul_(y, orient(A, op), x)
+        assert relerr(y, O.nearmul(core, x, op)) < RTOL, (op, "near")
+        y = np.zeros((n_out, 4))
+        h2g.mul_(y, orient(A, op), x)
+        assert relerr(y, O.mul(core, x, op).real) < RTOL, (op, "mul")
+        X = rng.standard_normal((n_in, 32)).astype(np.float32)
+        ym = orient(Am, op) @ X
+        assert relerr(ym, O.mul(core, X.astype(np.float64), op).real) < 1e-6, (op, "mixed")
+
+
+def test_helmholtz_device_near_field(gpu):
+    """Helmholtz k = 10 on the sphere: the near field filled on the device (ComplexF64 kernel
+    values) and applied alone, every orientation, against the oracle (no transfer kernel runs)."""
+    import h2g
+    from h2g import method as PM
+
+    pts = np.random.default_rng(8).standard_normal((3000, 3))
+    pts = 0.5 + 0.45 * pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=10.0,
+                         leaf_size=48)
+    arrays = PM.build_map(pts, p=3, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=10.0,
+                          leaf_size=48, host_values=False)
+    A = h2g.H2Workspace(h2g.GpuH2Core(arrays), max_rhs=3)
+    rng = np.random.default_rng(5)
+    for nrhs in (1, 3):
+        x = rng.standard_normal((3000, nrhs)) + 1j * rng.standard_normal((3000, nrhs))
+        for op in "NTC":
+            y = np.zeros((3000, nrhs), dtype=np.complex128)
+            h2g.nearmul_(y, orient(A, op), x)
+            assert relerr(y, O.nearmul(core, x, op)) < RTOL, (op, nrhs)

@@ -99,3 +99,44 @@ def test_direct_sum_matches_double_loop():
     ref = np.array([sum(x[j] / (4 * np.pi * np.linalg.norm(pts[i] - pts[j])) for j in range(40) if j != i)
                     for i in range(40)])
     assert np.allclose(O.direct_sum("laplace", 0.0, pts, pts, x), ref, rtol=1e-13)
+
+
+def test_reference_farmul_fixture():
+    """test/farinteractions/test_farmul.jl:11-97: the hand-derived far matrix F (F[left, right] =
+    F[right, left] = c = 2 - i) in every orientation and the 5-argument accumulation."""
+    core, _, F = OB.farmul_fixture_core()
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(8) + 1j * rng.standard_normal(8)
+    assert relerr(O.farmul(core, x, "N"), F @ x) < 1e-14
+    assert relerr(O.farmul(core, x, "T"), F.T @ x) < 1e-14
+    assert relerr(O.farmul(core, x, "C"), F.conj().T @ x) < 1e-14
+    y = rng.standard_normal(8) + 1j * rng.standard_normal(8)
+    a, b = 0.7 + 0.3j, -0.4 + 0.2j
+    assert relerr(O.farmul(core, x, "N", a, b, y), a * F @ x + b * y) < 1e-14
+    # a conjugation dropped from the adjoint would give F^T x instead
+    assert relerr(F.T @ x, F.conj().T @ x) > 0.5
+
+
+def test_farmul5_without_far_pairs_writes_no_row():
+    """With no far pair every receiving leaf is uninitialised and the 5-argument farmul! leaves y
+    exactly as it was -- not even scaled by beta (src/farinteractions/traversalhelpers.jl:103-112)."""
+    core = OB.no_far_core()
+    assert len(core.pairs) == 0 and len(core.near_pairs) > 0
+    y0 = np.arange(5) + 1j
+    assert np.array_equal(O.farmul(core, np.ones(5) + 0j, "N", 2.0, 3.0, y0), y0)
+    assert np.array_equal(O.farmul(core, np.ones(5) + 0j, "N"), np.zeros(5))
+
+
+def test_helmholtz_method_accuracy_vs_direct_sum():
+    """Helmholtz k = 10 on the sphere, Chebyshev p = 5: the H2 product against the O(N^2) sum."""
+    pts = np.random.default_rng(8).standard_normal((1500, 3))
+    pts = 0.5 + 0.45 * pts / np.linalg.norm(pts, axis=1, keepdims=True)
+    x = np.random.default_rng(3).standard_normal(1500) + 1j * np.random.default_rng(4).standard_normal(1500)
+    core = OB.build_core(pts, p=5, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=10.0,
+                         leaf_size=32)
+    for op, conj in (("N", False), ("C", True)):
+        y = O.mul(core, x, op)
+        yd = O.direct_sum("helmholtz", 10.0, pts, pts, x)
+        if conj:  # the Helmholtz kernel matrix is symmetric: adjoint = conj(A)
+            yd = np.conj(O.direct_sum("helmholtz", 10.0, pts, pts, np.conj(x)))
+        assert relerr(y, yd) < 1e-3, op

@@ -119,3 +119,43 @@ def test_permuted_spectra_and_dense_equal_direct(offset):
     if not np.array_equal(perm[perm], np.arange(L ** 3)):
         wrong = M.toeplitz_permutation((L, L, L), M.inverse_symmetry(S))
         assert not np.allclose(Dc[wrong], Dd, rtol=1e-6)
+
+
+def test_helmholtz_kernel_is_the_spherical_hankel_function():
+    """exp(ikr)/(4 pi r) = (ik/4 pi) h0(kr) with h0 = j0 + i y0 (scipy.special), 0 at r = 0; a sign
+    slip (exp(-ikr)) or a missing 1/(4 pi) fails here."""
+    from scipy.special import spherical_jn, spherical_yn
+
+    r = np.random.default_rng(0).uniform(1e-3, 3.0, 200)
+    for k in (2.0, 10.0, 37.5):
+        h0 = spherical_jn(0, k * r) + 1j * spherical_yn(0, k * r)
+        assert np.allclose(M.kernel_of_r("helmholtz", k, r), 1j * k / (4 * np.pi) * h0, rtol=1e-13, atol=0)
+    xt = np.random.default_rng(1).random((30, 3))
+    G = M.kernel_matrix("helmholtz", 10.0, xt, xt)
+    d = np.linalg.norm(xt[:, None] - xt[None], axis=-1)
+    off = ~np.eye(30, dtype=bool)
+    assert np.allclose(G[off], M.kernel_of_r("helmholtz", 10.0, d[off]), rtol=1e-14)
+    assert np.all(np.diag(G) == 0)
+
+
+def test_helmholtz_tends_to_laplace():
+    r = np.linspace(0.01, 2.0, 50)
+    lap = M.kernel_of_r("laplace", 0.0, r)
+    assert np.allclose(M.kernel_of_r("helmholtz", 1e-9, r), lap, rtol=1e-8, atol=0)
+    assert np.array_equal(M.kernel_of_r("helmholtz", 0.0, r).real, lap)
+
+
+@pytest.mark.parametrize("offset", [(2, 0, 0), (3, -2, 1)])
+def test_helmholtz_toeplitz_spectrum_equals_dense_kernel(offset):
+    """The Helmholtz branch of the equispaced M2L: dense == multilevel Toeplitz == FFT application;
+    the transpose is the same (symmetric kernel), the adjoint conjugates."""
+    p, w, k = 4, 0.125, 10.0
+    nodes = M.nodes1d(p, "equi")
+    dense = M.m2l_dense("helmholtz", k, nodes, w, offset)
+    kappa = M.toeplitz_kappa("helmholtz", k, p, w, offset)
+    assert np.allclose(M.toeplitz_dense(kappa, p), dense, rtol=1e-13, atol=0)
+    D = np.fft.fftn(kappa)
+    x = np.random.default_rng(0).standard_normal(p ** 3) + 1j * np.random.default_rng(1).standard_normal(p ** 3)
+    for op, R in (("N", dense), ("T", dense.T), ("C", dense.conj().T)):
+        y = M.modulate_out(M.hadamard_transform(D, op) * M.modulate_in(x, p, 2 * p - 1, op), p, op)
+        assert np.allclose(y, R @ x, rtol=1e-12, atol=1e-14), op

@@ -60,13 +60,25 @@ def test_continuous_batching_same_alone_and_in_mix(gpu):
         M = {"N": A, "T": h2g.transpose(A), "C": h2g.adjoint(A)}[r.op]
         y = M @ r.x
         assert np.linalg.norm(r.result - y) <= 1e-6 * np.linalg.norm(y), (r.map_name, r.op, r.batch_size)
-    # and the served results match the FP64 plain product to the mixed-precision tolerance
-    plain = h2g.H2Workspace(srv.maps["lap"].core, max_rhs=1, precision="plain")
-    for r in reqs[:6]:
-        if r.map_name != "lap":
-            continue
-        M = plain if r.op == "N" else h2g.transpose(plain)
-        y64 = M @ r.x.astype(np.float64)
-        assert np.linalg.norm(r.result - y64) <= 1e-5 * np.linalg.norm(y64)
+    for A in alone.values():
+        A.close()
+    # and every served result against the FP64 oracle on the same points, within the 1e-6 gate
+    ref = _oracle_cores()
+    for r in reqs:
+        y64 = O.mul(ref[r.map_name], r.x.astype(np.complex128 if r.map_name == "hel" else np.float64), r.op)
+        y64 = y64 if r.map_name == "hel" else y64.real
+        assert np.linalg.norm(r.result - y64) <= 1e-6 * np.linalg.norm(y64), (r.map_name, r.op, r.batch_size)
     srv.close()
     assert not srv.maps
+
+
+def _oracle_cores():
+    from oracle import build as OB
+
+    rng = np.random.default_rng(0)
+    cube = rng.random((3000, 3))
+    sph = rng.standard_normal((1600, 3))
+    sph = 0.5 + 0.45 * sph / np.linalg.norm(sph, axis=1, keepdims=True)
+    return {"lap": OB.build_core(cube, p=4, nodekind="equi", m2l_kind="toeplitz", leaf_size=32),
+            "hel": OB.build_core(sph, p=3, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=2.0,
+                                 leaf_size=32)}
