@@ -8,6 +8,7 @@
 #include <complex>
 #include <set>
 #include <unordered_map>
+#include <mutex>
 
 #include "h2g_internal.h"
 
@@ -1088,6 +1089,164 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     if (join) H2G_HIP(hipStreamWaitEvent(s, join, 0));
 }
 
+// Device buffers of one workspace of `R` columns (bytes): tree-ordered x and y in the core type,
+// the moment pools in the compute type and, for Toeplitz couplings, the spectra.
+struct WsLayout {
+    size_t xs = 0, ys = 0, mom = 0, loc = 0, spec_in = 0, spec_out = 0;
+    size_t total(size_t align) const {
+        size_t s = 0;
+        for (size_t b : {xs, ys, mom, loc, spec_in, spec_out}) s += (b + align - 1) / align * align;
+        return s;
+    }
+};
+
+WsLayout ws_layout(const Core &c, bool mixed, int R) {
+    WsLayout L;
+    const int64_t nmax = std::max(c.test.n_dofs, c.trial->n_dofs);
+    const int64_t bmax = std::max(c.test.n_boxes, c.trial->n_boxes);
+    const size_t se = elem_size(c.dtype);
+    const size_t ce = mixed ? se / 2 : se;
+    L.xs = L.ys = (size_t)nmax * R * se;
+    L.mom = L.loc = (size_t)bmax * c.K * R * ce;
+    if (is_toeplitz(c)) {
+        const size_t cc = mixed ? 8 : 16;
+        // per column, the largest level's (boxes x stored frequencies) of either orientation
+        int64_t fin = 0, fout = 0;
+        spectrum_sizes(c, c.fwd, *c.trial, fin, fout);
+        spectrum_sizes(c, c.rev, c.test, fin, fout);
+        // source spectra of FP32 batches of >= 4 columns may be stored with rows padded to 32
+        // columns (the direct M2L kernel's layout)
+        const int64_t rin = mixed && R >= 4 ? (R + 31) / 32 * 32 : R;
+        L.spec_in = (size_t)fin * rin * cc;
+        L.spec_out = (size_t)fout * R * cc;
+    }
+    return L;
+}
+
+// y = a op(A) x + b y (five) or y = op(A) x over column blocks of the workspace's capacity, all on
+// the workspace stream.  x and y are device pointers.  overlap_near: the near field runs on the
+// workspace's side stream, concurrent with the far field up to its first write of the result.
+void run_passes(Workspace &w, int op, int which, int dtype, int64_t nrhs, const void *dx, int64_t lx, void *dy,
+                int64_t ly, cplx<double> a, cplx<double> b, bool five, bool overlap_near, bool use_graph) {
+    Core &c = *w.core;
+    hipStream_t s = w.stream;
+    const size_t xe = elem_size(dtype), se = elem_size(c.dtype);
+    const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
+    const Side &src = *P.src, &dst = *P.dst;
+    const bool mixed_ws = w.opts.precision == H2G_PREC_MIXED;
+    for (int64_t c0 = 0; c0 < nrhs; c0 += w.R_cap) {
+        const int R = (int)std::min<int64_t>(w.R_cap, nrhs - c0);
+        w.mark_begin(0, -1);
+        launch_gather(c.dtype, dtype, (const char *)dx + (size_t)c0 * lx * xe, lx, src.d_perm.as<int64_t>(),
+                      src.d_iperm.as<int64_t>(), src.n_dofs, R, w.xs.ptr, s);
+        w.mark_end();
+        auto body = [&]() {
+            launch_fill_zero(w.ys.ptr, (size_t)dst.n_dofs * R * se, s);
+            if ((which & H2G_NEARMUL) && (which & H2G_FARMUL) && overlap_near && !w.opts.timing) {
+                // the near field (which sets its rows of the result) on the side stream,
+                // concurrent with the far field up to its first write of the result
+                H2G_HIP(hipEventRecord(w.ev_fork, s));
+                H2G_HIP(hipStreamWaitEvent(w.side, w.ev_fork, 0));
+                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, mixed_ws, w.side);
+                H2G_HIP(hipEventRecord(w.ev_near, w.side));
+                far_pass(w, op, R, w.ev_near);
+                return;
+            }
+            if (which & H2G_NEARMUL) {
+                w.mark_begin(1, -1);
+                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, mixed_ws, s);
+                w.mark_end();
+            }
+            if (which & H2G_FARMUL) far_pass(w, op, R, nullptr);
+        };
+        if (use_graph) {
+            GraphKey key{op, which, R};
+            auto it = w.graphs.find(key);
+            if (it == w.graphs.end()) {
+                hipGraph_t g;
+                H2G_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    body();
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &g);
+                    throw;
+                }
+                H2G_HIP(hipStreamEndCapture(s, &g));
+                hipGraphExec_t ge;
+                H2G_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(g);
+                it = w.graphs.emplace(key, ge).first;
+            }
+            H2G_HIP(hipGraphLaunch(it->second, s));
+        } else {
+            body();
+        }
+        const uint8_t *mask = (which == H2G_FARMUL && five) ? P.out_mask.as<uint8_t>() : nullptr;
+        w.mark_begin(9, -1);
+        launch_scatter(c.dtype, dtype, w.ys.ptr, dst.d_perm.as<int64_t>(), dst.d_iperm.as<int64_t>(), dst.n_dofs, R,
+                       (char *)dy + (size_t)c0 * ly * xe, ly, a, five ? b : cplx<double>{0.0, 0.0}, mask, s);
+        w.mark_end();
+    }
+}
+
+// Argument checks shared by the product entry points; returns the scalars (1, 0 without them).
+void check_call(const Core &c, int op, int which, int dtype, int64_t nrhs, const void *x, int64_t ldx, const void *y,
+                int64_t ldy, const void *alpha, const void *beta, cplx<double> &a, cplx<double> &b) {
+    if (op < 0 || op > 2) fail(H2G_EINVAL, "op must be H2G_OP_N, H2G_OP_T or H2G_OP_C");
+    if (which < 1 || which > 3) fail(H2G_EINVAL, "which must be H2G_MUL, H2G_FARMUL or H2G_NEARMUL");
+    if (dtype < 0 || dtype > 3) fail(H2G_EINVAL, "unknown vector dtype");
+    if (c.dtype == H2G_F64 && is_complex_dtype(dtype)) fail(H2G_EINVAL, "a real map takes real vectors");
+    if ((alpha == nullptr) != (beta == nullptr)) fail(H2G_EINVAL, "give both alpha and beta or neither");
+    if (nrhs < 0) fail(H2G_EINVAL, "nrhs must be >= 0");
+    const int64_t n_in = op == H2G_OP_N ? c.trial->n_dofs : c.test.n_dofs;
+    const int64_t n_out = op == H2G_OP_N ? c.test.n_dofs : c.trial->n_dofs;
+    if (nrhs > 0 && (ldx < n_in || ldy < n_out)) fail(H2G_EDIM, "leading dimension smaller than the vector length");
+    if (nrhs > 0 && (!x || !y)) fail(H2G_EINVAL, "null x or y");
+    a = {1.0, 0.0};
+    b = {0.0, 0.0};
+    if (alpha) {
+        // scalars are host values of the vectors' element type
+        auto rd = [&](const void *p) -> cplx<double> {
+            switch (dtype) {
+            case H2G_F64: return {*(const double *)p, 0.0};
+            case H2G_F32: return {(double)*(const float *)p, 0.0};
+            case H2G_C64: return {((const double *)p)[0], ((const double *)p)[1]};
+            default: return {(double)((const float *)p)[0], (double)((const float *)p)[1]};
+            }
+        };
+        a = rd(alpha);
+        b = rd(beta);
+    }
+}
+
+}  // namespace
+
+}  // namespace h2g
+
+// Scratch of h2g_matvec_noalloc: the workspace buffers (x, y, moments, spectra) of its products
+// are views into this array.  Each GPU holds its own copy, at its own address, until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress returns per device.
+static __device__ __align__(256) unsigned char scratch[size_t(128) << 20];
+
+namespace h2g {
+namespace {
+constexpr size_t NOALLOC_BYTES = size_t(128) << 20, NOALLOC_ALIGN = 256;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::array<std::atomic<bool>, NOALLOC_MAX_DEVICES> g_noalloc_busy{};
+
+// Device address of `scratch` on the current device `device`, looked up once per device.
+void *noalloc_scratch(int device) {
+    static std::mutex mu;
+    static std::array<void *, NOALLOC_MAX_DEVICES> ptrs{};
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[device]) {
+        void *p = nullptr;
+        H2G_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(H2G_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        ptrs[device] = p;
+    }
+    return ptrs[device];
+}
 }  // namespace
 
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -1279,28 +1438,13 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             H2G_HIP(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_near, hipEventDisableTiming));
-            const Core &c = *core;
-            const int64_t nmax = std::max(c.test.n_dofs, c.trial->n_dofs);
-            const int64_t bmax = std::max(c.test.n_boxes, c.trial->n_boxes);
-            const size_t se = elem_size(c.dtype);
-            const bool mixed = o.precision == H2G_PREC_MIXED;
-            const size_t ce = mixed ? se / 2 : se;
-            w->xs.alloc((size_t)nmax * w->R_cap * se);
-            w->ys.alloc((size_t)nmax * w->R_cap * se);
-            w->mom.alloc((size_t)bmax * c.K * w->R_cap * ce);
-            w->loc.alloc((size_t)bmax * c.K * w->R_cap * ce);
-            if (is_toeplitz(c)) {
-                const size_t cc = mixed ? 8 : 16;
-                // per column, the largest level's (boxes x stored frequencies) of either orientation
-                int64_t fin = 0, fout = 0;
-                spectrum_sizes(c, c.fwd, *c.trial, fin, fout);
-                spectrum_sizes(c, c.rev, c.test, fin, fout);
-                // source spectra of FP32 batches of >= 4 columns may be stored with rows padded to
-                // 32 columns (the direct M2L kernel's layout)
-                const int64_t rin = mixed && w->R_cap >= 4 ? (w->R_cap + 31) / 32 * 32 : w->R_cap;
-                w->spec_in.alloc((size_t)fin * rin * cc);
-                w->spec_out.alloc((size_t)fout * w->R_cap * cc);
-            }
+            const WsLayout L = ws_layout(*core, o.precision == H2G_PREC_MIXED, w->R_cap);
+            w->xs.alloc(L.xs);
+            w->ys.alloc(L.ys);
+            w->mom.alloc(L.mom);
+            w->loc.alloc(L.loc);
+            w->spec_in.alloc(L.spec_in);
+            w->spec_out.alloc(L.spec_out);
         } catch (...) {
             if (w->ev_near) (void)hipEventDestroy(w->ev_near);
             if (w->ev_fork) (void)hipEventDestroy(w->ev_fork);
@@ -1389,39 +1533,15 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
     if (rc != H2G_OK) return rc;
     rc = guarded([&] {
         Core &c = *ws->core;
-        if (op < 0 || op > 2) fail(H2G_EINVAL, "op must be H2G_OP_N, H2G_OP_T or H2G_OP_C");
-        if (which < 1 || which > 3) fail(H2G_EINVAL, "which must be H2G_MUL, H2G_FARMUL or H2G_NEARMUL");
-        if (dtype < 0 || dtype > 3) fail(H2G_EINVAL, "unknown vector dtype");
-        if (c.dtype == H2G_F64 && is_complex_dtype(dtype))
-            fail(H2G_EINVAL, "a real map takes real vectors");
-        if ((alpha == nullptr) != (beta == nullptr)) fail(H2G_EINVAL, "give both alpha and beta or neither");
-        if (nrhs < 0) fail(H2G_EINVAL, "nrhs must be >= 0");
-        const int64_t n_in = op == H2G_OP_N ? c.trial->n_dofs : c.test.n_dofs;
-        const int64_t n_out = op == H2G_OP_N ? c.test.n_dofs : c.trial->n_dofs;
-        if (nrhs > 0 && (ldx < n_in || ldy < n_out)) fail(H2G_EDIM, "leading dimension smaller than the vector length");
-        if (nrhs > 0 && (!x || !y)) fail(H2G_EINVAL, "null x or y");
-        cplx<double> a{1.0, 0.0}, b{0.0, 0.0};
+        cplx<double> a, b;
+        check_call(c, op, which, dtype, nrhs, x, ldx, y, ldy, alpha, beta, a, b);
         const bool five = alpha != nullptr;
-        if (five) {
-            const bool host = flags & H2G_HOST_PTRS;
-            auto rd = [&](const void *p) -> cplx<double> {
-                if (!host) {
-                    // scalars are host-side in every mode; device scalars are not supported
-                }
-                switch (dtype) {
-                case H2G_F64: return {*(const double *)p, 0.0};
-                case H2G_F32: return {(double)*(const float *)p, 0.0};
-                case H2G_C64: return {((const double *)p)[0], ((const double *)p)[1]};
-                default: return {(double)((const float *)p)[0], (double)((const float *)p)[1]};
-                }
-            };
-            a = rd(alpha);
-            b = rd(beta);
-        }
         if (nrhs == 0) return;
         H2G_HIP(hipSetDevice(c.device));
         hipStream_t s = ws->stream;
         const size_t xe = elem_size(dtype);
+        const int64_t n_in = op == H2G_OP_N ? c.trial->n_dofs : c.test.n_dofs;
+        const int64_t n_out = op == H2G_OP_N ? c.test.n_dofs : c.trial->n_dofs;
         const void *dx = x;
         void *dy = y;
         int64_t lx = ldx, ly = ldy;
@@ -1441,67 +1561,9 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
             lx = n_in;
             ly = n_out;
         }
-        const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
-        const Side &src = *P.src, &dst = *P.dst;
-        const size_t se = elem_size(c.dtype);
-        const bool use_graph = ws->opts.use_graph && !ws->opts.timing;
-        const bool mixed_ws = ws->opts.precision == H2G_PREC_MIXED;
         ws->timings.clear();
         ws->n_events_used = 0;
-        for (int64_t c0 = 0; c0 < nrhs; c0 += ws->R_cap) {
-            const int R = (int)std::min<int64_t>(ws->R_cap, nrhs - c0);
-            ws->mark_begin(0, -1);
-            launch_gather(c.dtype, dtype, (const char *)dx + (size_t)c0 * lx * xe, lx, src.d_perm.as<int64_t>(),
-                          src.d_iperm.as<int64_t>(),
-                          src.n_dofs, R, ws->xs.ptr, s);
-            ws->mark_end();
-            auto body = [&]() {
-                launch_fill_zero(ws->ys.ptr, (size_t)dst.n_dofs * R * se, s);
-                if ((which & H2G_NEARMUL) && (which & H2G_FARMUL) && !ws->opts.timing) {
-                    // the near field (which sets its rows of the result) on the side stream,
-                    // concurrent with the far field up to its first write of the result
-                    H2G_HIP(hipEventRecord(ws->ev_fork, s));
-                    H2G_HIP(hipStreamWaitEvent(ws->side, ws->ev_fork, 0));
-                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, mixed_ws, ws->side);
-                    H2G_HIP(hipEventRecord(ws->ev_near, ws->side));
-                    far_pass(*ws, op, R, ws->ev_near);
-                    return;
-                }
-                if (which & H2G_NEARMUL) {
-                    ws->mark_begin(1, -1);
-                    launch_near(c.dtype, op, c, ws->xs.ptr, ws->ys.ptr, R, mixed_ws, s);
-                    ws->mark_end();
-                }
-                if (which & H2G_FARMUL) far_pass(*ws, op, R, nullptr);
-            };
-            if (use_graph) {
-                GraphKey key{op, which, R};
-                auto it = ws->graphs.find(key);
-                if (it == ws->graphs.end()) {
-                    hipGraph_t g;
-                    H2G_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    try {
-                        body();
-                    } catch (...) {
-                        (void)hipStreamEndCapture(s, &g);
-                        throw;
-                    }
-                    H2G_HIP(hipStreamEndCapture(s, &g));
-                    hipGraphExec_t ge;
-                    H2G_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-                    (void)hipGraphDestroy(g);
-                    it = ws->graphs.emplace(key, ge).first;
-                }
-                H2G_HIP(hipGraphLaunch(it->second, s));
-            } else {
-                body();
-            }
-            const uint8_t *mask = (which == H2G_FARMUL && five) ? P.out_mask.as<uint8_t>() : nullptr;
-            ws->mark_begin(9, -1);
-            launch_scatter(c.dtype, dtype, ws->ys.ptr, dst.d_perm.as<int64_t>(), dst.d_iperm.as<int64_t>(), dst.n_dofs, R,
-                           (char *)dy + (size_t)c0 * ly * xe, ly, a, five ? b : cplx<double>{0.0, 0.0}, mask, s);
-            ws->mark_end();
-        }
+        run_passes(*ws, op, which, dtype, nrhs, dx, lx, dy, ly, a, b, five, true, ws->opts.use_graph && !ws->opts.timing);
         if (host) {
             H2G_HIP(hipMemcpy2DAsync(y, ldy * xe, ws->yhost.ptr, n_out * xe, n_out * xe, nrhs, hipMemcpyDeviceToHost, s));
             if (!(flags & H2G_ASYNC) || ws->opts.timing) H2G_HIP(hipStreamSynchronize(s));
@@ -1514,5 +1576,68 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
     h2g_ws_release(ws);
     return rc;
 }
+
+int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dtype, int64_t nrhs, const void *x,
+                       int64_t ldx, void *y, int64_t ldy, const void *alpha, const void *beta, void *stream,
+                       int flags) {
+    if (!core) {
+        set_error("null core");
+        return H2G_EINVAL;
+    }
+    if (core->device < 0 || core->device >= NOALLOC_MAX_DEVICES) {
+        set_error("h2g_matvec_noalloc: device ordinal out of range");
+        return H2G_EINVAL;
+    }
+    std::atomic<bool> &busy = g_noalloc_busy[core->device];
+    bool expected = false;
+    if (!busy.compare_exchange_strong(expected, true)) {
+        set_error("h2g_matvec_noalloc: another call is using this GPU's scratch");
+        return H2G_EBUSY;
+    }
+    const int rc = guarded([&] {
+        Core &c = *core;
+        if (precision != H2G_PREC_PLAIN && precision != H2G_PREC_MIXED) fail(H2G_EINVAL, "unknown precision");
+        if (!(c.precisions & (1 << precision)))
+            fail(H2G_EINVAL, "the core was built without this precision (h2g_core_desc.precisions)");
+        if (flags & ~H2G_ASYNC) fail(H2G_EINVAL, "h2g_matvec_noalloc takes device pointers only (flags: 0 or H2G_ASYNC)");
+        cplx<double> a, b;
+        check_call(c, op, which, dtype, nrhs, x, ldx, y, ldy, alpha, beta, a, b);
+        if (nrhs == 0) return;
+        H2G_HIP(hipSetDevice(c.device));
+        const bool mixed = precision == H2G_PREC_MIXED;
+        // the widest column block whose buffers fit the scratch
+        int R = (int)std::min<int64_t>(nrhs, 1 << 16);
+        while (R > 1 && ws_layout(c, mixed, R).total(NOALLOC_ALIGN) > NOALLOC_BYTES) R = R > 8 ? R * 3 / 4 : R - 1;
+        const WsLayout L = ws_layout(c, mixed, R);
+        if (L.total(NOALLOC_ALIGN) > NOALLOC_BYTES)
+            fail(H2G_ENOMEM, "h2g_matvec_noalloc: one column of this map needs " + std::to_string(L.total(NOALLOC_ALIGN)) +
+                                 " bytes of scratch, more than the " + std::to_string(NOALLOC_BYTES) +
+                                 " bytes it holds; use h2g_matvec with a workspace");
+        unsigned char *base = static_cast<unsigned char *>(noalloc_scratch(c.device));
+        Workspace w;
+        w.core = &c;
+        w.opts.precision = precision;
+        w.opts.max_rhs = R;
+        w.R_cap = R;
+        w.stream = static_cast<hipStream_t>(stream);
+        w.side = w.stream;
+        size_t off = 0;
+        auto carve = [&](DevBuf &buf, size_t n) {
+            buf.borrow(base + off, n);
+            off += (n + NOALLOC_ALIGN - 1) / NOALLOC_ALIGN * NOALLOC_ALIGN;
+        };
+        carve(w.xs, L.xs);
+        carve(w.ys, L.ys);
+        carve(w.mom, L.mom);
+        carve(w.loc, L.loc);
+        carve(w.spec_in, L.spec_in);
+        carve(w.spec_out, L.spec_out);
+        run_passes(w, op, which, dtype, nrhs, x, ldx, y, ldy, a, b, alpha != nullptr, false, false);
+        if (!(flags & H2G_ASYNC)) H2G_HIP(hipStreamSynchronize(w.stream));
+    });
+    busy.store(false);
+    return rc;
+}
+
 
 }  // extern "C"

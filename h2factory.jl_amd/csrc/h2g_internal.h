@@ -138,6 +138,7 @@ inline size_t elem_size(int dtype) {
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool owned = true;  // false for a view (borrow) of memory the buffer does not free
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -157,10 +158,18 @@ struct DevBuf {
         if (n) H2G_HIP(hipMemcpy(ptr, host, n, hipMemcpyHostToDevice));
     }
     template <class T> void upload_vec(const std::vector<T> &v) { upload(v.data(), v.size() * sizeof(T)); }
+    // a view of n bytes at p (not freed by this buffer)
+    void borrow(void *p, size_t n) {
+        release();
+        ptr = n ? p : nullptr;
+        bytes = n;
+        owned = false;
+    }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && owned) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
+        owned = true;
     }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };

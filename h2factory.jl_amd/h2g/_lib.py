@@ -25,7 +25,7 @@ EXPORTED = [
     "h2g_core_create", "h2g_core_retain", "h2g_core_release", "h2g_core_size", "h2g_core_device_bytes",
     "h2g_ws_create", "h2g_ws_destroy", "h2g_ws_device_bytes", "h2g_ws_stream", "h2g_ws_acquire",
     "h2g_ws_release", "h2g_matvec", "h2g_ws_sync", "h2g_ws_timing", "h2g_last_error", "h2g_version",
-    "h2g_method_fill", "h2g_ws_query",
+    "h2g_method_fill", "h2g_ws_query", "h2g_matvec_noalloc",
 ]
 
 VP = C.c_void_p
@@ -107,6 +107,8 @@ def lib():
         L.h2g_ws_release.argtypes = [VP]
         L.h2g_matvec.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.c_int64, VP, C.c_int64, VP, C.c_int64,
                                  VP, VP, C.c_int]
+        L.h2g_matvec_noalloc.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, VP, C.c_int64, VP,
+                                         C.c_int64, VP, VP, VP, C.c_int]
         L.h2g_ws_sync.argtypes = [VP]
         L.h2g_ws_query.argtypes = [VP]
         L.h2g_ws_timing.argtypes = [VP, C.POINTER(TimingRecord), C.c_int32, I32P]

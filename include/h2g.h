@@ -200,6 +200,22 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs,
                const void *x, int64_t ldx, void *y, int64_t ldy,
                const void *alpha, const void *beta, int flags);
 
+/* The same products as h2g_matvec without allocating anything: no workspace, no hipMalloc or
+ * hipFree, no allocating library call.  It has no counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The buffers of the product (x and y in tree order, moments,
+ * spectra) are views into one static device array of 128 MiB per GPU, which the library holds
+ * until the process ends; wider x is processed in column blocks that fit, and a map of which even
+ * one column does not fit is refused with H2G_ENOMEM.  `precision` is H2G_PREC_PLAIN or
+ * H2G_PREC_MIXED (the core must support it); x and y are device pointers; the product runs on
+ * `stream` (a hipStream_t, NULL for the null stream), synchronously unless flags is H2G_ASYNC.
+ * Every call on one GPU shares that array: a call made while another is inside the library
+ * returns H2G_EBUSY, and calls whose device work overlaps (H2G_ASYNC on different streams) race,
+ * so order them on one stream. */
+int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dtype, int64_t nrhs,
+                       const void *x, int64_t ldx, void *y, int64_t ldy, const void *alpha, const void *beta,
+                       void *stream, int flags);
+
 /* Exclusive ownership of a workspace across several calls: acquireworkspace! /
  * releaseworkspace! (src/maps/workspace.jl:304-317).  h2g_matvec on a held workspace
  * returns H2G_EBUSY, like a nested withworkspace in the reference. */

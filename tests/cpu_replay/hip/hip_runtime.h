@@ -118,6 +118,11 @@ inline hipError_t hipMalloc(void **p, size_t n) {
     return *p ? hipSuccess : hipErrorMemoryAllocation;
 }
 inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
+    *p = const_cast<void *>(symbol);
+    return hipSuccess;
+}
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) {
     if (n) std::memcpy(d, s, n);
     return hipSuccess;

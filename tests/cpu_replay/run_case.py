@@ -90,6 +90,30 @@ def main(name):
         A = h2g.H2Workspace(c, max_rhs=3)
         x = np.random.default_rng(9).standard_normal((1200, 3))
         ok = _products(name, A, core, x, "NT", ("mul", "nearmul", "farmul"))
+    elif name == "noalloc":
+        # h2g_matvec_noalloc (buffers in the static scratch) against the workspace product and the
+        # oracle: two calls in a row, the second on a smaller map with fewer columns
+        import ctypes as C
+
+        ok = True
+        for n, nrhs, seed in ((1500, 3, 1), (700, 2, 2)):
+            pts = np.random.default_rng(seed).random((n, 3))
+            core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="dense", leaf_size=40)
+            c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
+            A = h2g.H2Workspace(c, max_rhs=nrhs)
+            for op, code in (("N", H.H2G_OP_N), ("T", H.H2G_OP_T)):
+                x = np.asfortranarray(np.random.default_rng(seed + 5).standard_normal((n, nrhs)))
+                y = np.asfortranarray(np.full((n, nrhs), np.nan))
+                rc = H.lib().h2g_matvec_noalloc(c.handle, H.H2G_PREC_PLAIN, code, H.H2G_MUL, H.H2G_F64, nrhs,
+                                                H.ptr(x), n, H.ptr(y), n, None, None, None, 0)
+                assert rc == 0, H.last_error()
+                e1 = relerr(y, orient(A, op) @ x)
+                e2 = relerr(y, O.mul(core, x, op).real)
+                print(f"{name} n={n} op={op}: vs workspace product {e1:.2e}, vs oracle {e2:.2e}", flush=True)
+                ok &= e1 < 1e-12 and e2 < 1e-12
+            A.close()
+        c.close()
+        sys.exit(0 if ok else 1)
     else:
         core, x = {"helmholtz_sphere": helmholtz_sphere, "random_petrov_symdense": random_petrov_symdense}[name]()
         c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
