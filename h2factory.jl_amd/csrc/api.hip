@@ -8,6 +8,7 @@
 #include <complex>
 #include <set>
 #include <unordered_map>
+#include <map>
 #include <mutex>
 
 #include "h2g_internal.h"
@@ -358,7 +359,8 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
                          const std::vector<uint8_t> &partner);
 
 void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, const int32_t *recv_of_pair,
-                  const int32_t *part_of_pair, const int32_t *pair_op, const int32_t *pair_sym, int64_t n_pairs) {
+                  const int32_t *part_of_pair, const int32_t *pair_op, const int32_t *pair_sym,
+                  const int32_t *pair_fop, int64_t n_pairs) {
     P.src = &src;
     P.dst = &dst;
     // M2L CSR over receiving (dst) boxes, partners sorted for a deterministic order
@@ -482,7 +484,7 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
 
     // M2L items
     std::vector<int64_t> iptr;
-    std::vector<int32_t> isrc, iop, isym;
+    std::vector<int32_t> isrc, iop, isym, ifop;
     P.m2l_base.assign(dst.n_levels, 0);
     P.min_level = dst.n_levels;
     P.max_level = -1;
@@ -498,6 +500,7 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
                 isrc.push_back(part_of_pair[k]);
                 iop.push_back(pair_op[k]);
                 if (pair_sym) isym.push_back(pair_sym[k]);
+                if (pair_fop) ifop.push_back(pair_fop[k]);
             }
         }
         iptr.push_back((int64_t)isrc.size());
@@ -514,6 +517,7 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
     P.m2l_src.upload_vec(isrc);
     P.m2l_op.upload_vec(iop);
     if (pair_sym) P.m2l_sym.upload_vec(isym);
+    if (pair_fop) P.m2l_fop.upload_vec(ifop);
     P.lattice = std::vector<LatticeLevel>(dst.n_levels);
     if (c.m2l_kind == H2G_M2L_TOEPLITZ || c.m2l_kind == H2G_M2L_SYMTOEPLITZ)
         for (int l = 0; l < dst.n_levels; ++l) {
@@ -1016,9 +1020,13 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
         const int64_t *iptr = P.m2l_ptr.as<int64_t>() + P.m2l_base[l];
         if (!is_toeplitz(c)) {
             w.mark_begin(5, l);
-            launch_boxgemm(cd, opA, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
-                           sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), c.perms_inv.as<int32_t>(),
-                           ops, K, w.mom.ptr, w.loc.ptr, R, 0, nullptr, s);
+            if (c.dense_ok)
+                launch_dense_m2l(c, mixed, op, recv, n, iptr, P.m2l_src.as<int32_t>(), P.m2l_fop.as<int32_t>(),
+                                 w.mom.ptr, w.loc.ptr, R, s);
+            else
+                launch_boxgemm(cd, opA, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
+                               sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(),
+                               c.perms_inv.as<int32_t>(), ops, K, w.mom.ptr, w.loc.ptr, R, 0, nullptr, s);
             w.mark_end();
         } else {
             // forward / adjoint: fft in, ifft out; transpose swaps the two plans
@@ -1378,8 +1386,26 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
                 c->twiddle.upload_vec(tw);
             }
             build_near(*c, d);
-            build_orient(*c, c->fwd, *c->trial, c->test, d.pair_test, d.pair_trial, d.pair_op, d.pair_sym, d.n_pairs);
-            build_orient(*c, c->rev, c->test, *c->trial, d.pair_trial, d.pair_test, d.pair_op, d.pair_sym, d.n_pairs);
+            // dense kinds: the distinct (translation id, symmetry) operators of the pairs
+            std::vector<int32_t> pair_fop, fop_op, fop_sym;
+            if (!toep && d.n_pairs > 0) {
+                std::map<std::pair<int32_t, int32_t>, int32_t> fops;
+                pair_fop.resize(d.n_pairs);
+                for (int64_t k = 0; k < d.n_pairs; ++k) {
+                    const auto key = std::make_pair(d.pair_op[k], d.pair_sym ? d.pair_sym[k] : -1);
+                    auto it = fops.find(key);
+                    if (it == fops.end()) {
+                        it = fops.emplace(key, (int32_t)fop_op.size()).first;
+                        fop_op.push_back(key.first);
+                        fop_sym.push_back(key.second);
+                    }
+                    pair_fop[k] = it->second;
+                }
+                build_dense_frags(*c, fop_op.data(), d.pair_sym ? fop_sym.data() : nullptr, (int64_t)fop_op.size());
+            }
+            const int32_t *pf = c->dense_ok ? pair_fop.data() : nullptr;
+            build_orient(*c, c->fwd, *c->trial, c->test, d.pair_test, d.pair_trial, d.pair_op, d.pair_sym, pf, d.n_pairs);
+            build_orient(*c, c->rev, c->test, *c->trial, d.pair_trial, d.pair_test, d.pair_op, d.pair_sym, pf, d.n_pairs);
         } catch (...) {
             delete c;
             throw;

@@ -314,8 +314,9 @@ struct OrientPlan {
     std::vector<Span> l2l_par;            // per level: span of the parent list in l2lp_box
     std::vector<int64_t> l2lp_base;       // per level: offset of the level's parents into l2lp_ptr
     DevBuf l2lp_box, l2lp_ptr, l2lp_child, l2lp_mat, l2lp_acc;
-    // M2L items: CSR over receiving boxes (concatenated by level)
-    DevBuf m2l_ptr, m2l_src, m2l_op, m2l_sym;
+    // M2L items: CSR over receiving boxes (concatenated by level); m2l_fop: the item's expanded
+    // (translation id, symmetry) operator of the dense MFMA kernel
+    DevBuf m2l_ptr, m2l_src, m2l_op, m2l_sym, m2l_fop;
     std::vector<int64_t> m2l_base;
     int min_level = 0, max_level = -1;
     int64_t max_src_level_boxes = 0, max_recv_level_boxes = 0;
@@ -351,6 +352,13 @@ struct Core {
     int precisions = 1;
     OrientPlan fwd, rev;
     DevBuf twiddle;  // c64 [L]: exp(-2 pi i n / L)
+    // dense / symmetry-reduced dense M2L on MFMA: every distinct (translation id, symmetry) of the
+    // pairs expanded into MFMA fragment order per orientation (0: N, 1: T, 2: C for complex maps),
+    // real form of Kr = K (real) or 2K (complex) rows in KrT tiles of 16
+    bool dense_ok = false;
+    int dense_Kr = 0, dense_KrT = 0;
+    int64_t n_fops = 0;
+    DevBuf dfrag64[3], dfrag32[3];
     bool half_spectrum = false;  // real map with Hermitian spectra: real-to-complex modulation
     int64_t device_bytes() const;
 };
@@ -442,6 +450,15 @@ void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t
                       bool half, bool want32, LatticeLevel &LL);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
                     int R, void *loc, const void *twiddle, hipStream_t s);
+// Dense M2L on MFMA (dense_m2l.hip): at most 4 * DENSE_MAX_RTW row tiles of 16 (Kr <= 448).
+constexpr int DENSE_MAX_RTW = 7;
+// Expands the distinct operators fop (translation id fop_op[e], symmetry fop_sym[e] or none) into
+// the fragment matrices of every orientation and precision of the core; sets c.dense_ok.
+void build_dense_frags(Core &c, const int32_t *fop_op, const int32_t *fop_sym, int64_t n_fops);
+// loc[t] = sum over the items of each receiving box t of A_fop(item) mom[src(item)] (overwrite)
+void launch_dense_m2l(const Core &c, bool mixed, int op, const int32_t *recv, int n_recv, const int64_t *item_ptr,
+                      const int32_t *item_src, const int32_t *item_fop, const void *mom, void *loc, int R,
+                      hipStream_t s);
 
 }  // namespace h2g
 

@@ -2,7 +2,8 @@
 //
 // The MFMA near-field and leaf kernels (leaf_mfma.hip) and the lattice Toeplitz kernels
 // (m2l_lattice.hip, m2l_mfma.hip) use gfx950 instructions and cannot run on the host.  These
-// stand-ins let the replay link: the MFMA launchers decline (as they do on the GPU for complex
+// stand-ins let the replay link: the MFMA launchers decline (the dense coupling kernel too, so
+// dense maps take the generic box GEMM here) (as they do on the GPU for complex
 // element types, so a complex map takes the same generic kernels there and here), the FP32 value
 // copy runs as a host loop, and the lattice entry points refuse -- the replay covers maps whose
 // couplings run on the generic kernels (dense, symmetry-reduced dense, and Toeplitz maps built
@@ -22,6 +23,13 @@ bool launch_l2p_mfma(int, const int32_t *, int, const int64_t *, const int64_t *
 }
 void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t) {
     for (int64_t i = 0; i < n; ++i) dst[i] = (float)src[i];
+}
+
+// the dense MFMA coupling kernel is not built: dense maps take the generic box GEMM
+void build_dense_frags(Core &c, const int32_t *, const int32_t *, int64_t) { c.dense_ok = false; }
+void launch_dense_m2l(const Core &, bool, int, const int32_t *, int, const int64_t *, const int32_t *, const int32_t *,
+                      const void *, void *, int, hipStream_t) {
+    throw Error{H2G_EUNSUPPORTED, "host replay: the dense MFMA coupling kernel runs on the GPU only"};
 }
 
 [[noreturn]] static void lattice_refused() {
