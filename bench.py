@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--p", type=int, default=6)
     ap.add_argument("--m2l", default="toeplitz")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "bf16x2", "bf16x3"],
+                    help="batched form: FP32 (mixed), or dense couplings on split BF16 (2 or 3 planes)")
     ap.add_argument("--plain-rhs", type=int, default=2, help="columns timed in the plain FP64 form")
     ap.add_argument("--cpu-n", type=int, default=0,
                     help="points of the CPU oracle sample (default N/8: the same cube one tree level coarser)")
@@ -153,7 +155,7 @@ def main():
     t_build = time.perf_counter() - t0
     log(f"map built on host in {t_build:.1f}s: {arrays.stats}")
     t0 = time.perf_counter()
-    core = h2g.GpuH2Core(arrays, precisions=("plain", "mixed"))
+    core = h2g.GpuH2Core(arrays, precisions=("plain", args.precision))
     torch.cuda.synchronize()
     t_core = time.perf_counter() - t0
     log(f"core on device in {t_core:.1f}s, {core.device_bytes() / 1e9:.1f} GB")
@@ -162,7 +164,7 @@ def main():
     X = torch.randn((R, N), generator=g, dtype=torch.float32).to(dev).t()  # column-major N x R
     Y = torch.zeros((R, N), dtype=torch.float32, device=dev).t()
 
-    A = h2g.H2Workspace(core, max_rhs=R, precision="mixed", use_graph=not args.no_graph)
+    A = h2g.H2Workspace(core, max_rhs=R, precision=args.precision, use_graph=not args.no_graph)
     log(f"batched workspace {A.device_bytes() / 1e9:.1f} GB")
     for i in range(args.warmup):
         h2g.mul_(Y, A, X)
@@ -209,7 +211,7 @@ def main():
     log(f"direct sum on 1000 targets: plain {err_direct_plain:.2e}, batched {err_direct_batched:.2e}")
 
     # per-phase device times (HIP events on the workspace stream), same product, eager
-    Tm = h2g.H2Workspace(core, max_rhs=R, precision="mixed", timing=True)
+    Tm = h2g.H2Workspace(core, max_rhs=R, precision=args.precision, timing=True)
     h2g.mul_(Y, Tm, X)
     recs = []
     for _ in range(2):
@@ -246,8 +248,13 @@ def main():
     else:
         achieved = work / (dms * 1e-3) / 1e12
         # the batched coupling stage runs on FP32 MFMA (FP32 operands, FP64 partial-sum accumulation):
-        # priced against the dense FP32 MFMA peak
+        # priced against the dense FP32 MFMA peak; the split-BF16 forms run 3 (hi + lo) or 6
+        # (hi + mid + lo) BF16 MFMA products per FP32 product, priced as that many BF16 products
         pk = PEAK["fp32"]
+        if args.precision != "mixed" and arrays.m2l_kind in ("dense", "symdense") and dph == "m2l_apply":
+            nprod = 3 if args.precision == "bf16x2" else 6
+            work *= nprod
+            pk = PEAK["bf16"]
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": pk, "unit": "TFLOP/s",
                 "frac": round(achieved / pk, 4), "traffic": traffic_of(f"{dph} level {dlv}", N, args.p, R)}
     roof["kernel"] = f"{dph} level {dlv}"
@@ -291,9 +298,10 @@ def main():
         "metric": "H2 matvec RHS/s, N=2M p=6 32 RHS, rel.err<=1e-6 vs FP64, vs plain; p99 latency",
         "value": round(value, 2), "unit": "RHS/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "mixed" else args.precision, "data": "synthetic",
         "config": {"workload": f"laplace equispaced p={args.p} {args.m2l} M2L, N={N} uniform cube, leaf<=64, "
-                               f"{R} RHS FP32 batched (FP32-stored near values, FP64 near arithmetic + final sum)",
+                               f"{R} RHS FP32 batched (FP32-stored near values, FP64 near arithmetic + final sum)"
+                               + ("" if args.precision == "mixed" else f", couplings on split BF16 ({args.precision})"),
                    "n": N, "nrhs": R, "p": args.p, "m2l": args.m2l, "leaf_size": 64},
         "plain_fp64_rhs_per_s": round(plain_rhs_per_s, 2),
         "speedup_vs_plain": round(value / plain_rhs_per_s, 2),

@@ -837,7 +837,7 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
     d_op.upload_vec(tq_op);
     if (sym) d_sym.upload_vec(tq_sym);
     build_lattice_ea(d_op.as<int32_t>(), sym ? d_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(), c.ops.ptr, c.L,
-                     c.half_spectrum, (c.precisions & (1 << H2G_PREC_MIXED)) != 0, LL);
+                     c.half_spectrum, c.reduced_precisions(), LL);
     LL.ok = true;
 }
 
@@ -906,7 +906,7 @@ void build_near(Core &c, const h2g_core_desc &d) {
         c.near_vals.upload(d.near_values, (size_t)d.n_near_values * elem_size(d.dtype));
     else
         c.near_vals.alloc((size_t)d.n_near_values * elem_size(d.dtype));
-    if (d.dtype == H2G_F64 && (c.precisions & (1 << H2G_PREC_MIXED)) && d.n_near_values > 0) {
+    if (d.dtype == H2G_F64 && c.reduced_precisions() && d.n_near_values > 0) {
         // the mixed form reads the near values rounded to FP32 (half the HBM traffic of the
         // near field) and still multiplies and accumulates in FP64
         c.near_vals32.alloc((size_t)d.n_near_values * sizeof(float));
@@ -969,7 +969,7 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     Core &c = *w.core;
     const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
     const Side &src = *P.src, &dst = *P.dst;
-    const bool mixed = w.opts.precision == H2G_PREC_MIXED;
+    const bool mixed = w.opts.precision != H2G_PREC_PLAIN;
     const int cd = mixed ? (c.dtype == H2G_F64 ? H2G_F32 : H2G_C32) : c.dtype;
     const bool conj = op == H2G_OP_C;
     const int32_t *L = P.lists.as<int32_t>();
@@ -1021,8 +1021,8 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
         if (!is_toeplitz(c)) {
             w.mark_begin(5, l);
             if (c.dense_ok)
-                launch_dense_m2l(c, mixed, op, recv, n, iptr, P.m2l_src.as<int32_t>(), P.m2l_fop.as<int32_t>(),
-                                 w.mom.ptr, w.loc.ptr, R, s);
+                launch_dense_m2l(c, w.opts.precision, op, recv, n, iptr, P.m2l_src.as<int32_t>(),
+                                 P.m2l_fop.as<int32_t>(), w.mom.ptr, w.loc.ptr, R, s);
             else
                 launch_boxgemm(cd, opA, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
                                sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(),
@@ -1141,7 +1141,7 @@ void run_passes(Workspace &w, int op, int which, int dtype, int64_t nrhs, const 
     const size_t xe = elem_size(dtype), se = elem_size(c.dtype);
     const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
     const Side &src = *P.src, &dst = *P.dst;
-    const bool mixed_ws = w.opts.precision == H2G_PREC_MIXED;
+    const bool mixed_ws = w.opts.precision != H2G_PREC_PLAIN;
     for (int64_t c0 = 0; c0 < nrhs; c0 += w.R_cap) {
         const int R = (int)std::min<int64_t>(w.R_cap, nrhs - c0);
         w.mark_begin(0, -1);
@@ -1333,7 +1333,7 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
             c->n_perms = d.n_perms;
             c->alpha = {d.alpha_re, d.alpha_im};
             c->precisions = d.precisions ? d.precisions : (1 << H2G_PREC_PLAIN);
-            const bool want32 = (c->precisions & (1 << H2G_PREC_MIXED)) != 0;
+            const bool want32 = c->reduced_precisions();
             copy_side(c->test, d.test, d.dtype, d.K, want32);
             if (c->galerkin) {
                 c->trial = &c->test;
@@ -1451,7 +1451,7 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
         o.max_rhs = 1;
         if (opts) o = *opts;
         if (o.max_rhs < 1) o.max_rhs = 1;
-        if (o.precision != H2G_PREC_PLAIN && o.precision != H2G_PREC_MIXED) fail(H2G_EINVAL, "unknown precision");
+        if (o.precision < H2G_PREC_PLAIN || o.precision > H2G_PREC_BF16X3) fail(H2G_EINVAL, "unknown precision");
         if (!(core->precisions & (1 << o.precision)))
             fail(H2G_EINVAL, "the core was built without this precision (h2g_core_desc.precisions)");
         H2G_HIP(hipSetDevice(core->device));
@@ -1464,7 +1464,7 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             H2G_HIP(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_near, hipEventDisableTiming));
-            const WsLayout L = ws_layout(*core, o.precision == H2G_PREC_MIXED, w->R_cap);
+            const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap);
             w->xs.alloc(L.xs);
             w->ys.alloc(L.ys);
             w->mom.alloc(L.mom);
@@ -1622,7 +1622,7 @@ int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dty
     }
     const int rc = guarded([&] {
         Core &c = *core;
-        if (precision != H2G_PREC_PLAIN && precision != H2G_PREC_MIXED) fail(H2G_EINVAL, "unknown precision");
+        if (precision < H2G_PREC_PLAIN || precision > H2G_PREC_BF16X3) fail(H2G_EINVAL, "unknown precision");
         if (!(c.precisions & (1 << precision)))
             fail(H2G_EINVAL, "the core was built without this precision (h2g_core_desc.precisions)");
         if (flags & ~H2G_ASYNC) fail(H2G_EINVAL, "h2g_matvec_noalloc takes device pointers only (flags: 0 or H2G_ASYNC)");
@@ -1630,7 +1630,7 @@ int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dty
         check_call(c, op, which, dtype, nrhs, x, ldx, y, ldy, alpha, beta, a, b);
         if (nrhs == 0) return;
         H2G_HIP(hipSetDevice(c.device));
-        const bool mixed = precision == H2G_PREC_MIXED;
+        const bool mixed = precision != H2G_PREC_PLAIN;
         // the widest column block whose buffers fit the scratch
         int R = (int)std::min<int64_t>(nrhs, 1 << 16);
         while (R > 1 && ws_layout(c, mixed, R).total(NOALLOC_ALIGN) > NOALLOC_BYTES) R = R > 8 ? R * 3 / 4 : R - 1;

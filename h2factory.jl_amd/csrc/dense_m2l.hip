@@ -27,7 +27,9 @@
 //   * FP64 (plain form): v_mfma_f64_16x16x4f64, FP64 accumulators;
 //   * FP32 (mixed form): v_mfma_f32_16x16x4f32 on FP32 operands; every 4 k groups (64 products
 //     per accumulator) the FP32 partial sums are added into FP64 accumulators, so the error stays
-//     at the FP32 rounding of the operands instead of growing with the ~190 partners x K terms.
+//     at the FP32 rounding of the operands instead of growing with the ~190 partners x K terms;
+//   * split BF16 (precisions BF16X2 / BF16X3): the operands as 2 or 3 BF16 planes on
+//     v_mfma_f32_16x16x32_bf16 (3 or 6 plane products), FP64 accumulation as for FP32.
 #include <algorithm>
 #include <cstdint>
 #include <map>
@@ -44,8 +46,27 @@ typedef float float4_d __attribute__((ext_vector_type(4)));
 
 constexpr int DNT = 256;  // threads per workgroup (4 waves)
 
-// Fragment matrix of expanded operator e for orientation `orient` (0: A, 1: A^T, 2: A^H), from the
-// canonical couplings `ops` (core type, column-major K x K, doubles; complex interleaved).
+// Element (i, j) of the real-form operator e for orientation `orient` (0: A, 1: A^T, 2: A^H), from
+// the canonical couplings `ops` (core type, column-major K x K, doubles; complex interleaved):
+// A_e[p][q] = C[pi^-1(p)][pi^-1(q)] for the symmetry of e (none: the identity).
+template <bool CPLX>
+__device__ inline double dense_value(const double *__restrict__ ops, int K, int Kr, int64_t e, int i, int j,
+                                     const int32_t *__restrict__ fop_op, const int32_t *__restrict__ fop_sym,
+                                     const int32_t *__restrict__ perms_inv, int orient) {
+    if (i >= Kr || j >= Kr) return 0.0;
+    const int ii = CPLX ? i >> 1 : i, jj = CPLX ? j >> 1 : j;
+    const int p = orient == 0 ? ii : jj, q = orient == 0 ? jj : ii;
+    const int sym = fop_sym ? fop_sym[e] : -1;
+    const int pp = sym >= 0 ? perms_inv[(int64_t)sym * K + p] : p;
+    const int qq = sym >= 0 ? perms_inv[(int64_t)sym * K + q] : q;
+    const int64_t at = (int64_t)fop_op[e] * K * K + pp + (int64_t)K * qq;
+    if (!CPLX) return ops[at];
+    const double re = ops[2 * at], im = orient == 2 ? -ops[2 * at + 1] : ops[2 * at + 1];
+    const int a = i & 1, b = j & 1;
+    return a == b ? re : (a == 0 ? -im : im);
+}
+
+// Fragment matrices for the 16 x 16 x 4 MFMAs (FP64 / FP32): [e][rt][g][lane][kk].
 template <class OUT, bool CPLX>
 __global__ void dense_frag_kernel(const double *__restrict__ ops, int K, int Kr, int KrT, int64_t n_fops,
                                   const int32_t *__restrict__ fop_op, const int32_t *__restrict__ fop_sym,
@@ -59,23 +80,36 @@ __global__ void dense_frag_kernel(const double *__restrict__ ops, int K, int Kr,
         const int kk = rem & 3, lane = (rem >> 2) & 63, tg = rem >> 8;
         const int rt = tg / KrT, g = tg - rt * KrT;
         const int i = 16 * rt + (lane & 15), j = 16 * g + 4 * (lane >> 4) + kk;
-        double v = 0.0;
-        if (i < Kr && j < Kr) {
-            const int ii = CPLX ? i >> 1 : i, jj = CPLX ? j >> 1 : j;
-            const int p = orient == 0 ? ii : jj, q = orient == 0 ? jj : ii;
-            const int sym = fop_sym ? fop_sym[e] : -1;
-            const int pp = sym >= 0 ? perms_inv[(int64_t)sym * K + p] : p;
-            const int qq = sym >= 0 ? perms_inv[(int64_t)sym * K + q] : q;
-            const int64_t at = (int64_t)fop_op[e] * K * K + pp + (int64_t)K * qq;
-            if (CPLX) {
-                const double re = ops[2 * at], im = orient == 2 ? -ops[2 * at + 1] : ops[2 * at + 1];
-                const int a = i & 1, b = j & 1;
-                v = a == b ? re : (a == 0 ? -im : im);
-            } else {
-                v = ops[at];
-            }
-        }
-        frag[idx] = (OUT)v;
+        frag[idx] = (OUT)dense_value<CPLX>(ops, K, Kr, e, i, j, fop_op, fop_sym, perms_inv, orient);
+    }
+}
+
+// Fragment matrices for the 16 x 16 x 32 BF16 MFMAs: three planes (hi, mid, lo) of
+// [e][rt][g][lane][8] with G2 = ceil(KrT / 2) groups of 32 reductions; lane (m + 16 k'') holds
+// A[16 rt + m][32 g + 8 k'' + u], u = 0..7.  The FP32 value x is split as hi = bf16(x),
+// mid = bf16(x - hi), lo = bf16(x - hi - mid) (each difference exact in FP32).
+template <bool CPLX>
+__global__ void dense_frag_bf16_kernel(const double *__restrict__ ops, int K, int Kr, int KrT, int64_t n_fops,
+                                       const int32_t *__restrict__ fop_op, const int32_t *__restrict__ fop_sym,
+                                       const int32_t *__restrict__ perms_inv, int orient, __bf16 *__restrict__ frag) {
+    const int G2 = (KrT + 1) / 2;
+    const int64_t per = (int64_t)KrT * G2 * 512;
+    const int64_t total = n_fops * per;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = idx / per;
+        const int rem = (int)(idx - e * per);
+        const int u = rem & 7, lane = (rem >> 3) & 63, tg = rem >> 9;
+        const int rt = tg / G2, g = tg - rt * G2;
+        const int i = 16 * rt + (lane & 15), j = 32 * g + 8 * (lane >> 4) + u;
+        const float x = (float)dense_value<CPLX>(ops, K, Kr, e, i, j, fop_op, fop_sym, perms_inv, orient);
+        const __bf16 hi = (__bf16)x;
+        const float r1 = x - (float)hi;
+        const __bf16 mid = (__bf16)r1;
+        const __bf16 lo = (__bf16)(r1 - (float)mid);
+        frag[idx] = hi;
+        frag[total + idx] = mid;
+        frag[2 * total + idx] = lo;
     }
 }
 
@@ -200,6 +234,158 @@ __global__ void __launch_bounds__(DNT, 2)
     }
 }
 
+typedef __bf16 bf16x8_d __attribute__((ext_vector_type(8)));
+
+// 8 consecutive k of an FP32 column (zero beyond Kr), split into NPL BF16 planes
+template <int NPL>
+__device__ inline void load_split8(const float *__restrict__ col, int k0, int Kr, bool valid, bf16x8_d (&out)[NPL]) {
+    float x[8];
+    if (valid && (Kr & 3) == 0 && k0 + 8 <= Kr) {
+        const float4_d v0 = *reinterpret_cast<const float4_d *>(col + k0);
+        const float4_d v1 = *reinterpret_cast<const float4_d *>(col + k0 + 4);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            x[u] = v0[u];
+            x[u + 4] = v1[u];
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = valid && k0 + u < Kr ? col[k0 + u] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        float r = x[u];
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl) {
+            const __bf16 h = (__bf16)r;
+            out[pl][u] = h;
+            r -= (float)h;
+        }
+    }
+}
+
+// Split BF16 form of the mixed precision: A and B split into NPL BF16 planes; the products of
+// plane pairs (pa, pb) with pa + pb < NPL (3 for hi + lo, 6 for hi + mid + lo) on
+// v_mfma_f32_16x16x32_bf16 into FP32 partial sums, added into FP64 every 2 groups (64 reductions).
+template <int NPL, int RTW, int NCT>
+__global__ void __launch_bounds__(DNT, 2)
+    dense_m2l_bf16_kernel(const int32_t *__restrict__ recv, const int64_t *__restrict__ item_ptr,
+                          const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_fop,
+                          const bf16x8_d *__restrict__ frag, int64_t plane, int Kr, int KrT,
+                          const float *__restrict__ mom, float *__restrict__ loc, int R) {
+    const int ti = blockIdx.x;
+    const int t = recv[ti];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = lane & 15, kq = lane >> 4;
+    const int r0 = blockIdx.y * 16 * NCT;
+    const int G2 = (KrT + 1) / 2;
+    const int ntw = (KrT - wave + 3) >> 2;
+    const int64_t fsz = (int64_t)KrT * G2 * 64;  // bf16x8 per operator and plane
+    double acc[RTW][NCT][4];
+    float4_d part[RTW][NCT];
+#pragma unroll
+    for (int j = 0; j < RTW; ++j)
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            part[j][c] = float4_d{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[j][c][v] = 0.0;
+        }
+    const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
+    for (int64_t it = i0; it < i1; ++it) {
+        const bf16x8_d *A0 = frag + (int64_t)item_fop[it] * fsz + lane;
+        const int64_t s = item_src[it];
+        const float *Bc[NCT];
+        bool bval[NCT];
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const int col = r0 + 16 * c + n;
+            bval[c] = col < R;
+            Bc[c] = mom + (s * R + (bval[c] ? col : 0)) * (int64_t)Kr;
+        }
+        for (int g = 0; g < G2; ++g) {
+            bf16x8_d a[RTW][NPL], b[NCT][NPL];
+#pragma unroll
+            for (int j = 0; j < RTW; ++j)
+#pragma unroll
+                for (int pl = 0; pl < NPL; ++pl)
+                    if (j < ntw) a[j][pl] = A0[pl * plane + ((int64_t)(wave + 4 * j) * G2 + g) * 64];
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) load_split8<NPL>(Bc[c], 32 * g + 8 * kq, Kr, bval[c], b[c]);
+#pragma unroll
+            for (int j = 0; j < RTW; ++j) {
+                if (j >= ntw) continue;
+#pragma unroll
+                for (int c = 0; c < NCT; ++c) {
+                    float4_d d = part[j][c];
+                    // the small products first
+#pragma unroll
+                    for (int ord = NPL - 1; ord >= 0; --ord)
+#pragma unroll
+                        for (int pa = 0; pa <= ord; ++pa)
+                            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j][pa], b[c][ord - pa], d, 0, 0, 0);
+                    part[j][c] = d;
+                }
+            }
+            if ((g & 1) == 1 || g == G2 - 1) {
+#pragma unroll
+                for (int j = 0; j < RTW; ++j)
+#pragma unroll
+                    for (int c = 0; c < NCT; ++c) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[j][c][v] += (double)part[j][c][v];
+                        part[j][c] = float4_d{0.f, 0.f, 0.f, 0.f};
+                    }
+            }
+        }
+    }
+    // D layout of the 16 x 16 f32 MFMAs: D[m][n] in lane n + 16 (m / 4), element m % 4
+#pragma unroll
+    for (int j = 0; j < RTW; ++j) {
+        if (j >= ntw) continue;
+        const int rt = wave + 4 * j;
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const int col = r0 + 16 * c + n;
+            if (col >= R) continue;
+            float *o = loc + ((int64_t)t * R + col) * Kr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * rt + 4 * kq + v;
+                if (row < Kr) o[row] = (float)acc[j][c][v];
+            }
+        }
+    }
+}
+
+template <int NPL, int RTW>
+void dense_bf16_go_nct(int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
+                       const int32_t *item_fop, const void *frag, int64_t plane, int Kr, int KrT, const void *mom,
+                       void *loc, int R, hipStream_t s) {
+    const dim3 grid((unsigned)n_recv, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
+    if (nct == 1)
+        hipLaunchKernelGGL((dense_m2l_bf16_kernel<NPL, RTW, 1>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
+                           (const bf16x8_d *)frag, plane, Kr, KrT, (const float *)mom, (float *)loc, R);
+    else
+        hipLaunchKernelGGL((dense_m2l_bf16_kernel<NPL, RTW, 2>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
+                           (const bf16x8_d *)frag, plane, Kr, KrT, (const float *)mom, (float *)loc, R);
+}
+
+template <int NPL>
+void dense_bf16_go(int rtw, int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr,
+                   const int32_t *item_src, const int32_t *item_fop, const void *frag, int64_t plane, int Kr, int KrT,
+                   const void *mom, void *loc, int R, hipStream_t s) {
+    switch (rtw) {
+    case 1: dense_bf16_go_nct<NPL, 1>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    case 2: dense_bf16_go_nct<NPL, 2>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    case 3: dense_bf16_go_nct<NPL, 3>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    case 4: dense_bf16_go_nct<NPL, 4>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    case 5: dense_bf16_go_nct<NPL, 5>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    case 6: dense_bf16_go_nct<NPL, 6>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    default: dense_bf16_go_nct<NPL, 7>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    }
+}
+
 template <int MODE, int RTW>
 void dense_go_nct(int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
                   const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
@@ -263,24 +449,48 @@ void build_dense_frags(Core &c, const int32_t *fop_op, const int32_t *fop_sym, i
         };
         if (c.precisions & (1 << H2G_PREC_PLAIN)) build(0.0, c.dfrag64[o]);
         if (c.precisions & (1 << H2G_PREC_MIXED)) build(0.0f, c.dfrag32[o]);
+        if (c.precisions & ((1 << H2G_PREC_BF16X2) | (1 << H2G_PREC_BF16X3))) {
+            const int64_t per_bf = (int64_t)KrT * ((KrT + 1) / 2) * 512;
+            const int64_t tb = n_fops * per_bf;
+            c.dfragbf[o].alloc(3 * (size_t)tb * sizeof(__bf16));
+            const unsigned bb = (unsigned)std::min<int64_t>((tb + 255) / 256, 1 << 20);
+            if (cplx)
+                hipLaunchKernelGGL((dense_frag_bf16_kernel<true>), bb, 256, 0, 0, c.ops.as<double>(), c.K, Kr, KrT,
+                                   n_fops, d_op.as<int32_t>(), fop_sym ? d_sym.as<int32_t>() : nullptr,
+                                   c.perms_inv.as<int32_t>(), o, c.dfragbf[o].as<__bf16>());
+            else
+                hipLaunchKernelGGL((dense_frag_bf16_kernel<false>), bb, 256, 0, 0, c.ops.as<double>(), c.K, Kr, KrT,
+                                   n_fops, d_op.as<int32_t>(), fop_sym ? d_sym.as<int32_t>() : nullptr,
+                                   c.perms_inv.as<int32_t>(), o, c.dfragbf[o].as<__bf16>());
+            H2G_CHECK_LAUNCH();
+        }
     }
     H2G_HIP(hipDeviceSynchronize());
     c.dense_ok = true;
 }
 
-void launch_dense_m2l(const Core &c, bool mixed, int op, const int32_t *recv, int n_recv, const int64_t *item_ptr,
+void launch_dense_m2l(const Core &c, int precision, int op, const int32_t *recv, int n_recv, const int64_t *item_ptr,
                       const int32_t *item_src, const int32_t *item_fop, const void *mom, void *loc, int R,
                       hipStream_t s) {
     if (n_recv == 0) return;
     const int o = op == H2G_OP_N ? 0 : (op == H2G_OP_T || c.dtype != H2G_C64 ? 1 : 2);
     const int rtw = (c.dense_KrT + 3) / 4;
     const int nct = R > 16 ? 2 : 1;
-    if (mixed)
+    if (precision == H2G_PREC_BF16X2 || precision == H2G_PREC_BF16X3) {
+        const int64_t plane = c.n_fops * (int64_t)c.dense_KrT * ((c.dense_KrT + 1) / 2) * 64;  // in bf16x8
+        if (precision == H2G_PREC_BF16X2)
+            dense_bf16_go<2>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfragbf[o].ptr, plane, c.dense_Kr,
+                             c.dense_KrT, mom, loc, R, s);
+        else
+            dense_bf16_go<3>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfragbf[o].ptr, plane, c.dense_Kr,
+                             c.dense_KrT, mom, loc, R, s);
+    } else if (precision == H2G_PREC_MIXED) {
         dense_go<1>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfrag32[o].ptr, c.dense_Kr, c.dense_KrT,
                     mom, loc, R, s);
-    else
+    } else {
         dense_go<0>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfrag64[o].ptr, c.dense_Kr, c.dense_KrT,
                     mom, loc, R, s);
+    }
     H2G_CHECK_LAUNCH();
 }
 

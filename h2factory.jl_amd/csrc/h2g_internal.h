@@ -350,6 +350,8 @@ struct Core {
     NearPlan near_fwd_plan, near_rev_plan;
     cplx<double> alpha{1.0, 0.0};
     int precisions = 1;
+    // a workspace precision other than PLAIN is supported (FP32 copies of the operators)
+    bool reduced_precisions() const { return (precisions & ~(1 << H2G_PREC_PLAIN)) != 0; }
     OrientPlan fwd, rev;
     DevBuf twiddle;  // c64 [L]: exp(-2 pi i n / L)
     // dense / symmetry-reduced dense M2L on MFMA: every distinct (translation id, symmetry) of the
@@ -359,6 +361,7 @@ struct Core {
     int dense_Kr = 0, dense_KrT = 0;
     int64_t n_fops = 0;
     DevBuf dfrag64[3], dfrag32[3];
+    DevBuf dfragbf[3];  // BF16 planes hi, mid, lo of the 16 x 16 x 32 fragment order (split precisions)
     bool half_spectrum = false;  // real map with Hermitian spectra: real-to-complex modulation
     int64_t device_bytes() const;
 };
@@ -456,7 +459,7 @@ constexpr int DENSE_MAX_RTW = 7;
 // the fragment matrices of every orientation and precision of the core; sets c.dense_ok.
 void build_dense_frags(Core &c, const int32_t *fop_op, const int32_t *fop_sym, int64_t n_fops);
 // loc[t] = sum over the items of each receiving box t of A_fop(item) mom[src(item)] (overwrite)
-void launch_dense_m2l(const Core &c, bool mixed, int op, const int32_t *recv, int n_recv, const int64_t *item_ptr,
+void launch_dense_m2l(const Core &c, int precision, int op, const int32_t *recv, int n_recv, const int64_t *item_ptr,
                       const int32_t *item_src, const int32_t *item_fop, const void *mom, void *loc, int R,
                       hipStream_t s);
 

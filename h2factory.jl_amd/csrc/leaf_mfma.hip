@@ -308,6 +308,22 @@ __global__ void mfma_f32_probe_kernel(const float *__restrict__ A, const float *
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
     for (int v = 0; v < 4; ++v) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = acc[v];
 }
+// probe of the BF16 16x16x32 layout the split-BF16 dense M2L kernel uses: lane l holds
+// A[l % 16][8 (l / 16) + u] and B[8 (l / 16) + u][l % 16], u = 0..7; D[m][n] in lane
+// n + 16 (m / 4), element m % 4
+__global__ void mfma_bf16_probe_kernel(const float *__restrict__ A, const float *__restrict__ B, float *__restrict__ D) {
+    typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+    typedef float float4_t __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    bf16x8 a, b;
+    for (int u = 0; u < 8; ++u) {
+        a[u] = (__bf16)A[(lane & 15) * 32 + 8 * (lane >> 4) + u];
+        b[u] = (__bf16)B[(8 * (lane >> 4) + u) * 16 + (lane & 15)];
+    }
+    float4_t acc = float4_t{0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    for (int v = 0; v < 4; ++v) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = acc[v];
+}
 }  // namespace
 
 bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
@@ -426,6 +442,24 @@ extern "C" int h2g_probe_mfma_f32(const float *A, const float *B, float *D) {
         b.upload(B, 64 * sizeof(float));
         d.alloc(256 * sizeof(float));
         mfma_f32_probe_kernel<<<1, 64>>>(a.as<float>(), b.as<float>(), d.as<float>());
+        H2G_HIP(hipGetLastError());
+        H2G_HIP(hipMemcpy(D, d.ptr, 256 * sizeof(float), hipMemcpyDeviceToHost));
+        return H2G_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    }
+}
+
+// Test hook: the BF16 layout probe (A 16x32, B 32x16 row-major; D 16x16).
+extern "C" int h2g_probe_mfma_bf16(const float *A, const float *B, float *D) {
+    using namespace h2g;
+    try {
+        DevBuf a, b, d;
+        a.upload(A, 512 * sizeof(float));
+        b.upload(B, 512 * sizeof(float));
+        d.alloc(256 * sizeof(float));
+        hipLaunchKernelGGL((mfma_bf16_probe_kernel), 1, 64, 0, 0, a.as<float>(), b.as<float>(), d.as<float>());
         H2G_HIP(hipGetLastError());
         H2G_HIP(hipMemcpy(D, d.ptr, 256 * sizeof(float), hipMemcpyDeviceToHost));
         return H2G_OK;

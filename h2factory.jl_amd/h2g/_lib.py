@@ -18,7 +18,8 @@ H2G_F64, H2G_C64, H2G_F32, H2G_C32 = range(4)
 H2G_OP_N, H2G_OP_T, H2G_OP_C = range(3)
 H2G_NEARMUL, H2G_FARMUL, H2G_MUL = 1, 2, 3
 H2G_M2L_DENSE, H2G_M2L_TOEPLITZ, H2G_M2L_SYMDENSE, H2G_M2L_SYMTOEPLITZ = range(4)
-H2G_PREC_PLAIN, H2G_PREC_MIXED = 0, 1
+H2G_PREC_PLAIN, H2G_PREC_MIXED, H2G_PREC_BF16X2, H2G_PREC_BF16X3 = 0, 1, 2, 3
+PRECISIONS = {"plain": H2G_PREC_PLAIN, "mixed": H2G_PREC_MIXED, "bf16x2": H2G_PREC_BF16X2, "bf16x3": H2G_PREC_BF16X3}
 H2G_HOST_PTRS, H2G_ASYNC = 1, 2
 
 EXPORTED = [

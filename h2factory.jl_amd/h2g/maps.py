@@ -102,7 +102,7 @@ class GpuH2Core:
         d.near_values = H.ptr(nv) if nv is not None else None
         a = complex(arrays.alpha)
         d.alpha_re, d.alpha_im = a.real, a.imag
-        d.precisions = sum(1 << {"plain": 0, "mixed": 1}[p] for p in precisions)
+        d.precisions = sum(1 << H.PRECISIONS[p] for p in set(precisions))
         d.device = device
         self.desc = d
         self.dtype = arrays.dtype
@@ -211,7 +211,7 @@ class H2Workspace:
     def __init__(self, core: GpuH2Core, max_rhs=1, precision="plain", use_graph=False, timing=False):
         self.core = core
         o = H.WsOpts()
-        o.precision = {"plain": 0, "mixed": 1}[precision]
+        o.precision = H.PRECISIONS[precision]
         o.max_rhs = int(max_rhs)
         o.use_graph = int(bool(use_graph))
         o.timing = int(bool(timing))

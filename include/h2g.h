@@ -66,8 +66,13 @@ enum { H2G_M2L_DENSE = 0, H2G_M2L_TOEPLITZ = 1, H2G_M2L_SYMDENSE = 2, H2G_M2L_SY
  *   MIXED  P2M, M2M, M2L, L2L, L2P in FP32 (ComplexF32); the near field of real cores reads an
  *          FP32 copy of the near values (made at core creation, or by h2g_method_fill) and
  *          multiplies and accumulates in FP64; the near+far sum and all precomputed spectra stay
- *          FP64. */
-enum { H2G_PREC_PLAIN = 0, H2G_PREC_MIXED = 1 };
+ *          FP64.
+ *   BF16X2 / BF16X3  MIXED, with the dense (and symmetry-reduced dense) couplings on BF16 MFMA:
+ *          each FP32 operand is split into BF16 planes, hi + lo (X2: products hi.hi + hi.lo +
+ *          lo.hi) or hi + mid + lo (X3: the six products of total order <= 2), accumulated in
+ *          FP32 by the MFMA and added into FP64 every 64 reductions.  Toeplitz couplings run as in
+ *          MIXED. */
+enum { H2G_PREC_PLAIN = 0, H2G_PREC_MIXED = 1, H2G_PREC_BF16X2 = 2, H2G_PREC_BF16X3 = 3 };
 
 /* Flags of h2g_matvec. */
 enum {

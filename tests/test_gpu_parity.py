@@ -421,3 +421,55 @@ def test_helmholtz_device_near_field(gpu):
             y = np.zeros((3000, nrhs), dtype=np.complex128)
             h2g.nearmul_(y, orient(A, op), x)
             assert relerr(y, O.nearmul(core, x, op)) < RTOL, (op, nrhs)
+
+
+def test_mfma_bf16_lane_layout(gpu):
+    """The lane layout the split-BF16 dense M2L kernel assumes for v_mfma_f32_16x16x32_bf16 (lane
+    l holds A[l % 16][8 (l / 16) + u], B[8 (l / 16) + u][l % 16]; D[m][n] in lane n + 16 (m / 4),
+    element m % 4); small integers, exact in BF16 and in the FP32 sums."""
+    import ctypes as C
+
+    from h2g import _lib as H
+
+    rng = np.random.default_rng(2)
+    A = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    B = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+    D = np.zeros((16, 16), dtype=np.float32)
+    lib = H.lib()
+    lib.h2g_probe_mfma_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    assert lib.h2g_probe_mfma_bf16(H.ptr(A), H.ptr(B), H.ptr(D)) == 0
+    assert np.array_equal(D, A @ B)
+
+
+@pytest.mark.parametrize("case", ["laplace", "helmholtz"])
+def test_split_bf16_dense_m2l(gpu, case):
+    """Dense couplings on split-BF16 MFMA against the plain FP64 form: hi + mid + lo (six plane
+    products) within the 1e-6 gate; hi + lo (three products) within its error budget of 1e-4
+    (each operand kept to about 2^-17 relative, the dropped products about 2^-16)."""
+    import h2g
+    from convert import to_arrays
+
+    rng = np.random.default_rng(41)
+    if case == "laplace":
+        pts = rng.random((6000, 3))
+        core = OB.build_core(pts, p=5, nodekind="cheb", m2l_kind="dense", leaf_size=64)
+        x = rng.standard_normal((6000, 20))
+    else:
+        pts = rng.standard_normal((5000, 3))
+        pts = 0.5 + 0.45 * pts / np.linalg.norm(pts, axis=1, keepdims=True)
+        core = OB.build_core(pts, p=4, nodekind="cheb", m2l_kind="dense",
+                             kernel="helmholtz", wavenumber=10.0, leaf_size=48)
+        x = rng.standard_normal((5000, 20)) + 1j * rng.standard_normal((5000, 20))
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed", "bf16x2", "bf16x3"))
+    plain = h2g.H2Workspace(c, max_rhs=20)
+    errs = {}
+    for prec, tol in (("mixed", 1e-6), ("bf16x3", 1e-6), ("bf16x2", 1e-4)):
+        ws = h2g.H2Workspace(c, max_rhs=20, precision=prec)
+        for op in ("NTC" if case == "helmholtz" else "NT"):
+            xx = x.astype(np.complex64 if case == "helmholtz" else np.float32)
+            y = orient(ws, op) @ xx
+            ref = orient(plain, op) @ x.astype(xx.dtype).astype(np.complex128 if case == "helmholtz" else np.float64)
+            errs[(prec, op)] = relerr(y, ref)
+            assert errs[(prec, op)] < tol, (prec, op, errs[(prec, op)])
+        ws.close()
+    print("split-BF16 errors vs plain:", {k: f"{v:.2e}" for k, v in errs.items()})
