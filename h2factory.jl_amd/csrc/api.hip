@@ -963,6 +963,23 @@ void build_near(Core &c, const h2g_core_desc &d) {
 // ------------------------------------------------------------------------------------------
 bool is_toeplitz(const Core &c) { return c.m2l_kind == H2G_M2L_TOEPLITZ || c.m2l_kind == H2G_M2L_SYMTOEPLITZ; }
 
+// representations of dense operators a core needs for its precisions (split BF16 only for couplings)
+int dense_reps(int precisions, bool couplings) {
+    int r = 0;
+    if (precisions & (1 << H2G_PREC_PLAIN)) r |= DENSE_REP_F64;
+    if (precisions & ~(1 << H2G_PREC_PLAIN)) r |= DENSE_REP_F32;
+    if (couplings && (precisions & ((1 << H2G_PREC_BF16X2) | (1 << H2G_PREC_BF16X3)))) r |= DENSE_REP_BF16;
+    return r;
+}
+
+// representation of a workspace precision for the dense kernel
+int dense_rep(int precision, bool couplings) {
+    if (precision == H2G_PREC_PLAIN) return DENSE_REP_F64;
+    if (couplings && precision == H2G_PREC_BF16X2) return DENSE_REP_BF16X2;
+    if (couplings && precision == H2G_PREC_BF16X3) return DENSE_REP_BF16X3;
+    return DENSE_REP_F32;
+}
+
 // join: an event the first write of the result (L2P) waits for (the near field on the side
 // stream), or null
 void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
@@ -972,6 +989,7 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     const bool mixed = w.opts.precision != H2G_PREC_PLAIN;
     const int cd = mixed ? (c.dtype == H2G_F64 ? H2G_F32 : H2G_C32) : c.dtype;
     const bool conj = op == H2G_OP_C;
+    const bool cmap = c.dtype == H2G_C64;
     const int32_t *L = P.lists.as<int32_t>();
     hipStream_t s = w.stream;
     const void *scoll = mixed ? src.coll32.ptr : src.coll.ptr;
@@ -1002,6 +1020,10 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
                                     P.m2m_mat.as<int32_t>(), mixed ? src.factors32.ptr : src.factors.ptr, (l + 1) * 8,
                                     src.sep_p,
                                     w.mom.ptr, w.mom.ptr, R, 0, nullptr, s);
+            else if (src.tr_dense.ok)
+                launch_dense(src.tr_dense, conj && cmap ? 3 : 0, dense_rep(w.opts.precision, false), L + P.m2m[l].begin,
+                             P.m2m[l].count, P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
+                             P.m2m_mat.as<int32_t>(), w.mom.ptr, w.mom.ptr, R, nullptr, s);
             else
                 launch_boxgemm(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
                                P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
@@ -1020,9 +1042,10 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
         const int64_t *iptr = P.m2l_ptr.as<int64_t>() + P.m2l_base[l];
         if (!is_toeplitz(c)) {
             w.mark_begin(5, l);
-            if (c.dense_ok)
-                launch_dense_m2l(c, w.opts.precision, op, recv, n, iptr, P.m2l_src.as<int32_t>(),
-                                 P.m2l_fop.as<int32_t>(), w.mom.ptr, w.loc.ptr, R, s);
+            if (c.m2l_dense.ok)
+                launch_dense(c.m2l_dense, op == H2G_OP_N ? 0 : (op == H2G_OP_T || !cmap ? 1 : 2),
+                             dense_rep(w.opts.precision, true), recv, n, iptr, P.m2l_src.as<int32_t>(),
+                             P.m2l_fop.as<int32_t>(), w.mom.ptr, w.loc.ptr, R, nullptr, s);
             else
                 launch_boxgemm(cd, opA, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
                                sym ? P.m2l_sym.as<int32_t>() : nullptr, c.perms.as<int32_t>(),
@@ -1076,6 +1099,10 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
                                P.l2lp_ptr.as<int64_t>() + P.l2lp_base[l], P.l2lp_child.as<int32_t>(),
                                P.l2lp_mat.as<int32_t>(), P.l2lp_acc.as<uint8_t>(),
                                mixed ? dst.factors32.ptr : dst.factors.ptr, l * 8, dst.sep_p, dst.sep_shared, w.loc.ptr, R, s);
+            else if (dst.tr_dense.ok)
+                launch_dense(dst.tr_dense, conj && cmap ? 2 : 1, dense_rep(w.opts.precision, false), L + P.l2l[l].begin,
+                             P.l2l[l].count, P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),
+                             P.l2l_mat.as<int32_t>(), w.loc.ptr, w.loc.ptr, R, P.l2l_acc.as<uint8_t>(), s);
             else
                 launch_boxgemm(cd, conj ? 2 : 1, P.l2l[l].count, L + P.l2l[l].begin,
                                P.l2l_ptr.as<int64_t>() + P.l2l_base[l], P.l2l_src.as<int32_t>(),
@@ -1401,9 +1428,19 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
                     }
                     pair_fop[k] = it->second;
                 }
-                build_dense_frags(*c, fop_op.data(), d.pair_sym ? fop_sym.data() : nullptr, (int64_t)fop_op.size());
+                build_dense_ops(c->m2l_dense, d.dtype, d.K, c->ops.ptr, fop_op.data(),
+                                d.pair_sym ? fop_sym.data() : nullptr, c->perms_inv.as<int32_t>(),
+                                (int64_t)fop_op.size(), d.dtype == H2G_C64 ? 0x7 : 0x3, dense_reps(c->precisions, true));
             }
-            const int32_t *pf = c->dense_ok ? pair_fop.data() : nullptr;
+            const int32_t *pf = c->m2l_dense.ok ? pair_fop.data() : nullptr;
+            // transfers that are not separable (complex maps, general operators) on the dense kernel
+            for (Side *S : {&c->test, c->galerkin ? nullptr : &c->trial_own}) {
+                if (!S || S->separable || S->n_levels < 2) continue;
+                std::vector<int32_t> mats(S->n_levels * 8);
+                for (int i = 0; i < (int)mats.size(); ++i) mats[i] = i;
+                build_dense_ops(S->tr_dense, d.dtype, d.K, S->transfers.ptr, mats.data(), nullptr, nullptr,
+                                (int64_t)mats.size(), d.dtype == H2G_C64 ? 0xF : 0x3, dense_reps(c->precisions, false));
+            }
             build_orient(*c, c->fwd, *c->trial, c->test, d.pair_test, d.pair_trial, d.pair_op, d.pair_sym, pf, d.n_pairs);
             build_orient(*c, c->rev, c->test, *c->trial, d.pair_trial, d.pair_test, d.pair_op, d.pair_sym, pf, d.n_pairs);
         } catch (...) {

@@ -46,7 +46,8 @@ typedef float float4_d __attribute__((ext_vector_type(4)));
 
 constexpr int DNT = 256;  // threads per workgroup (4 waves)
 
-// Element (i, j) of the real-form operator e for orientation `orient` (0: A, 1: A^T, 2: A^H), from
+// Element (i, j) of the real-form operator e for orientation `orient` (0: A, 1: A^T, 2: A^H,
+// 3: conj(A)), from
 // the canonical couplings `ops` (core type, column-major K x K, doubles; complex interleaved):
 // A_e[p][q] = C[pi^-1(p)][pi^-1(q)] for the symmetry of e (none: the identity).
 template <bool CPLX>
@@ -55,13 +56,14 @@ __device__ inline double dense_value(const double *__restrict__ ops, int K, int 
                                      const int32_t *__restrict__ perms_inv, int orient) {
     if (i >= Kr || j >= Kr) return 0.0;
     const int ii = CPLX ? i >> 1 : i, jj = CPLX ? j >> 1 : j;
-    const int p = orient == 0 ? ii : jj, q = orient == 0 ? jj : ii;
+    const bool tr = orient == 1 || orient == 2;
+    const int p = tr ? jj : ii, q = tr ? ii : jj;
     const int sym = fop_sym ? fop_sym[e] : -1;
     const int pp = sym >= 0 ? perms_inv[(int64_t)sym * K + p] : p;
     const int qq = sym >= 0 ? perms_inv[(int64_t)sym * K + q] : q;
     const int64_t at = (int64_t)fop_op[e] * K * K + pp + (int64_t)K * qq;
     if (!CPLX) return ops[at];
-    const double re = ops[2 * at], im = orient == 2 ? -ops[2 * at + 1] : ops[2 * at + 1];
+    const double re = ops[2 * at], im = orient >= 2 ? -ops[2 * at + 1] : ops[2 * at + 1];
     const int a = i & 1, b = j & 1;
     return a == b ? re : (a == 0 ? -im : im);
 }
@@ -130,6 +132,30 @@ __device__ inline typename dvec<V>::type load_k4(const V *__restrict__ col, int 
     return v;
 }
 
+// Accumulators started from the current output (the accumulate form, e.g. L2L into a box that
+// already holds couplings), in the D layout of the 16 x 16 MFMAs: FP32 -- D[m][n] in lane
+// n + 16 (m / 4), element m % 4; FP64 -- D[m][n] in lane n + 16 (m % 4), element m / 4.
+template <bool F64LAYOUT, int RTW, int NCT, class V>
+__device__ inline void init_from_out(double (&acc)[RTW][NCT][4], const V *__restrict__ out, int t, int R, int Kr,
+                                     int KrT, int r0, int wave, int ntw, int n, int kq) {
+#pragma unroll
+    for (int j = 0; j < RTW; ++j) {
+        if (j >= ntw) continue;
+        const int rt = wave + 4 * j;
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const int col = r0 + 16 * c + n;
+            if (col >= R) continue;
+            const V *o = out + ((int64_t)t * R + col) * Kr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * rt + (F64LAYOUT ? kq + 4 * v : 4 * kq + v);
+                if (row < Kr) acc[j][c][v] = (double)o[row];
+            }
+        }
+    }
+}
+
 // MODE 0: FP64 operands on FP64 MFMA; MODE 1: FP32 operands on FP32 MFMA with FP64 accumulation
 // of the partial sums.  RTW: row tiles per wave (16 rows each), NCT: column tiles (16 columns).
 template <int MODE, int RTW, int NCT>
@@ -138,7 +164,8 @@ __global__ void __launch_bounds__(DNT, 2)
                      const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_fop,
                      const typename std::conditional<MODE == 0, double, float>::type *__restrict__ frag, int Kr,
                      int KrT, const typename std::conditional<MODE == 0, double, float>::type *__restrict__ mom,
-                     typename std::conditional<MODE == 0, double, float>::type *__restrict__ loc, int R) {
+                     typename std::conditional<MODE == 0, double, float>::type *__restrict__ loc, int R,
+                     const uint8_t *__restrict__ acc_flags) {
     using V = typename std::conditional<MODE == 0, double, float>::type;
     using V4 = typename dvec<V>::type;
     const int ti = blockIdx.x;
@@ -162,6 +189,8 @@ __global__ void __launch_bounds__(DNT, 2)
 #pragma unroll
         for (int c = 0; c < NCT; ++c) part[j][c] = V4{0, 0, 0, 0};
     const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
+    if (acc_flags && i1 > i0 && acc_flags[i0])
+        init_from_out<MODE == 0>(acc, loc, t, R, Kr, KrT, r0, wave, ntw, n, kq);
     for (int64_t it = i0; it < i1; ++it) {
         const V *A0 = frag + (int64_t)item_fop[it] * fsz + (int64_t)lane * 4;
         const int64_t s = item_src[it];
@@ -272,7 +301,8 @@ __global__ void __launch_bounds__(DNT, 2)
     dense_m2l_bf16_kernel(const int32_t *__restrict__ recv, const int64_t *__restrict__ item_ptr,
                           const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_fop,
                           const bf16x8_d *__restrict__ frag, int64_t plane, int Kr, int KrT,
-                          const float *__restrict__ mom, float *__restrict__ loc, int R) {
+                          const float *__restrict__ mom, float *__restrict__ loc, int R,
+                          const uint8_t *__restrict__ acc_flags) {
     const int ti = blockIdx.x;
     const int t = recv[ti];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -292,6 +322,7 @@ __global__ void __launch_bounds__(DNT, 2)
             for (int v = 0; v < 4; ++v) acc[j][c][v] = 0.0;
         }
     const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
+    if (acc_flags && i1 > i0 && acc_flags[i0]) init_from_out<false>(acc, loc, t, R, Kr, KrT, r0, wave, ntw, n, kq);
     for (int64_t it = i0; it < i1; ++it) {
         const bf16x8_d *A0 = frag + (int64_t)item_fop[it] * fsz + lane;
         const int64_t s = item_src[it];
@@ -361,135 +392,134 @@ __global__ void __launch_bounds__(DNT, 2)
 template <int NPL, int RTW>
 void dense_bf16_go_nct(int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
                        const int32_t *item_fop, const void *frag, int64_t plane, int Kr, int KrT, const void *mom,
-                       void *loc, int R, hipStream_t s) {
+                       void *loc, int R, const uint8_t *acc, hipStream_t s) {
     const dim3 grid((unsigned)n_recv, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
     if (nct == 1)
         hipLaunchKernelGGL((dense_m2l_bf16_kernel<NPL, RTW, 1>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
-                           (const bf16x8_d *)frag, plane, Kr, KrT, (const float *)mom, (float *)loc, R);
+                           (const bf16x8_d *)frag, plane, Kr, KrT, (const float *)mom, (float *)loc, R, acc);
     else
         hipLaunchKernelGGL((dense_m2l_bf16_kernel<NPL, RTW, 2>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
-                           (const bf16x8_d *)frag, plane, Kr, KrT, (const float *)mom, (float *)loc, R);
+                           (const bf16x8_d *)frag, plane, Kr, KrT, (const float *)mom, (float *)loc, R, acc);
 }
 
 template <int NPL>
 void dense_bf16_go(int rtw, int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr,
                    const int32_t *item_src, const int32_t *item_fop, const void *frag, int64_t plane, int Kr, int KrT,
-                   const void *mom, void *loc, int R, hipStream_t s) {
+                   const void *mom, void *loc, int R, const uint8_t *acc, hipStream_t s) {
     switch (rtw) {
-    case 1: dense_bf16_go_nct<NPL, 1>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
-    case 2: dense_bf16_go_nct<NPL, 2>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
-    case 3: dense_bf16_go_nct<NPL, 3>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
-    case 4: dense_bf16_go_nct<NPL, 4>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
-    case 5: dense_bf16_go_nct<NPL, 5>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
-    case 6: dense_bf16_go_nct<NPL, 6>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
-    default: dense_bf16_go_nct<NPL, 7>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, s); break;
+    case 1: dense_bf16_go_nct<NPL, 1>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, acc, s); break;
+    case 2: dense_bf16_go_nct<NPL, 2>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, acc, s); break;
+    case 3: dense_bf16_go_nct<NPL, 3>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, acc, s); break;
+    case 4: dense_bf16_go_nct<NPL, 4>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, acc, s); break;
+    case 5: dense_bf16_go_nct<NPL, 5>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, acc, s); break;
+    case 6: dense_bf16_go_nct<NPL, 6>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, acc, s); break;
+    default: dense_bf16_go_nct<NPL, 7>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, plane, Kr, KrT, mom, loc, R, acc, s); break;
     }
 }
 
 template <int MODE, int RTW>
 void dense_go_nct(int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
                   const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
-                  hipStream_t s) {
+                  const uint8_t *acc, hipStream_t s) {
     using V = typename std::conditional<MODE == 0, double, float>::type;
     const dim3 grid((unsigned)n_recv, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
     if (nct == 1)
         hipLaunchKernelGGL((dense_m2l_kernel<MODE, RTW, 1>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
-                           (const V *)frag, Kr, KrT, (const V *)mom, (V *)loc, R);
+                           (const V *)frag, Kr, KrT, (const V *)mom, (V *)loc, R, acc);
     else
         hipLaunchKernelGGL((dense_m2l_kernel<MODE, RTW, 2>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
-                           (const V *)frag, Kr, KrT, (const V *)mom, (V *)loc, R);
+                           (const V *)frag, Kr, KrT, (const V *)mom, (V *)loc, R, acc);
 }
 
 template <int MODE>
 void dense_go(int rtw, int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
               const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
-              hipStream_t s) {
+              const uint8_t *acc, hipStream_t s) {
     switch (rtw) {
-    case 1: dense_go_nct<MODE, 1>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s); break;
-    case 2: dense_go_nct<MODE, 2>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s); break;
-    case 3: dense_go_nct<MODE, 3>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s); break;
-    case 4: dense_go_nct<MODE, 4>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s); break;
-    case 5: dense_go_nct<MODE, 5>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s); break;
-    case 6: dense_go_nct<MODE, 6>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s); break;
-    default: dense_go_nct<MODE, 7>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s); break;
+    case 1: dense_go_nct<MODE, 1>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 2: dense_go_nct<MODE, 2>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 3: dense_go_nct<MODE, 3>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 4: dense_go_nct<MODE, 4>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 5: dense_go_nct<MODE, 5>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 6: dense_go_nct<MODE, 6>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    default: dense_go_nct<MODE, 7>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
     }
 }
 
 }  // namespace
 
-void build_dense_frags(Core &c, const int32_t *fop_op, const int32_t *fop_sym, int64_t n_fops) {
-    c.dense_ok = false;
-    const bool cplx = c.dtype == H2G_C64;
-    const int Kr = cplx ? 2 * c.K : c.K;
+void build_dense_ops(DenseOps &D, int dtype, int K, const void *ops, const int32_t *fop_op, const int32_t *fop_sym,
+                     const int32_t *perms_inv, int64_t n, int orient_mask, int reps) {
+    D = DenseOps{};
+    const bool cplx = dtype == H2G_C64;
+    const int Kr = cplx ? 2 * K : K;
     const int KrT = (Kr + 15) / 16;
-    if (KrT > 4 * DENSE_MAX_RTW || n_fops <= 0) return;
-    c.dense_Kr = Kr;
-    c.dense_KrT = KrT;
-    c.n_fops = n_fops;
+    if (KrT > 4 * DENSE_MAX_RTW || n <= 0) return;
+    D.Kr = Kr;
+    D.KrT = KrT;
+    D.n = n;
     DevBuf d_op, d_sym;
-    d_op.upload(fop_op, (size_t)n_fops * sizeof(int32_t));
-    if (fop_sym) d_sym.upload(fop_sym, (size_t)n_fops * sizeof(int32_t));
+    d_op.upload(fop_op, (size_t)n * sizeof(int32_t));
+    if (fop_sym) d_sym.upload(fop_sym, (size_t)n * sizeof(int32_t));
+    const int32_t *dsym = fop_sym ? d_sym.as<int32_t>() : nullptr;
     const size_t per = (size_t)(16 * KrT) * (16 * KrT);
-    const int norient = cplx ? 3 : 2;
-    const int64_t total = n_fops * (int64_t)per;
+    const int64_t total = n * (int64_t)per;
     const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 1 << 20);
-    for (int o = 0; o < norient; ++o) {
+    for (int o = 0; o < 4; ++o) {
+        if (!(orient_mask & (1 << o))) continue;
         auto build = [&](auto tag, DevBuf &dst) {
             using OUT = decltype(tag);
-            dst.alloc(per * n_fops * sizeof(OUT));
+            dst.alloc(per * n * sizeof(OUT));
             if (cplx)
-                hipLaunchKernelGGL((dense_frag_kernel<OUT, true>), blocks, 256, 0, 0, c.ops.as<double>(), c.K, Kr, KrT,
-                                   n_fops, d_op.as<int32_t>(), fop_sym ? d_sym.as<int32_t>() : nullptr,
-                                   c.perms_inv.as<int32_t>(), o, dst.as<OUT>());
+                hipLaunchKernelGGL((dense_frag_kernel<OUT, true>), blocks, 256, 0, 0, (const double *)ops, K, Kr, KrT, n,
+                                   d_op.as<int32_t>(), dsym, perms_inv, o, dst.as<OUT>());
             else
-                hipLaunchKernelGGL((dense_frag_kernel<OUT, false>), blocks, 256, 0, 0, c.ops.as<double>(), c.K, Kr,
-                                   KrT, n_fops, d_op.as<int32_t>(), fop_sym ? d_sym.as<int32_t>() : nullptr,
-                                   c.perms_inv.as<int32_t>(), o, dst.as<OUT>());
+                hipLaunchKernelGGL((dense_frag_kernel<OUT, false>), blocks, 256, 0, 0, (const double *)ops, K, Kr, KrT,
+                                   n, d_op.as<int32_t>(), dsym, perms_inv, o, dst.as<OUT>());
             H2G_CHECK_LAUNCH();
         };
-        if (c.precisions & (1 << H2G_PREC_PLAIN)) build(0.0, c.dfrag64[o]);
-        if (c.precisions & (1 << H2G_PREC_MIXED)) build(0.0f, c.dfrag32[o]);
-        if (c.precisions & ((1 << H2G_PREC_BF16X2) | (1 << H2G_PREC_BF16X3))) {
-            const int64_t per_bf = (int64_t)KrT * ((KrT + 1) / 2) * 512;
-            const int64_t tb = n_fops * per_bf;
-            c.dfragbf[o].alloc(3 * (size_t)tb * sizeof(__bf16));
+        if (reps & DENSE_REP_F64) build(0.0, D.f64[o]);
+        if (reps & DENSE_REP_F32) build(0.0f, D.f32[o]);
+        if (reps & DENSE_REP_BF16) {
+            const int64_t tb = n * (int64_t)KrT * ((KrT + 1) / 2) * 512;
+            D.bf[o].alloc(3 * (size_t)tb * sizeof(__bf16));
             const unsigned bb = (unsigned)std::min<int64_t>((tb + 255) / 256, 1 << 20);
             if (cplx)
-                hipLaunchKernelGGL((dense_frag_bf16_kernel<true>), bb, 256, 0, 0, c.ops.as<double>(), c.K, Kr, KrT,
-                                   n_fops, d_op.as<int32_t>(), fop_sym ? d_sym.as<int32_t>() : nullptr,
-                                   c.perms_inv.as<int32_t>(), o, c.dfragbf[o].as<__bf16>());
+                hipLaunchKernelGGL((dense_frag_bf16_kernel<true>), bb, 256, 0, 0, (const double *)ops, K, Kr, KrT, n,
+                                   d_op.as<int32_t>(), dsym, perms_inv, o, D.bf[o].as<__bf16>());
             else
-                hipLaunchKernelGGL((dense_frag_bf16_kernel<false>), bb, 256, 0, 0, c.ops.as<double>(), c.K, Kr, KrT,
-                                   n_fops, d_op.as<int32_t>(), fop_sym ? d_sym.as<int32_t>() : nullptr,
-                                   c.perms_inv.as<int32_t>(), o, c.dfragbf[o].as<__bf16>());
+                hipLaunchKernelGGL((dense_frag_bf16_kernel<false>), bb, 256, 0, 0, (const double *)ops, K, Kr, KrT, n,
+                                   d_op.as<int32_t>(), dsym, perms_inv, o, D.bf[o].as<__bf16>());
             H2G_CHECK_LAUNCH();
         }
     }
     H2G_HIP(hipDeviceSynchronize());
-    c.dense_ok = true;
+    D.ok = true;
 }
 
-void launch_dense_m2l(const Core &c, int precision, int op, const int32_t *recv, int n_recv, const int64_t *item_ptr,
-                      const int32_t *item_src, const int32_t *item_fop, const void *mom, void *loc, int R,
-                      hipStream_t s) {
+void launch_dense(const DenseOps &D, int orient, int rep, const int32_t *recv, int n_recv, const int64_t *item_ptr,
+                  const int32_t *item_src, const int32_t *item_fop, const void *in, void *out, int R,
+                  const uint8_t *acc_flags, hipStream_t s) {
     if (n_recv == 0) return;
-    const int o = op == H2G_OP_N ? 0 : (op == H2G_OP_T || c.dtype != H2G_C64 ? 1 : 2);
-    const int rtw = (c.dense_KrT + 3) / 4;
+    const int rtw = (D.KrT + 3) / 4;
     const int nct = R > 16 ? 2 : 1;
-    if (precision == H2G_PREC_BF16X2 || precision == H2G_PREC_BF16X3) {
-        const int64_t plane = c.n_fops * (int64_t)c.dense_KrT * ((c.dense_KrT + 1) / 2) * 64;  // in bf16x8
-        if (precision == H2G_PREC_BF16X2)
-            dense_bf16_go<2>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfragbf[o].ptr, plane, c.dense_Kr,
-                             c.dense_KrT, mom, loc, R, s);
+    if (rep == DENSE_REP_BF16X2 || rep == DENSE_REP_BF16X3) {
+        const int64_t plane = D.n * (int64_t)D.KrT * ((D.KrT + 1) / 2) * 64;  // in bf16x8
+        if (!D.bf[orient].ptr) throw Error{H2G_EINVAL, "split-BF16 operators were not built for this core"};
+        if (rep == DENSE_REP_BF16X2)
+            dense_bf16_go<2>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.bf[orient].ptr, plane, D.Kr,
+                             D.KrT, in, out, R, acc_flags, s);
         else
-            dense_bf16_go<3>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfragbf[o].ptr, plane, c.dense_Kr,
-                             c.dense_KrT, mom, loc, R, s);
-    } else if (precision == H2G_PREC_MIXED) {
-        dense_go<1>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfrag32[o].ptr, c.dense_Kr, c.dense_KrT,
-                    mom, loc, R, s);
+            dense_bf16_go<3>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.bf[orient].ptr, plane, D.Kr,
+                             D.KrT, in, out, R, acc_flags, s);
+    } else if (rep == DENSE_REP_F32) {
+        if (!D.f32[orient].ptr) throw Error{H2G_EINVAL, "FP32 operators were not built for this core"};
+        dense_go<1>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.f32[orient].ptr, D.Kr, D.KrT, in, out, R,
+                    acc_flags, s);
     } else {
-        dense_go<0>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, c.dfrag64[o].ptr, c.dense_Kr, c.dense_KrT,
-                    mom, loc, R, s);
+        if (!D.f64[orient].ptr) throw Error{H2G_EINVAL, "FP64 operators were not built for this core"};
+        dense_go<0>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.f64[orient].ptr, D.Kr, D.KrT, in, out, R,
+                    acc_flags, s);
     }
     H2G_CHECK_LAUNCH();
 }

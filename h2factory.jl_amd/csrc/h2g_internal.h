@@ -10,6 +10,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "h2g.h"
@@ -180,6 +181,31 @@ struct Span {
     int32_t count = 0;
 };
 
+// A family of K x K operators in MFMA fragment order for the dense kernels (dense_m2l.hip): the
+// real form (Kr = K rows, or 2K for complex operators) in KrT tiles of 16, per orientation
+// (0: A, 1: A^T, 2: A^H, 3: conj(A)) and representation (FP64, FP32, split-BF16 planes).
+constexpr int DENSE_MAX_RTW = 7;  // at most 4 * 7 row tiles of 16 (Kr <= 448)
+enum { DENSE_REP_F64 = 1, DENSE_REP_F32 = 2, DENSE_REP_BF16 = 4, DENSE_REP_BF16X2 = 8, DENSE_REP_BF16X3 = 16 };
+struct DenseOps {
+    bool ok = false;
+    int Kr = 0, KrT = 0;
+    int64_t n = 0;
+    DevBuf f64[4], f32[4], bf[4];
+    DenseOps() = default;
+    DenseOps &operator=(DenseOps &&o) noexcept {
+        ok = o.ok;
+        Kr = o.Kr;
+        KrT = o.KrT;
+        n = o.n;
+        for (int i = 0; i < 4; ++i) {
+            std::swap(f64[i].ptr, o.f64[i].ptr), std::swap(f64[i].bytes, o.f64[i].bytes);
+            std::swap(f32[i].ptr, o.f32[i].ptr), std::swap(f32[i].bytes, o.f32[i].bytes);
+            std::swap(bf[i].ptr, o.bf[i].ptr), std::swap(bf[i].bytes, o.bf[i].bytes);
+        }
+        return *this;
+    }
+};
+
 // One side of the map.
 struct Side {
     int64_t n_dofs = 0;
@@ -196,6 +222,7 @@ struct Side {
     bool sep_shared = false;        // ... with one factor per axis and child bit (stored per sector)
     int sep_p = 0;
     DevBuf factors, factors32;      // [n_levels*8][3][p][p]
+    DenseOps tr_dense;              // the transfers on the dense MFMA kernel (when not separable)
     int level_of(int b) const {
         int l = 0;
         while (l + 1 < (int)level_offsets.size() && b >= level_offsets[l + 1]) ++l;
@@ -355,13 +382,8 @@ struct Core {
     OrientPlan fwd, rev;
     DevBuf twiddle;  // c64 [L]: exp(-2 pi i n / L)
     // dense / symmetry-reduced dense M2L on MFMA: every distinct (translation id, symmetry) of the
-    // pairs expanded into MFMA fragment order per orientation (0: N, 1: T, 2: C for complex maps),
-    // real form of Kr = K (real) or 2K (complex) rows in KrT tiles of 16
-    bool dense_ok = false;
-    int dense_Kr = 0, dense_KrT = 0;
-    int64_t n_fops = 0;
-    DevBuf dfrag64[3], dfrag32[3];
-    DevBuf dfragbf[3];  // BF16 planes hi, mid, lo of the 16 x 16 x 32 fragment order (split precisions)
+    // pairs expanded into MFMA fragment order
+    DenseOps m2l_dense;
     bool half_spectrum = false;  // real map with Hermitian spectra: real-to-complex modulation
     int64_t device_bytes() const;
 };
@@ -453,15 +475,17 @@ void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t
                       bool half, bool want32, LatticeLevel &LL);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,
                     int R, void *loc, const void *twiddle, hipStream_t s);
-// Dense M2L on MFMA (dense_m2l.hip): at most 4 * DENSE_MAX_RTW row tiles of 16 (Kr <= 448).
-constexpr int DENSE_MAX_RTW = 7;
-// Expands the distinct operators fop (translation id fop_op[e], symmetry fop_sym[e] or none) into
-// the fragment matrices of every orientation and precision of the core; sets c.dense_ok.
-void build_dense_frags(Core &c, const int32_t *fop_op, const int32_t *fop_sym, int64_t n_fops);
-// loc[t] = sum over the items of each receiving box t of A_fop(item) mom[src(item)] (overwrite)
-void launch_dense_m2l(const Core &c, int precision, int op, const int32_t *recv, int n_recv, const int64_t *item_ptr,
-                      const int32_t *item_src, const int32_t *item_fop, const void *mom, void *loc, int R,
-                      hipStream_t s);
+// Builds D from n operators on the device (`ops`: core type, column-major K x K; operator e is
+// ops[fop_op[e]] with symmetry fop_sym[e] applied, A_e[k][j] = C[perms_inv(k)][perms_inv(j)]),
+// for the orientations of orient_mask and the representations of reps (DENSE_REP_F64 / F32 / BF16).
+void build_dense_ops(DenseOps &D, int dtype, int K, const void *ops, const int32_t *fop_op, const int32_t *fop_sym,
+                     const int32_t *perms_inv, int64_t n, int orient_mask, int reps);
+// out[t] = (acc_flags && acc_flags[first item of t] ? out[t] : 0) + sum over the items of t of
+// A_{orient}(fop(item)) in[src(item)], per receiving box t of `recv`; rep: DENSE_REP_F64 (FP64
+// data), DENSE_REP_F32 (FP32 data) or DENSE_REP_BF16X2 / X3 (FP32 data, split-BF16 products).
+void launch_dense(const DenseOps &D, int orient, int rep, const int32_t *recv, int n_recv, const int64_t *item_ptr,
+                  const int32_t *item_src, const int32_t *item_fop, const void *in, void *out, int R,
+                  const uint8_t *acc_flags, hipStream_t s);
 
 }  // namespace h2g
 

@@ -25,11 +25,14 @@ void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t) {
     for (int64_t i = 0; i < n; ++i) dst[i] = (float)src[i];
 }
 
-// the dense MFMA coupling kernel is not built: dense maps take the generic box GEMM
-void build_dense_frags(Core &c, const int32_t *, const int32_t *, int64_t) { c.dense_ok = false; }
-void launch_dense_m2l(const Core &, bool, int, const int32_t *, int, const int64_t *, const int32_t *, const int32_t *,
-                      const void *, void *, int, hipStream_t) {
-    throw Error{H2G_EUNSUPPORTED, "host replay: the dense MFMA coupling kernel runs on the GPU only"};
+// the dense MFMA kernels are not built: dense maps and general transfers take the generic box GEMM
+void build_dense_ops(DenseOps &D, int, int, const void *, const int32_t *, const int32_t *, const int32_t *, int64_t,
+                     int, int) {
+    D.ok = false;
+}
+void launch_dense(const DenseOps &, int, int, const int32_t *, int, const int64_t *, const int32_t *, const int32_t *,
+                  const void *, void *, int, const uint8_t *, hipStream_t) {
+    throw Error{H2G_EUNSUPPORTED, "host replay: the dense MFMA kernels run on the GPU only"};
 }
 
 [[noreturn]] static void lattice_refused() {
