@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=384,
                     help="single-RHS requests of the serving measurement (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the requests (1/s)")
+    ap.add_argument("--workload", default="laplace", choices=["laplace", "helmholtz"],
+                    help="helmholtz: N = 4M points on the unit sphere, k = 10, Chebyshev p = 6, dense complex "
+                         "couplings, 16 complex RHS (ComplexF32 and split-BF16 forms, forward and adjoint)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy five times (HBM counter calibration)")
     return ap.parse_args()
@@ -124,8 +127,116 @@ def cpu_baseline(n, reps=1, seed_pts=0, seed_rhs=1, p=6):
     return dt / reps
 
 
+def helmholtz_main(args):
+    """Helmholtz workload: e^{ikr}/(4 pi r), k = 10, N points uniform on the unit sphere (seed 0),
+    octree on [-1, 1]^3 with leaves of at most 64 points, Chebyshev p = 6 (K = 216), dense complex
+    couplings, R complex-normal right-hand sides (seed 1).  Times the ComplexF32 form and the split
+    BF16 (hi + mid + lo) form, forward and adjoint, each against the plain ComplexF64 form one column
+    at a time, and reports every form's error against the O(N^2) FP64 direct sum on 1,000 targets."""
+    import numpy as np
+    import torch
+
+    import h2g
+    from h2g import method as PM
+
+    dev = torch.device("cuda:0")
+    N = args.n if args.n != 2_000_000 else 4_000_000
+    R = args.nrhs if args.nrhs != 32 else 16
+    k = 10.0
+    t0 = time.perf_counter()
+    pts = np.random.default_rng(0).standard_normal((N, 3))
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    kw = dict(p=6, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=k, leaf_size=64,
+              origin=(-1.0, -1.0, -1.0), size=2.0)
+    arrays = PM.build_map(pts, **kw)
+    t_build = time.perf_counter() - t0
+    log(f"map built on host in {t_build:.1f}s: {arrays.stats}")
+    t0 = time.perf_counter()
+    core = h2g.GpuH2Core(arrays, precisions=("plain", "mixed", "bf16x3"))
+    torch.cuda.synchronize()
+    t_core = time.perf_counter() - t0
+    log(f"core on device in {t_core:.1f}s, {core.device_bytes() / 1e9:.1f} GB")
+    g = torch.Generator(device="cpu").manual_seed(1)
+    Xh = (torch.randn((R, N), generator=g, dtype=torch.float32)
+          + 1j * torch.randn((R, N), generator=g, dtype=torch.float32)) / np.sqrt(2.0)
+    X = Xh.to(dev).t()  # column-major N x R complex64
+    ncol = max(1, min(args.plain_rhs, R))
+    X64 = X[:, :ncol].to(torch.complex128).t().contiguous().t()
+    # plain ComplexF64 form, one column at a time
+    P = h2g.H2Workspace(core, max_rhs=1, precision="plain")
+    plain = {}
+    for op, M in (("N", P), ("C", h2g.adjoint(P))):
+        Y64 = torch.zeros((ncol, N), dtype=torch.complex128, device=dev).t()
+        h2g.mul_(Y64[:, :1], M, X64[:, :1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for j in range(ncol):
+            h2g.mul_(Y64[:, j:j + 1], M, X64[:, j:j + 1])
+        torch.cuda.synchronize()
+        plain[op] = (Y64, (time.perf_counter() - t0) / ncol)
+        log(f"plain C64 {op}: {1e3 * plain[op][1]:.1f} ms per RHS")
+    del P
+    # O(N^2) direct sum on 1,000 targets: forward G x, adjoint conj(G) x = conj(G conj(x))
+    rows = torch.from_numpy(np.random.default_rng(2).choice(N, 1000, replace=False)).to(dev)
+    P_d = torch.from_numpy(pts).to(dev)
+    yd = {"N": torch.zeros((1000, ncol), dtype=torch.complex128, device=dev)}
+    yd["C"] = torch.zeros_like(yd["N"])
+    for c0 in range(0, N, 200_000):
+        diff = P_d[rows][:, None, :] - P_d[None, c0:c0 + 200_000, :]
+        d = torch.sqrt((diff * diff).sum(-1))
+        dd = torch.where(d > 0, d, torch.ones_like(d))
+        G = torch.where(d > 0, torch.exp(1j * k * dd) / (4 * np.pi * dd), torch.zeros_like(dd, dtype=torch.complex128))
+        yd["N"] += G @ X64[c0:c0 + 200_000]
+        yd["C"] += torch.conj(G @ torch.conj(X64[c0:c0 + 200_000]))
+        del diff, d, dd, G
+    rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    out_forms = {}
+    for prec in ("mixed", "bf16x3"):
+        A = h2g.H2Workspace(core, max_rhs=R, precision=prec, use_graph=True)
+        form = {}
+        for op, M in (("N", A), ("C", h2g.adjoint(A))):
+            Y = torch.zeros((R, N), dtype=torch.complex64, device=dev).t()
+            for _ in range(args.warmup):
+                h2g.mul_(Y, M, X)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                h2g.mul_(Y, M, X)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+            Yc = Y[:, :ncol].to(torch.complex128)
+            form[op] = {"ms_per_product": round(1e3 * dt, 2), "rhs_per_s": round(R / dt, 2),
+                        "rel_err_vs_plain_c64": rel(Yc, plain[op][0]),
+                        "rel_err_vs_direct_sum_1000": rel(Yc[rows], yd[op]),
+                        "speedup_vs_plain": round(plain[op][1] * R / dt, 2)}
+            log(f"{prec} {op}: {form[op]}")
+        del A
+        Tm = h2g.H2Workspace(core, max_rhs=R, precision=prec, timing=True)
+        h2g.mul_(Y, Tm, X)
+        h2g.mul_(Y, Tm, X)
+        ph = {}
+        for (phi, lvl, ms) in Tm.timings():
+            ph[PHASES[phi]] = round(ph.get(PHASES[phi], 0.0) + ms, 3)
+        form["phase_ms"] = ph
+        del Tm
+        out_forms[prec] = form
+    out = {
+        "metric": "Helmholtz H2 matvec RHS/s (k=10, unit sphere, Chebyshev p=6, dense complex M2L), forward + adjoint",
+        "workload": f"helmholtz k={k} N={N} unit sphere, leaf<=64, p=6 Chebyshev dense, {R} complex RHS",
+        "n": N, "nrhs": R, "data": "synthetic", "stats": arrays.stats,
+        "plain_c64_ms_per_rhs": {op: round(1e3 * v[1], 2) for op, v in plain.items()},
+        "plain_rel_err_vs_direct_sum_1000": {op: rel(plain[op][0][rows], yd[op]) for op in plain},
+        "forms": out_forms,
+        "build_s": {"host_plan": round(t_build, 1), "device_core": round(t_core, 1)},
+        "device_bytes": {"core": core.device_bytes()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "helmholtz" and args.gpus == 1:
+        return helmholtz_main(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "of this path is a design in DESIGN.md, not a measured configuration", file=sys.stderr)

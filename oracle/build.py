@@ -36,9 +36,9 @@ def _collection(tree, nodes):
 
 
 def build_core(points_test, points_trial=None, p=4, nodekind="cheb", kernel="laplace",
-               wavenumber=0.0, m2l_kind="dense", leaf_size=64, alpha=1.0):
-    test = build_tree(points_test, leaf_size)
-    trial = test if points_trial is None else build_tree(points_trial, leaf_size)
+               wavenumber=0.0, m2l_kind="dense", leaf_size=64, alpha=1.0, origin=(0.0, 0.0, 0.0), size=1.0):
+    test = build_tree(points_test, leaf_size, origin, size)
+    trial = test if points_trial is None else build_tree(points_trial, leaf_size, origin, size)
     nodes = M.nodes1d(p, nodekind)
     K = p ** 3
     pairs = m2l_pairs(test, trial)

@@ -473,3 +473,33 @@ def test_split_bf16_dense_m2l(gpu, case):
             assert errs[(prec, op)] < tol, (prec, op, errs[(prec, op)])
         ws.close()
     print("split-BF16 errors vs plain:", {k: f"{v:.2e}" for k, v in errs.items()})
+
+
+def test_helmholtz_unit_sphere_p6(gpu):
+    """The Helmholtz workload's shape at small N: points on the unit sphere (octree on [-1, 1]^3),
+    k = 10, Chebyshev p = 6, dense complex couplings, 16 right-hand sides.  Plain ComplexF64 ==
+    oracle to 1e-12 in every orientation; ComplexF32 and split BF16 (hi + mid + lo) within the 1e-6
+    gate of the plain form, forward and adjoint (device-filled near values and collections)."""
+    import h2g
+    from h2g import method as PM
+
+    rng = np.random.default_rng(44)
+    pts = rng.standard_normal((4000, 3))
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    kw = dict(p=6, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=10.0, leaf_size=64,
+              origin=(-1.0, -1.0, -1.0), size=2.0)
+    core = OB.build_core(pts, **kw)
+    arrays = PM.build_map(pts, host_values=False, **kw)
+    c = h2g.GpuH2Core(arrays, precisions=("plain", "mixed", "bf16x3"))
+    plain = h2g.H2Workspace(c, max_rhs=16)
+    x = rng.standard_normal((4000, 16)) + 1j * rng.standard_normal((4000, 16))
+    for op in "NTC":
+        y = orient(plain, op) @ x
+        assert relerr(y, O.mul(core, x, op)) < RTOL, op
+    x32 = x.astype(np.complex64)
+    for prec in ("mixed", "bf16x3"):
+        ws = h2g.H2Workspace(c, max_rhs=16, precision=prec, use_graph=True)
+        for op in "NC":
+            ref = orient(plain, op) @ x32.astype(np.complex128)
+            assert relerr(orient(ws, op) @ x32, ref) < 1e-6, (prec, op)
+        ws.close()
