@@ -48,9 +48,15 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=2, help="products timed in the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=384,
+    ap.add_argument("--serve-requests", type=int, default=4000,
                     help="single-RHS requests of the serving measurement (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="Poisson arrival rate of the requests (1/s)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the requests (1/s)")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="HBM budget of the resident maps (GB)")
+    ap.add_argument("--serve-seed", type=int, default=0, help="seed of the serving map mix")
+    ap.add_argument("--serve-rate-low", type=float, default=200.0,
+                    help="second operating point below capacity (requests/s; 0: skip)")
+    ap.add_argument("--serve-requests-low", type=int, default=1000)
+    ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (one JSON line)")
     ap.add_argument("--workload", default="laplace", choices=["laplace", "helmholtz"],
                     help="helmholtz: N = 4M points on the unit sphere, k = 10, Chebyshev p = 6, dense complex "
                          "couplings, 16 complex RHS (ComplexF32 and split-BF16 forms, forward and adjoint)")
@@ -233,10 +239,159 @@ def helmholtz_main(args):
     print(json.dumps(out), flush=True)
 
 
+def serving_mix(seed=0, n_maps=8):
+    """The serving workload's resident maps, drawn from a seed: Laplace (equispaced p, Toeplitz
+    couplings) or Helmholtz (k in {2, 10}; Chebyshev p, dense complex couplings), N log-uniform in
+    [50k, 4M], points uniform in the unit cube or on the unit sphere, p in {4, 6}."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n_maps):
+        kernel = str(rng.choice(["laplace", "helmholtz"]))
+        k = float(rng.choice([2.0, 10.0]))
+        n = int(round(np.exp(rng.uniform(np.log(5e4), np.log(4e6)))))
+        geom = str(rng.choice(["cube", "sphere"]))
+        p = int(rng.choice([4, 6]))
+        specs.append(dict(kernel=kernel, k=k if kernel == "helmholtz" else 0.0, n=n, geom=geom, p=p))
+    return specs
+
+
+def build_spec(spec, seed):
+    import numpy as np
+
+    from h2g import method as PM
+
+    rng = np.random.default_rng(seed)
+    if spec["geom"] == "cube":
+        pts, box = rng.random((spec["n"], 3)), dict(origin=(0.0, 0.0, 0.0), size=1.0)
+    else:
+        pts = rng.standard_normal((spec["n"], 3))
+        pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+        box = dict(origin=(-1.0, -1.0, -1.0), size=2.0)
+    if spec["kernel"] == "laplace":
+        return PM.build_map(pts, p=spec["p"], nodekind="equi", m2l_kind="toeplitz", leaf_size=64, **box)
+    return PM.build_map(pts, p=spec["p"], nodekind="cheb", m2l_kind="dense", kernel="helmholtz",
+                        wavenumber=spec["k"], leaf_size=64, **box)
+
+
+def serving_leg(args):
+    """Continuous batching over the resident map mix: a seeded Poisson stream of single-column
+    forward / transpose / adjoint requests (FP32 or ComplexF32 N(0,1)), batched per map up to 32
+    columns, maps admitted against the HBM budget; against the plain FP64 form serving one request
+    at a time (its service time per map, i.e. its throughput capacity with no queueing)."""
+    import numpy as np
+    import torch
+
+    import h2g
+    from h2g.serve import AdmissionError, Request, Server
+
+    specs = serving_mix(args.serve_seed)
+    srv = Server(budget_bytes=int(args.serve_budget_gb * 1e9), max_batch=32, ws_per_map=2, precision="mixed",
+                 n_streams=4)
+    admitted, rejected = [], []
+    t0 = time.perf_counter()
+    for i, s in enumerate(specs):
+        arrays = build_spec(s, 100 + i)
+        name = f"m{i}"
+        try:
+            rm = srv.add_map(name, arrays)
+            admitted.append(dict(name=name, **s, gb=round(rm.device_bytes() / 1e9, 1)))
+            log(f"serving: admitted {name} {s}: {rm.device_bytes() / 1e9:.1f} GB, total {srv.used_bytes() / 1e9:.1f} GB")
+        except AdmissionError as e:
+            rejected.append(dict(name=name, **s, reason=str(e)))
+            log(f"serving: rejected {name} {s}: {e}")
+        del arrays
+    t_build = time.perf_counter() - t0
+    names = [a["name"] for a in admitted]
+    rng = np.random.default_rng(7)
+
+    def stream(n_req):
+        reqs = []
+        for _ in range(n_req):
+            name = names[int(rng.integers(len(names)))]
+            rm = srv.maps[name]
+            x = rng.standard_normal(rm.n).astype(np.float32)
+            if rm.dtype == np.complex64:
+                x = (x + 1j * rng.standard_normal(rm.n)).astype(np.complex64)
+            op = "NTC"[int(rng.integers(3))]
+            n_out, n_in = rm.core.shape
+            reqs.append(Request(name, op, x[:n_in if op == "N" else n_out]))
+        return reqs
+
+    def per_map_stats(reqs):
+        out = {}
+        for name in names:
+            lat = np.array([r.latency for r in reqs if r.map_name == name]) * 1e3
+            bs = [r.batch_size for r in reqs if r.map_name == name]
+            out[name] = {"requests": len(lat), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 1), "mean_batch": round(float(np.mean(bs)), 1)}
+        return out
+
+    # (each map's first workspace ran a 1-column and a full-width product at admission)
+    reqs = stream(args.serve_requests)
+    srv.done.clear()
+    st = srv.run_stream(reqs, rate_hz=args.serve_rate, seed=6)
+    per_map = per_map_stats(reqs)
+    st["accounting"] = dict(srv.last_accounting)
+    log(f"serving: {st}")
+    low = None
+    if args.serve_rate_low > 0 and args.serve_requests_low > 0:
+        reqs_low = stream(args.serve_requests_low)
+        srv.done.clear()
+        sl = srv.run_stream(reqs_low, rate_hz=args.serve_rate_low, seed=8)
+        low = {"requests": sl["requests"], "arrival_rate_hz": args.serve_rate_low,
+               "matvecs_per_s": round(sl["matvecs_per_s"], 1), "p50_ms": round(sl["p50_ms"], 1),
+               "p99_ms": round(sl["p99_ms"], 1), "mean_batch": round(sl["mean_batch"], 2),
+               "per_map": per_map_stats(reqs_low), "accounting": dict(srv.last_accounting)}
+        log(f"serving at {args.serve_rate_low}/s: {low}")
+    # plain FP64 one request at a time: service time per map; and each map's served results (two
+    # per map) against the plain product
+    svc, errs = {}, {}
+    for name in names:
+        rm = srv.maps[name]
+        P = h2g.H2Workspace(rm.core, max_rhs=1, precision="plain")
+        mine = [r for r in reqs if r.map_name == name][:2]
+        e = 0.0
+        for r in mine:
+            M = {"N": P, "T": h2g.transpose(P), "C": h2g.adjoint(P)}[r.op]
+            y64 = M @ r.x.astype(np.complex128 if rm.dtype == np.complex64 else np.float64)
+            e = max(e, float(np.linalg.norm(r.result - y64) / np.linalg.norm(y64)))
+        errs[name] = e
+        xd = torch.from_numpy(mine[0].x.astype(np.complex128 if rm.dtype == np.complex64 else np.float64)).cuda()
+        n_out = rm.core.shape[0] if mine[0].op == "N" else rm.core.shape[1]
+        yd = torch.zeros(n_out, dtype=xd.dtype, device="cuda")
+        M = {"N": P, "T": h2g.transpose(P), "C": h2g.adjoint(P)}[mine[0].op]
+        h2g.mul_(yd, M, xd)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(2):
+            h2g.mul_(yd, M, xd)
+        torch.cuda.synchronize()
+        svc[name] = (time.perf_counter() - ts) / 2
+        P.close()
+        log(f"serving: {name} plain FP64 {1e3 * svc[name]:.1f} ms per request, served vs plain {e:.2e}")
+    srv.close()
+    mean_svc = float(np.mean([svc[r.map_name] for r in reqs]))
+    return {"maps_admitted": admitted, "maps_rejected": rejected, "budget_gb": args.serve_budget_gb,
+            "requests": st["requests"], "arrival_rate_hz": args.serve_rate, "streams": 4, "max_batch": 32,
+            "matvecs_per_s": round(st["matvecs_per_s"], 1), "wall_s": round(st["wall_s"], 2),
+            "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1),
+            "mean_batch": round(st["mean_batch"], 2), "per_map": per_map, "accounting": st["accounting"],
+            "plain_fp64_one_at_a_time": {"service_ms": {k: round(1e3 * v, 2) for k, v in svc.items()},
+                                         "capacity_matvecs_per_s": round(1.0 / mean_svc, 2)},
+            "throughput_vs_plain": round(st["matvecs_per_s"] * mean_svc, 2),
+            "max_rel_err_vs_plain_fp64": max(errs.values()), "build_s": round(t_build, 1),
+            "below_capacity": low}
+
+
 def main():
     args = parse()
     if args.workload == "helmholtz" and args.gpus == 1:
         return helmholtz_main(args)
+    if args.serve_only and args.gpus == 1:
+        print(json.dumps({"serving": serving_leg(args)}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "of this path is a design in DESIGN.md, not a measured configuration", file=sys.stderr)
@@ -371,26 +526,13 @@ def main():
     roof["kernel"] = f"{dph} level {dlv}"
     roof["launch_ms"] = round(dms, 3)
 
-    # serving: single-RHS requests with Poisson arrivals against the resident map, continuously
-    # batched (up to R per product, two workspaces in flight); latency = completion - arrival
+    # serving: the resident map mix under a Poisson stream of single-column requests
     serve = None
     if args.serve_requests > 0:
-        from h2g.serve import Request, Server
-
-        del A, Tm, P
+        del A, Tm, P, core
         torch.cuda.synchronize()
-        srv = Server(budget_bytes=280 * 2 ** 30, max_batch=R, ws_per_map=2)
-        srv.add_core("bench", core)
-        rng = np.random.default_rng(5)
-        reqs = [Request("bench", "N", rng.standard_normal(N).astype(np.float32)) for _ in range(args.serve_requests)]
-        for r in reqs[: 2 * R]:  # warm the two workspaces (graphs captured per batch width)
-            srv.submit(Request("bench", "N", r.x))
-        srv.drain()
-        st = srv.run_stream(reqs, rate_hz=args.serve_rate, seed=6)
-        srv.close()
-        serve = {"requests": st["requests"], "arrival_rate_hz": args.serve_rate,
-                 "throughput_rhs_per_s": round(st["matvecs_per_s"], 1), "p50_ms": round(st["p50_ms"], 2),
-                 "p99_ms": round(st["p99_ms"], 2), "mean_batch": round(st["mean_batch"], 2)}
+        torch.cuda.empty_cache()
+        serve = serving_leg(args)
         log(f"serving: {serve}")
 
     cpu = None

@@ -1306,10 +1306,15 @@ void Workspace::mark_end() {
 
 int64_t Core::device_bytes() const {
     int64_t b = 0;
+    auto dense = [&](const DenseOps &D) {
+        for (int o = 0; o < 4; ++o) b += D.f64[o].bytes + D.f32[o].bytes + D.bf[o].bytes;
+    };
     auto side = [&](const Side &S) {
         b += S.d_perm.bytes + S.d_iperm.bytes + S.d_begin.bytes + S.d_end.bytes + S.coll.bytes + S.coll32.bytes + S.transfers.bytes +
-             S.transfers32.bytes;
+             S.transfers32.bytes + S.factors.bytes + S.factors32.bytes;
+        dense(S.tr_dense);
     };
+    dense(m2l_dense);
     side(test);
     if (!galerkin) side(trial_own);
     b += ops.bytes + ops32.bytes + perms.bytes + perms_inv.bytes + near_vals.bytes + near_vals32.bytes;
@@ -1497,10 +1502,16 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             w->core = core;
             w->opts = o;
             w->R_cap = o.max_rhs;
-            H2G_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
-            H2G_HIP(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
+            if (o.stream) {
+                w->stream = w->side = static_cast<hipStream_t>(o.stream);
+                w->own_streams = false;
+            } else {
+                H2G_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+                H2G_HIP(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
+            }
             H2G_HIP(hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_near, hipEventDisableTiming));
+            H2G_HIP(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
             const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap);
             w->xs.alloc(L.xs);
             w->ys.alloc(L.ys);
@@ -1509,10 +1520,11 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             w->spec_in.alloc(L.spec_in);
             w->spec_out.alloc(L.spec_out);
         } catch (...) {
+            if (w->ev_done) (void)hipEventDestroy(w->ev_done);
             if (w->ev_near) (void)hipEventDestroy(w->ev_near);
             if (w->ev_fork) (void)hipEventDestroy(w->ev_fork);
-            if (w->side) (void)hipStreamDestroy(w->side);
-            if (w->stream) (void)hipStreamDestroy(w->stream);
+            if (w->own_streams && w->side) (void)hipStreamDestroy(w->side);
+            if (w->own_streams && w->stream) (void)hipStreamDestroy(w->stream);
             delete w;
             throw;
         }
@@ -1524,17 +1536,19 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
 int h2g_ws_destroy(h2g_ws *ws) {
     if (!ws) return H2G_EINVAL;
     (void)hipSetDevice(ws->core->device);
-    if (ws->stream) (void)hipStreamSynchronize(ws->stream);
-    if (ws->side) (void)hipStreamSynchronize(ws->side);
+    if (ws->ev_done) (void)hipEventSynchronize(ws->ev_done);
+    if (ws->own_streams && ws->stream) (void)hipStreamSynchronize(ws->stream);
+    if (ws->own_streams && ws->side) (void)hipStreamSynchronize(ws->side);
     for (auto &kv : ws->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto &e : ws->events) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    if (ws->ev_done) (void)hipEventDestroy(ws->ev_done);
     if (ws->ev_near) (void)hipEventDestroy(ws->ev_near);
     if (ws->ev_fork) (void)hipEventDestroy(ws->ev_fork);
-    if (ws->side) (void)hipStreamDestroy(ws->side);
-    if (ws->stream) (void)hipStreamDestroy(ws->stream);
+    if (ws->own_streams && ws->side) (void)hipStreamDestroy(ws->side);
+    if (ws->own_streams && ws->stream) (void)hipStreamDestroy(ws->stream);
     h2g_core *c = static_cast<h2g_core *>(ws->core);
     delete ws;
     return h2g_core_release(c);
@@ -1547,6 +1561,26 @@ int h2g_ws_device_bytes(const h2g_ws *ws, int64_t *bytes) {
 }
 
 void *h2g_ws_stream(h2g_ws *ws) { return ws ? (void *)ws->stream : nullptr; }
+
+int h2g_ws_set_stream(h2g_ws *ws, void *stream) {
+    if (!ws || !stream) return H2G_EINVAL;
+    if (ws->own_streams) {
+        set_error("h2g_ws_set_stream: the workspace runs on streams of its own");
+        return H2G_EINVAL;
+    }
+    if (ws->busy.load()) {
+        set_error("ConcurrentWorkspaceError: the workspace is already in use");
+        return H2G_EBUSY;
+    }
+    return guarded([&] {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (s != ws->stream) {
+            H2G_HIP(hipSetDevice(ws->core->device));
+            H2G_HIP(hipStreamWaitEvent(s, ws->ev_done, 0));
+            ws->stream = ws->side = s;
+        }
+    });
+}
 
 int h2g_ws_acquire(h2g_ws *ws) {
     if (!ws) return H2G_EINVAL;
@@ -1566,12 +1600,12 @@ int h2g_ws_release(h2g_ws *ws) {
 
 int h2g_ws_sync(h2g_ws *ws) {
     if (!ws) return H2G_EINVAL;
-    return guarded([&] { H2G_HIP(hipStreamSynchronize(ws->stream)); });
+    return guarded([&] { H2G_HIP(hipEventSynchronize(ws->ev_done)); });
 }
 
 int h2g_ws_query(h2g_ws *ws) {
     if (!ws) return H2G_EINVAL;
-    const hipError_t e = hipStreamQuery(ws->stream);
+    const hipError_t e = hipEventQuery(ws->ev_done);
     if (e == hipSuccess) return H2G_OK;
     if (e == hipErrorNotReady) return H2G_EAGAIN;
     set_error(std::string("hipStreamQuery: ") + hipGetErrorString(e));
@@ -1627,12 +1661,10 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
         ws->timings.clear();
         ws->n_events_used = 0;
         run_passes(*ws, op, which, dtype, nrhs, dx, lx, dy, ly, a, b, five, true, ws->opts.use_graph && !ws->opts.timing);
-        if (host) {
+        if (host)
             H2G_HIP(hipMemcpy2DAsync(y, ldy * xe, ws->yhost.ptr, n_out * xe, n_out * xe, nrhs, hipMemcpyDeviceToHost, s));
-            if (!(flags & H2G_ASYNC) || ws->opts.timing) H2G_HIP(hipStreamSynchronize(s));
-        } else if (!(flags & H2G_ASYNC) || ws->opts.timing) {
-            H2G_HIP(hipStreamSynchronize(s));
-        }
+        H2G_HIP(hipEventRecord(ws->ev_done, s));
+        if (!(flags & H2G_ASYNC) || ws->opts.timing) H2G_HIP(hipEventSynchronize(ws->ev_done));
         for (size_t i = 0; i < ws->n_events_used && i < ws->timings.size(); ++i)
             H2G_HIP(hipEventElapsedTime(&ws->timings[i].ms, ws->events[i].first, ws->events[i].second));
     });

@@ -404,6 +404,8 @@ struct Workspace {
     // M2L run on `stream`; the first far-field write of the result waits for it (ev_near)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_near = nullptr;
+    hipEvent_t ev_done = nullptr;  // recorded after each product (h2g_ws_query / h2g_ws_sync)
+    bool own_streams = true;       // false: stream (== side) belongs to the caller
     int R_cap = 1;
     DevBuf xs, ys;        // tree-ordered FP64/C64, [n][R]
     DevBuf mom, loc;      // [box][R][K] compute type

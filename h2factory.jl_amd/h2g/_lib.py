@@ -26,7 +26,7 @@ EXPORTED = [
     "h2g_core_create", "h2g_core_retain", "h2g_core_release", "h2g_core_size", "h2g_core_device_bytes",
     "h2g_ws_create", "h2g_ws_destroy", "h2g_ws_device_bytes", "h2g_ws_stream", "h2g_ws_acquire",
     "h2g_ws_release", "h2g_matvec", "h2g_ws_sync", "h2g_ws_timing", "h2g_last_error", "h2g_version",
-    "h2g_method_fill", "h2g_ws_query", "h2g_matvec_noalloc",
+    "h2g_method_fill", "h2g_ws_query", "h2g_matvec_noalloc", "h2g_ws_set_stream",
 ]
 
 VP = C.c_void_p
@@ -77,7 +77,8 @@ class MethodDesc(C.Structure):
 
 
 class WsOpts(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("max_rhs", C.c_int32), ("use_graph", C.c_int32), ("timing", C.c_int32)]
+    _fields_ = [("precision", C.c_int32), ("max_rhs", C.c_int32), ("use_graph", C.c_int32), ("timing", C.c_int32),
+                ("stream", VP)]
 
 
 class TimingRecord(C.Structure):
@@ -104,6 +105,7 @@ def lib():
         L.h2g_ws_device_bytes.argtypes = [VP, I64P]
         L.h2g_ws_stream.argtypes = [VP]
         L.h2g_ws_stream.restype = VP
+        L.h2g_ws_set_stream.argtypes = [VP, VP]
         L.h2g_ws_acquire.argtypes = [VP]
         L.h2g_ws_release.argtypes = [VP]
         L.h2g_matvec.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.c_int64, VP, C.c_int64, VP, C.c_int64,

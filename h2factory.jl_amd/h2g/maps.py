@@ -208,14 +208,17 @@ class H2Workspace:
 
     op = H.H2G_OP_N
 
-    def __init__(self, core: GpuH2Core, max_rhs=1, precision="plain", use_graph=False, timing=False):
+    def __init__(self, core: GpuH2Core, max_rhs=1, precision="plain", use_graph=False, timing=False, stream=None):
+        """``stream``: a HIP stream handle (int) the workspace runs on, shared with other workspaces,
+        or None for streams of its own."""
         self.core = core
         o = H.WsOpts()
+        o.stream = stream
         o.precision = H.PRECISIONS[precision]
         o.max_rhs = int(max_rhs)
         o.use_graph = int(bool(use_graph))
         o.timing = int(bool(timing))
-        self.opts = dict(max_rhs=max_rhs, precision=precision, use_graph=use_graph, timing=timing)
+        self.opts = dict(max_rhs=max_rhs, precision=precision, use_graph=use_graph, timing=timing, stream=stream)
         h = C.c_void_p()
         _raise(H.lib().h2g_ws_create(core.handle, C.byref(o), C.byref(h)))
         self.handle = h
@@ -245,6 +248,11 @@ class H2Workspace:
 
     def stream(self):
         return H.lib().h2g_ws_stream(self.handle)
+
+    def set_stream(self, stream):
+        """Move a workspace created with ``stream=`` to another caller stream (ordered after its
+        last product)."""
+        _raise(H.lib().h2g_ws_set_stream(self.handle, C.c_void_p(stream)))
 
     def acquire(self):
         _raise(H.lib().h2g_ws_acquire(self.handle))

@@ -162,6 +162,10 @@ typedef struct h2g_ws_opts {
     int32_t use_graph;            /* capture each (op, parts, nrhs) pass into a hipGraph      */
     int32_t timing;               /* record per-phase / per-level device times (HIP events on the
                                      workspace stream; passes are then not captured in graphs) */
+    void *stream;                 /* a hipStream_t the workspace runs on (shared by several workspaces,
+                                     e.g. a serving loop's pool of as many streams as the device has
+                                     hardware queues; the near field then runs in order with the far
+                                     field), or NULL for two streams of its own */
 } h2g_ws_opts;
 
 /* One per-phase timing record, shaped like the reference's TimingRecord
@@ -193,6 +197,11 @@ int h2g_ws_destroy(h2g_ws *ws);
 int h2g_ws_device_bytes(const h2g_ws *ws, int64_t *bytes);
 /* The workspace's HIP stream (hipStream_t). */
 void *h2g_ws_stream(h2g_ws *ws);
+/* Moves a workspace created with a caller stream (h2g_ws_opts.stream) to another caller stream;
+ * its next product starts after its last one (the new stream waits for the workspace's
+ * completion event).  A serving loop uses it to send each batch to the least-loaded stream of
+ * its pool.  H2G_EINVAL for a workspace with streams of its own. */
+int h2g_ws_set_stream(h2g_ws *ws, void *stream);
 
 /* y = op(A) x (alpha == beta == NULL) or y = alpha op(A) x + beta y, for `which` in
  * H2G_MUL / H2G_FARMUL / H2G_NEARMUL, exactly as the reference's 3- and 5-argument methods.
@@ -227,9 +236,10 @@ int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dty
 int h2g_ws_acquire(h2g_ws *ws);
 int h2g_ws_release(h2g_ws *ws);
 
-/* Waits for the workspace stream (after H2G_ASYNC calls). */
+/* Waits for the last product enqueued on the workspace (after H2G_ASYNC calls). */
 int h2g_ws_sync(h2g_ws *ws);
-/* H2G_OK when every product enqueued on the workspace has finished, H2G_EAGAIN otherwise. */
+/* H2G_OK when the last product enqueued on the workspace has finished, H2G_EAGAIN otherwise
+ * (a completion event per workspace, so workspaces sharing a stream are told apart). */
 int h2g_ws_query(h2g_ws *ws);
 
 /* Timing records of the last matvec (opts.timing != 0). */

@@ -148,6 +148,8 @@ inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipE
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete reinterpret_cast<char *>(e); return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
     auto *g = new h2g_cpu::Graph;
     h2g_cpu::stream_of(s)->capture = &g->ops;

@@ -37,7 +37,8 @@ def test_continuous_batching_same_alone_and_in_mix(gpu):
     from h2g.serve import Request, Server
 
     lap, hel = _maps()
-    srv = Server(budget_bytes=50 * 2 ** 30, max_batch=8, ws_per_map=2)
+    # four workspaces on a pool of two streams: completion is told apart per workspace
+    srv = Server(budget_bytes=50 * 2 ** 30, max_batch=8, ws_per_map=2, n_streams=2)
     srv.add_map("lap", lap)
     srv.add_map("hel", hel)
     rng = np.random.default_rng(1)
