@@ -8,8 +8,9 @@ batched mixed-precision form (FP32 far field; near values stored in FP32, near p
 final sum in FP64; hipGraph replay).
 The same run times the plain FP64 form one right-hand side at a time, checks the batched
 result against it, reports the error against the O(N^2) direct sum on 1,000 sampled targets,
-measures the dominant kernel's per-launch time with HIP events for the roofline figure and
-times the CPU oracle on a bounded sample.
+measures the dominant kernel's per-launch time with HIP events for the roofline figure, serves
+a seeded Poisson stream over a mix of resident maps, and times the C/OpenMP restatement of the
+reference traversal on the host cores (the CPU baseline).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2]
 """
@@ -43,9 +44,7 @@ def parse():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "bf16x2", "bf16x3"],
                     help="batched form: FP32 (mixed), or dense couplings on split BF16 (2 or 3 planes)")
     ap.add_argument("--plain-rhs", type=int, default=2, help="columns timed in the plain FP64 form")
-    ap.add_argument("--cpu-n", type=int, default=0,
-                    help="points of the CPU oracle sample (default N/8: the same cube one tree level coarser)")
-    ap.add_argument("--cpu-reps", type=int, default=2, help="products timed in the CPU oracle sample")
+    ap.add_argument("--cpu-reps", type=int, default=2, help="products timed by the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=4000,
@@ -111,26 +110,34 @@ def traffic_of(kernel, n, p, nrhs):
     return d["traffic_bytes_per_launch"] if d.get("kernel") == kernel else None
 
 
-def cpu_baseline(n, reps=1, seed_pts=0, seed_rhs=1, p=6):
-    """The oracle (NumPy port of the reference traversal, FP64, one thread) on a bounded sample.
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    Returns the mean time of one single-column product over ``reps`` products."""
-    import numpy as np
-    from threadpoolctl import threadpool_limits
 
+def cpu_baseline(arrays, x, reps=2):
+    """The C/OpenMP restatement of the reference traversal (oracle/h2cpu.c: FP64, one column,
+    every OpenMP thread of the host) on the same core data and the same column as the GPU:
+    one warm-up product, then the mean of ``reps`` timed products.  Returns (seconds per product,
+    threads, the product)."""
     sys.path.insert(0, ROOT)
-    from oracle import build as OB
-    from oracle import h2 as O
+    from oracle.cpu import CpuH2
 
-    pts = np.random.default_rng(seed_pts).random((n, 3))
-    xs = np.random.default_rng(seed_rhs).standard_normal((reps, n))
-    core = OB.build_core(pts, p=p, nodekind="equi", m2l_kind="toeplitz", leaf_size=64)
-    with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        for x in xs:
-            O.mul(core, x)
-        dt = time.perf_counter() - t0
-    return dt / reps
+    cpu = CpuH2(arrays)
+    y = cpu.mul(x)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y = cpu.mul(x)
+    dt = (time.perf_counter() - t0) / reps
+    threads = cpu.threads
+    cpu.close()
+    return dt, threads, y
 
 
 def helmholtz_main(args):
@@ -526,6 +533,9 @@ def main():
     roof["kernel"] = f"{dph} level {dlv}"
     roof["launch_ms"] = round(dms, 3)
 
+    x0_host = X64[:, 0].cpu().numpy().copy()
+    y0_plain_host = Y64[:, 0].cpu().numpy().copy()
+
     # serving: the resident map mix under a Poisson stream of single-column requests
     serve = None
     if args.serve_requests > 0:
@@ -537,15 +547,17 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        # N/8 points in the unit cube have the leaf occupancy of N points one tree level coarser,
-        # so the work per point matches and the time scales linearly in N.
-        cpu_n = args.cpu_n or N // 8
-        dt_cpu = cpu_baseline(cpu_n, args.cpu_reps)
-        cpu_rhs_s = 1.0 / (dt_cpu * (N / cpu_n))
-        cpu = {"value": round(cpu_rhs_s, 6), "unit": "RHS/s", "cores": 1, "kind": "port",
-               "sample": f"oracle (NumPy, FP64) forward product, {args.cpu_reps} products of 1 RHS, N={cpu_n} "
-                         f"same distribution/p/leaf size, {dt_cpu:.1f}s each, scaled linearly to N={N}"}
-        log(f"cpu oracle sample: {dt_cpu:.1f}s per product at N={cpu_n}")
+        import numpy as np
+
+        dt_cpu, threads, y_cpu = cpu_baseline(arrays, x0_host, args.cpu_reps)
+        err_cpu = float(np.linalg.norm(y_cpu - y0_plain_host) / np.linalg.norm(y0_plain_host))
+        cpu = {"value": round(1.0 / dt_cpu, 4), "unit": "RHS/s", "cores": threads, "kind": "port",
+               "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+               "sample": f"C/OpenMP restatement of the reference traversal (FP64), the full N={N} forward product "
+                         f"of one column on {threads} threads, mean of {args.cpu_reps} after a warm-up "
+                         f"({dt_cpu:.2f}s each); same core data and column as the GPU",
+               "rel_err_vs_gpu_plain_fp64": err_cpu}
+        log(f"cpu restatement: {dt_cpu:.2f}s per product on {threads} threads; GPU plain FP64 vs CPU {err_cpu:.2e}")
 
     out = {
         "metric": "H2 matvec RHS/s, N=2M p=6 32 RHS, rel.err<=1e-6 vs FP64, vs plain; p99 latency",
