@@ -95,7 +95,7 @@ void validate_side(const h2g_side_desc &s, const char *name, int K) {
 void validate_desc(const h2g_core_desc &d) {
     if (d.dtype != H2G_F64 && d.dtype != H2G_C64) fail(H2G_EINVAL, "core dtype must be H2G_F64 or H2G_C64");
     if (d.K < 1) fail(H2G_EINVAL, "K must be positive");
-    if (d.m2l_kind < 0 || d.m2l_kind > 3) fail(H2G_EINVAL, "unknown m2l_kind");
+    if (d.m2l_kind < 0 || d.m2l_kind > H2G_M2L_DIAGONAL) fail(H2G_EINVAL, "unknown m2l_kind");
     const bool toep = d.m2l_kind == H2G_M2L_TOEPLITZ || d.m2l_kind == H2G_M2L_SYMTOEPLITZ;
     const bool sym = d.m2l_kind == H2G_M2L_SYMDENSE || d.m2l_kind == H2G_M2L_SYMTOEPLITZ;
     if (toep) {
@@ -1040,7 +1040,12 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
         if (n == 0) continue;
         const int32_t *recv = L + P.m2l_recv[l].begin;
         const int64_t *iptr = P.m2l_ptr.as<int64_t>() + P.m2l_base[l];
-        if (!is_toeplitz(c)) {
+        if (c.m2l_kind == H2G_M2L_DIAGONAL) {
+            w.mark_begin(5, l);
+            launch_diag_m2l(cd, conj, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(), ops, K, w.mom.ptr,
+                            w.loc.ptr, R, s);
+            w.mark_end();
+        } else if (!is_toeplitz(c)) {
             w.mark_begin(5, l);
             if (c.m2l_dense.ok)
                 launch_dense(c.m2l_dense, op == H2G_OP_N ? 0 : (op == H2G_OP_T || !cmap ? 1 : 2),
@@ -1374,7 +1379,8 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
                 c->trial = &c->trial_own;
             }
             const bool toep = is_toeplitz(*c);
-            const size_t opelems = toep ? (size_t)d.n_ops * c->L3 : (size_t)d.n_ops * d.K * d.K;
+            const bool diag = d.m2l_kind == H2G_M2L_DIAGONAL;
+            const size_t opelems = toep ? (size_t)d.n_ops * c->L3 : (size_t)d.n_ops * d.K * (diag ? 1 : d.K);
             const int opdtype = toep ? H2G_C64 : d.dtype;
             c->ops.upload(d.ops, opelems * elem_size(opdtype));
             if (want32) {
@@ -1420,7 +1426,7 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
             build_near(*c, d);
             // dense kinds: the distinct (translation id, symmetry) operators of the pairs
             std::vector<int32_t> pair_fop, fop_op, fop_sym;
-            if (!toep && d.n_pairs > 0) {
+            if (!toep && !diag && d.n_pairs > 0) {
                 std::map<std::pair<int32_t, int32_t>, int32_t> fops;
                 pair_fop.resize(d.n_pairs);
                 for (int64_t k = 0; k < d.n_pairs; ++k) {

@@ -451,6 +451,11 @@ void launch_hadamard(int cdtype, bool conj, int n_recv, const int64_t *item_ptr,
                      const int32_t *item_op, const int32_t *item_sym, const int32_t *perms, const void *ops,
                      int L3, int src_slot_base, const void *spec_in, void *spec_out, int R, hipStream_t s);
 void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+// Diagonal couplings: out[t][r][k] = sum over the items of t of c(op)[k] in[src][r][k], c conjugated
+// for the adjoint (overwrites out[t])
+void launch_diag_m2l(int cdtype, bool conj, int n_recv, const int32_t *recv, const int64_t *item_ptr,
+                     const int32_t *item_src, const int32_t *item_op, const void *ops, int K, const void *in, void *out,
+                     int R, hipStream_t s);
 // M2M of all children of each target (shared factors: Side::sep_shared), overwriting the target
 void launch_sep_m2m(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
                     const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,

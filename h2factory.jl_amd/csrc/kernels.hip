@@ -554,11 +554,43 @@ __global__ void fill_zero_kernel(uint32_t *__restrict__ p, int64_t n) {
     default: throw Error{H2G_EINVAL, "unknown vector element type"};          \
     }
 
+// Diagonal (elementwise) couplings, one workgroup per receiving box, threads over (r, k): the
+// coefficients of a pair are read once per column block from L2, the moments stream from HBM.
+template <class T>
+__global__ void diag_m2l_kernel(const int32_t *__restrict__ recv, const int64_t *__restrict__ item_ptr,
+                                const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_op,
+                                const T *__restrict__ ops, int K, const T *__restrict__ in, T *__restrict__ out,
+                                int R, bool conj) {
+    const int ti = blockIdx.x;
+    const int t = recv[ti];
+    const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
+    for (int e = threadIdx.x; e < R * K; e += blockDim.x) {
+        const int r = e / K, k = e - r * K;
+        T acc = zero_v<T>();
+        for (int64_t it = i0; it < i1; ++it) {
+            const T c = maybe_conj(ops[(int64_t)item_op[it] * K + k], conj);
+            fmacc(acc, c, in[((int64_t)item_src[it] * R + r) * K + k]);
+        }
+        out[((int64_t)t * R + r) * K + k] = acc;
+    }
+}
+
 // Zero fill by kernel (usable under stream capture, unlike a memset node).
 void launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
     const int64_t n = (int64_t)(bytes / 4);
     if (n == 0) return;
     hipLaunchKernelGGL((fill_zero_kernel), grid1(n), TPB, 0, s, (uint32_t *)p, n);
+    H2G_CHECK_LAUNCH();
+}
+
+void launch_diag_m2l(int cdtype, bool conj, int n_recv, const int32_t *recv, const int64_t *item_ptr,
+                     const int32_t *item_src, const int32_t *item_op, const void *ops, int K, const void *in, void *out,
+                     int R, hipStream_t s) {
+    if (n_recv == 0) return;
+    DISPATCH_T(cdtype, {
+        hipLaunchKernelGGL((diag_m2l_kernel<T>), n_recv, TPB, 0, s, recv, item_ptr, item_src, item_op, (const T *)ops, K,
+                           (const T *)in, (T *)out, R, conj);
+    });
     H2G_CHECK_LAUNCH();
 }
 

@@ -26,7 +26,7 @@ import numpy as np
 from . import tree as T
 
 FOUR_PI = 4.0 * np.pi
-KINDS = {"dense": 0, "toeplitz": 1, "symdense": 2, "symtoeplitz": 3}
+KINDS = {"dense": 0, "toeplitz": 1, "symdense": 2, "symtoeplitz": 3, "diagonal": 4}
 
 
 def nodes1d(p: int, kind: str) -> np.ndarray:

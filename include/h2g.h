@@ -58,8 +58,13 @@ enum { H2G_MUL = 3, H2G_FARMUL = 2, H2G_NEARMUL = 1 };
  *   SYMDENSE     canonical K x K matrices + grid permutation per pair
  *                (src/farinteractions/i2otranslations/symmetryreduceddense.jl:16-293)
  *   SYMTOEPLITZ  canonical spectra + spectrum permutation per pair
- *                (src/farinteractions/i2otranslations/symmetryreducedcollection.jl:11-35,356-387) */
-enum { H2G_M2L_DENSE = 0, H2G_M2L_TOEPLITZ = 1, H2G_M2L_SYMDENSE = 2, H2G_M2L_SYMTOEPLITZ = 3 };
+ *                (src/farinteractions/i2otranslations/symmetryreducedcollection.jl:11-35,356-387)
+ *   DIAGONAL     one vector of K coefficients per translation id, applied elementwise (the
+ *                MLFMA-style coupling): out (+)= coef .* in for forward and transpose,
+ *                conj(coef) .* in for the adjoint
+ *                (src/farinteractions/i2otranslations/i2otranslationcollection.jl:415-495,
+ *                 src/moments/abstractmoment.jl:420-510) */
+enum { H2G_M2L_DENSE = 0, H2G_M2L_TOEPLITZ = 1, H2G_M2L_SYMDENSE = 2, H2G_M2L_SYMTOEPLITZ = 3, H2G_M2L_DIAGONAL = 4 };
 
 /* Arithmetic of the far field in a workspace.
  *   PLAIN  everything in the core's type (Float64 / ComplexF64): the drop-in.
@@ -130,7 +135,8 @@ typedef struct h2g_core_desc {
     const int32_t *pair_sym;      /* permutation id (SYM kinds), else NULL                    */
     int64_t n_ops;
     const void *ops;              /* DENSE/SYMDENSE: n_ops K x K, core type.
-                                     TOEPLITZ/SYMTOEPLITZ: n_ops L^3 ComplexF64 spectra, x fastest */
+                                     TOEPLITZ/SYMTOEPLITZ: n_ops L^3 ComplexF64 spectra, x fastest
+                                     DIAGONAL: n_ops K-vectors, core type                      */
     int32_t n_perms;
     const int32_t *perms;         /* SYMDENSE: [n_perms][K] grid permutation (0-based):
                                        t[perm[i], perm[j]] = canonical[i, j]

@@ -111,7 +111,8 @@ Core._galerkin_fill = _galerkin_fill
 
 def random_core(points_test, points_trial=None, p=2, m2l_kind="dense", leaf_size=8,
                 seed=0, complex_data=True, alpha=1.0):
-    """Same tree/plan structure with random (complex) operators of every kind."""
+    """Same tree/plan structure with random (complex) operators of every kind, including the
+    diagonal (elementwise, MLFMA-style) couplings: one random K-vector per translation id."""
     rng = np.random.default_rng(seed)
     test = build_tree(points_test, leaf_size)
     trial = test if points_trial is None else build_tree(points_trial, leaf_size)
@@ -150,6 +151,8 @@ def random_core(points_test, points_trial=None, p=2, m2l_kind="dense", leaf_size
             keys[key] = len(ops)
             if m2l_kind in ("dense", "symdense"):
                 ops.append(rnd(K, K) / K)
+            elif m2l_kind == "diagonal":
+                ops.append(rnd(K))
             else:
                 ops.append(np.fft.fftn(rnd(L, L, L) / K))
         pair_op[k] = keys[key]
@@ -162,8 +165,8 @@ def random_core(points_test, points_trial=None, p=2, m2l_kind="dense", leaf_size
     if ops:
         ops = np.stack(ops)
     else:
-        ops = np.zeros((0, K, K) if m2l_kind in ("dense", "symdense") else (0, L, L, L),
-                       dtype=np.complex128 if (complex_data or "toeplitz" in m2l_kind) else np.float64)
+        shape = (0, K, K) if m2l_kind in ("dense", "symdense") else ((0, K) if m2l_kind == "diagonal" else (0, L, L, L))
+        ops = np.zeros(shape, dtype=np.complex128 if (complex_data or "toeplitz" in m2l_kind) else np.float64)
     npairs = near_pairs(test, trial)
     vals = [rnd(test.end[b] - test.begin[b], trial.end[c] - trial.begin[c]) for b, c in npairs]
     ct = coll(test)

@@ -62,8 +62,8 @@ class Core:
     transfers_trial: list
     pairs: np.ndarray                # (P, 2) forward (test box, trial box)
     pair_op: np.ndarray              # (P,)
-    m2l_kind: str                    # dense | toeplitz | symdense | symtoeplitz
-    ops: np.ndarray                  # dense (n, K, K) | spectra (n, L, L, L)
+    m2l_kind: str                    # dense | toeplitz | symdense | symtoeplitz | diagonal
+    ops: np.ndarray                  # dense (n, K, K) | spectra (n, L, L, L) | diagonal (n, K)
     pair_sym: np.ndarray = None      # (P,) permutation id (symmetry-reduced kinds)
     perms: np.ndarray = None         # (n_sym, K) grid perms | (n_sym, L^3) spectrum perms
     near_pairs: np.ndarray = None    # (B, 2) (test leaf, trial leaf)
@@ -96,6 +96,8 @@ def _dense_coupling(core: Core, k: int):
         t = np.empty_like(c)
         t[np.ix_(pi, pi)] = c
         return t
+    if core.m2l_kind == "diagonal":
+        return np.diag(core.ops[core.pair_op[k]])
     raise ValueError(core.m2l_kind)
 
 
@@ -107,7 +109,13 @@ def _spectrum(core: Core, k: int):
 
 
 def _apply_dense_pair(core, k, m, op):
-    """``op(M)*m`` following the dense / symmetry-reduced dense kernels."""
+    """``op(M)*m`` following the dense / symmetry-reduced dense / diagonal kernels."""
+    if core.m2l_kind == "diagonal":
+        # elementwise coupling (i2otranslationcollection.jl:415-495, abstractmoment.jl:420-510):
+        # forward and transpose use the coefficients as stored, the adjoint conjugates them
+        d = core.ops[core.pair_op[k]]
+        d = np.conj(d) if op == "C" else d
+        return d[:, None] * m if m.ndim == 2 else d * m
     if core.m2l_kind == "dense":
         Mk = core.ops[core.pair_op[k]]
         if op == "N":
@@ -302,7 +310,7 @@ def _in_basis(side: Tree, coll, transfers, b):
 
 def pair_matrix(core: Core, k: int):
     """Dense forward coupling of pair k, built without the FFT path for Toeplitz kinds."""
-    if core.m2l_kind in ("dense", "symdense"):
+    if core.m2l_kind in ("dense", "symdense", "diagonal"):
         return _dense_coupling(core, k)
     kappa = np.fft.ifftn(_spectrum(core, k))
     return M.toeplitz_dense(kappa, core.p)

@@ -20,13 +20,15 @@ def _side(tree, coll, transfers, dt):
 
 def to_arrays(core) -> MapArrays:
     cplx = np.iscomplexobj(core.coll_test) or np.iscomplexobj(core.transfers_test[1][0] if len(core.transfers_test) > 1 else 0) \
-        or (core.m2l_kind in ("dense", "symdense") and np.iscomplexobj(core.ops)) \
+        or (core.m2l_kind in ("dense", "symdense", "diagonal") and np.iscomplexobj(core.ops)) \
         or any(np.iscomplexobj(v) for v in core.near_values) or np.iscomplexobj(core.alpha) and complex(core.alpha).imag != 0
     dt = np.complex128 if cplx else np.float64
     ts = _side(core.test, core.coll_test, core.transfers_test, dt)
     cs = ts if core.galerkin else _side(core.trial, core.coll_trial, core.transfers_trial, dt)
     if core.m2l_kind in ("dense", "symdense"):
         ops = np.ascontiguousarray(np.swapaxes(np.asarray(core.ops), 1, 2), dtype=dt)
+    elif core.m2l_kind == "diagonal":
+        ops = np.ascontiguousarray(core.ops, dtype=dt)
     else:
         ops = np.ascontiguousarray(core.ops, dtype=np.complex128)
     sizes = np.array([v.size for v in core.near_values], dtype=np.int64)

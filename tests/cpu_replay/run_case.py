@@ -53,6 +53,15 @@ def random_petrov_symdense():
     return core, x
 
 
+def random_petrov_diagonal():
+    """Random complex operators with diagonal (elementwise) couplings, separate trees."""
+    rng = np.random.default_rng(9)
+    core = OB.random_core(rng.random((240, 3)), rng.random((200, 3)), p=2, m2l_kind="diagonal", leaf_size=8,
+                          seed=12, alpha=1.1 + 0.4j)
+    x = rng.standard_normal((200, 2)) + 1j * rng.standard_normal((200, 2))
+    return core, x
+
+
 def _products(name, A, core, x, ops, fns=("mul",)):
     ok = True
     for op in ops:
@@ -115,7 +124,8 @@ def main(name):
         c.close()
         sys.exit(0 if ok else 1)
     else:
-        core, x = {"helmholtz_sphere": helmholtz_sphere, "random_petrov_symdense": random_petrov_symdense}[name]()
+        core, x = {"helmholtz_sphere": helmholtz_sphere, "random_petrov_symdense": random_petrov_symdense,
+                   "random_petrov_diagonal": random_petrov_diagonal}[name]()
         c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
         A = h2g.H2Workspace(c, max_rhs=2)
         ok = _products(name, A, core, x, "NTC", ("mul", "farmul", "nearmul"))

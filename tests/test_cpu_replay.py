@@ -26,7 +26,8 @@ def replay_lib():
     return B.build(), B.asan_runtime()
 
 
-@pytest.mark.parametrize("case", ["helmholtz_sphere", "laplace_device_fill", "random_petrov_symdense", "noalloc"])
+@pytest.mark.parametrize("case", ["helmholtz_sphere", "laplace_device_fill", "random_petrov_symdense",
+                                  "random_petrov_diagonal", "noalloc"])
 def test_replay_in_bounds_and_equal_to_oracle(replay_lib, case):
     """helmholtz_sphere is the map of the skipped GPU Helmholtz test (ComplexF64, p = 3 Chebyshev,
     dense couplings); laplace_device_fill compares the device-filled near values and collections

@@ -38,7 +38,7 @@ PTS_A = RNG.random((260, 3))
 PTS_B = RNG.random((190, 3))
 
 
-@pytest.mark.parametrize("kind", ["dense", "toeplitz", "symdense", "symtoeplitz"])
+@pytest.mark.parametrize("kind", ["dense", "toeplitz", "symdense", "symtoeplitz", "diagonal"])
 @pytest.mark.parametrize("petrov", [False, True])
 def test_random_core_all_products(gpu, kind, petrov):
     """Random complex operators: every orientation x product x 3/5-argument form."""

@@ -19,7 +19,7 @@ PA = RNG.random((300, 3))
 PB = RNG.random((240, 3))
 
 
-@pytest.mark.parametrize("kind", ["dense", "toeplitz", "symdense", "symtoeplitz"])
+@pytest.mark.parametrize("kind", ["dense", "toeplitz", "symdense", "symtoeplitz", "diagonal"])
 @pytest.mark.parametrize("petrov", [False, True])
 def test_traversal_equals_dense_assembly(kind, petrov):
     """Random complex operators: traversal in every orientation == composite-basis assembly."""
