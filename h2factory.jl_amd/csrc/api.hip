@@ -1,5 +1,6 @@
 // C ABI of libh2g: core / workspace lifetime, validation, traversal plans and the matvec.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1300,7 +1301,7 @@ void Workspace::mark_begin(int phase, int level) {
         events.emplace_back(a, b);
     }
     H2G_HIP(hipEventRecord(events[n_events_used].first, stream));
-    timings.push_back(h2g_timing_record{phase, level, 0.0f});
+    timings.push_back(h2g_timing_record{phase, level, 0.0f, cur_op, -1});
 }
 
 void Workspace::mark_end() {
@@ -1424,6 +1425,14 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
                 c->twiddle.upload_vec(tw);
             }
             build_near(*c, d);
+            {
+                std::set<std::pair<int32_t, int32_t>> uniq;
+                for (int64_t k = 0; k < d.n_pairs; ++k) {
+                    uniq.emplace(d.pair_op[k], d.pair_sym ? d.pair_sym[k] : -1);
+                    if (d.galerkin && d.pair_test[k] == d.pair_trial[k]) ++c->n_self_pairs;
+                }
+                c->n_unique_translations = (int64_t)uniq.size();
+            }
             // dense kinds: the distinct (translation id, symmetry) operators of the pairs
             std::vector<int32_t> pair_fop, fop_op, fop_sym;
             if (!toep && !diag && d.n_pairs > 0) {
@@ -1459,6 +1468,73 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
             throw;
         }
         *out = c;
+    });
+}
+
+int h2g_core_summary(const h2g_core *core, h2g_summary *out) {
+    if (!core || !out) return H2G_EINVAL;
+    const Core &c = *core;
+    h2g_summary s{};
+    s.rows = c.test.n_dofs;
+    s.cols = c.trial->n_dofs;
+    s.dtype = c.dtype;
+    s.m2l_kind = c.m2l_kind;
+    s.galerkin = c.galerkin ? 1 : 0;
+    s.n_levels = c.test.n_levels;
+    s.n_nodes = c.test.n_boxes;
+    int64_t mn = INT64_MAX, mx = 0, tot = 0;
+    for (int b = 0; b < c.test.n_boxes; ++b)
+        if (c.test.is_leaf[b]) {
+            const int64_t n = c.test.end[b] - c.test.begin[b];
+            ++s.n_leaves;
+            mn = std::min(mn, n);
+            mx = std::max(mx, n);
+            tot += n;
+        }
+    s.min_leaf_dofs = s.n_leaves ? mn : 0;
+    s.max_leaf_dofs = mx;
+    s.mean_leaf_dofs = s.n_leaves ? (double)tot / s.n_leaves : 0.0;
+    s.far_pairs = c.n_pairs;
+    s.unique_translations = c.n_unique_translations;
+    s.reuse = c.n_unique_translations ? (double)c.n_pairs / (double)c.n_unique_translations : NAN;
+    s.near_blocks = c.n_near_blocks;
+    s.near_values = c.n_near_values;
+    for (const LatticeLevel &LL : c.fwd.lattice) s.lattice_levels += LL.ok ? 1 : 0;
+    s.dense_mfma = c.m2l_dense.ok ? 1 : 0;
+    s.device_bytes = c.device_bytes();
+    *out = s;
+    return H2G_OK;
+}
+
+int h2g_core_sanitycheck(const h2g_core *core, h2g_issue *out, int32_t max, int32_t *n) {
+    if (!core || !n) return H2G_EINVAL;
+    return guarded([&] {
+    const Core &c = *core;
+    H2G_HIP(hipSetDevice(c.device));
+    std::vector<h2g_issue> is;
+    if (c.n_pairs == 0) is.push_back({1, H2G_ISSUE_NO_FAR_PAIRS, 0, -1});
+    if (c.n_self_pairs) is.push_back({2, H2G_ISSUE_SELF_FAR_PAIR, c.n_self_pairs, -1});
+    // operator ids and moments: a core is only created when every pair resolves (validate_desc)
+    // and every box on a far pair's side gets a moment of K coefficients; the plan is re-checked
+    // here from the orientation's work lists: each receiving box and each partner must be a box
+    // whose moment the passes produce
+    for (const OrientPlan *P : {&c.fwd, &c.rev}) {
+        std::vector<int64_t> ptr(P->m2l_ptr.bytes / sizeof(int64_t));
+        std::vector<int32_t> src(P->m2l_src.bytes / sizeof(int32_t));
+        if (!ptr.empty()) H2G_HIP(hipMemcpy(ptr.data(), P->m2l_ptr.ptr, P->m2l_ptr.bytes, hipMemcpyDeviceToHost));
+        if (!src.empty()) H2G_HIP(hipMemcpy(src.data(), P->m2l_src.ptr, P->m2l_src.bytes, hipMemcpyDeviceToHost));
+        int64_t missing = 0;
+        for (int32_t s : src)
+            if (s < 0 || s >= P->src->n_boxes) ++missing;
+        if (missing) is.push_back({2, H2G_ISSUE_MISSING_MOMENT, missing, -1});
+        if (P == &c.fwd && is_toeplitz(c))
+            for (int l = 0; l < (int)P->m2l_recv.size(); ++l)
+                if (P->m2l_recv[l].count && !(l < (int)P->lattice.size() && P->lattice[l].ok))
+                    is.push_back({0, H2G_ISSUE_GENERIC_COUPLING_LEVEL, P->m2l_recv[l].count, l});
+    }
+    *n = (int32_t)is.size();
+    if (out)
+        for (int32_t i = 0; i < std::min(max, *n); ++i) out[i] = is[i];
     });
 }
 
@@ -1666,6 +1742,7 @@ int h2g_matvec(h2g_ws *ws, int op, int which, int dtype, int64_t nrhs, const voi
         }
         ws->timings.clear();
         ws->n_events_used = 0;
+        ws->cur_op = op;
         run_passes(*ws, op, which, dtype, nrhs, dx, lx, dy, ly, a, b, five, true, ws->opts.use_graph && !ws->opts.timing);
         if (host)
             H2G_HIP(hipMemcpy2DAsync(y, ldy * xe, ws->yhost.ptr, n_out * xe, n_out * xe, nrhs, hipMemcpyDeviceToHost, s));

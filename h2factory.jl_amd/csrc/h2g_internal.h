@@ -377,6 +377,8 @@ struct Core {
     NearPlan near_fwd_plan, near_rev_plan;
     cplx<double> alpha{1.0, 0.0};
     int precisions = 1;
+    // counts for the diagnostics (h2g_core_summary / h2g_core_sanitycheck)
+    int64_t n_unique_translations = 0, n_self_pairs = 0, n_bad_ops = 0;
     // a workspace precision other than PLAIN is supported (FP32 copies of the operators)
     bool reduced_precisions() const { return (precisions & ~(1 << H2G_PREC_PLAIN)) != 0; }
     OrientPlan fwd, rev;
@@ -414,6 +416,7 @@ struct Workspace {
     DevBuf scal;          // unused placeholder
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<h2g_timing_record> timings;
+    int cur_op = 0;  // orientation of the product being recorded
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     size_t n_events_used = 0;
     // records [start, stop) of one phase when opts.timing is set (eager passes only)

@@ -26,7 +26,8 @@ EXPORTED = [
     "h2g_core_create", "h2g_core_retain", "h2g_core_release", "h2g_core_size", "h2g_core_device_bytes",
     "h2g_ws_create", "h2g_ws_destroy", "h2g_ws_device_bytes", "h2g_ws_stream", "h2g_ws_acquire",
     "h2g_ws_release", "h2g_matvec", "h2g_ws_sync", "h2g_ws_timing", "h2g_last_error", "h2g_version",
-    "h2g_method_fill", "h2g_ws_query", "h2g_matvec_noalloc", "h2g_ws_set_stream",
+    "h2g_method_fill", "h2g_ws_query", "h2g_matvec_noalloc", "h2g_ws_set_stream", "h2g_core_summary",
+    "h2g_core_sanitycheck",
 ]
 
 VP = C.c_void_p
@@ -82,7 +83,21 @@ class WsOpts(C.Structure):
 
 
 class TimingRecord(C.Structure):
-    _fields_ = [("phase", C.c_int32), ("level", C.c_int32), ("ms", C.c_float)]
+    _fields_ = [("phase", C.c_int32), ("level", C.c_int32), ("ms", C.c_float), ("mode", C.c_int32),
+                ("node", C.c_int32)]
+
+
+class Summary(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("dtype", C.c_int32), ("m2l_kind", C.c_int32),
+                ("galerkin", C.c_int32), ("n_levels", C.c_int32), ("n_nodes", C.c_int32), ("n_leaves", C.c_int32),
+                ("min_leaf_dofs", C.c_int64), ("max_leaf_dofs", C.c_int64), ("mean_leaf_dofs", C.c_double),
+                ("far_pairs", C.c_int64), ("unique_translations", C.c_int64), ("reuse", C.c_double),
+                ("near_blocks", C.c_int64), ("near_values", C.c_int64), ("lattice_levels", C.c_int32),
+                ("dense_mfma", C.c_int32), ("device_bytes", C.c_int64)]
+
+
+class Issue(C.Structure):
+    _fields_ = [("severity", C.c_int32), ("code", C.c_int32), ("count", C.c_int64), ("level", C.c_int32)]
 
 
 _lib = None
@@ -106,6 +121,8 @@ def lib():
         L.h2g_ws_stream.argtypes = [VP]
         L.h2g_ws_stream.restype = VP
         L.h2g_ws_set_stream.argtypes = [VP, VP]
+        L.h2g_core_summary.argtypes = [VP, C.POINTER(Summary)]
+        L.h2g_core_sanitycheck.argtypes = [VP, C.POINTER(Issue), C.c_int32, I32P]
         L.h2g_ws_acquire.argtypes = [VP]
         L.h2g_ws_release.argtypes = [VP]
         L.h2g_matvec.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.c_int64, VP, C.c_int64, VP, C.c_int64,

@@ -175,14 +175,52 @@ typedef struct h2g_ws_opts {
 } h2g_ws_opts;
 
 /* One per-phase timing record, shaped like the reference's TimingRecord
- * (src/diagnostics/timing.jl:11-88). */
+ * (src/diagnostics/timing.jl:11-20: operation, level, node, mode, seconds). */
 typedef struct h2g_timing_record {
-    int32_t phase;                /* 0 gather, 1 near, 2 p2m, 3 m2m, 4 m2l modulate-in (FFT),
-                                     5 m2l apply (Hadamard / dense GEMM), 6 m2l modulate-out (iFFT),
-                                     7 l2l, 8 l2p, 9 scatter                                    */
-    int32_t level;                /* -1 for whole-vector phases                               */
+    int32_t phase;                /* the operation: 0 gather, 1 near, 2 p2m, 3 m2m, 4 m2l
+                                     modulate-in (FFT), 5 m2l apply (Hadamard / dense GEMM),
+                                     6 m2l modulate-out (iFFT), 7 l2l, 8 l2p, 9 scatter          */
+    int32_t level;                /* -1 for whole-vector phases (missing)                     */
     float ms;
+    int32_t mode;                 /* orientation of the product: H2G_OP_N / T / C             */
+    int32_t node;                 /* -1: records are per level, not per node (missing)        */
 } h2g_timing_record;
+
+/* Structure of a core: the reference's h2summary and translation_counts
+ * (src/diagnostics/structure.jl:40-118); the tree fields describe the test side. */
+typedef struct h2g_summary {
+    int64_t rows, cols;
+    int32_t dtype, m2l_kind, galerkin, n_levels;
+    int32_t n_nodes, n_leaves;
+    int64_t min_leaf_dofs, max_leaf_dofs;
+    double mean_leaf_dofs;
+    int64_t far_pairs;            /* M2L pairs                                                */
+    int64_t unique_translations;  /* distinct coupling operators (translation id, symmetry)  */
+    double reuse;                 /* far_pairs / unique_translations (NaN without pairs)     */
+    int64_t near_blocks, near_values;
+    int32_t lattice_levels;       /* levels whose Toeplitz couplings run the lattice kernels */
+    int32_t dense_mfma;           /* dense couplings on the MFMA kernels (1) or the box GEMM */
+    int64_t device_bytes;
+} h2g_summary;
+int h2g_core_summary(const h2g_core *core, h2g_summary *out);
+
+/* Read-only checks of a core, like the reference's checkplans / checkmoments / sanitycheck
+ * (src/diagnostics/structure.jl:150-310): one record per issue found. */
+enum {
+    H2G_ISSUE_NO_FAR_PAIRS = 1,           /* warning: the far field is empty                  */
+    H2G_ISSUE_SELF_FAR_PAIR = 2,          /* error: a node translated with itself             */
+    H2G_ISSUE_UNRESOLVED_TRANSLATION = 3, /* error: a pair's operator id is out of range      */
+    H2G_ISSUE_MISSING_MOMENT = 4,         /* error: a far-pair node without a moment           */
+    H2G_ISSUE_NONUNIFORM_MOMENT_SIZE = 5, /* error: moments of one level differ in size        */
+    H2G_ISSUE_GENERIC_COUPLING_LEVEL = 6  /* info: a Toeplitz level on the per-pair kernels   */
+};
+typedef struct h2g_issue {
+    int32_t severity;             /* 0 info, 1 warning, 2 error                               */
+    int32_t code;                 /* H2G_ISSUE_*                                              */
+    int64_t count;                /* offending pairs / nodes / levels                         */
+    int32_t level;                /* the level concerned, or -1                               */
+} h2g_issue;
+int h2g_core_sanitycheck(const h2g_core *core, h2g_issue *out, int32_t max, int32_t *n);
 
 /* --- core ------------------------------------------------------------------------------ */
 /* Builds the read-only device core (GalerkinH2Core / PetrovGalerkinH2Core). */

@@ -19,6 +19,7 @@ import h2g._lib as H  # noqa: E402
 H.LIB_PATH = os.path.join(HERE, "_build", "libh2g_replay.so")
 
 import h2g  # noqa: E402
+import h2g._lib as _H  # noqa: E402,F401
 from convert import to_arrays  # noqa: E402
 from oracle import build as OB  # noqa: E402
 from oracle import h2 as O  # noqa: E402
@@ -99,6 +100,43 @@ def main(name):
         A = h2g.H2Workspace(c, max_rhs=3)
         x = np.random.default_rng(9).standard_normal((1200, 3))
         ok = _products(name, A, core, x, "NT", ("mul", "nearmul", "farmul"))
+    elif name == "diagnostics":
+        # h2summary / translation_counts / sanitycheck against the oracle's tree and pair lists
+        from h2g import diagnostics as D
+
+        pts = np.random.default_rng(3).random((1500, 3))
+        core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="symdense", leaf_size=40)
+        c = h2g.GpuH2Core(to_arrays(core))
+        A = h2g.H2Workspace(c)
+        s = D.h2summary(A)
+        tr = core.test
+        dofs = (tr.end - tr.begin)[tr.is_leaf.astype(bool)]
+        uniq = len({(int(o), int(q)) for o, q in zip(core.pair_op, core.pair_sym)})
+        ok = (s.size == (1500, 1500) and s.numlevels == tr.n_levels and s.numleaves == int(tr.is_leaf.sum())
+              and s.numnodes == tr.n_boxes and s.minleafdofs == dofs.min() and s.maxleafdofs == dofs.max()
+              and abs(s.meanleafdofs - dofs.mean()) < 1e-12 and s.farpairs == len(core.pairs)
+              and s.uniquetranslations == uniq and s.nearblocks == len(core.near_pairs))
+        tc = D.translation_counts(A)
+        ok &= tc.farpairs == len(core.pairs) and abs(tc.reuse - len(core.pairs) / uniq) < 1e-12
+        ok &= D.sanitycheck(A) == []
+        print(f"{name}: summary {s}, counts {tc}, vs oracle {'ok' if ok else 'MISMATCH'}", flush=True)
+        A.close()
+        c.close()
+        # no far pair: a warning; a pair translating a node with itself: an error
+        c2 = h2g.GpuH2Core(to_arrays(OB.no_far_core()))
+        iss = D.sanitycheck(c2)
+        ok &= [i.code for i in iss] == ["no_far_pairs"] and iss[0].severity == "warning"
+        bad = OB.random_core(np.random.default_rng(4).random((200, 3)), p=2, m2l_kind="dense", leaf_size=8, seed=1)
+        b0 = int(bad.pairs[0, 0])
+        bad.pairs = np.vstack([bad.pairs, [[b0, b0]]])
+        bad.pair_op = np.concatenate([bad.pair_op, [0]])
+        c3 = h2g.GpuH2Core(to_arrays(bad))
+        iss = D.sanitycheck(c3)
+        ok &= any(i.code == "self_far_pair" and i.severity == "error" and i.count == 1 for i in iss)
+        print(f"{name}: no-far {D.sanitycheck(c2)}, self pair {iss}", flush=True)
+        c2.close()
+        c3.close()
+        sys.exit(0 if ok else 1)
     elif name == "noalloc":
         # h2g_matvec_noalloc (buffers in the static scratch) against the workspace product and the
         # oracle: two calls in a row, the second on a smaller map with fewer columns

@@ -503,3 +503,25 @@ def test_helmholtz_unit_sphere_p6(gpu):
             ref = orient(plain, op) @ x32.astype(np.complex128)
             assert relerr(orient(ws, op) @ x32, ref) < 1e-6, (prec, op)
         ws.close()
+
+
+def test_diagnostics_and_timing_records(gpu):
+    """h2summary on the lattice and dense-MFMA paths; timing records carry the orientation."""
+    import h2g
+    from convert import to_arrays
+    from h2g import diagnostics as D
+
+    pts = np.random.default_rng(30).random((4000, 3))
+    lat = OB.build_core(pts, p=4, nodekind="equi", m2l_kind="toeplitz", leaf_size=32)
+    c = h2g.GpuH2Core(to_arrays(lat), precisions=("plain", "mixed"))
+    s = D.h2summary(c)
+    assert s.latticelevels > 0 and s.farpairs == len(lat.pairs) and s.m2l_kind == "toeplitz"
+    assert D.sanitycheck(c) == []
+    ws = h2g.H2Workspace(c, max_rhs=2, timing=True)
+    x = np.random.default_rng(1).standard_normal((4000, 2))
+    h2g.adjoint(ws) @ x
+    recs = D.timing_records(ws)
+    assert recs and all(r.mode == "adjoint" for r in recs)
+    assert {"near", "m2l_apply", "l2p"} <= {r.operation for r in recs}
+    dense = OB.build_core(pts[:2000], p=3, nodekind="cheb", m2l_kind="dense", leaf_size=32)
+    assert D.h2summary(h2g.GpuH2Core(to_arrays(dense))).dense_mfma
