@@ -157,14 +157,15 @@ class _ResidentMap:
 
 
 def estimate_core_bytes(arrays, precisions=("plain", "mixed")):
-    """Device bytes of a core before it is built (admission check): near values (and the FP32 copy
-    of real ones), collections, couplings, transfers, and for dense couplings and non-separable
-    (complex) transfers the MFMA fragment matrices per orientation and representation."""
+    """Device bytes of a core before it is built (admission check): near values (and for real maps
+    with a reduced precision their FP32 row-major copies per orientation, columns padded to 16),
+    collections, couplings, transfers, and for dense couplings and non-separable (complex)
+    transfers the MFMA fragment matrices per orientation and representation."""
     cplx = arrays.dtype == "c64"
     es = 16 if cplx else 8
     reduced = any(p != "plain" for p in precisions)
     n = arrays.test.tree.points.shape[0] + (0 if arrays.galerkin else arrays.trial.tree.points.shape[0])
-    b = arrays.n_near_values * (es + (4 if reduced and not cplx else 0))
+    b = arrays.n_near_values * (es + (2 * 4 * 1.05 if reduced and not cplx else 0))
     coll = n * arrays.K * es
     b += coll * (1.5 if reduced else 1.0)
     b += arrays.ops.nbytes * (1.5 if reduced else 1.0)
@@ -191,8 +192,6 @@ class Server:
         """``use_graph``: capture each (orientation, batch width) product of a workspace in a hipGraph
         on first use (worth it for maps whose products are launch-bound; a stream of single
         requests produces every width from 1 to ``max_batch``)."""
-        import torch
-
         self.budget = int(budget_bytes)
         self.max_batch = max_batch
         self.ws_per_map = ws_per_map
@@ -200,12 +199,21 @@ class Server:
         self.use_graph = use_graph
         self.maps = {}
         self.done = []
-        self._streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
+        self._n_streams = max(1, n_streams)
+        self._pool = None  # the streams, created with the first admitted map
         self._next_stream = 0
-        self._free_at = [0.0] * len(self._streams)  # expected drain time of each stream
+        self._free_at = [0.0] * self._n_streams  # expected drain time of each stream
         # accounting of a stream: host seconds staging batches in / results out, and the device
         # seconds the batches were expected to take (per-map cost model)
         self.host_launch_s = self.host_harvest_s = self.device_est_s = 0.0
+
+    @property
+    def _streams(self):
+        if self._pool is None:
+            import torch
+
+            self._pool = [torch.cuda.Stream() for _ in range(self._n_streams)]
+        return self._pool
 
     def _stream(self):
         s = self._streams[self._next_stream % len(self._streams)]
@@ -217,8 +225,6 @@ class Server:
 
     def add_map(self, name, arrays):
         """Admit and build a map, or raise AdmissionError when it does not fit the budget."""
-        import torch
-
         est = estimate_core_bytes(arrays)
         n = arrays.test.tree.points.shape[0]
         es = 16 if arrays.dtype == "c64" else 8
