@@ -846,13 +846,19 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
 // order[p[t+1]-1], all with the same output points), the source columns of all its blocks in
 // order, and one task per 64 output rows.  Tasks are grouped by row-tile count, largest first.
 // Source indices must fit int32; otherwise the plan stays empty and the block kernels run.
+// With `frag`, also the padded source-point tables and each task's place in the FP32 row-major
+// copy (filled by build_near_frags once the values are set).
 void build_near_plan(NearPlan &plan, bool trans, const std::vector<int64_t> &p, const std::vector<int32_t> &order,
                      const std::vector<int64_t> &rb, const std::vector<int64_t> &cb, const std::vector<int32_t> &nr,
-                     const std::vector<int32_t> &nc, const std::vector<int64_t> &off) {
+                     const std::vector<int32_t> &nc, const std::vector<int64_t> &off, bool frag) {
     plan.task.release();
     plan.col.release();
+    plan.frag.release();
+    plan.xrow.release();
+    plan.n_frag = 0;
     plan.ok = false;
     std::vector<NearCol> cols;
+    std::vector<int32_t> xrow;
     std::vector<NearTask> byrt[4];
     const int64_t ntgt = (int64_t)p.size() - 1;
     for (int64_t t = 0; t < ntgt; ++t) {
@@ -876,6 +882,10 @@ void build_near_plan(NearPlan &plan, bool trans, const std::vector<int64_t> &p, 
         const int64_t ncol = (int64_t)cols.size() - col0;
         if (ncol == 0) continue;  // nothing to add: the output rows stay zero
         if (ncol > INT32_MAX) return;
+        const int64_t xoff = (int64_t)xrow.size();
+        if (frag) {
+            for (int64_t j = 0; j < near_ncolp(ncol); ++j) xrow.push_back(cols[col0 + std::min(j, ncol - 1)].xrow);
+        }
         for (int r0 = 0; r0 < n_out; r0 += 64) {
             NearTask k;
             k.out0 = out0;
@@ -884,6 +894,8 @@ void build_near_plan(NearPlan &plan, bool trans, const std::vector<int64_t> &p, 
             k.ncol = (int32_t)ncol;
             k.row0 = r0;
             k.pad = 0;
+            k.foff = 0;
+            k.xoff = xoff;
             byrt[(k.nrow + 15) / 16 - 1].push_back(k);
         }
     }
@@ -893,6 +905,15 @@ void build_near_plan(NearPlan &plan, bool trans, const std::vector<int64_t> &p, 
         tasks.insert(tasks.end(), byrt[3 - i].begin(), byrt[3 - i].end());
     }
     plan.cls[4] = (int64_t)tasks.size();
+    if (frag) {
+        int64_t f = 0;
+        for (NearTask &k : tasks) {
+            k.foff = f;
+            f += (int64_t)k.nrow * near_ncolp(k.ncol);
+        }
+        plan.n_frag = f;
+        plan.xrow.upload_vec(xrow);
+    }
     plan.task.upload_vec(tasks);
     plan.col.upload_vec(cols);
     plan.ok = true;
@@ -907,15 +928,6 @@ void build_near(Core &c, const h2g_core_desc &d) {
         c.near_vals.upload(d.near_values, (size_t)d.n_near_values * elem_size(d.dtype));
     else
         c.near_vals.alloc((size_t)d.n_near_values * elem_size(d.dtype));
-    if (d.dtype == H2G_F64 && c.reduced_precisions() && d.n_near_values > 0) {
-        // the mixed form reads the near values rounded to FP32 (half the HBM traffic of the
-        // near field) and still multiplies and accumulates in FP64
-        c.near_vals32.alloc((size_t)d.n_near_values * sizeof(float));
-        if (d.near_values) {
-            launch_narrow(c.near_vals.as<double>(), c.near_vals32.as<float>(), d.n_near_values, nullptr);
-            H2G_HIP(hipDeviceSynchronize());
-        }
-    }
     std::vector<int64_t> rb(nb), cb(nb), off(nb);
     std::vector<int32_t> nr(nb), nc(nb);
     for (int64_t q = 0; q < nb; ++q) {
@@ -951,12 +963,16 @@ void build_near(Core &c, const h2g_core_desc &d) {
         tgt.upload_vec(targets);
         ptr.upload_vec(p);
         blk.upload_vec(order);
-        if (c.dtype == H2G_F64) build_near_plan(plan, trans, p, order, rb, cb, nr, nc, off);
+        // the mixed form of real maps reads the near values rounded to FP32, per task in row-major
+        // order (half the HBM traffic of the near field; products and sums stay FP64)
+        if (c.dtype == H2G_F64)
+            build_near_plan(plan, trans, p, order, rb, cb, nr, nc, off, c.reduced_precisions() && nb > 0);
     };
     csr(d.near_test, d.test.n_boxes, c.near_fwd_tgt, c.near_fwd_ptr, c.near_fwd_blk, c.n_near_fwd_tgt,
         c.near_fwd_plan, false);
     csr(d.near_trial, tr.n_boxes, c.near_rev_tgt, c.near_rev_ptr, c.near_rev_blk, c.n_near_rev_tgt,
         c.near_rev_plan, true);
+    if (d.near_values) build_near_frags(c);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1133,10 +1149,10 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
 // Device buffers of one workspace of `R` columns (bytes): tree-ordered x and y in the core type,
 // the moment pools in the compute type and, for Toeplitz couplings, the spectra.
 struct WsLayout {
-    size_t xs = 0, ys = 0, mom = 0, loc = 0, spec_in = 0, spec_out = 0;
+    size_t xs = 0, ys = 0, mom = 0, loc = 0, spec_in = 0, spec_out = 0, x32 = 0;
     size_t total(size_t align) const {
         size_t s = 0;
-        for (size_t b : {xs, ys, mom, loc, spec_in, spec_out}) s += (b + align - 1) / align * align;
+        for (size_t b : {xs, ys, mom, loc, spec_in, spec_out, x32}) s += (b + align - 1) / align * align;
         return s;
     }
 };
@@ -1149,6 +1165,8 @@ WsLayout ws_layout(const Core &c, bool mixed, int R) {
     const size_t ce = mixed ? se / 2 : se;
     L.xs = L.ys = (size_t)nmax * R * se;
     L.mom = L.loc = (size_t)bmax * c.K * R * ce;
+    if (mixed && (c.near_fwd_plan.n_frag > 0 || c.near_rev_plan.n_frag > 0))
+        L.x32 = (size_t)nmax * near_x32_pitch(R) * sizeof(float);
     if (is_toeplitz(c)) {
         const size_t cc = mixed ? 8 : 16;
         // per column, the largest level's (boxes x stored frequencies) of either orientation
@@ -1175,6 +1193,7 @@ void run_passes(Workspace &w, int op, int which, int dtype, int64_t nrhs, const 
     const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
     const Side &src = *P.src, &dst = *P.dst;
     const bool mixed_ws = w.opts.precision != H2G_PREC_PLAIN;
+    float *x32 = mixed_ws && w.x32.ptr ? w.x32.as<float>() : nullptr;
     for (int64_t c0 = 0; c0 < nrhs; c0 += w.R_cap) {
         const int R = (int)std::min<int64_t>(w.R_cap, nrhs - c0);
         w.mark_begin(0, -1);
@@ -1188,14 +1207,14 @@ void run_passes(Workspace &w, int op, int which, int dtype, int64_t nrhs, const 
                 // concurrent with the far field up to its first write of the result
                 H2G_HIP(hipEventRecord(w.ev_fork, s));
                 H2G_HIP(hipStreamWaitEvent(w.side, w.ev_fork, 0));
-                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, mixed_ws, w.side);
+                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, x32, w.side);
                 H2G_HIP(hipEventRecord(w.ev_near, w.side));
                 far_pass(w, op, R, w.ev_near);
                 return;
             }
             if (which & H2G_NEARMUL) {
                 w.mark_begin(1, -1);
-                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, mixed_ws, s);
+                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, x32, s);
                 w.mark_end();
             }
             if (which & H2G_FARMUL) far_pass(w, op, R, nullptr);
@@ -1323,13 +1342,15 @@ int64_t Core::device_bytes() const {
     dense(m2l_dense);
     side(test);
     if (!galerkin) side(trial_own);
-    b += ops.bytes + ops32.bytes + perms.bytes + perms_inv.bytes + near_vals.bytes + near_vals32.bytes;
+    b += ops.bytes + ops32.bytes + perms.bytes + perms_inv.bytes + near_vals.bytes;
+    b += near_fwd_plan.frag.bytes + near_fwd_plan.xrow.bytes + near_rev_plan.frag.bytes + near_rev_plan.xrow.bytes;
     b += near_fwd_plan.task.bytes + near_fwd_plan.col.bytes + near_rev_plan.task.bytes + near_rev_plan.col.bytes;
     return b;
 }
 
 int64_t Workspace::device_bytes() const {
-    return xs.bytes + ys.bytes + mom.bytes + loc.bytes + spec_in.bytes + spec_out.bytes + xhost.bytes + yhost.bytes;
+    return xs.bytes + ys.bytes + mom.bytes + loc.bytes + spec_in.bytes + spec_out.bytes + x32.bytes + xhost.bytes +
+           yhost.bytes;
 }
 
 }  // namespace h2g
@@ -1597,6 +1618,7 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap);
             w->xs.alloc(L.xs);
             w->ys.alloc(L.ys);
+            w->x32.alloc(L.x32);
             w->mom.alloc(L.mom);
             w->loc.alloc(L.loc);
             w->spec_in.alloc(L.spec_in);
@@ -1806,6 +1828,7 @@ int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dty
         };
         carve(w.xs, L.xs);
         carve(w.ys, L.ys);
+        carve(w.x32, L.x32);
         carve(w.mom, L.mom);
         carve(w.loc, L.loc);
         carve(w.spec_in, L.spec_in);

@@ -308,17 +308,28 @@ struct LatticeLevel {
 // all the leaf's near blocks, concatenated.  Column j of A starts at value aoff (row 0) with row
 // stride rstride (1 for a block applied as stored, the block's row count for its transpose), and
 // multiplies source point xrow.
+//
+// For the mixed form of real maps the values are also kept per task in FP32, row-major with the
+// columns padded to a multiple of 16 (zeros): row r of the task at foff + r * ncolp, so that a lane
+// reads 4 consecutive columns with one 16-byte load; xoff is the task's first entry in the padded
+// table of source points (ncolp int32 per leaf, padding entries repeat a valid point).
 struct NearTask {
     int64_t out0;  // first output point of the leaf
     int64_t col0;  // first entry in the column table
     int32_t nrow, ncol, row0, pad;
+    int64_t foff;  // first FP32 value of the task's row-major copy
+    int64_t xoff;  // first entry of the leaf's padded source-point table
 };
+__host__ __device__ inline int64_t near_ncolp(int64_t ncol) { return (ncol + 15) & ~int64_t(15); }
 struct NearCol {
     int64_t aoff;
     int32_t xrow, rstride;
 };
 struct NearPlan {
     DevBuf task, col;
+    DevBuf frag;      // float: the tasks' row-major FP32 copies (mixed cores of real maps)
+    DevBuf xrow;      // int32: the leaves' padded source-point tables
+    int64_t n_frag = 0;
     int64_t cls[5] = {0, 0, 0, 0, 0};  // tasks of 4, 3, 2, 1 row tiles: [cls[i], cls[i+1])
     bool ok = false;
 };
@@ -366,7 +377,6 @@ struct Core {
     // near: blocks
     int64_t n_near_blocks = 0, n_near_values = 0;
     DevBuf near_vals;
-    DevBuf near_vals32;  // FP32 copy of real near values (cores with the mixed precision)
     DevBuf near_fwd_ptr, near_fwd_blk;   // CSR over test leaves (by list index)
     DevBuf near_rev_ptr, near_rev_blk;   // CSR over trial leaves
     DevBuf near_fwd_tgt, near_rev_tgt;   // target leaf lists
@@ -410,6 +420,7 @@ struct Workspace {
     bool own_streams = true;       // false: stream (== side) belongs to the caller
     int R_cap = 1;
     DevBuf xs, ys;        // tree-ordered FP64/C64, [n][R]
+    DevBuf x32;           // FP32 source columns of the mixed near field of real maps
     DevBuf mom, loc;      // [box][R][K] compute type
     DevBuf spec_in, spec_out;  // toeplitz spectra [slot][R][L3] complex compute type
     DevBuf xhost, yhost;  // device staging for host pointers
@@ -435,8 +446,16 @@ void launch_gather(int dtype_core, int dtype_x, const void *x, int64_t ldx, cons
 void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *perm, const int64_t *iperm, int64_t n,
                     int R, void *y, int64_t ldy, cplx<double> a, cplx<double> b, const uint8_t *mask, hipStream_t s);
 // vals32: read the FP32 copy of the near values where the core has one (mixed precision)
-void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, bool vals32, hipStream_t s);
-void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s);
+// x32: for the mixed form of real maps, a workspace buffer of near_x32_elems(n, R) floats that the
+// near field fills with its FP32 source columns (null: the FP64 path)
+void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, float *x32, hipStream_t s);
+// FP32 row-major near values of both orientations' tasks, from the FP64 values (after they are set)
+void build_near_frags(Core &c);
+inline int near_x32_nct(int R) { return R <= 16 ? 1 : R <= 32 ? 2 : 4; }
+inline int64_t near_x32_pitch(int R) {
+    const int w = 16 * near_x32_nct(R);
+    return (int64_t)(R + w - 1) / w * w;
+}
 void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
                 const void *coll, int K, const void *xs, int R, void *mom, hipStream_t s);
 void launch_l2p(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,

@@ -142,6 +142,94 @@ __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__res
         }
 }
 
+// The mixed near field of real maps on FP64 MFMA from FP32 operands in load order.  As above, one
+// wave per task (RT row tiles) and NCT 16-column tiles per wave, but the k-steps of a group of 16
+// columns are assigned so that each lane's operands are contiguous: at k-step u of group g, lane
+// (m, q) = (lane % 16, lane / 16) contributes column 16 g + 4 q + u.  Its matrix values of 4
+// k-steps are then one 16-byte load per row tile from the task's row-major FP32 copy, its 4
+// source points one 16-byte load from the padded point table, and its NCT source columns of a
+// point one load from the interleaved FP32 copy of x.  Rows past nrow read row nrow - 1 (their
+// results are not stored), padding columns have zero values and repeat a valid point, so the loop
+// needs no masks.  The point table runs two groups ahead of its use, the operands one group.
+template <int RT, int NCT>
+__global__ void __launch_bounds__(TPB, 2) near_frag_kernel(const NearTask *__restrict__ tasks, int64_t t0, int64_t t1,
+                                                        const int32_t *__restrict__ xrow,
+                                                        const float *__restrict__ frag,
+                                                        const float *__restrict__ x32, int64_t pitch,
+                                                        double *__restrict__ ys, int R) {
+    typedef float float4_t __attribute__((ext_vector_type(4)));
+    typedef int int4_t __attribute__((ext_vector_type(4)));
+    typedef float floatn_t __attribute__((ext_vector_type(NCT)));
+    int wg = blockIdx.x;
+    const int n8 = gridDim.x & ~7;
+    if (wg < n8) wg = (wg & 7) * (n8 >> 3) + (wg >> 3);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t ti = t0 + (int64_t)wg * (TPB / 64) + wave;
+    if (ti >= t1) return;
+    const NearTask tk = tasks[ti];
+    const int nrow = tk.nrow;
+    const int ncp = (int)near_ncolp(tk.ncol);
+    const int ng = ncp >> 4;
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15, q = lane >> 4;
+    const int c0 = blockIdx.y * NCT * 16;
+    const float *__restrict__ ap[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) ap[rt] = frag + tk.foff + (int64_t)min(16 * rt + m, nrow - 1) * ncp + 4 * q;
+    const int32_t *__restrict__ xq = xrow + tk.xoff + 4 * q;
+    const float *__restrict__ xb = x32 + c0 + NCT * m;
+    auto load_pt = [&](int g) { return *reinterpret_cast<const int4_t *>(xq + 16 * min(g, ng - 1)); };
+    float4_t a[RT];
+    floatn_t bv[4];
+    auto load_ops = [&](int g, const int4_t &pt) {
+        const int gc = min(g, ng - 1);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) a[rt] = *reinterpret_cast<const float4_t *>(ap[rt] + 16 * gc);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bv[u] = *reinterpret_cast<const floatn_t *>(xb + (int64_t)pt[u] * pitch);
+    };
+    double4_t acc[RT][NCT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) acc[rt][t] = double4_t{0.0, 0.0, 0.0, 0.0};
+    int4_t pt1 = load_pt(1);
+    load_ops(0, load_pt(0));
+    for (int g = 0; g < ng; ++g) {
+        float4_t ac[RT];
+        floatn_t bc[4];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) ac[rt] = a[rt];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) bc[u] = bv[u];
+        const int4_t pt2 = load_pt(g + 2);
+        load_ops(g + 1, pt1);
+        // keep the next group's loads ahead of this group's MFMAs
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int t = 0; t < NCT; ++t)
+                    acc[rt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ac[rt][u], (double)bc[u][t], acc[rt][t],
+                                                                      0, 0, 0);
+        pt1 = pt2;
+    }
+    double *__restrict__ yo = ys + (tk.out0 + tk.row0) * R;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) {
+            const int col = c0 + 16 * t + m;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * rt + mf_d_row(lane, v);
+                if (row < nrow && col < R) yo[(int64_t)row * R + col] = acc[rt][t][v];
+            }
+        }
+}
+
 // Leaf expansions on FP64 MFMA (real maps, R >= MFMA_MIN_R), FP64 accumulation.
 //   L2P: Y_leaf (n x R) += alpha C^T (n x K) L (K x R)      rows = points, contraction over k
 //   P2M: M (K x R)      = C (K x n) X (n x R)                rows = k, contraction over points
@@ -281,10 +369,41 @@ __global__ void __launch_bounds__(P2M_NT)
     }
 }
 
-// FP32 copy of FP64 values (grid-stride)
-__global__ void narrow_values_kernel(const double *__restrict__ src, float *__restrict__ dst, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = (float)src[i];
+// The FP32 row-major copy of one task per workgroup: row r, column j at foff + r * ncolp + j, the
+// value of row row0 + r of column j's block (zero for the padding columns j >= ncol).
+__global__ void near_frag_fill_kernel(const NearTask *__restrict__ tasks, const NearCol *__restrict__ cols,
+                                      const double *__restrict__ vals, float *__restrict__ frag) {
+    const NearTask tk = tasks[blockIdx.x];
+    const int64_t ncp = near_ncolp(tk.ncol);
+    const int64_t n = (int64_t)tk.nrow * ncp;
+    float *__restrict__ dst = frag + tk.foff;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const int64_t r = e / ncp, j = e - r * ncp;
+        float v = 0.f;
+        if (j < tk.ncol) {
+            const NearCol cc = cols[tk.col0 + j];
+            v = (float)vals[cc.aoff + (int64_t)(tk.row0 + r) * cc.rstride];
+        }
+        dst[e] = v;
+    }
+}
+
+// FP32 source columns of the mixed near field: x32 [point][pitch] from xs [point][R].  Within each
+// chunk of W = 16 NCT columns, column chunk + 16 t + m sits at chunk + NCT m + t, so the lane of
+// MFMA column m reads its NCT columns (m, m + 16, ...) of a point with one load; columns >= R are
+// zero.
+template <int NCT>
+__global__ void near_x32_kernel(const double *__restrict__ xs, int R, int64_t n, int64_t pitch,
+                                float *__restrict__ x32) {
+    constexpr int W = 16 * NCT;
+    const int64_t total = n * pitch;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pt = i / pitch;
+        const int q = (int)(i - pt * pitch);
+        const int rem = q % W;
+        const int c = q - rem + (rem % NCT) * 16 + rem / NCT;
+        x32[i] = c < R ? (float)xs[pt * R + c] : 0.f;
+    }
 }
 
 // probe of the operand / result lane layout used above: D = A (16x4) * B (4x16)
@@ -375,36 +494,67 @@ bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
     return true;
 }
 
-bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, bool vals32, hipStream_t s) {
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, float *x32, hipStream_t s) {
     if (c.dtype != H2G_F64 || R < MFMA_MIN_R) return false;
     const bool trans = op != H2G_OP_N;
     const NearPlan &P = trans ? c.near_rev_plan : c.near_fwd_plan;
     if (!P.ok) return false;
+    using std::integral_constant;
+    if (x32 && P.frag.ptr) {
+        // mixed form: FP32 source columns, then the tasks on their row-major FP32 values
+        const int nct = near_x32_nct(R);
+        const int64_t pitch = near_x32_pitch(R);
+        const int64_t n_src = (trans ? c.rev : c.fwd).src->n_dofs;
+        const unsigned gx = (unsigned)std::min<int64_t>((n_src * pitch + TPB - 1) / TPB, 1 << 20);
+        if (gx > 0) {
+            switch (nct) {
+            case 1: hipLaunchKernelGGL((near_x32_kernel<1>), gx, TPB, 0, s, (const double *)xs, R, n_src, pitch, x32); break;
+            case 2: hipLaunchKernelGGL((near_x32_kernel<2>), gx, TPB, 0, s, (const double *)xs, R, n_src, pitch, x32); break;
+            default: hipLaunchKernelGGL((near_x32_kernel<4>), gx, TPB, 0, s, (const double *)xs, R, n_src, pitch, x32); break;
+            }
+            H2G_CHECK_LAUNCH();
+        }
+        const unsigned gy = (unsigned)(pitch / (16 * nct));
+        for (int i = 0; i < 4; ++i) {  // tasks of 4, 3, 2, 1 row tiles
+            const int64_t t0 = P.cls[i], t1 = P.cls[i + 1];
+            if (t1 == t0) continue;
+            const dim3 grid((unsigned)((t1 - t0 + TPB / 64 - 1) / (TPB / 64)), gy);
+            auto go = [&](auto rt_c, auto nct_c) {
+                constexpr int RT = decltype(rt_c)::value, NC = decltype(nct_c)::value;
+                hipLaunchKernelGGL((near_frag_kernel<RT, NC>), grid, TPB, 0, s, P.task.as<NearTask>(), t0, t1,
+                                   P.xrow.as<int32_t>(), P.frag.as<float>(), (const float *)x32, pitch, (double *)ys, R);
+            };
+            auto by_nct = [&](auto rt_c) {
+                switch (nct) {
+                case 1: go(rt_c, integral_constant<int, 1>{}); break;
+                case 2: go(rt_c, integral_constant<int, 2>{}); break;
+                default: go(rt_c, integral_constant<int, 4>{}); break;
+                }
+            };
+            switch (4 - i) {
+            case 1: by_nct(integral_constant<int, 1>{}); break;
+            case 2: by_nct(integral_constant<int, 2>{}); break;
+            case 3: by_nct(integral_constant<int, 3>{}); break;
+            default: by_nct(integral_constant<int, 4>{}); break;
+            }
+            H2G_CHECK_LAUNCH();
+        }
+        return true;
+    }
     // NCT column tiles per workgroup (up to 64 RHS); wider batches take several workgroups
     const int nct = std::min(4, (R + 15) / 16);
     const unsigned gy = (unsigned)((R + 16 * nct - 1) / (16 * nct));
-    const bool f32 = vals32 && c.near_vals32.ptr;
     for (int i = 0; i < 4; ++i) {  // tasks of 4, 3, 2, 1 row tiles
         const int64_t t0 = P.cls[i], t1 = P.cls[i + 1];
         if (t1 == t0) continue;
         const dim3 grid((unsigned)((t1 - t0 + TPB / 64 - 1) / (TPB / 64)), gy);
-        auto go = [&](auto kern, auto vv) {
-            using V = decltype(vv);
-            kern<<<grid, TPB, 0, s>>>(P.task.as<NearTask>(), t0, t1, P.col.as<NearCol>(),
-                                      f32 ? (const V *)c.near_vals32.ptr : (const V *)c.near_vals.ptr,
-                                      (const double *)xs, (double *)ys, R);
-        };
-        // the value type (FP32 copy or FP64) and the orientation select the instantiation
+        // the orientation selects the instantiation
         auto pick = [&](auto rt_c, auto ku_c, auto nct_c) {
             constexpr int RT = decltype(rt_c)::value, KU = decltype(ku_c)::value, NC = decltype(nct_c)::value;
-            if (f32)
-                trans ? go(near_task_kernel<true, RT, KU, NC, float>, 0.f)
-                      : go(near_task_kernel<false, RT, KU, NC, float>, 0.f);
-            else
-                trans ? go(near_task_kernel<true, RT, KU, NC, double>, 0.0)
-                      : go(near_task_kernel<false, RT, KU, NC, double>, 0.0);
+            auto kern = trans ? near_task_kernel<true, RT, KU, NC, double> : near_task_kernel<false, RT, KU, NC, double>;
+            hipLaunchKernelGGL(kern, grid, TPB, 0, s, P.task.as<NearTask>(), t0, t1, P.col.as<NearCol>(),
+                               c.near_vals.as<double>(), (const double *)xs, (double *)ys, R);
         };
-        using std::integral_constant;
         auto by_nct = [&](auto rt_c, auto ku_c) {
             switch (nct) {
             case 1: pick(rt_c, ku_c, integral_constant<int, 1>{}); break;
@@ -424,10 +574,18 @@ bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, bo
     return true;
 }
 
-void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t s) {
-    if (n <= 0) return;
-    narrow_values_kernel<<<(unsigned)std::min<int64_t>((n + TPB - 1) / TPB, 1 << 20), TPB, 0, s>>>(src, dst, n);
-    H2G_CHECK_LAUNCH();
+void build_near_frags(Core &c) {
+    if (c.dtype != H2G_F64 || !c.near_vals.ptr) return;
+    for (NearPlan *P : {&c.near_fwd_plan, &c.near_rev_plan}) {
+        if (!P->ok || P->n_frag == 0) continue;
+        if (!P->frag.ptr) P->frag.alloc((size_t)P->n_frag * sizeof(float));
+        const int64_t nt = P->cls[4];
+        if (nt > 0)
+            hipLaunchKernelGGL(near_frag_fill_kernel, (unsigned)nt, TPB, 0, 0, P->task.as<NearTask>(), P->col.as<NearCol>(),
+                               c.near_vals.as<double>(), P->frag.as<float>());
+        H2G_CHECK_LAUNCH();
+    }
+    H2G_HIP(hipDeviceSynchronize());
 }
 
 }  // namespace h2g

@@ -154,8 +154,7 @@ extern "C" int h2g_method_fill(h2g_core *core, const h2g_method_desc *desc) {
                     c.near_vals.as<cplx<double>>());
             H2G_HIP(hipGetLastError());
             H2G_HIP(hipDeviceSynchronize());
-            if (c.near_vals32.ptr)
-                launch_narrow(c.near_vals.as<double>(), c.near_vals32.as<float>(), c.n_near_values, nullptr);
+            if (c.dtype == H2G_F64) build_near_frags(c);
             H2G_HIP(hipDeviceSynchronize());
         }
         if (d.fill_collections) {

@@ -2,17 +2,18 @@
 //
 // The MFMA near-field and leaf kernels (leaf_mfma.hip) and the lattice Toeplitz kernels
 // (m2l_lattice.hip, m2l_mfma.hip) use gfx950 instructions and cannot run on the host.  These
-// stand-ins let the replay link: the MFMA launchers decline (the dense coupling kernel too, so
-// dense maps take the generic box GEMM here) (as they do on the GPU for complex
-// element types, so a complex map takes the same generic kernels there and here), the FP32 value
-// copy runs as a host loop, and the lattice entry points refuse -- the replay covers maps whose
-// couplings run on the generic kernels (dense, symmetry-reduced dense, and Toeplitz maps built
-// without lattice coordinates).
+// stand-ins let the replay link: the MFMA launchers decline, as they do on the GPU for complex
+// element types, so a real map takes the same generic kernels here that a complex map takes
+// there; the dense coupling kernel declines too, so dense maps take the generic box GEMM here; and
+// the lattice entry points refuse -- the replay covers maps whose couplings run on the generic
+// kernels (dense, symmetry-reduced dense, and Toeplitz maps built without lattice coordinates).
 #include "h2g_internal.h"
 
 namespace h2g {
 
-bool launch_near_mfma(int, const Core &, const void *, void *, int, bool, hipStream_t) { return false; }
+bool launch_near_mfma(int, const Core &, const void *, void *, int, float *, hipStream_t) { return false; }
+// the FP32 row-major near values feed only the MFMA near kernel, which declines here
+void build_near_frags(Core &) {}
 bool launch_p2m_mfma(int, const int32_t *, int, const int64_t *, const int64_t *, const void *, int, const void *,
                      int, void *, hipStream_t) {
     return false;
@@ -20,9 +21,6 @@ bool launch_p2m_mfma(int, const int32_t *, int, const int64_t *, const int64_t *
 bool launch_l2p_mfma(int, const int32_t *, int, const int64_t *, const int64_t *, const void *, int, const void *,
                      int, void *, double, bool, hipStream_t) {
     return false;
-}
-void launch_narrow(const double *src, float *dst, int64_t n, hipStream_t) {
-    for (int64_t i = 0; i < n; ++i) dst[i] = (float)src[i];
 }
 
 // the dense MFMA kernels are not built: dense maps and general transfers take the generic box GEMM
