@@ -151,7 +151,10 @@ __global__ void __launch_bounds__(TPB, 2) near_task_kernel(const NearTask *__res
 // point one load from the interleaved FP32 copy of x.  Rows past nrow read row nrow - 1 (their
 // results are not stored), padding columns have zero values and repeat a valid point, so the loop
 // needs no masks.  The point table runs two groups ahead of its use, the operands one group.
-template <int RT, int NCT>
+// M32: the products on v_mfma_f32_16x16x4f32 (twice the FP64 MFMA rate on gfx950) with FP32
+// partial sums of FLUSH groups (16 FLUSH products per output) added into FP64 accumulators; else
+// on v_mfma_f64_16x16x4f64 with FP64 accumulation throughout.
+template <int RT, int NCT, bool M32>
 __global__ void __launch_bounds__(TPB, 2) near_frag_kernel(const NearTask *__restrict__ tasks, int64_t t0, int64_t t1,
                                                         const int32_t *__restrict__ xrow,
                                                         const float *__restrict__ frag,
@@ -188,11 +191,26 @@ __global__ void __launch_bounds__(TPB, 2) near_frag_kernel(const NearTask *__res
 #pragma unroll
         for (int u = 0; u < 4; ++u) bv[u] = *reinterpret_cast<const floatn_t *>(xb + (int64_t)pt[u] * pitch);
     };
+    constexpr int FLUSH = 2;
     double4_t acc[RT][NCT];
+    float4_t acc32[RT][NCT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int t = 0; t < NCT; ++t) acc[rt][t] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < NCT; ++t) {
+            acc[rt][t] = double4_t{0.0, 0.0, 0.0, 0.0};
+            acc32[rt][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+        }
+    auto flush = [&]() {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int t = 0; t < NCT; ++t) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[rt][t][v] += (double)acc32[rt][t][v];
+                acc32[rt][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+            }
+    };
     int4_t pt1 = load_pt(1);
     load_ops(0, load_pt(0));
     for (int g = 0; g < ng; ++g) {
@@ -206,16 +224,28 @@ __global__ void __launch_bounds__(TPB, 2) near_frag_kernel(const NearTask *__res
         load_ops(g + 1, pt1);
         // keep the next group's loads ahead of this group's MFMAs
         __asm__ volatile("" ::: "memory");
+        if constexpr (M32) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt)
+                for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-                for (int t = 0; t < NCT; ++t)
-                    acc[rt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ac[rt][u], (double)bc[u][t], acc[rt][t],
-                                                                      0, 0, 0);
+                    for (int t = 0; t < NCT; ++t)
+                        acc32[rt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[rt][u], bc[u][t], acc32[rt][t], 0, 0, 0);
+            if (g % FLUSH == FLUSH - 1) flush();
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                    for (int t = 0; t < NCT; ++t)
+                        acc[rt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ac[rt][u], (double)bc[u][t],
+                                                                          acc[rt][t], 0, 0, 0);
+        }
         pt1 = pt2;
     }
+    if constexpr (M32) flush();
     double *__restrict__ yo = ys + (tk.out0 + tk.row0) * R;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -224,7 +254,8 @@ __global__ void __launch_bounds__(TPB, 2) near_frag_kernel(const NearTask *__res
             const int col = c0 + 16 * t + m;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = 16 * rt + mf_d_row(lane, v);
+                // D[m][n] in lane n + 16 (m / 4), element m % 4 (FP32); lane n + 16 (m % 4), element m / 4 (FP64)
+                const int row = 16 * rt + (M32 ? 4 * (lane >> 4) + v : mf_d_row(lane, v));
                 if (row < nrow && col < R) yo[(int64_t)row * R + col] = acc[rt][t][v];
             }
         }
@@ -521,7 +552,7 @@ bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, fl
             const dim3 grid((unsigned)((t1 - t0 + TPB / 64 - 1) / (TPB / 64)), gy);
             auto go = [&](auto rt_c, auto nct_c) {
                 constexpr int RT = decltype(rt_c)::value, NC = decltype(nct_c)::value;
-                hipLaunchKernelGGL((near_frag_kernel<RT, NC>), grid, TPB, 0, s, P.task.as<NearTask>(), t0, t1,
+                hipLaunchKernelGGL((near_frag_kernel<RT, NC, true>), grid, TPB, 0, s, P.task.as<NearTask>(), t0, t1,
                                    P.xrow.as<int32_t>(), P.frag.as<float>(), (const float *)x32, pitch, (double *)ys, R);
             };
             auto by_nct = [&](auto rt_c) {
