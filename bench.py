@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=2, help="products timed by the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (RHS block and tile shapes); use the library's defaults")
     ap.add_argument("--serve-requests", type=int, default=4000,
                     help="single-RHS requests of the serving measurement (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the requests (1/s)")
@@ -99,15 +101,18 @@ def algorithmic_work(arrays, nrhs, phase, level, mixed):
     return "hbm", 2.0 * n * nrhs * 8
 
 
-def traffic_of(kernel, n, p, nrhs):
+def traffic_of(kernel, n, p, nrhs, groups):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE measurement of this workload (scaled by the copy-kernel calibration), else None."""
+    WRITE_SIZE measurement of this workload and tile shape (scaled by the copy-kernel
+    calibration), else None."""
     path = os.path.join(ROOT, "profiles", "m2l_direct_traffic_n2m_p6_r32.json")
     if (n, p, nrhs) != (2_000_000, 6, 32) or not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    return d["traffic_bytes_per_launch"] if d.get("kernel") == kernel else None
+    if d.get("kernel") != kernel or d.get("m2l_groups", 512) != (groups or 512):
+        return None
+    return d["traffic_bytes_per_launch"]
 
 
 def cpu_model():
@@ -295,7 +300,7 @@ def serving_leg(args):
 
     specs = serving_mix(args.serve_seed)
     srv = Server(budget_bytes=int(args.serve_budget_gb * 1e9), max_batch=32, ws_per_map=2, precision="mixed",
-                 n_streams=4)
+                 n_streams=4, autotune=not args.no_tune)
     admitted, rejected = [], []
     t0 = time.perf_counter()
     for i, s in enumerate(specs):
@@ -303,7 +308,7 @@ def serving_leg(args):
         name = f"m{i}"
         try:
             rm = srv.add_map(name, arrays)
-            admitted.append(dict(name=name, **s, gb=round(rm.device_bytes() / 1e9, 1)))
+            admitted.append(dict(name=name, **s, gb=round(rm.device_bytes() / 1e9, 1), tile=rm.tile))
             log(f"serving: admitted {name} {s}: {rm.device_bytes() / 1e9:.1f} GB, total {srv.used_bytes() / 1e9:.1f} GB")
         except AdmissionError as e:
             rejected.append(dict(name=name, **s, reason=str(e)))
@@ -437,7 +442,18 @@ def main():
     X = torch.randn((R, N), generator=g, dtype=torch.float32).to(dev).t()  # column-major N x R
     Y = torch.zeros((R, N), dtype=torch.float32, device=dev).t()
 
-    A = h2g.H2Workspace(core, max_rhs=R, precision=args.precision, use_graph=not args.no_graph)
+    # RHS block and tile shapes: measured on this map by the autotuner (outside the timed region)
+    tile = {"max_rhs": R, "m2l_groups": 0, "near_cols": 0}
+    tune_table = []
+    if not args.no_tune:
+        from h2g.autotune import tune
+
+        t0 = time.perf_counter()
+        tile, tune_table = tune(core, nrhs=R, precision=args.precision, blocks=(16, R))
+        log(f"autotune in {time.perf_counter() - t0:.1f}s: {tile}; " +
+            ", ".join(f"{c['max_rhs']}/{c['m2l_groups']}/{c['near_cols']}: {ms * R:.2f} ms" for c, ms in tune_table))
+    R_launch = min(R, tile["max_rhs"])
+    A = h2g.H2Workspace(core, precision=args.precision, use_graph=not args.no_graph, **tile)
     log(f"batched workspace {A.device_bytes() / 1e9:.1f} GB")
     for i in range(args.warmup):
         h2g.mul_(Y, A, X)
@@ -484,7 +500,7 @@ def main():
     log(f"direct sum on 1000 targets: plain {err_direct_plain:.2e}, batched {err_direct_batched:.2e}")
 
     # per-phase device times (HIP events on the workspace stream), same product, eager
-    Tm = h2g.H2Workspace(core, max_rhs=R, precision=args.precision, timing=True)
+    Tm = h2g.H2Workspace(core, precision=args.precision, timing=True, **tile)
     h2g.mul_(Y, Tm, X)
     recs = []
     for _ in range(2):
@@ -513,7 +529,7 @@ def main():
     log("plain per-phase ms (1 RHS): " + ", ".join(f"{k}={v:.2f}" for k, v in sorted(ptot.items(), key=lambda t: -t[1])))
     del Tp
     (dph, dlv), dms = max(((k, float(np.mean(v))) for k, v in per.items()), key=lambda t: t[1])
-    kind, work = algorithmic_work(arrays, R, dph, dlv, True)
+    kind, work = algorithmic_work(arrays, R_launch, dph, dlv, True)
     if kind == "hbm":
         achieved = work / (dms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK["hbm"], "unit": "GB/s",
@@ -529,7 +545,8 @@ def main():
             work *= nprod
             pk = PEAK["bf16"]
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": pk, "unit": "TFLOP/s",
-                "frac": round(achieved / pk, 4), "traffic": traffic_of(f"{dph} level {dlv}", N, args.p, R)}
+                "frac": round(achieved / pk, 4), "traffic": traffic_of(f"{dph} level {dlv}", N, args.p, R_launch,
+                                                                       tile["m2l_groups"])}
     roof["kernel"] = f"{dph} level {dlv}"
     roof["launch_ms"] = round(dms, 3)
 
@@ -567,7 +584,9 @@ def main():
         "config": {"workload": f"laplace equispaced p={args.p} {args.m2l} M2L, N={N} uniform cube, leaf<=64, "
                                f"{R} RHS FP32 batched (FP32-stored near values, FP64 near arithmetic + final sum)"
                                + ("" if args.precision == "mixed" else f", couplings on split BF16 ({args.precision})"),
-                   "n": N, "nrhs": R, "p": args.p, "m2l": args.m2l, "leaf_size": 64},
+                   "n": N, "nrhs": R, "p": args.p, "m2l": args.m2l, "leaf_size": 64,
+                   "tile": tile},
+        "autotune": [{"config": c, "ms_per_product": round(ms * R, 3)} for c, ms in tune_table],
         "plain_fp64_rhs_per_s": round(plain_rhs_per_s, 2),
         "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
         "rel_err_vs_plain_fp64": err_vs_plain,

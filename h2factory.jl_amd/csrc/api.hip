@@ -1090,7 +1090,7 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
                                  c.twiddle.ptr, s);
                 w.mark_end();
                 w.mark_begin(5, l);
-                launch_m2l_lattice(cd, conj, LL, nsrc, n, w.spec_in.ptr, w.spec_out.ptr, R, s);
+                launch_m2l_lattice(cd, conj, LL, nsrc, n, w.spec_in.ptr, w.spec_out.ptr, R, w.opts.m2l_groups, s);
                 w.mark_end();
                 w.mark_begin(6, l);
                 launch_fft_out_fm(cd, half, !inv_in, recv, n, w.spec_out.ptr, K, c.p, c.L, R, w.loc.ptr,
@@ -1157,7 +1157,7 @@ struct WsLayout {
     }
 };
 
-WsLayout ws_layout(const Core &c, bool mixed, int R) {
+WsLayout ws_layout(const Core &c, bool mixed, int R, int near_cols) {
     WsLayout L;
     const int64_t nmax = std::max(c.test.n_dofs, c.trial->n_dofs);
     const int64_t bmax = std::max(c.test.n_boxes, c.trial->n_boxes);
@@ -1166,7 +1166,7 @@ WsLayout ws_layout(const Core &c, bool mixed, int R) {
     L.xs = L.ys = (size_t)nmax * R * se;
     L.mom = L.loc = (size_t)bmax * c.K * R * ce;
     if (mixed && (c.near_fwd_plan.n_frag > 0 || c.near_rev_plan.n_frag > 0))
-        L.x32 = (size_t)nmax * near_x32_pitch(R) * sizeof(float);
+        L.x32 = (size_t)nmax * near_x32_pitch(R, near_cols) * sizeof(float);
     if (is_toeplitz(c)) {
         const size_t cc = mixed ? 8 : 16;
         // per column, the largest level's (boxes x stored frequencies) of either orientation
@@ -1207,14 +1207,14 @@ void run_passes(Workspace &w, int op, int which, int dtype, int64_t nrhs, const 
                 // concurrent with the far field up to its first write of the result
                 H2G_HIP(hipEventRecord(w.ev_fork, s));
                 H2G_HIP(hipStreamWaitEvent(w.side, w.ev_fork, 0));
-                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, x32, w.side);
+                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, x32, w.opts.near_cols, w.side);
                 H2G_HIP(hipEventRecord(w.ev_near, w.side));
                 far_pass(w, op, R, w.ev_near);
                 return;
             }
             if (which & H2G_NEARMUL) {
                 w.mark_begin(1, -1);
-                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, x32, s);
+                launch_near(c.dtype, op, c, w.xs.ptr, w.ys.ptr, R, x32, w.opts.near_cols, s);
                 w.mark_end();
             }
             if (which & H2G_FARMUL) far_pass(w, op, R, nullptr);
@@ -1597,6 +1597,9 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
         if (opts) o = *opts;
         if (o.max_rhs < 1) o.max_rhs = 1;
         if (o.precision < H2G_PREC_PLAIN || o.precision > H2G_PREC_BF16X3) fail(H2G_EINVAL, "unknown precision");
+        if (o.m2l_groups < 0) fail(H2G_EINVAL, "m2l_groups must be >= 0");
+        if (o.near_cols != 0 && o.near_cols != 16 && o.near_cols != 32 && o.near_cols != 64)
+            fail(H2G_EINVAL, "near_cols must be 0, 16, 32 or 64");
         if (!(core->precisions & (1 << o.precision)))
             fail(H2G_EINVAL, "the core was built without this precision (h2g_core_desc.precisions)");
         H2G_HIP(hipSetDevice(core->device));
@@ -1615,7 +1618,7 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             H2G_HIP(hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_near, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
-            const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap);
+            const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap, o.near_cols);
             w->xs.alloc(L.xs);
             w->ys.alloc(L.ys);
             w->x32.alloc(L.x32);
@@ -1807,8 +1810,8 @@ int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dty
         const bool mixed = precision != H2G_PREC_PLAIN;
         // the widest column block whose buffers fit the scratch
         int R = (int)std::min<int64_t>(nrhs, 1 << 16);
-        while (R > 1 && ws_layout(c, mixed, R).total(NOALLOC_ALIGN) > NOALLOC_BYTES) R = R > 8 ? R * 3 / 4 : R - 1;
-        const WsLayout L = ws_layout(c, mixed, R);
+        while (R > 1 && ws_layout(c, mixed, R, 0).total(NOALLOC_ALIGN) > NOALLOC_BYTES) R = R > 8 ? R * 3 / 4 : R - 1;
+        const WsLayout L = ws_layout(c, mixed, R, 0);
         if (L.total(NOALLOC_ALIGN) > NOALLOC_BYTES)
             fail(H2G_ENOMEM, "h2g_matvec_noalloc: one column of this map needs " + std::to_string(L.total(NOALLOC_ALIGN)) +
                                  " bytes of scratch, more than the " + std::to_string(NOALLOC_BYTES) +

@@ -448,12 +448,19 @@ void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *
 // vals32: read the FP32 copy of the near values where the core has one (mixed precision)
 // x32: for the mixed form of real maps, a workspace buffer of near_x32_elems(n, R) floats that the
 // near field fills with its FP32 source columns (null: the FP64 path)
-void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, float *x32, hipStream_t s);
+// near_cols: columns per workgroup of the MFMA near kernels (16, 32, 64; 0: the default)
+void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, float *x32, int near_cols,
+                 hipStream_t s);
 // FP32 row-major near values of both orientations' tasks, from the FP64 values (after they are set)
 void build_near_frags(Core &c);
-inline int near_x32_nct(int R) { return R <= 16 ? 1 : R <= 32 ? 2 : 4; }
-inline int64_t near_x32_pitch(int R) {
-    const int w = 16 * near_x32_nct(R);
+// column tiles per workgroup of the mixed near kernel: the requested width, else the narrowest
+// of 1, 2, 4 tiles that holds the batch
+inline int near_x32_nct(int R, int near_cols = 0) {
+    if (near_cols == 16 || near_cols == 32 || near_cols == 64) return near_cols / 16;
+    return R <= 16 ? 1 : R <= 32 ? 2 : 4;
+}
+inline int64_t near_x32_pitch(int R, int near_cols = 0) {
+    const int w = 16 * near_x32_nct(R, near_cols);
     return (int64_t)(R + w - 1) / w * w;
 }
 void launch_p2m(int cdtype, bool conj, const int32_t *leaves, int n, const int64_t *begin, const int64_t *end,
@@ -498,8 +505,9 @@ void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv,
 // most 16 columns; a multiple of 32: the permuted layout of launch_fft_in_fm, perm).  0 when the
 // level does not run the direct kernel.
 int m2l_direct_rows(int cdtype, const LatticeLevel &LL, int n_src, int R);
+// groups: workgroups per frequency of the direct kernel (0: the default)
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
-                        int R, hipStream_t s);
+                        int R, int groups, hipStream_t s);
 void build_lattice_ea(const int32_t *tq_op, const int32_t *tq_sym, const int32_t *perms, const void *ops, int L,
                       bool half, bool want32, LatticeLevel &LL);
 void launch_fft_out(int cdtype, bool inverse, const int32_t *boxes, int n, const void *spec, int K, int p, int L,

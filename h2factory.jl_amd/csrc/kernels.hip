@@ -620,10 +620,12 @@ void launch_scatter(int dtype_core, int dtype_y, const void *ys, const int64_t *
     H2G_CHECK_LAUNCH();
 }
 
-bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, float *x32, hipStream_t s);
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, float *x32, int near_cols,
+                      hipStream_t s);
 
-void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, float *x32, hipStream_t s) {
-    if (launch_near_mfma(op, c, xs, ys, R, x32, s)) return;
+void launch_near(int dtype_core, int op, const Core &c, const void *xs, void *ys, int R, float *x32, int near_cols,
+                 hipStream_t s) {
+    if (launch_near_mfma(op, c, xs, ys, R, x32, near_cols, s)) return;
     const bool trans = op != H2G_OP_N;
     const int ntgt = trans ? c.n_near_rev_tgt : c.n_near_fwd_tgt;
     if (ntgt == 0) return;

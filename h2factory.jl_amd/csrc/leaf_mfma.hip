@@ -525,7 +525,8 @@ bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
     return true;
 }
 
-bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, float *x32, hipStream_t s) {
+bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, float *x32, int near_cols,
+                      hipStream_t s) {
     if (c.dtype != H2G_F64 || R < MFMA_MIN_R) return false;
     const bool trans = op != H2G_OP_N;
     const NearPlan &P = trans ? c.near_rev_plan : c.near_fwd_plan;
@@ -533,8 +534,8 @@ bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, fl
     using std::integral_constant;
     if (x32 && P.frag.ptr) {
         // mixed form: FP32 source columns, then the tasks on their row-major FP32 values
-        const int nct = near_x32_nct(R);
-        const int64_t pitch = near_x32_pitch(R);
+        const int nct = near_x32_nct(R, near_cols);
+        const int64_t pitch = near_x32_pitch(R, near_cols);
         const int64_t n_src = (trans ? c.rev : c.fwd).src->n_dofs;
         const unsigned gx = (unsigned)std::min<int64_t>((n_src * pitch + TPB - 1) / TPB, 1 << 20);
         if (gx > 0) {
@@ -573,7 +574,7 @@ bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, fl
         return true;
     }
     // NCT column tiles per workgroup (up to 64 RHS); wider batches take several workgroups
-    const int nct = std::min(4, (R + 15) / 16);
+    const int nct = near_cols ? near_cols / 16 : std::min(4, (R + 15) / 16);
     const unsigned gy = (unsigned)((R + 16 * nct - 1) / (16 * nct));
     for (int i = 0; i < 4; ++i) {  // tasks of 4, 3, 2, 1 row tiles
         const int64_t t0 = P.cls[i], t1 = P.cls[i + 1];

@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(NT_LIST)
 // 4 slot pairs per partial sum, each batch's partial sums added in FP64.  116-120 registers and
 // 35 KB of LDS per workgroup leave room for 4 workgroups (16 waves) per CU.
 // ---------------------------------------------------------------------------------------------
-constexpr int DR_NT = 256, DR_MAXB = 16;  // threads; batches of a block's list held in LDS
+constexpr int DR_NT = 512, DR_MAXB = 16;  // threads; batches of a block's list held in LDS
 // workgroups per frequency (groups of consecutive blocks); measured on the N = 2M, p = 6 map with
 // 4 workgroups per CU: 128 groups 33.7 ms, 256 32.5 ms, 512 to 1024 32.0 to 32.3 ms (levels 4-6)
 constexpr int DR_GROUPS = 512;
@@ -427,7 +427,7 @@ constexpr int DR_MIN_R = 4;
 constexpr bool DR_NCH1 = true;
 
 template <bool CONJ, int NCH>
-__global__ void __launch_bounds__(DR_NT, 4)
+__global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
     m2l_direct_kernel(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr, const int4 *__restrict__ list,
                       int n_list, const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg, int n_src,
                       int n_recv, const cplx<float> *S, int Rrow, cplx<float> *__restrict__ O, int R) {
@@ -666,16 +666,16 @@ int m2l_direct_rows(int cdtype, const LatticeLevel &LL, int n_src, int R) {
 }
 
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
-                        int R, hipStream_t s) {
+                        int R, int groups, hipStream_t s) {
     const bool f64 = cdtype == H2G_F64 || cdtype == H2G_C64;
     if (!LL.EA64.ptr) throw Error{H2G_EINVAL, "lattice coupling table missing"};
     if (const int Rrow = m2l_direct_rows(cdtype, LL, n_src, R)) {
         const LatticeDirect &D = LL.direct;
-        // groups of consecutive blocks: at least two blocks per wave (the coupling table is
-        // loaded once per group), and about DR_GROUPS groups per frequency, so that the
+        // groups of consecutive blocks: at least one block per wave (the coupling table is
+        // loaded once per group), and about `groups` groups per frequency, so that the
         // workgroups resident on an XCD work on few frequencies at a time (L2 reuse of source
         // rows)
-        const int gsz = std::max(2 * DR_NT / 64, D.n_blocks / DR_GROUPS);
+        const int gsz = std::max(DR_NT / 64, D.n_blocks / (groups > 0 ? groups : DR_GROUPS));
         const int ng = (D.n_blocks + gsz - 1) / gsz;
         const int ch = Rrow == 16 ? 16 : 32;  // columns per chunk (NCH 16-column tiles)
         const dim3 grid((unsigned)((ng + 7) / 8 * 8), (unsigned)LL.F, (unsigned)(Rrow / ch));

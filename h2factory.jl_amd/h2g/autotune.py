@@ -1,13 +1,25 @@
-"""Autotuner for the batched product: picks the RHS block (columns per pass) of a workspace.
+"""Autotuner for the batched product: picks a workspace's RHS block and tile shapes per map.
 
-A multi-RHS product over ``nrhs`` columns runs as ceil(nrhs / block) passes of the whole
-hierarchy.  Wider blocks amortise every read of the near field, the collections and the
-coupling spectra over more columns and give the per-frequency Hadamard more lanes; narrower
-blocks need less workspace memory (moments and spectra scale with the block) and keep the
-spectra slices of one frequency L2-resident.  The best block depends on the map (N, p, tree
-depth, near-field density), so it is measured: each candidate block runs the same product on
-synthetic columns, timed on the device, and the fastest per column wins.  Results are cached
-per map signature in a JSON file, so a serving process tunes a map once.
+What is tuned (each a workspace option of ``h2g_ws_opts``):
+
+* ``max_rhs`` -- the RHS block, columns per pass.  A product over ``nrhs`` columns runs as
+  ceil(nrhs / block) passes of the whole hierarchy.  Wider blocks amortise every read of the
+  near field, the collections and the couplings over more columns; narrower blocks need less
+  workspace memory (moments and spectra scale with the block).
+* ``m2l_groups`` -- workgroups per frequency of the lattice Toeplitz coupling kernel (groups of
+  Morton-consecutive parent blocks).  Fewer, larger groups load each frequency's coupling table
+  into LDS fewer times; more, smaller groups keep fewer frequencies in flight per XCD, so the
+  source spectra of a frequency stay in that XCD's L2.
+* ``near_cols`` -- columns per workgroup of the near-field kernel (one, two or four 16-column
+  MFMA tiles per wave): wider tiles reuse every matrix value for more columns, narrower ones
+  run more workgroups with fewer registers each.
+
+The best values depend on the map (N, p, tree depth, near-field density, coupling kind) and the
+device, so they are measured: each candidate runs the same product on synthetic columns, timed
+on the device, and the fastest per column wins.  The search is coordinate-wise (RHS block with
+the default shapes, then the coupling groups, then the near-field columns), which needs a few
+products per candidate instead of the full grid.  Results are cached per map signature in a
+JSON file, so a serving process tunes a map once.
 """
 
 from __future__ import annotations
@@ -17,11 +29,11 @@ import json
 import os
 import time
 
-import numpy as np
-
 from .maps import H2Workspace, mul_
 
-DEFAULT_CANDIDATES = (1, 4, 8, 16, 32)
+DEFAULT_BLOCKS = (8, 16, 32)
+DEFAULT_M2L_GROUPS = (256, 512, 768)
+DEFAULT_NEAR_COLS = (16, 32)
 
 
 def signature(core, precision: str, nrhs: int) -> str:
@@ -33,9 +45,15 @@ def signature(core, precision: str, nrhs: int) -> str:
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
 
-def tune(core, nrhs: int = 32, precision: str = "mixed", candidates=DEFAULT_CANDIDATES, repeats: int = 3,
-         cache_path: str | None = None, use_graph: bool = True):
-    """Returns (best_block, {block: ms_per_column}); measured on the core's device."""
+def _lattice(core):
+    """Whether the map's couplings run on the lattice Toeplitz kernels (m2l_groups matters)."""
+    return core.arrays.m2l_kind in ("toeplitz", "symtoeplitz")
+
+
+def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, m2l_groups=DEFAULT_M2L_GROUPS,
+         near_cols=DEFAULT_NEAR_COLS, repeats: int = 3, cache_path: str | None = None, use_graph: bool = True):
+    """Returns (best, table): ``best`` = {"max_rhs", "m2l_groups", "near_cols"} (0 = the
+    library's default) and ``table`` = [(config, ms per column)] for every candidate timed."""
     import torch
 
     sig = signature(core, precision, nrhs)
@@ -44,7 +62,7 @@ def tune(core, nrhs: int = 32, precision: str = "mixed", candidates=DEFAULT_CAND
             cache = json.load(fh)
         if sig in cache:
             hit = cache[sig]
-            return int(hit["best"]), {int(k): v for k, v in hit["ms_per_column"].items()}
+            return dict(hit["best"]), [(dict(c), ms) for c, ms in hit["table"]]
     n_out, n_in = core.shape
     cplx = core.dtype == "c64"
     tdt = torch.complex64 if cplx else torch.float32
@@ -54,46 +72,53 @@ def tune(core, nrhs: int = 32, precision: str = "mixed", candidates=DEFAULT_CAND
         X = X + 1j * torch.randn((nrhs, n_in), generator=g, dtype=torch.float32)
     X = X.to(tdt).cuda().t()
     Y = torch.zeros((nrhs, n_out), dtype=tdt, device="cuda").t()
-    results = {}
-    for block in candidates:
-        if block > nrhs:
-            continue
-        ws = H2Workspace(core, max_rhs=block, precision=precision, use_graph=use_graph)
-        mul_(Y, ws, X)  # warm (captures graphs)
-        torch.cuda.synchronize()
-        best = float("inf")
-        for _ in range(repeats):
-            t0 = time.perf_counter()
-            mul_(Y, ws, X)
+    table = []
+    timed = {}
+
+    def measure(cfg):
+        key = (cfg["max_rhs"], cfg["m2l_groups"], cfg["near_cols"])
+        if key in timed:
+            return timed[key]
+        ws = H2Workspace(core, precision=precision, use_graph=use_graph, **cfg)
+        try:
+            mul_(Y, ws, X)  # warm (captures the graphs)
             torch.cuda.synchronize()
-            best = min(best, time.perf_counter() - t0)
-        results[block] = 1e3 * best / nrhs
-        ws.close()
-    best_block = min(results, key=results.get)
+            best = float("inf")
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                mul_(Y, ws, X)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+        finally:
+            ws.close()
+        ms = 1e3 * best / nrhs
+        timed[key] = ms
+        table.append((dict(cfg), ms))
+        return ms
+
+    best = {"max_rhs": min(nrhs, max(blocks)), "m2l_groups": 0, "near_cols": 0}
+    for axis, values in (("max_rhs", [b for b in blocks if b <= nrhs] or [nrhs]),
+                         ("m2l_groups", list(m2l_groups) if _lattice(core) and not cplx else []),
+                         ("near_cols", list(near_cols) if not cplx else [])):
+        if not values:
+            continue
+        scores = {}
+        for v in values:
+            cfg = dict(best, **{axis: v})
+            scores[v] = measure(cfg)
+        best[axis] = min(scores, key=scores.get)
     if cache_path:
         cache = {}
         if os.path.exists(cache_path):
             with open(cache_path) as fh:
                 cache = json.load(fh)
-        cache[sig] = {"best": best_block, "ms_per_column": {str(k): v for k, v in results.items()}}
+        cache[sig] = {"best": best, "table": table}
         with open(cache_path, "w") as fh:
             json.dump(cache, fh, indent=1)
-    return best_block, results
+    return best, table
 
 
-def tuned_workspace(core, nrhs: int = 32, precision: str = "mixed", **kw):
-    """A workspace with the tuned RHS block."""
-    block, _ = tune(core, nrhs=nrhs, precision=precision, **kw)
-    return H2Workspace(core, max_rhs=block, precision=precision, use_graph=True)
-
-
-def _candidates_fit(core, candidates, budget_bytes):
-    """Drop blocks whose workspace would not fit ``budget_bytes`` (estimated)."""
-    a = core.arrays
-    es = 8
-    out = []
-    for b in candidates:
-        ws = b * (a.test.tree.n_boxes * a.K * es * 2 + 2 * (2 * a.p - 1) ** 3 * a.test.tree.n_boxes * es)
-        if ws <= budget_bytes:
-            out.append(b)
-    return tuple(out) or (min(candidates),)
+def tuned_workspace(core, nrhs: int = 32, precision: str = "mixed", use_graph: bool = True, stream=None, **kw):
+    """A workspace with the tuned RHS block and tile shapes."""
+    best, _ = tune(core, nrhs=nrhs, precision=precision, **kw)
+    return H2Workspace(core, precision=precision, use_graph=use_graph, stream=stream, **best)

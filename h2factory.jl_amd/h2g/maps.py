@@ -208,9 +208,11 @@ class H2Workspace:
 
     op = H.H2G_OP_N
 
-    def __init__(self, core: GpuH2Core, max_rhs=1, precision="plain", use_graph=False, timing=False, stream=None):
+    def __init__(self, core: GpuH2Core, max_rhs=1, precision="plain", use_graph=False, timing=False, stream=None,
+                 m2l_groups=0, near_cols=0):
         """``stream``: a HIP stream handle (int) the workspace runs on, shared with other workspaces,
-        or None for streams of its own."""
+        or None for streams of its own.  ``m2l_groups`` / ``near_cols``: tile shapes (0: the
+        library's defaults; ``h2g.autotune.tune`` measures them for a map)."""
         self.core = core
         o = H.WsOpts()
         o.stream = stream
@@ -218,7 +220,10 @@ class H2Workspace:
         o.max_rhs = int(max_rhs)
         o.use_graph = int(bool(use_graph))
         o.timing = int(bool(timing))
-        self.opts = dict(max_rhs=max_rhs, precision=precision, use_graph=use_graph, timing=timing, stream=stream)
+        o.m2l_groups = int(m2l_groups)
+        o.near_cols = int(near_cols)
+        self.opts = dict(max_rhs=max_rhs, precision=precision, use_graph=use_graph, timing=timing, stream=stream,
+                         m2l_groups=m2l_groups, near_cols=near_cols)
         h = C.c_void_p()
         _raise(H.lib().h2g_ws_create(core.handle, C.byref(o), C.byref(h)))
         self.handle = h

@@ -92,15 +92,18 @@ def _pinned(shape, dtype):
 
 
 class _ResidentMap:
-    def __init__(self, name, core, n_ws, max_batch, precision, dtype, streams, use_graph):
+    def __init__(self, name, core, n_ws, max_batch, precision, dtype, streams, use_graph, tile=None):
         self.name = name
         self.core = core
         self.queues = {op: deque() for op in OPS}
         self.slots = []
+        self.tile = dict(tile or {})
         n_out, n_in = core.shape
         n = max(n_out, n_in)
+        shapes = {k: self.tile.get(k, 0) for k in ("m2l_groups", "near_cols")}
         for _ in range(n_ws):
-            ws = H2Workspace(core, max_rhs=max_batch, precision=precision, use_graph=use_graph, stream=streams())
+            ws = H2Workspace(core, max_rhs=max_batch, precision=precision, use_graph=use_graph, stream=streams(),
+                             **shapes)
             xh, tx = _pinned((n, max_batch), dtype)
             yh, ty = _pinned((n, max_batch), dtype)
             self.slots.append(_Slot(ws, xh, yh, (tx, ty)))
@@ -188,15 +191,18 @@ class Server:
     """Continuous-batching request loop over resident maps."""
 
     def __init__(self, budget_bytes: int, max_batch: int = 32, ws_per_map: int = 2, precision: str = "mixed",
-                 n_streams: int = 4, use_graph: bool = False):
+                 n_streams: int = 4, use_graph: bool = False, autotune: bool = False):
         """``use_graph``: capture each (orientation, batch width) product of a workspace in a hipGraph
         on first use (worth it for maps whose products are launch-bound; a stream of single
-        requests produces every width from 1 to ``max_batch``)."""
+        requests produces every width from 1 to ``max_batch``).  ``autotune``: measure each
+        admitted map's tile shapes (``h2g.autotune.tune`` at ``max_batch`` columns) before its
+        workspaces are made."""
         self.budget = int(budget_bytes)
         self.max_batch = max_batch
         self.ws_per_map = ws_per_map
         self.precision = precision
         self.use_graph = use_graph
+        self.autotune = autotune
         self.maps = {}
         self.done = []
         self._n_streams = max(1, n_streams)
@@ -237,7 +243,7 @@ class Server:
         core = GpuH2Core(arrays, precisions=("plain", self.precision))
         dtype = np.complex64 if arrays.dtype == "c64" else np.float32
         rm = _ResidentMap(name, core, self.ws_per_map, self.max_batch, self.precision, dtype, self._stream,
-                          self.use_graph)
+                          self.use_graph, self._tile(core))
         self.maps[name] = rm
         if self.used_bytes() > self.budget:
             rm.close()
@@ -258,10 +264,19 @@ class Server:
                                  f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
         dtype = np.complex64 if a.dtype == "c64" else np.float32
         rm = _ResidentMap(name, core, self.ws_per_map, self.max_batch, self.precision, dtype, self._stream,
-                          self.use_graph)
+                          self.use_graph, self._tile(core))
         self.maps[name] = rm
         rm.calibrate()
         return rm
+
+    def _tile(self, core):
+        if not self.autotune:
+            return None
+        from .autotune import tune
+
+        best, _ = tune(core, nrhs=self.max_batch, precision=self.precision, blocks=(self.max_batch,), repeats=2,
+                       use_graph=self.use_graph)
+        return best
 
     def remove_map(self, name):
         self.maps.pop(name).close()

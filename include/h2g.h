@@ -172,6 +172,13 @@ typedef struct h2g_ws_opts {
                                      e.g. a serving loop's pool of as many streams as the device has
                                      hardware queues; the near field then runs in order with the far
                                      field), or NULL for two streams of its own */
+    /* Tile shapes (0: the library's default; the autotuner in h2g.autotune measures them per map): */
+    int32_t m2l_groups;           /* lattice Toeplitz M2L of wide FP32 batches: workgroups per
+                                     frequency (groups of Morton-consecutive parent blocks, at least
+                                     4 blocks each); default 512                                   */
+    int32_t near_cols;            /* near field of real maps: columns per workgroup, 16, 32 or 64
+                                     (one to four 16-column MFMA tiles per wave); default
+                                     min(64, 16 ceil(nrhs / 16))                                   */
 } h2g_ws_opts;
 
 /* One per-phase timing record, shaped like the reference's TimingRecord

@@ -11,7 +11,7 @@
 
 namespace h2g {
 
-bool launch_near_mfma(int, const Core &, const void *, void *, int, float *, hipStream_t) { return false; }
+bool launch_near_mfma(int, const Core &, const void *, void *, int, float *, int, hipStream_t) { return false; }
 // the FP32 row-major near values feed only the MFMA near kernel, which declines here
 void build_near_frags(Core &) {}
 bool launch_p2m_mfma(int, const int32_t *, int, const int64_t *, const int64_t *, const void *, int, const void *,
@@ -37,7 +37,7 @@ void launch_dense(const DenseOps &, int, int, const int32_t *, int, const int64_
     throw Error{H2G_EUNSUPPORTED, "host replay: the lattice Toeplitz kernels run on the GPU only"};
 }
 int m2l_direct_rows(int, const LatticeLevel &, int, int) { lattice_refused(); }
-void launch_m2l_lattice(int, bool, const LatticeLevel &, int, int, const void *, void *, int, hipStream_t) {
+void launch_m2l_lattice(int, bool, const LatticeLevel &, int, int, const void *, void *, int, int, hipStream_t) {
     lattice_refused();
 }
 void build_lattice_ea(const int32_t *, const int32_t *, const int32_t *, const void *, int, bool, bool, LatticeLevel &) {

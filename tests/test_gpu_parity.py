@@ -153,6 +153,81 @@ def test_laplace_method_high_order(gpu, kind, p, nodes):
             assert relerr(y32, y64) < 1e-6, (op, nrhs)
 
 
+def _tile_case():
+    pts = np.random.default_rng(31).random((12000, 3))
+    core = OB.build_core(pts, p=4, nodekind="equi", m2l_kind="toeplitz", leaf_size=32)
+    return pts, core
+
+
+@pytest.mark.parametrize("nrhs", [32, 40])
+def test_tile_shapes_do_not_change_the_product(gpu, nrhs):
+    """The workspace tile shapes (coupling-kernel groups per frequency, near-field columns per
+    workgroup) change only the schedule: every shape gives the default shape's mixed product to
+    the FP32 rounding level, and the plain FP64 product within the 1e-6 gate."""
+    import h2g
+    from convert import to_arrays
+
+    pts, core = _tile_case()
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
+    from h2g.diagnostics import h2summary
+
+    assert h2summary(c).latticelevels > 0
+    X = np.random.default_rng(32).standard_normal((12000, nrhs)).astype(np.float32)
+    plain = h2g.H2Workspace(c, max_rhs=nrhs, precision="plain")
+    for op in "NT":
+        y64 = np.zeros((12000, nrhs))
+        h2g.mul_(y64, orient(plain, op), X.astype(np.float64))
+        ref = None
+        for groups, cols in ((0, 0), (3, 64), (64, 16), (1024, 32), (100000, 0)):
+            A = h2g.H2Workspace(c, max_rhs=nrhs, precision="mixed", use_graph=True, m2l_groups=groups,
+                                near_cols=cols)
+            y = np.zeros((12000, nrhs), dtype=np.float32)
+            h2g.mul_(y, orient(A, op), X)
+            A.close()
+            assert relerr(y, y64) < 1e-6, (op, groups, cols)
+            if ref is None:
+                ref = y
+            assert relerr(y, ref) < 1e-6, (op, groups, cols)
+
+
+def test_tile_shape_arguments_are_checked(gpu):
+    import h2g
+    from convert import to_arrays
+
+    _, core = _tile_case()
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
+    for kw in ({"near_cols": 24}, {"m2l_groups": -1}):
+        with pytest.raises(h2g.ArgumentError):
+            h2g.H2Workspace(c, max_rhs=8, precision="mixed", **kw)
+
+
+def test_autotune_picks_a_measured_shape(gpu, tmp_path):
+    """tune() times every candidate of each axis, returns the fastest, caches it per map, and the
+    tuned workspace computes the default workspace's product."""
+    import h2g
+    from convert import to_arrays
+    from h2g.autotune import tune, tuned_workspace
+
+    _, core = _tile_case()
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
+    cache = str(tmp_path / "tune.json")
+    best, table = tune(c, nrhs=32, precision="mixed", blocks=(16, 32), m2l_groups=(128, 512), near_cols=(16, 32),
+                       repeats=1, cache_path=cache)
+    assert best["max_rhs"] in (16, 32) and best["m2l_groups"] in (128, 512) and best["near_cols"] in (16, 32)
+    tried = {(t["max_rhs"], t["m2l_groups"], t["near_cols"]) for t, _ in table}
+    assert (best["max_rhs"], best["m2l_groups"], best["near_cols"]) in tried
+    assert all(ms > 0 for _, ms in table) and len(table) >= 5
+    best2, table2 = tune(c, nrhs=32, precision="mixed", cache_path=cache)  # served from the cache
+    assert best2 == best and len(table2) == len(table)
+    X = np.random.default_rng(33).standard_normal((12000, 32)).astype(np.float32)
+    y0 = np.zeros((12000, 32), dtype=np.float32)
+    y1 = np.zeros((12000, 32), dtype=np.float32)
+    h2g.mul_(y0, h2g.H2Workspace(c, max_rhs=32, precision="mixed"), X)
+    T = tuned_workspace(c, nrhs=32, precision="mixed", cache_path=cache)
+    h2g.mul_(y1, T, X)
+    assert relerr(y1, y0) < 1e-6
+
+
 @pytest.mark.skip(reason="faulted the GPU twice (memory aperture violation) with the separable-transfer kernel "
                          "for ComplexF64; that path is switched off for complex maps and this case stays off the "
                          "GPU until the cause is shown")
