@@ -27,7 +27,7 @@ EXPORTED = [
     "h2g_ws_create", "h2g_ws_destroy", "h2g_ws_device_bytes", "h2g_ws_stream", "h2g_ws_acquire",
     "h2g_ws_release", "h2g_matvec", "h2g_ws_sync", "h2g_ws_timing", "h2g_last_error", "h2g_version",
     "h2g_method_fill", "h2g_ws_query", "h2g_matvec_noalloc", "h2g_ws_set_stream", "h2g_core_summary",
-    "h2g_core_sanitycheck",
+    "h2g_core_sanitycheck", "h2g_plan_build", "h2g_plan_info", "h2g_plan_copy", "h2g_plan_destroy",
 ]
 
 VP = C.c_void_p
@@ -96,6 +96,29 @@ class Summary(C.Structure):
                 ("dense_mfma", C.c_int32), ("device_bytes", C.c_int64)]
 
 
+class PlanOpts(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("size", C.c_double), ("leaf_size", C.c_int32),
+                ("max_level", C.c_int32), ("symmetric", C.c_int32), ("device", C.c_int32)]
+
+
+class PlanCounts(C.Structure):
+    _fields_ = [("n_dofs", C.c_int64 * 2), ("n_levels", C.c_int32 * 2), ("n_boxes", C.c_int32 * 2),
+                ("n_pairs", C.c_int64), ("n_keys", C.c_int64), ("n_near_pairs", C.c_int64),
+                ("n_near_values", C.c_int64), ("galerkin", C.c_int32)]
+
+
+class PlanTreeOut(C.Structure):
+    _fields_ = [("perm", I64P), ("level_offsets", I64P), ("level", I32P), ("coord", I32P),
+                ("key", C.POINTER(C.c_uint64)), ("parent", I32P), ("sector", I32P), ("begin", I64P),
+                ("end", I64P), ("is_leaf", U8P)]
+
+
+class PlanOut(C.Structure):
+    _fields_ = [("test", PlanTreeOut), ("trial", PlanTreeOut), ("pair_test", I32P), ("pair_trial", I32P),
+                ("pair_op", I32P), ("pair_sym", I32P), ("keys", I32P), ("near_test", I32P),
+                ("near_trial", I32P), ("near_offset", I64P)]
+
+
 class Issue(C.Structure):
     _fields_ = [("severity", C.c_int32), ("code", C.c_int32), ("count", C.c_int64), ("level", C.c_int32)]
 
@@ -134,6 +157,10 @@ def lib():
         L.h2g_ws_timing.argtypes = [VP, C.POINTER(TimingRecord), C.c_int32, I32P]
         L.h2g_last_error.argtypes = [C.c_char_p, C.c_size_t]
         L.h2g_method_fill.argtypes = [VP, C.POINTER(MethodDesc)]
+        L.h2g_plan_build.argtypes = [F64P, C.c_int64, F64P, C.c_int64, C.POINTER(PlanOpts), C.POINTER(VP)]
+        L.h2g_plan_info.argtypes = [VP, C.POINTER(PlanCounts)]
+        L.h2g_plan_copy.argtypes = [VP, C.POINTER(PlanOut)]
+        L.h2g_plan_destroy.argtypes = [VP]
         _lib = L
     return _lib
 

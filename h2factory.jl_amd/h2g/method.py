@@ -200,26 +200,14 @@ def _transfers(tree: T.Octree, nodes):
     return out
 
 
-def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="laplace", wavenumber=0.0,
-              m2l_kind="toeplitz", leaf_size=64, alpha=1.0, host_values=False, origin=(0.0, 0.0, 0.0),
-              size=1.0) -> MapArrays:
-    """Tree, plans and operators for the interpolation method.
-
-    ``host_values`` computes the near values and the leaf collections on the host (small
-    problems, exact comparisons); otherwise they are filled on the device by
-    ``h2g_method_fill`` after the core is created.
-    """
-    if m2l_kind in ("toeplitz", "symtoeplitz") and nodekind != "equi":
-        raise ValueError("Toeplitz couplings need equispaced nodes")
-    test = T.build_octree(points_test, leaf_size, origin, size)
-    trial = test if points_trial is None else T.build_octree(points_trial, leaf_size, origin, size)
-    nodes = nodes1d(p, nodekind)
-    K = p ** 3
-    pairs = T.m2l_pairs(test, trial)
+def translation_ids(test, trial, pairs, symmetric):
+    """Translation id of each M2L pair: the rank of its (level, offset) -- (level, canonical
+    offset) and a symmetry id when ``symmetric`` -- among the present ones.  Returns
+    (pair_op, pair_sym or None, the (level, offset) rows of the ids)."""
     offs = trial.coord[pairs[:, 1]] - test.coord[pairs[:, 0]]
     lv = test.level[pairs[:, 0]]
     sym = None
-    if m2l_kind.startswith("sym"):
+    if symmetric:
         canon, sym = canonicalize(offs)
         keyarr = np.concatenate([lv[:, None], canon], axis=1)
     else:
@@ -232,7 +220,38 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
         uniq = keyarr[first]
     else:
         uniq, pair_op = np.unique(keyarr, axis=0, return_inverse=True)
-    pair_op = pair_op.reshape(-1)
+    return pair_op.reshape(-1), sym, uniq
+
+
+def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="laplace", wavenumber=0.0,
+              m2l_kind="toeplitz", leaf_size=64, alpha=1.0, host_values=False, origin=(0.0, 0.0, 0.0),
+              size=1.0, plan="host") -> MapArrays:
+    """Tree, plans and operators for the interpolation method.
+
+    ``host_values`` computes the near values and the leaf collections on the host (small
+    problems, exact comparisons); otherwise they are filled on the device by
+    ``h2g_method_fill`` after the core is created.  ``plan="device"`` builds the trees, the
+    M2L pairs with their translation ids and the near pairs on the GPU (``h2g_plan_build``; the
+    same arrays as the host builder, bit for bit).
+    """
+    if m2l_kind in ("toeplitz", "symtoeplitz") and nodekind != "equi":
+        raise ValueError("Toeplitz couplings need equispaced nodes")
+    if plan not in ("host", "device"):
+        raise ValueError(f"plan must be 'host' or 'device', not {plan!r}")
+    nodes = nodes1d(p, nodekind)
+    K = p ** 3
+    dplan = None
+    if plan == "device":
+        from .plan import device_plan
+
+        dplan = device_plan(points_test, points_trial, leaf_size, origin, size, symmetric=m2l_kind.startswith("sym"))
+        test, trial = dplan.test, dplan.trial
+        pairs, pair_op, sym, uniq = dplan.pairs, dplan.pair_op, dplan.pair_sym, dplan.keys
+    else:
+        test = T.build_octree(points_test, leaf_size, origin, size)
+        trial = test if points_trial is None else T.build_octree(points_trial, leaf_size, origin, size)
+        pairs = T.m2l_pairs(test, trial)
+        pair_op, sym, uniq = translation_ids(test, trial, pairs, m2l_kind.startswith("sym"))
     ops = []
     for l in np.unique(uniq[:, 0]) if len(uniq) else []:
         sel = uniq[:, 0] == l
@@ -257,12 +276,15 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
     elif m2l_kind == "symtoeplitz":
         perms = np.stack([toeplitz_perm(2 * p - 1, g) for g in GROUP])
 
-    npairs = T.near_pairs(test, trial)
+    npairs = dplan.near_pairs if dplan is not None else T.near_pairs(test, trial)
     nr = test.end[npairs[:, 0]] - test.begin[npairs[:, 0]]
     nc = trial.end[npairs[:, 1]] - trial.begin[npairs[:, 1]]
     sizes = nr * nc
-    near_offset = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if len(sizes) else np.zeros(0, np.int64)
-    n_near_values = int(sizes.sum())
+    if dplan is not None:
+        near_offset, n_near_values = dplan.near_offset, dplan.n_near_values
+    else:
+        near_offset = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if len(sizes) else np.zeros(0, np.int64)
+        n_near_values = int(sizes.sum())
     near_values = None
     dt = np.complex128 if is_complex else np.float64
     ts = SideArrays(test, transfers=_transfers(test, nodes))

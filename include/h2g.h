@@ -324,6 +324,80 @@ typedef struct h2g_method_desc {
 
 int h2g_method_fill(h2g_core *core, const h2g_method_desc *desc);
 
+/* ---- Tree and traversal plans, built on the device ----------------------------------------
+ * The octree of each side and the pair lists a core description needs (h2g_core_desc: the side
+ * arrays, pair_test / pair_trial / pair_op / pair_sym, near_test / near_trial / near_offset).
+ * In the reference they come from H2Trees, which the method hands to H2Factory: the tree, the
+ * plans of assembleplans (src/maps/interactionstrategy.jl:50-60: isnear and the translating
+ * nodes) and the translation ids of translationmetadata
+ * (src/farinteractions/i2otranslations/i2otranslationcollection.jl:19-84).  Definitions:
+ *   - points get 21-bit integer coordinates floor((x - origin) / size * 2^21) (clipped) and are
+ *     put in tree order by a stable sort of their Morton codes (x in the lowest bit);
+ *   - a box is split while it holds more than leaf_size points and its level is below
+ *     max_level; only non-empty children exist; boxes are stored level by level, Morton order
+ *     within a level; a child's sector is bx + 2 by + 4 bz;
+ *   - M2L pairs: same-level boxes whose parents are adjacent and that are not adjacent, grouped
+ *     by test box, trial boxes ascending; translation id = rank of the pair's (level, offset)
+ *     among the present ones, or of (level, canonical offset) for symmetric plans, where the
+ *     canonical offset sorts |offset| ascending and pair_sym is the index of the signed
+ *     permutation S (S canonical = offset) in the 48-element group enumerated as permutations of
+ *     (0, 1, 2) in lexicographic order times sign patterns (+++), (++-), ..., (---);
+ *   - near pairs: a test leaf and a trial leaf whose ancestors at the coarser of the two leaf
+ *     levels are adjacent, grouped by test leaf, trial ascending; near_offset = exclusive prefix
+ *     sum of the block sizes n_rows * n_cols.
+ * Points are host arrays [n][3].  h2g_plan_info gives the sizes; h2g_plan_copy copies into
+ * host arrays of those sizes (NULL fields are skipped). */
+typedef struct h2g_plan_opts {
+    double origin[3];             /* lower corner of the root cube                            */
+    double size;                  /* edge of the root cube                                    */
+    int32_t leaf_size;            /* split while a box holds more points than this            */
+    int32_t max_level;            /* deepest level (1..21); 0 means 21                        */
+    int32_t symmetric;            /* 1: translation ids over canonical offsets + pair_sym     */
+    int32_t device;               /* HIP device ordinal                                       */
+} h2g_plan_opts;
+
+typedef struct h2g_plan h2g_plan;
+
+typedef struct h2g_plan_counts {
+    int64_t n_dofs[2];            /* [test, trial] (trial == test for a Galerkin plan)         */
+    int32_t n_levels[2];
+    int32_t n_boxes[2];
+    int64_t n_pairs;              /* M2L pairs                                                */
+    int64_t n_keys;               /* distinct translation ids                                 */
+    int64_t n_near_pairs;
+    int64_t n_near_values;        /* sum of the near block sizes                              */
+    int32_t galerkin;
+} h2g_plan_counts;
+
+typedef struct h2g_plan_tree_out {
+    int64_t *perm;                /* [n_dofs] tree position -> original point                 */
+    int64_t *level_offsets;       /* [n_levels + 1]                                           */
+    int32_t *level;               /* [n_boxes]                                                */
+    int32_t *coord;               /* [n_boxes][3] integer coordinate at the box's level       */
+    uint64_t *key;                /* [n_boxes] Morton key at the box's level                  */
+    int32_t *parent;              /* [n_boxes], -1 at the root                                */
+    int32_t *sector;              /* [n_boxes], -1 at the root                                */
+    int64_t *begin, *end;         /* [n_boxes] tree-order point range                         */
+    uint8_t *is_leaf;             /* [n_boxes]                                                */
+} h2g_plan_tree_out;
+
+typedef struct h2g_plan_out {
+    h2g_plan_tree_out test;
+    h2g_plan_tree_out trial;      /* ignored for a Galerkin plan                              */
+    int32_t *pair_test, *pair_trial, *pair_op;  /* [n_pairs]                                  */
+    int32_t *pair_sym;            /* [n_pairs], symmetric plans only                          */
+    int32_t *keys;                /* [n_keys][4]: (level, ox, oy, oz) of each translation id  */
+    int32_t *near_test, *near_trial;            /* [n_near_pairs]                             */
+    int64_t *near_offset;         /* [n_near_pairs]                                           */
+} h2g_plan_out;
+
+/* test_points [n_test][3]; trial_points NULL for a Galerkin plan (one tree for both sides). */
+int h2g_plan_build(const double *test_points, int64_t n_test, const double *trial_points, int64_t n_trial,
+                   const h2g_plan_opts *opts, h2g_plan **out);
+int h2g_plan_info(const h2g_plan *plan, h2g_plan_counts *out);
+int h2g_plan_copy(const h2g_plan *plan, const h2g_plan_out *out);
+int h2g_plan_destroy(h2g_plan *plan);
+
 /* Last error message of the calling thread. */
 int h2g_last_error(char *buf, size_t len);
 /* Library version (major * 10000 + minor * 100 + patch). */

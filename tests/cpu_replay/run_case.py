@@ -88,6 +88,17 @@ def _in_for(core, x, op):
 
 
 def main(name):
+    if name == "device_plan":
+        # trees and pair lists of the plan kernels (plan.hip) against the oracle
+        import plan_check as PC
+        from h2g.plan import device_plan
+
+        for case, a, b, leaf, origin, size, sym in PC.point_sets():
+            dp = device_plan(a, b, leaf, origin, size, symmetric=sym)
+            PC.check_against_oracle(dp, a, b, leaf, origin, size, sym)
+            print(f"{name} {case}: {dp.test.n_boxes} boxes, {len(dp.pairs)} M2L pairs, {len(dp.near_pairs)} near "
+                  f"pairs: equal to the oracle (vs oracle 0)", flush=True)
+        sys.exit(0)
     if name == "laplace_device_fill":
         # near values and leaf collections filled by the library (h2g_method_fill) against the
         # oracle's host assembly on the same points

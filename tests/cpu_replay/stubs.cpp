@@ -53,3 +53,43 @@ void launch_fft_out_fm(int, bool, bool, const int32_t *, int, const void *, int,
 }
 
 }  // namespace h2g
+
+// Device-wide sort and scan of the plan builder (prims.hip runs them on rocPRIM): host stand-ins.
+// The radix sorts order keys by their low end_bit bits, stably; the callers' keys have no bits above.
+#include <numeric>
+
+namespace h2g {
+
+void prim_sort_keys_u64(const uint64_t *in, uint64_t *out, int64_t n, int, hipStream_t) {
+    std::vector<uint64_t> v(in, in + n);
+    std::stable_sort(v.begin(), v.end());
+    std::copy(v.begin(), v.end(), out);
+}
+
+void prim_sort_pairs_u64(const uint64_t *kin, uint64_t *kout, const int64_t *vin, int64_t *vout, int64_t n, int,
+                         hipStream_t) {
+    std::vector<int64_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return kin[a] < kin[b]; });
+    std::vector<uint64_t> k(n);
+    std::vector<int64_t> v(n);
+    for (int64_t i = 0; i < n; ++i) k[i] = kin[idx[i]], v[i] = vin[idx[i]];
+    std::copy(k.begin(), k.end(), kout);
+    std::copy(v.begin(), v.end(), vout);
+}
+
+void prim_inclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t) {
+    int64_t s = 0;
+    for (int64_t i = 0; i < n; ++i) out[i] = (s += in[i]);
+}
+
+void prim_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t) {
+    int64_t s = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t v = in[i];
+        out[i] = s;
+        s += v;
+    }
+}
+
+}  // namespace h2g
