@@ -48,7 +48,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plan", default="device", choices=["device", "host"],
-                    help="build the trees and pair lists on the GPU (h2g_plan_build) or with the host builder")
+                    help="build the trees, pair lists and operators on the GPU (h2g_plan_build, "
+                         "h2g_method_operators) or with the host builders")
+    ap.add_argument("--tile", default=None, help="RHS block / coupling groups / near-field columns, e.g. 32/256/32: "
+                                                  "use this tile instead of autotuning (0 = the library default)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (RHS block and tile shapes); use the library's defaults")
     ap.add_argument("--serve-requests", type=int, default=4000,
@@ -112,9 +115,10 @@ def traffic_of(kernel, n, p, nrhs, groups):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    if d.get("kernel") != kernel or d.get("m2l_groups", 512) != (groups or 512):
-        return None
-    return d["traffic_bytes_per_launch"]
+    for m in d.get("measurements", []):
+        if m.get("kernel") == kernel and m.get("m2l_groups") == (groups or 512):
+            return m["traffic_bytes_per_launch"]
+    return None
 
 
 def cpu_model():
@@ -168,7 +172,7 @@ def helmholtz_main(args):
     pts /= np.linalg.norm(pts, axis=1, keepdims=True)
     kw = dict(p=6, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=k, leaf_size=64,
               origin=(-1.0, -1.0, -1.0), size=2.0)
-    arrays = PM.build_map(pts, plan=args.plan, **kw)
+    arrays = PM.build_map(pts, plan=args.plan, ops=args.plan, **kw)
     t_build = time.perf_counter() - t0
     log(f"map built ({args.plan} plan) in {t_build:.1f}s: {arrays.stats}")
     t0 = time.perf_counter()
@@ -285,9 +289,9 @@ def build_spec(spec, seed):
         box = dict(origin=(-1.0, -1.0, -1.0), size=2.0)
     if spec["kernel"] == "laplace":
         return PM.build_map(pts, p=spec["p"], nodekind="equi", m2l_kind="toeplitz", leaf_size=64, plan="device",
-                            **box)
+                            ops="device", **box)
     return PM.build_map(pts, p=spec["p"], nodekind="cheb", m2l_kind="dense", kernel="helmholtz",
-                        wavenumber=spec["k"], leaf_size=64, plan="device", **box)
+                        wavenumber=spec["k"], leaf_size=64, plan="device", ops="device", **box)
 
 
 def serving_leg(args):
@@ -432,7 +436,8 @@ def main():
 
     t0 = time.perf_counter()
     pts = np.random.default_rng(0).random((N, 3))
-    arrays = PM.build_map(pts, p=args.p, nodekind="equi", m2l_kind=args.m2l, leaf_size=64, plan=args.plan)
+    arrays = PM.build_map(pts, p=args.p, nodekind="equi", m2l_kind=args.m2l, leaf_size=64, plan=args.plan,
+                          ops=args.plan)
     t_build = time.perf_counter() - t0
     log(f"map built ({args.plan} plan) in {t_build:.1f}s: {arrays.stats}")
     t0 = time.perf_counter()
@@ -448,7 +453,10 @@ def main():
     # RHS block and tile shapes: measured on this map by the autotuner (outside the timed region)
     tile = {"max_rhs": R, "m2l_groups": 0, "near_cols": 0}
     tune_table = []
-    if not args.no_tune:
+    if args.tile:
+        mr, mg, nc = (int(v) for v in args.tile.split("/"))
+        tile = {"max_rhs": mr, "m2l_groups": mg, "near_cols": nc}
+    elif not args.no_tune:
         from h2g.autotune import tune
 
         t0 = time.perf_counter()

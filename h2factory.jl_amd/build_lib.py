@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["api.hip", "kernels.hip", "leaf_mfma.hip", "method.hip", "m2l_lattice.hip", "m2l_mfma.hip",
-           "dense_m2l.hip", "noalloc.hip", "plan.hip", "prims.hip"]
+           "dense_m2l.hip", "noalloc.hip", "plan.hip", "prims.hip", "operators.hip"]
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

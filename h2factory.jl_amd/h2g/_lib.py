@@ -28,6 +28,7 @@ EXPORTED = [
     "h2g_ws_release", "h2g_matvec", "h2g_ws_sync", "h2g_ws_timing", "h2g_last_error", "h2g_version",
     "h2g_method_fill", "h2g_ws_query", "h2g_matvec_noalloc", "h2g_ws_set_stream", "h2g_core_summary",
     "h2g_core_sanitycheck", "h2g_plan_build", "h2g_plan_info", "h2g_plan_copy", "h2g_plan_destroy",
+    "h2g_method_operators",
 ]
 
 VP = C.c_void_p
@@ -96,6 +97,16 @@ class Summary(C.Structure):
                 ("dense_mfma", C.c_int32), ("device_bytes", C.c_int64)]
 
 
+class OpsDesc(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("wavenumber", C.c_double), ("p", C.c_int32), ("nodes", F64P),
+                ("root_size", C.c_double), ("m2l_kind", C.c_int32), ("n_keys", C.c_int64), ("keys", I32P),
+                ("device", C.c_int32)]
+
+
+class OpsOut(C.Structure):
+    _fields_ = [("ops", VP), ("transfers", F64P), ("perms", I32P)]
+
+
 class PlanOpts(C.Structure):
     _fields_ = [("origin", C.c_double * 3), ("size", C.c_double), ("leaf_size", C.c_int32),
                 ("max_level", C.c_int32), ("symmetric", C.c_int32), ("device", C.c_int32)]
@@ -161,6 +172,7 @@ def lib():
         L.h2g_plan_info.argtypes = [VP, C.POINTER(PlanCounts)]
         L.h2g_plan_copy.argtypes = [VP, C.POINTER(PlanOut)]
         L.h2g_plan_destroy.argtypes = [VP]
+        L.h2g_method_operators.argtypes = [C.POINTER(OpsDesc), C.POINTER(OpsOut)]
         _lib = L
     return _lib
 

@@ -170,6 +170,7 @@ class MapArrays:
     near_offset: np.ndarray
     n_near_values: int
     near_values: np.ndarray = None     # None: device fill
+    ops_keys: np.ndarray = None        # (n_ops, 4): level and (canonical) offset of each translation id
     alpha: complex = 1.0
     # method data for device assembly
     kernel: str = "laplace"
@@ -192,9 +193,11 @@ def _side_collection(tree: T.Octree, nodes, K):
     return C
 
 
-def _transfers(tree: T.Octree, nodes):
+def _transfers(tree: T.Octree, nodes, per=None):
+    """Per level and sector T^T (column-major T); ``per``: the 8 sector matrices already built."""
     K = len(nodes) ** 3
-    per = np.stack([transfer_matrix(nodes, s).T for s in range(8)])
+    if per is None:
+        per = np.stack([transfer_matrix(nodes, s).T for s in range(8)])
     out = np.broadcast_to(per, (tree.n_levels, 8, K, K)).copy()
     out[0] = np.eye(K)
     return out
@@ -225,19 +228,22 @@ def translation_ids(test, trial, pairs, symmetric):
 
 def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="laplace", wavenumber=0.0,
               m2l_kind="toeplitz", leaf_size=64, alpha=1.0, host_values=False, origin=(0.0, 0.0, 0.0),
-              size=1.0, plan="host") -> MapArrays:
+              size=1.0, plan="host", ops="host") -> MapArrays:
     """Tree, plans and operators for the interpolation method.
 
     ``host_values`` computes the near values and the leaf collections on the host (small
     problems, exact comparisons); otherwise they are filled on the device by
     ``h2g_method_fill`` after the core is created.  ``plan="device"`` builds the trees, the
     M2L pairs with their translation ids and the near pairs on the GPU (``h2g_plan_build``; the
-    same arrays as the host builder, bit for bit).
+    same arrays as the host builder, bit for bit).  ``ops="device"`` builds the couplings, the
+    level transfers and the symmetry permutations on the GPU (``h2g_method_operators``).
     """
     if m2l_kind in ("toeplitz", "symtoeplitz") and nodekind != "equi":
         raise ValueError("Toeplitz couplings need equispaced nodes")
     if plan not in ("host", "device"):
         raise ValueError(f"plan must be 'host' or 'device', not {plan!r}")
+    if ops not in ("host", "device"):
+        raise ValueError(f"ops must be 'host' or 'device', not {ops!r}")
     nodes = nodes1d(p, nodekind)
     K = p ** 3
     dplan = None
@@ -252,8 +258,13 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
         trial = test if points_trial is None else T.build_octree(points_trial, leaf_size, origin, size)
         pairs = T.m2l_pairs(test, trial)
         pair_op, sym, uniq = translation_ids(test, trial, pairs, m2l_kind.startswith("sym"))
+    dev_tr = None
+    if ops == "device":
+        from .operators import device_operators
+
+        dev_ops, dev_tr, dev_perms = device_operators(kernel, wavenumber, p, nodes, size, m2l_kind, uniq)
     ops = []
-    for l in np.unique(uniq[:, 0]) if len(uniq) else []:
+    for l in (np.unique(uniq[:, 0]) if len(uniq) else []) if dev_tr is None else []:
         sel = uniq[:, 0] == l
         width = test.size / 2.0 ** l
         if m2l_kind in ("dense", "symdense"):
@@ -262,7 +273,9 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
         else:
             ops.append(toeplitz_spectra(kernel, wavenumber, p, width, uniq[sel, 1:]))
     is_complex = kernel == "helmholtz"
-    if ops:
+    if dev_tr is not None:
+        ops = dev_ops
+    elif ops:
         ops = np.concatenate(ops)
     else:
         ops = np.zeros((0, K, K) if m2l_kind in ("dense", "symdense") else (0, 2 * p - 1, 2 * p - 1, 2 * p - 1))
@@ -271,7 +284,9 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
     else:
         ops = np.ascontiguousarray(ops.astype(np.complex128 if is_complex else np.float64))
     perms = None
-    if m2l_kind == "symdense":
+    if dev_tr is not None:
+        perms = dev_perms
+    elif m2l_kind == "symdense":
         perms = np.stack([grid_perm(p, g) for g in GROUP])
     elif m2l_kind == "symtoeplitz":
         perms = np.stack([toeplitz_perm(2 * p - 1, g) for g in GROUP])
@@ -287,8 +302,8 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
         n_near_values = int(sizes.sum())
     near_values = None
     dt = np.complex128 if is_complex else np.float64
-    ts = SideArrays(test, transfers=_transfers(test, nodes))
-    cs = ts if trial is test else SideArrays(trial, transfers=_transfers(trial, nodes))
+    ts = SideArrays(test, transfers=_transfers(test, nodes, dev_tr))
+    cs = ts if trial is test else SideArrays(trial, transfers=_transfers(trial, nodes, dev_tr))
     if host_values:
         near_values = np.zeros(n_near_values, dtype=dt)
         tp = test.points[test.perm]
@@ -310,5 +325,6 @@ def build_map(points_test, points_trial=None, p=6, nodekind="equi", kernel="lapl
                      m2l_kind=m2l_kind, pairs=pairs, pair_op=pair_op, pair_sym=sym, ops=ops, perms=perms,
                      near_pairs=npairs, near_offset=near_offset, n_near_values=n_near_values,
                      near_values=near_values, alpha=alpha, kernel=kernel, wavenumber=wavenumber, nodes=nodes,
+                     ops_keys=np.asarray(uniq, dtype=np.int64).reshape(-1, 4),
                      stats=dict(n_boxes=test.n_boxes, n_leaves=int(test.is_leaf.sum()), n_pairs=len(pairs),
                                 n_near_blocks=len(npairs), n_near_values=n_near_values, n_ops=len(ops)))

@@ -324,6 +324,43 @@ typedef struct h2g_method_desc {
 
 int h2g_method_fill(h2g_core *core, const h2g_method_desc *desc);
 
+/* ---- Operators of the point-kernel interpolation method, built on the device --------------
+ * The couplings of the given translation ids, the level transfers and the symmetry
+ * permutation tables, in the layouts h2g_core_desc takes.  The reference builds one operator per
+ * translation id from a representative pair (src/farinteractions/i2otranslations/
+ * i2otranslationcollection.jl:52-70), the Toeplitz spectra D = fft(kappa)
+ * (ext/H2Toeplitz/toeplitzi2ocollection.jl:14-31) and the permutations toeplitzpermutation /
+ * latticegridpermutation (src/farinteractions/i2otranslations/symmetryreducedcollection.jl:162-232).
+ *   DENSE kinds: ops[o] = M (K x K column-major), M[k, j] = G(x_k - y_j - width_l * offset_o),
+ *     x, y the p^3 grids (nodes scaled by width_l / 2, x fastest) of the receiving box at the
+ *     origin and of the translating box; Float64 for Laplace, ComplexF64 for Helmholtz.
+ *   TOEPLITZ kinds: ops[o] = D = fftn(kappa) (L^3 ComplexF64, x fastest, L = 2p-1),
+ *     kappa[d] = G(delta (d - (p-1) offset_o)) over wrap-order d, delta = width_l / (p-1).
+ *   transfers: [8] K x K column-major Float64, T_s[k, i] = parent basis k at child node i of
+ *     sector s (the core's transfers of every level but 0).
+ *   perms (SYM kinds): [48][K] grid permutations (SYMDENSE) or [48][L^3] spectrum permutations
+ *     (SYMTOEPLITZ), group enumerated as for h2g_plan_build's pair_sym.
+ * width_l = root_size / 2^level.  Outputs are host arrays; NULL fields are skipped. */
+typedef struct h2g_ops_desc {
+    int32_t kernel;               /* 0 = Laplace 1/(4 pi r), 1 = Helmholtz exp(i k r)/(4 pi r)  */
+    double wavenumber;
+    int32_t p;
+    const double *nodes;          /* [p] reference nodes in [-1, 1]                           */
+    double root_size;             /* edge of the level-0 box                                  */
+    int32_t m2l_kind;             /* H2G_M2L_DENSE, TOEPLITZ, SYMDENSE or SYMTOEPLITZ          */
+    int64_t n_keys;
+    const int32_t *keys;          /* [n_keys][4] (level, ox, oy, oz) of each translation id   */
+    int32_t device;
+} h2g_ops_desc;
+
+typedef struct h2g_ops_out {
+    void *ops;
+    double *transfers;
+    int32_t *perms;
+} h2g_ops_out;
+
+int h2g_method_operators(const h2g_ops_desc *desc, const h2g_ops_out *out);
+
 /* ---- Tree and traversal plans, built on the device ----------------------------------------
  * The octree of each side and the pair lists a core description needs (h2g_core_desc: the side
  * arrays, pair_test / pair_trial / pair_op / pair_sym, near_test / near_trial / near_offset).

@@ -27,3 +27,11 @@ def gpu():
     if not gpu_available():
         pytest.fail("GPU test collected without a GPU (run with -m 'not gpu' on the CPU)")
     return 0
+
+
+@pytest.fixture(autouse=True)
+def _gpu_runtime_first(request):
+    """GPU tests bring up torch's HIP runtime before the library's first call (torch does not
+    come up in a process where libh2g already initialised the device)."""
+    if request.node.get_closest_marker("gpu") is not None:
+        request.getfixturevalue("gpu")

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "h2factory.jl_amd", "csrc")
 OUT = os.path.join(HERE, "_build", "libh2g_replay.so")
-SOURCES = [os.path.join(CSRC, s) for s in ("api.hip", "kernels.hip", "method.hip", "plan.hip")] + [os.path.join(HERE, "stubs.cpp")]
+SOURCES = [os.path.join(CSRC, s) for s in ("api.hip", "kernels.hip", "method.hip", "plan.hip", "operators.hip")] + [os.path.join(HERE, "stubs.cpp")]
 
 
 def asan_runtime():

@@ -60,6 +60,7 @@ inline std::common_type_t<A, B> min(A a, B b) { return a < b ? a : b; }
 template <class A, class B>
 inline std::common_type_t<A, B> max(A a, B b) { return a < b ? b : a; }
 inline void sincos(double x, double *s, double *c) { *s = std::sin(x); *c = std::cos(x); }
+inline void sincospi(double x, double *s, double *c) { *s = std::sin(M_PI * x); *c = std::cos(M_PI * x); }
 
 // the dynamic LDS arrays the kernels declare (extern __shared__ unsigned char smem_raw[]),
 // in namespace h2g and in its unnamed namespace of each translation unit

@@ -99,6 +99,27 @@ def main(name):
             print(f"{name} {case}: {dp.test.n_boxes} boxes, {len(dp.pairs)} M2L pairs, {len(dp.near_pairs)} near "
                   f"pairs: equal to the oracle (vs oracle 0)", flush=True)
         sys.exit(0)
+    if name == "device_operators":
+        # couplings, spectra, transfers and permutations of the operator kernels (operators.hip)
+        import ops_check as OC
+
+        for case in OC.CASES:
+            e = OC.check(*case)
+            print(f"{name} {case}: largest deviation vs oracle {e:.2e}", flush=True)
+        from h2g import method as PM
+
+        pts = np.random.default_rng(6).random((1500, 3))
+        # (the lattice Toeplitz kernels run on the GPU only: a symmetry-reduced dense map here)
+        core = OB.build_core(pts, p=3, nodekind="cheb", m2l_kind="symdense", leaf_size=40)
+        arrays = PM.build_map(pts, p=3, nodekind="cheb", m2l_kind="symdense", leaf_size=40, plan="device",
+                              ops="device", host_values=True)
+        c = h2g.GpuH2Core(arrays)
+        A = h2g.H2Workspace(c, max_rhs=2)
+        x = np.random.default_rng(9).standard_normal((1500, 2))
+        ok = _products(name, A, core, x, "NTC", ("mul",))
+        A.close()
+        c.close()
+        sys.exit(0 if ok else 1)
     if name == "laplace_device_fill":
         # near values and leaf collections filled by the library (h2g_method_fill) against the
         # oracle's host assembly on the same points

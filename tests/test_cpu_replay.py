@@ -28,7 +28,7 @@ def replay_lib():
 
 @pytest.mark.parametrize("case", ["helmholtz_sphere", "laplace_device_fill", "random_petrov_symdense",
                                   "random_petrov_diagonal", "noalloc", "diagnostics",
-                                  "device_plan"])
+                                  "device_plan", "device_operators"])
 def test_replay_in_bounds_and_equal_to_oracle(replay_lib, case):
     """helmholtz_sphere is the map of the skipped GPU Helmholtz test (ComplexF64, p = 3 Chebyshev,
     dense couplings); laplace_device_fill compares the device-filled near values and collections
