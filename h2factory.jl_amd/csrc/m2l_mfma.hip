@@ -506,8 +506,11 @@ __global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
 #pragma unroll
         for (int g = 0; g < NCH; ++g) acc[g] = double4_m{0.0, 0.0, 0.0, 0.0};
         // a batch past the block's last (the trip's tail) loads clamped (valid) operands and
-        // skips its MFMAs
-        auto step = [&](int k, const float(&anc)[8], const BV(&bnc)[4], float(&ann)[8], BV(&bnn)[4]) {
+        // skips its MFMAs.  The FP32 partial sums f0 / f1 run over the DR_SETS batches of one
+        // trip (32 products each) and are added into the FP64 accumulators once per trip.
+        float4_m f0[NCH], f1[NCH];
+        auto step = [&](int k, auto u_, const float(&anc)[8], const BV(&bnc)[4], float(&ann)[8], BV(&bnn)[4]) {
+            constexpr int u = decltype(u_)::value;
             float ac[8], bc[8][NCH];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -537,17 +540,18 @@ __global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
             // (memory clobber) and not scheduled after the MFMAs (scheduling barrier)
             __asm__ volatile("" ::: "memory");
             __builtin_amdgcn_sched_barrier(0);
-            if (k < k1) {
-                // raised wave priority while this batch's MFMAs issue, so that they go out back to
-                // back rather than interleaved with other waves' loads (coupling stage 28.4 ->
-                // 27.4 ms on the N = 2M, p = 6 map)
-                __builtin_amdgcn_s_setprio(1);
-                float4_m f0[NCH], f1[NCH];
+            if constexpr (u == 0) {
 #pragma unroll
                 for (int g = 0; g < NCH; ++g) {
                     f0[g] = float4_m{0.f, 0.f, 0.f, 0.f};
                     f1[g] = f0[g];
                 }
+            }
+            if (k < k1) {
+                // raised wave priority while this batch's MFMAs issue, so that they go out back to
+                // back rather than interleaved with other waves' loads (coupling stage 28.4 ->
+                // 27.4 ms on the N = 2M, p = 6 map)
+                __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                 for (int i = 0; i < 8; i += 2)
 #pragma unroll
@@ -556,6 +560,8 @@ __global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
                         f1[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[i + 1], bc[i + 1][g], f1[g], 0, 0, 0);
                     }
                 __builtin_amdgcn_s_setprio(0);
+            }
+            if constexpr (u == DR_SETS - 1) {
 #pragma unroll
                 for (int g = 0; g < NCH; ++g)
 #pragma unroll
@@ -563,10 +569,10 @@ __global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
             }
         };
         for (int k = k0; k < k1; k += DR_SETS) {
-#pragma unroll
-            for (int u = 0; u < DR_SETS; ++u)
-                step(k + u, an[u], bn[u], an[(u + DR_SETS - 1) % DR_SETS], bn[(u + DR_SETS - 1) % DR_SETS]);
+            step(k, std::integral_constant<int, 0>{}, an[0], bn[0], an[DR_SETS - 1], bn[DR_SETS - 1]);
+            step(k + 1, std::integral_constant<int, 1>{}, an[1], bn[1], an[0], bn[0]);
         }
+        static_assert(DR_SETS == 2, "the trip above unrolls two batches");
 #pragma unroll
         for (int g = 0; g < NCH; ++g) {
             // FP32 16x16x4 result layout: rows 4 (l / 16) + v = (target 2 (l / 16) + v / 2, re / im v % 2)
