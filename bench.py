@@ -50,8 +50,9 @@ def parse():
     ap.add_argument("--plan", default="device", choices=["device", "host"],
                     help="build the trees, pair lists and operators on the GPU (h2g_plan_build, "
                          "h2g_method_operators) or with the host builders")
-    ap.add_argument("--tile", default=None, help="RHS block / coupling groups / near-field columns, e.g. 32/256/32: "
-                                                  "use this tile instead of autotuning (0 = the library default)")
+    ap.add_argument("--tile", default=None, help="RHS block / coupling groups / near-field columns [/ persistent], "
+                                                  "e.g. 32/256/32 or 32/512/32/1: use this tile instead of autotuning "
+                                                  "(0 = the library default)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (RHS block and tile shapes); use the library's defaults")
     ap.add_argument("--serve-requests", type=int, default=4000,
@@ -83,8 +84,9 @@ def algorithmic_work(arrays, nrhs, phase, level, mixed):
         n = tr.points.shape[0]
         return "hbm", arrays.n_near_values * 8.0 + 2 * n * nrhs * 8.0
     if phase == "m2l_apply":
+        # level -1: the persistent coupling stage over every level
         lv = tr.level[arrays.pairs[:, 0]]
-        npairs = int(np.sum(lv == level))
+        npairs = int(np.sum(lv == level)) if level >= 0 else int(np.sum(lv >= 2))
         if arrays.m2l_kind in ("toeplitz", "symtoeplitz"):
             # one complex multiply-add per stored frequency: for real maps the half spectrum
             # without the conjugate half of its plane fx = 0, ((L^2 + 1) / 2 + (p - 1) L^2)
@@ -451,18 +453,19 @@ def main():
     Y = torch.zeros((R, N), dtype=torch.float32, device=dev).t()
 
     # RHS block and tile shapes: measured on this map by the autotuner (outside the timed region)
-    tile = {"max_rhs": R, "m2l_groups": 0, "near_cols": 0}
+    tile = {"max_rhs": R, "m2l_groups": 0, "near_cols": 0, "persistent": 0}
     tune_table = []
     if args.tile:
-        mr, mg, nc = (int(v) for v in args.tile.split("/"))
-        tile = {"max_rhs": mr, "m2l_groups": mg, "near_cols": nc}
+        v = [int(t) for t in args.tile.split("/")]
+        tile = {"max_rhs": v[0], "m2l_groups": v[1], "near_cols": v[2], "persistent": v[3] if len(v) > 3 else 0}
     elif not args.no_tune:
         from h2g.autotune import tune
 
         t0 = time.perf_counter()
         tile, tune_table = tune(core, nrhs=R, precision=args.precision, blocks=(16, R))
         log(f"autotune in {time.perf_counter() - t0:.1f}s: {tile}; " +
-            ", ".join(f"{c['max_rhs']}/{c['m2l_groups']}/{c['near_cols']}: {ms * R:.2f} ms" for c, ms in tune_table))
+            ", ".join(f"{c['max_rhs']}/{c['m2l_groups']}/{c['near_cols']}/{c.get('persistent', 0)}: {ms * R:.2f} ms"
+                      for c, ms in tune_table))
     R_launch = min(R, tile["max_rhs"])
     A = h2g.H2Workspace(core, precision=args.precision, use_graph=not args.no_graph, **tile)
     log(f"batched workspace {A.device_bytes() / 1e9:.1f} GB")

@@ -340,13 +340,24 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
 // Spectrum elements per column one orientation needs: the largest level's source boxes (in) and
 // receiving boxes (out) times the frequencies the level stores (lattice levels: LL.F, half spectra
 // for real maps; the per-pair Hadamard levels: L^3).
-void spectrum_sizes(const Core &c, const OrientPlan &P, const Side &src, int64_t &fin, int64_t &fout) {
+// Spectrum elements per column: the largest level's (per-level launches) or, with `all_levels`
+// (the persistent coupling stage), the sum over the levels.
+void spectrum_sizes(const Core &c, const OrientPlan &P, const Side &src, int64_t &fin, int64_t &fout,
+                    bool all_levels = false) {
+    int64_t sin = 0, sout = 0;
     for (int l = 0; l < (int)P.m2l_recv.size(); ++l) {
         if (P.m2l_recv[l].count == 0) continue;
         const int64_t F = l < (int)P.lattice.size() && P.lattice[l].ok ? P.lattice[l].F : c.L3;
-        if (l < src.n_levels)
-            fin = std::max<int64_t>(fin, (int64_t)(src.level_offsets[l + 1] - src.level_offsets[l]) * F);
-        fout = std::max<int64_t>(fout, (int64_t)P.m2l_recv[l].count * F);
+        const int64_t a = l < src.n_levels ? (int64_t)(src.level_offsets[l + 1] - src.level_offsets[l]) * F : 0;
+        const int64_t b = (int64_t)P.m2l_recv[l].count * F;
+        fin = std::max<int64_t>(fin, a);
+        fout = std::max<int64_t>(fout, b);
+        sin += a;
+        sout += b;
+    }
+    if (all_levels) {
+        fin = std::max(fin, sin);
+        fout = std::max(fout, sout);
     }
 }
 
@@ -999,6 +1010,21 @@ int dense_rep(int precision, bool couplings) {
 
 // join: an event the first write of the result (L2P) waits for (the near field on the side
 // stream), or null
+// Whether the coupling stage runs as one persistent launch: a workspace asking for it, Toeplitz
+// couplings, and every level with coupling pairs a lattice level on the direct kernel's path.
+bool persistent_stage(const Workspace &w, const OrientPlan &P, const Side &src, const Side &dst, int cd, int R) {
+    if (!w.opts.persistent || !w.counters.ptr || !is_toeplitz(*w.core) || (cd != H2G_F32 && cd != H2G_C32)) return false;
+    bool any = false;
+    for (int l = 0; l < dst.n_levels; ++l) {
+        if (P.m2l_recv[l].count == 0) continue;
+        if (l >= src.n_levels || l >= (int)P.lattice.size() || !P.lattice[l].ok) return false;
+        const int nsrc = src.level_offsets[l + 1] - src.level_offsets[l];
+        if (!m2l_direct_rows(cd, P.lattice[l], nsrc, R)) return false;
+        any = true;
+    }
+    return any;
+}
+
 void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     Core &c = *w.core;
     const OrientPlan &P = op == H2G_OP_N ? c.fwd : c.rev;
@@ -1052,7 +1078,51 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     // couplings
     const int opA = op == H2G_OP_N ? 0 : (op == H2G_OP_T ? 1 : 2);
     const bool sym = c.m2l_kind == H2G_M2L_SYMDENSE || c.m2l_kind == H2G_M2L_SYMTOEPLITZ;
-    for (int l = 0; l < dst.n_levels; ++l) {
+    const bool persist = persistent_stage(w, P, src, dst, cd, R);
+    if (persist) {
+        // every coupling level is a lattice level on the direct path: modulate-in of all levels
+        // into their own parts of the spectrum buffers, one persistent coupling launch over the
+        // levels (largest first), modulate-out of all levels
+        const bool inv_in = op == H2G_OP_T;
+        std::vector<DirectLaunchLevel> lv;
+        std::vector<std::pair<int64_t, int>> order;
+        int64_t oin = 0, oout = 0;
+        for (int l = 0; l < dst.n_levels; ++l) {
+            const int n = P.m2l_recv[l].count;
+            if (n == 0) continue;
+            const int box0 = src.level_offsets[l];
+            const int nsrc = src.level_offsets[l + 1] - box0;
+            const LatticeLevel &LL = P.lattice[l];
+            const int drows = m2l_direct_rows(cd, LL, nsrc, R);
+            cplx<float> *S = w.spec_in.as<cplx<float>>() + oin;
+            cplx<float> *O = w.spec_out.as<cplx<float>>() + oout;
+            w.mark_begin(4, l);
+            launch_fft_in_fm(cd, c.half_spectrum, inv_in, box0, nsrc, w.mom.ptr, K, c.p, c.L, R, drows, drows > 16,
+                             S, c.twiddle.ptr, s);
+            w.mark_end();
+            order.push_back({(int64_t)LL.direct.n_blocks * LL.F, (int)lv.size()});
+            lv.push_back(DirectLaunchLevel{&LL, nsrc, n, S, O});
+            oin += (int64_t)LL.F * nsrc * drows;
+            oout += (int64_t)LL.F * n * R;
+        }
+        std::sort(order.begin(), order.end(), [](auto &a, auto &b) { return a.first > b.first; });
+        std::vector<DirectLaunchLevel> sorted;
+        for (auto &o : order) sorted.push_back(lv[o.second]);
+        w.mark_begin(5, -1);
+        launch_m2l_direct_persistent(conj, sorted, R, w.opts.m2l_groups, w.counters.as<int>(), s);
+        w.mark_end();
+        int li = 0;
+        for (int l = 0; l < dst.n_levels; ++l) {
+            const int n = P.m2l_recv[l].count;
+            if (n == 0) continue;
+            w.mark_begin(6, l);
+            launch_fft_out_fm(cd, c.half_spectrum, !inv_in, L + P.m2l_recv[l].begin, n, lv[li].O, K, c.p, c.L, R,
+                              w.loc.ptr, c.twiddle.ptr, s);
+            w.mark_end();
+            ++li;
+        }
+    }
+    for (int l = 0; l < dst.n_levels && !persist; ++l) {
         const int n = P.m2l_recv[l].count;
         if (n == 0) continue;
         const int32_t *recv = L + P.m2l_recv[l].begin;
@@ -1157,7 +1227,7 @@ struct WsLayout {
     }
 };
 
-WsLayout ws_layout(const Core &c, bool mixed, int R, int near_cols) {
+WsLayout ws_layout(const Core &c, bool mixed, int R, int near_cols, bool persistent) {
     WsLayout L;
     const int64_t nmax = std::max(c.test.n_dofs, c.trial->n_dofs);
     const int64_t bmax = std::max(c.test.n_boxes, c.trial->n_boxes);
@@ -1171,8 +1241,9 @@ WsLayout ws_layout(const Core &c, bool mixed, int R, int near_cols) {
         const size_t cc = mixed ? 8 : 16;
         // per column, the largest level's (boxes x stored frequencies) of either orientation
         int64_t fin = 0, fout = 0;
-        spectrum_sizes(c, c.fwd, *c.trial, fin, fout);
-        spectrum_sizes(c, c.rev, c.test, fin, fout);
+        const bool all = persistent && mixed && R >= 4;
+        spectrum_sizes(c, c.fwd, *c.trial, fin, fout, all);
+        spectrum_sizes(c, c.rev, c.test, fin, fout, all);
         // source spectra of FP32 batches of >= 4 columns may be stored with rows padded to 32
         // columns (the direct M2L kernel's layout)
         const int64_t rin = mixed && R >= 4 ? (R + 31) / 32 * 32 : R;
@@ -1618,7 +1689,8 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             H2G_HIP(hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_near, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
-            const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap, o.near_cols);
+            if (o.persistent != 0 && o.persistent != 1) fail(H2G_EINVAL, "persistent must be 0 or 1");
+            const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap, o.near_cols, o.persistent);
             w->xs.alloc(L.xs);
             w->ys.alloc(L.ys);
             w->x32.alloc(L.x32);
@@ -1626,6 +1698,7 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             w->loc.alloc(L.loc);
             w->spec_in.alloc(L.spec_in);
             w->spec_out.alloc(L.spec_out);
+            if (o.persistent) w->counters.alloc(8 * sizeof(int));
         } catch (...) {
             if (w->ev_done) (void)hipEventDestroy(w->ev_done);
             if (w->ev_near) (void)hipEventDestroy(w->ev_near);
@@ -1810,8 +1883,8 @@ int h2g_matvec_noalloc(h2g_core *core, int precision, int op, int which, int dty
         const bool mixed = precision != H2G_PREC_PLAIN;
         // the widest column block whose buffers fit the scratch
         int R = (int)std::min<int64_t>(nrhs, 1 << 16);
-        while (R > 1 && ws_layout(c, mixed, R, 0).total(NOALLOC_ALIGN) > NOALLOC_BYTES) R = R > 8 ? R * 3 / 4 : R - 1;
-        const WsLayout L = ws_layout(c, mixed, R, 0);
+        while (R > 1 && ws_layout(c, mixed, R, 0, false).total(NOALLOC_ALIGN) > NOALLOC_BYTES) R = R > 8 ? R * 3 / 4 : R - 1;
+        const WsLayout L = ws_layout(c, mixed, R, 0, false);
         if (L.total(NOALLOC_ALIGN) > NOALLOC_BYTES)
             fail(H2G_ENOMEM, "h2g_matvec_noalloc: one column of this map needs " + std::to_string(L.total(NOALLOC_ALIGN)) +
                                  " bytes of scratch, more than the " + std::to_string(NOALLOC_BYTES) +

@@ -423,6 +423,7 @@ struct Workspace {
     DevBuf x32;           // FP32 source columns of the mixed near field of real maps
     DevBuf mom, loc;      // [box][R][K] compute type
     DevBuf spec_in, spec_out;  // toeplitz spectra [slot][R][L3] complex compute type
+    DevBuf counters;           // 8 int32: task counters of the persistent coupling stage
     DevBuf xhost, yhost;  // device staging for host pointers
     DevBuf scal;          // unused placeholder
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -505,6 +506,16 @@ void launch_fft_out_fm(int cdtype, bool half, bool inverse, const int32_t *recv,
 // most 16 columns; a multiple of 32: the permuted layout of launch_fft_in_fm, perm).  0 when the
 // level does not run the direct kernel.
 int m2l_direct_rows(int cdtype, const LatticeLevel &LL, int n_src, int R);
+// The direct coupling kernel of several lattice levels (FP32 batches on the direct path) in one
+// persistent launch; counters: 8 int32 of device memory (zeroed on `s` by the launcher).
+struct DirectLaunchLevel {
+    const LatticeLevel *LL;
+    int n_src, n_recv;
+    const void *S;
+    void *O;
+};
+void launch_m2l_direct_persistent(bool conj, const std::vector<DirectLaunchLevel> &levels, int R, int groups,
+                                  int *counters, hipStream_t s);
 // groups: workgroups per frequency of the direct kernel (0: the default)
 void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src, int n_recv, const void *S, void *O,
                         int R, int groups, hipStream_t s);

@@ -426,23 +426,23 @@ constexpr int DR_MIN_R = 4;
 // batches of at most 16 columns run one 16-column tile per chunk (rows of 16 columns)
 constexpr bool DR_NCH1 = true;
 
+// One task of the direct kernel: block group `grp` of a level, frequency f, columns r0 .. r0 +
+// 16 NCH - 1.  `ea` (the frequency's coupling table) and `lsd` (per wave: the list of its current
+// block; the next block's list waits in registers and is written after the wave has finished
+// reading the current one) are the workgroup's LDS.  Every thread of the workgroup calls it with
+// the same arguments (the early return and the barrier are workgroup-uniform).
 template <bool CONJ, int NCH>
-__global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
-    m2l_direct_kernel(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr, const int4 *__restrict__ list,
-                      int n_list, const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg, int n_src,
-                      int n_recv, const cplx<float> *S, int Rrow, cplx<float> *__restrict__ O, int R) {
-    __shared__ __align__(16) float ea[216 * 32];
-    // per wave: the list of its current block (the next block's list waits in registers and is
-    // written after the wave has finished reading the current one)
-    __shared__ int4 lsd[DR_NT / 64][DR_MAXB * 8];
-    const int grp = group_of_workgroup();
+__device__ __forceinline__ void direct_task(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr,
+                                            const int4 *__restrict__ list, int n_list,
+                                            const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg,
+                                            int n_src, int n_recv, const cplx<float> *S, int Rrow,
+                                            cplx<float> *__restrict__ O, int R, int grp, int f, int r0, float *ea,
+                                            int4 (*lsd)[DR_MAXB * 8]) {
     const int bb0 = grp * gsz, bb1 = min(n_blocks, bb0 + gsz);
     if (bb0 >= bb1) return;
     // NCH 16-column tiles per chunk of columns: 2 (rows padded to 32 columns in the permuted
     // layout) or 1 (rows of 16 columns in natural order, batches of at most 16 columns)
-    constexpr int CH = 16 * NCH;
     using BV = typename std::conditional<NCH == 2, float4, float2>::type;
-    const int f = blockIdx.y, r0 = blockIdx.z * CH;
     fill_ea<float, DR_NT>(ea, EAg + (int64_t)f * 216 * 32);
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
@@ -596,6 +596,68 @@ __global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
     }
 }
 
+template <bool CONJ, int NCH>
+__global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
+    m2l_direct_kernel(int n_blocks, int gsz, const int32_t *__restrict__ blk_ptr, const int4 *__restrict__ list,
+                      int n_list, const int32_t *__restrict__ blk_tgt, const float *__restrict__ EAg, int n_src,
+                      int n_recv, const cplx<float> *S, int Rrow, cplx<float> *__restrict__ O, int R) {
+    __shared__ __align__(16) float ea[216 * 32];
+    __shared__ int4 lsd[DR_NT / 64][DR_MAXB * 8];
+    direct_task<CONJ, NCH>(n_blocks, gsz, blk_ptr, list, n_list, blk_tgt, EAg, n_src, n_recv, S, Rrow, O, R,
+                           group_of_workgroup(), blockIdx.y, blockIdx.z * 16 * NCH, ea, lsd);
+}
+
+// levels of one persistent launch
+constexpr int DP_MAXL = 8;
+struct DirectLevel {
+    int n_blocks, gsz, ng, n_list, n_src, n_recv, F, pad;
+    const int32_t *blk_ptr;
+    const int4 *list;
+    const int32_t *blk_tgt;
+    const float *EA;
+    const cplx<float> *S;
+    cplx<float> *O;
+};
+struct DirectPersist {
+    int n_levels, Rrow, R, pad;
+    DirectLevel lv[DP_MAXL];
+    int count[DP_MAXL][8];  // tasks of level i on XCD x
+    int total[8];
+};
+
+// The coupling stage of every lattice level in one persistent launch.  Each XCD x takes its own
+// task list: for each level (largest first), each 32-column chunk and each frequency, the block
+// groups [x ng / 8, (x + 1) ng / 8) of the level -- a contiguous range of blocks of one frequency
+// at a time per XCD, as in the per-level launches (workgroup b runs on XCD b mod 8).  A workgroup
+// takes the XCD's next task from counters[x] until the list is exhausted; tasks are independent,
+// so no workgroup waits for another and every workgroup exits once its counter passes the end.
+// The counters are zeroed on the stream before the launch.
+template <bool CONJ, int NCH>
+__global__ void __launch_bounds__(DR_NT, 1024 / DR_NT)
+    m2l_direct_persistent_kernel(DirectPersist P, int *__restrict__ counters) {
+    __shared__ __align__(16) float ea[216 * 32];
+    __shared__ int4 lsd[DR_NT / 64][DR_MAXB * 8];
+    __shared__ int task;
+    const int x = blockIdx.x & 7;
+    for (;;) {
+        __syncthreads();  // the previous task's readers of ea, lsd and task are done
+        if (threadIdx.x == 0) task = atomicAdd(&counters[x], 1);
+        __syncthreads();
+        int t = task;
+        if (t >= P.total[x]) break;
+        int i = 0;
+        while (i + 1 < P.n_levels && t >= P.count[i][x]) t -= P.count[i][x], ++i;
+        const DirectLevel &L = P.lv[i];
+        const int g0 = (int)((int64_t)x * L.ng / 8), g1 = (int)((int64_t)(x + 1) * L.ng / 8);
+        const int ngx = g1 - g0;
+        const int g = g0 + t % ngx;
+        const int fc = t / ngx;  // chunk-major, then frequency
+        const int f = fc % L.F, chunk = fc / L.F;
+        direct_task<CONJ, NCH>(L.n_blocks, L.gsz, L.blk_ptr, L.list, L.n_list, L.blk_tgt, L.EA, L.n_src, L.n_recv,
+                               L.S, P.Rrow, L.O, P.R, g, f, chunk * 16 * NCH, ea, lsd);
+    }
+}
+
 // EA[f][q][m][b] from the level's coupling table: D[op(t, q)] at frequency f, through the
 // symmetry permutation of (t, q) for the symmetry-reduced kinds; zero where (t, q) does not couple.
 template <class EA>
@@ -713,6 +775,67 @@ void launch_m2l_lattice(int cdtype, bool conj, const LatticeLevel &LL, int n_src
         else if (R == 1) cluster_go<cplx<float>, double, 1>(conj, LL, E, n_src, n_recv, S, O, R, s);
         else cluster_go<cplx<float>, double, 2>(conj, LL, E, n_src, n_recv, S, O, R, s);
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e)};
+}
+
+void launch_m2l_direct_persistent(bool conj, const std::vector<DirectLaunchLevel> &levels, int R, int groups,
+                                  int *counters, hipStream_t s) {
+    if (levels.empty()) return;
+    if ((int)levels.size() > DP_MAXL) throw Error{H2G_EINVAL, "persistent coupling stage: too many lattice levels"};
+    DirectPersist P{};
+    P.n_levels = (int)levels.size();
+    P.R = R;
+    P.Rrow = 0;
+    for (int i = 0; i < P.n_levels; ++i) {
+        const DirectLaunchLevel &d = levels[i];
+        const LatticeLevel &LL = *d.LL;
+        const int Rrow = m2l_direct_rows(H2G_F32, LL, d.n_src, R);
+        if (!Rrow || (P.Rrow && Rrow != P.Rrow)) throw Error{H2G_EINVAL, "persistent coupling stage: level not on the direct path"};
+        P.Rrow = Rrow;
+        const LatticeDirect &D = LL.direct;
+        DirectLevel &L = P.lv[i];
+        L.n_blocks = D.n_blocks;
+        L.gsz = std::max(DR_NT / 64, D.n_blocks / (groups > 0 ? groups : DR_GROUPS));
+        L.ng = (D.n_blocks + L.gsz - 1) / L.gsz;
+        L.n_list = (int)(D.list.bytes / sizeof(int4));
+        L.n_src = d.n_src;
+        L.n_recv = d.n_recv;
+        L.F = LL.F;
+        L.blk_ptr = D.ptr.as<int32_t>();
+        L.list = D.list.as<int4>();
+        L.blk_tgt = D.tgt.as<int32_t>();
+        L.EA = LL.EA32.as<float>();
+        L.S = (const cplx<float> *)d.S;
+        L.O = (cplx<float> *)d.O;
+    }
+    const int ch = P.Rrow == 16 ? 16 : 32;
+    const int chunks = P.Rrow / ch;
+    for (int x = 0; x < 8; ++x) {
+        int64_t tot = 0;
+        for (int i = 0; i < P.n_levels; ++i) {
+            const DirectLevel &L = P.lv[i];
+            const int64_t ngx = (int64_t)(x + 1) * L.ng / 8 - (int64_t)x * L.ng / 8;
+            const int64_t c = ngx * L.F * chunks;
+            P.count[i][x] = (int)c;
+            tot += c;
+        }
+        if (tot > INT32_MAX / 2) throw Error{H2G_EINVAL, "persistent coupling stage: too many tasks"};
+        P.total[x] = (int)tot;
+    }
+    auto kern = ch == 16 ? (conj ? m2l_direct_persistent_kernel<true, 1> : m2l_direct_persistent_kernel<false, 1>)
+                         : (conj ? m2l_direct_persistent_kernel<true, 2> : m2l_direct_persistent_kernel<false, 2>);
+    // as many workgroups as are resident at once, a multiple of the 8 XCDs (every instantiation
+    // has the same resources)
+    static const int grid = [&] {
+        int dev = 0, ncu = 0, per_cu = 0;
+        H2G_HIP(hipGetDevice(&dev));
+        H2G_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        H2G_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kern, DR_NT, 0));
+        return std::max(8, (ncu * std::max(1, per_cu)) / 8 * 8);
+    }();
+    launch_fill_zero(counters, 8 * sizeof(int), s);
+    kern<<<grid, DR_NT, 0, s>>>(P, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error{H2G_EHIP, std::string("launch: ") + hipGetErrorString(e)};
 }

@@ -13,6 +13,9 @@ What is tuned (each a workspace option of ``h2g_ws_opts``):
 * ``near_cols`` -- columns per workgroup of the near-field kernel (one, two or four 16-column
   MFMA tiles per wave): wider tiles reuse every matrix value for more columns, narrower ones
   run more workgroups with fewer registers each.
+* ``persistent`` -- the coupling stage of all lattice levels in one persistent launch (per-XCD
+  task lists, the small levels filling the tail of the large ones) or one launch per level; the
+  persistent form keeps the spectra of all levels at once (more workspace memory).
 
 The best values depend on the map (N, p, tree depth, near-field density, coupling kind) and the
 device, so they are measured: each candidate runs the same product on synthetic columns, timed
@@ -34,6 +37,7 @@ from .maps import H2Workspace, mul_
 DEFAULT_BLOCKS = (8, 16, 32)
 DEFAULT_M2L_GROUPS = (256, 512, 768)
 DEFAULT_NEAR_COLS = (16, 32)
+DEFAULT_PERSISTENT = (0, 1)
 
 
 def signature(core, precision: str, nrhs: int) -> str:
@@ -51,9 +55,11 @@ def _lattice(core):
 
 
 def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, m2l_groups=DEFAULT_M2L_GROUPS,
-         near_cols=DEFAULT_NEAR_COLS, repeats: int = 3, cache_path: str | None = None, use_graph: bool = True):
-    """Returns (best, table): ``best`` = {"max_rhs", "m2l_groups", "near_cols"} (0 = the
-    library's default) and ``table`` = [(config, ms per column)] for every candidate timed."""
+         near_cols=DEFAULT_NEAR_COLS, persistent=DEFAULT_PERSISTENT, repeats: int = 3, cache_path: str | None = None,
+         use_graph: bool = True):
+    """Returns (best, table): ``best`` = {"max_rhs", "m2l_groups", "near_cols", "persistent"}
+    (0 = the library's default) and ``table`` = [(config, ms per column)] for every candidate
+    timed."""
     import torch
 
     sig = signature(core, precision, nrhs)
@@ -76,7 +82,7 @@ def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, 
     timed = {}
 
     def measure(cfg):
-        key = (cfg["max_rhs"], cfg["m2l_groups"], cfg["near_cols"])
+        key = (cfg["max_rhs"], cfg["m2l_groups"], cfg["near_cols"], cfg["persistent"])
         if key in timed:
             return timed[key]
         ws = H2Workspace(core, precision=precision, use_graph=use_graph, **cfg)
@@ -96,10 +102,11 @@ def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, 
         table.append((dict(cfg), ms))
         return ms
 
-    best = {"max_rhs": min(nrhs, max(blocks)), "m2l_groups": 0, "near_cols": 0}
+    best = {"max_rhs": min(nrhs, max(blocks)), "m2l_groups": 0, "near_cols": 0, "persistent": 0}
     for axis, values in (("max_rhs", [b for b in blocks if b <= nrhs] or [nrhs]),
                          ("m2l_groups", list(m2l_groups) if _lattice(core) and not cplx else []),
-                         ("near_cols", list(near_cols) if not cplx else [])):
+                         ("near_cols", list(near_cols) if not cplx else []),
+                         ("persistent", list(persistent) if _lattice(core) and not cplx else [])):
         if not values:
             continue
         scores = {}

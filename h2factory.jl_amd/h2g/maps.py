@@ -209,10 +209,11 @@ class H2Workspace:
     op = H.H2G_OP_N
 
     def __init__(self, core: GpuH2Core, max_rhs=1, precision="plain", use_graph=False, timing=False, stream=None,
-                 m2l_groups=0, near_cols=0):
+                 m2l_groups=0, near_cols=0, persistent=0):
         """``stream``: a HIP stream handle (int) the workspace runs on, shared with other workspaces,
         or None for streams of its own.  ``m2l_groups`` / ``near_cols``: tile shapes (0: the
-        library's defaults; ``h2g.autotune.tune`` measures them for a map)."""
+        library's defaults; ``h2g.autotune.tune`` measures them for a map).  ``persistent``: the
+        lattice coupling stage of all levels in one persistent launch."""
         self.core = core
         o = H.WsOpts()
         o.stream = stream
@@ -222,8 +223,9 @@ class H2Workspace:
         o.timing = int(bool(timing))
         o.m2l_groups = int(m2l_groups)
         o.near_cols = int(near_cols)
+        o.persistent = int(persistent)
         self.opts = dict(max_rhs=max_rhs, precision=precision, use_graph=use_graph, timing=timing, stream=stream,
-                         m2l_groups=m2l_groups, near_cols=near_cols)
+                         m2l_groups=m2l_groups, near_cols=near_cols, persistent=persistent)
         h = C.c_void_p()
         _raise(H.lib().h2g_ws_create(core.handle, C.byref(o), C.byref(h)))
         self.handle = h

@@ -274,8 +274,9 @@ class Server:
             return None
         from .autotune import tune
 
+        # one launch per coupling level: the admission estimate sizes the spectra for the largest level
         best, _ = tune(core, nrhs=self.max_batch, precision=self.precision, blocks=(self.max_batch,), repeats=2,
-                       use_graph=self.use_graph)
+                       use_graph=self.use_graph, persistent=(0,))
         return best
 
     def remove_map(self, name):

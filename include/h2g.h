@@ -179,6 +179,12 @@ typedef struct h2g_ws_opts {
     int32_t near_cols;            /* near field of real maps: columns per workgroup, 16, 32 or 64
                                      (one to four 16-column MFMA tiles per wave); default
                                      min(64, 16 ceil(nrhs / 16))                                   */
+    int32_t persistent;           /* 1: the lattice Toeplitz coupling stage of every level in one
+                                     persistent launch (FP32 batches of >= 4 columns; each XCD
+                                     works through its own task list, largest level first); the
+                                     workspace then keeps the spectra of all lattice levels at
+                                     once (about 1.45x the spectrum memory of the per-level
+                                     launches).  0: one launch per level.                        */
 } h2g_ws_opts;
 
 /* One per-phase timing record, shaped like the reference's TimingRecord

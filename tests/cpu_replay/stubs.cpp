@@ -93,3 +93,10 @@ void prim_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStre
 }
 
 }  // namespace h2g
+
+namespace h2g {
+// the persistent coupling stage is a lattice kernel too
+void launch_m2l_direct_persistent(bool, const std::vector<DirectLaunchLevel> &, int, int, int *, hipStream_t) {
+    throw Error{H2G_EUNSUPPORTED, "host replay: the lattice Toeplitz kernels run on the GPU only"};
+}
+}  // namespace h2g

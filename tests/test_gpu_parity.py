@@ -190,13 +190,42 @@ def test_tile_shapes_do_not_change_the_product(gpu, nrhs):
             assert relerr(y, ref) < 1e-6, (op, groups, cols)
 
 
+@pytest.mark.parametrize("nrhs", [4, 16, 32, 40])
+def test_persistent_coupling_stage_equals_per_level(gpu, nrhs):
+    """The coupling stage of all lattice levels in one persistent launch computes the per-level
+    launches' products (the same task arithmetic, other workgroups): equal to FP32 rounding, in
+    every orientation, and within the 1e-6 gate of the plain FP64 product."""
+    import h2g
+    from convert import to_arrays
+
+    pts, core = _tile_case()
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
+    X = np.random.default_rng(34).standard_normal((12000, nrhs)).astype(np.float32)
+    plain = h2g.H2Workspace(c, max_rhs=nrhs, precision="plain")
+    for op in "NTC":
+        y64 = np.zeros((12000, nrhs))
+        h2g.mul_(y64, orient(plain, op), X.astype(np.float64))
+        ys = []
+        for persistent, groups in ((0, 0), (1, 0), (1, 64)):
+            A = h2g.H2Workspace(c, max_rhs=nrhs, precision="mixed", use_graph=True, persistent=persistent,
+                                m2l_groups=groups)
+            y = np.zeros((12000, nrhs), dtype=np.float32)
+            for _ in range(2):  # the second replay starts from counters the first one left behind
+                h2g.mul_(y, orient(A, op), X)
+            A.close()
+            ys.append(y)
+            assert relerr(y, y64) < 1e-6, (op, persistent, groups)
+        # each block's sums run in the same order in either launch form and any grouping
+        assert np.array_equal(ys[1], ys[0]) and np.array_equal(ys[2], ys[0]), op
+
+
 def test_tile_shape_arguments_are_checked(gpu):
     import h2g
     from convert import to_arrays
 
     _, core = _tile_case()
     c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
-    for kw in ({"near_cols": 24}, {"m2l_groups": -1}):
+    for kw in ({"near_cols": 24}, {"m2l_groups": -1}, {"persistent": 2}):
         with pytest.raises(h2g.ArgumentError):
             h2g.H2Workspace(c, max_rhs=8, precision="mixed", **kw)
 
