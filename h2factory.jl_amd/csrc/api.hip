@@ -813,6 +813,9 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
             const auto mine = sources_of(k);
             std::vector<std::pair<int, int>> cls[2];
             for (const auto &e : mine) cls[e.second & 1].push_back(e);
+            // each class in the source boxes' Morton order: the waves of a workgroup (Morton-
+            // consecutive blocks) then read the source rows they share at about the same time
+            for (auto &cl : cls) std::sort(cl.begin(), cl.end());
             std::vector<std::pair<int, int>> h0, h1;
             const size_t both = std::min(cls[0].size(), cls[1].size());
             for (size_t i = 0; i < both; ++i) {
