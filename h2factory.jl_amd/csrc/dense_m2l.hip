@@ -86,6 +86,33 @@ __global__ void dense_frag_kernel(const double *__restrict__ ops, int K, int Kr,
     }
 }
 
+// Complex fragments (complex maps, FP64 and FP32): the real form repeats every complex entry
+// twice (rows 2t and 2t + 1 of the real form read the same Re, Im), so the operator is stored
+// once, as complex values in the MFMA order of the real form's row pairs:
+//     cfrag[e][rt][g][tp + 8 kq][0..3] = (Re, Im) of M[t][s0], M[t][s1],
+//     t = 8 rt + tp (tp < 8), s0 = 8 g + 2 kq, s1 = s0 + 1,
+// half the bytes of the real-form fragments.  Lane (m + 16 kq) of the kernel loads entry
+// (m / 2 + 8 kq) -- lanes m and m ^ 1 the same 16 or 32 bytes -- and forms its four real-form
+// values A[16 rt + m][16 g + 4 kq + kk]: (Re0, -Im0, Re1, -Im1) for even m, (Im0, Re0, Im1, Re1)
+// for odd m.
+template <class OUT>
+__global__ void dense_cfrag_kernel(const double *__restrict__ ops, int K, int Kr, int KrT, int64_t n_fops,
+                                   const int32_t *__restrict__ fop_op, const int32_t *__restrict__ fop_sym,
+                                   const int32_t *__restrict__ perms_inv, int orient, OUT *__restrict__ frag) {
+    const int64_t per = (int64_t)KrT * KrT * 128;
+    const int64_t total = n_fops * per;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = idx / per;
+        const int rem = (int)(idx - e * per);
+        const int v = rem & 3, slot = (rem >> 2) & 31, tg = rem >> 7;
+        const int rt = tg / KrT, g = tg - rt * KrT;
+        const int t = 8 * rt + (slot & 7), s = 8 * g + 2 * (slot >> 3) + (v >> 1);
+        // real-form rows 2t (a = 0) and 2t + 1 (a = 1), column 2s (b = 0): Re and Im
+        frag[idx] = (OUT)dense_value<true>(ops, K, Kr, e, 2 * t + (v & 1), 2 * s, fop_op, fop_sym, perms_inv, orient);
+    }
+}
+
 // Fragment matrices for the 16 x 16 x 32 BF16 MFMAs: three planes (hi, mid, lo) of
 // [e][rt][g][lane][8] with G2 = ceil(KrT / 2) groups of 32 reductions; lane (m + 16 k'') holds
 // A[16 rt + m][32 g + 8 k'' + u], u = 0..7.  The FP32 value x is split as hi = bf16(x),
@@ -158,7 +185,7 @@ __device__ inline void init_from_out(double (&acc)[RTW][NCT][4], const V *__rest
 
 // MODE 0: FP64 operands on FP64 MFMA; MODE 1: FP32 operands on FP32 MFMA with FP64 accumulation
 // of the partial sums.  RTW: row tiles per wave (16 rows each), NCT: column tiles (16 columns).
-template <int MODE, int RTW, int NCT>
+template <int MODE, int RTW, int NCT, bool CF>
 __global__ void __launch_bounds__(DNT, 2)
     dense_m2l_kernel(const int32_t *__restrict__ recv, const int64_t *__restrict__ item_ptr,
                      const int32_t *__restrict__ item_src, const int32_t *__restrict__ item_fop,
@@ -175,7 +202,8 @@ __global__ void __launch_bounds__(DNT, 2)
     const int r0 = blockIdx.y * 16 * NCT;
     const int G = KrT;
     const int ntw = (KrT - wave + 3) >> 2;  // row tiles of this wave: wave, wave + 4, ...
-    const int64_t fsz = (int64_t)(16 * KrT) * (16 * KrT);
+    // real-form fragments: (16 KrT)^2 values per operator; complex fragments: half of that
+    const int64_t fsz = CF ? (int64_t)KrT * KrT * 128 : (int64_t)(16 * KrT) * (16 * KrT);
     double acc[RTW][NCT][4];
 #pragma unroll
     for (int j = 0; j < RTW; ++j)
@@ -192,7 +220,7 @@ __global__ void __launch_bounds__(DNT, 2)
     if (acc_flags && i1 > i0 && acc_flags[i0])
         init_from_out<MODE == 0>(acc, loc, t, R, Kr, KrT, r0, wave, ntw, n, kq);
     for (int64_t it = i0; it < i1; ++it) {
-        const V *A0 = frag + (int64_t)item_fop[it] * fsz + (int64_t)lane * 4;
+        const V *A0 = frag + (int64_t)item_fop[it] * fsz + (CF ? (int64_t)((n >> 1) + 8 * kq) : (int64_t)lane) * 4;
         const int64_t s = item_src[it];
         const V *Bc[NCT];
         bool bval[NCT];
@@ -205,9 +233,16 @@ __global__ void __launch_bounds__(DNT, 2)
         for (int g = 0; g < G; ++g) {
             V4 a[RTW], b[NCT];
 #pragma unroll
-            for (int j = 0; j < RTW; ++j)
-                a[j] = j < ntw ? *reinterpret_cast<const V4 *>(A0 + ((int64_t)(wave + 4 * j) * G + g) * 256)
-                               : V4{0, 0, 0, 0};
+            for (int j = 0; j < RTW; ++j) {
+                if constexpr (CF) {
+                    const V4 c = j < ntw ? *reinterpret_cast<const V4 *>(A0 + ((int64_t)(wave + 4 * j) * G + g) * 128)
+                                         : V4{0, 0, 0, 0};
+                    a[j] = (n & 1) ? V4{c[1], c[0], c[3], c[2]} : V4{c[0], -c[1], c[2], -c[3]};
+                } else {
+                    a[j] = j < ntw ? *reinterpret_cast<const V4 *>(A0 + ((int64_t)(wave + 4 * j) * G + g) * 256)
+                                   : V4{0, 0, 0, 0};
+                }
+            }
 #pragma unroll
             for (int c = 0; c < NCT; ++c) b[c] = load_k4<V>(Bc[c], 16 * g + 4 * kq, Kr, bval[c]);
 #pragma unroll
@@ -417,33 +452,44 @@ void dense_bf16_go(int rtw, int nct, const int32_t *recv, int n_recv, const int6
     }
 }
 
-template <int MODE, int RTW>
+template <int MODE, int RTW, bool CF>
 void dense_go_nct(int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
                   const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
                   const uint8_t *acc, hipStream_t s) {
     using V = typename std::conditional<MODE == 0, double, float>::type;
     const dim3 grid((unsigned)n_recv, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
     if (nct == 1)
-        hipLaunchKernelGGL((dense_m2l_kernel<MODE, RTW, 1>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
+        hipLaunchKernelGGL((dense_m2l_kernel<MODE, RTW, 1, CF>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
                            (const V *)frag, Kr, KrT, (const V *)mom, (V *)loc, R, acc);
     else
-        hipLaunchKernelGGL((dense_m2l_kernel<MODE, RTW, 2>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
+        hipLaunchKernelGGL((dense_m2l_kernel<MODE, RTW, 2, CF>), grid, DNT, 0, s, recv, item_ptr, item_src, item_fop,
                            (const V *)frag, Kr, KrT, (const V *)mom, (V *)loc, R, acc);
 }
 
-template <int MODE>
-void dense_go(int rtw, int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
-              const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
-              const uint8_t *acc, hipStream_t s) {
+template <int MODE, bool CF>
+void dense_go_cf(int rtw, int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr, const int32_t *item_src,
+                 const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
+                 const uint8_t *acc, hipStream_t s) {
     switch (rtw) {
-    case 1: dense_go_nct<MODE, 1>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
-    case 2: dense_go_nct<MODE, 2>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
-    case 3: dense_go_nct<MODE, 3>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
-    case 4: dense_go_nct<MODE, 4>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
-    case 5: dense_go_nct<MODE, 5>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
-    case 6: dense_go_nct<MODE, 6>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
-    default: dense_go_nct<MODE, 7>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 1: dense_go_nct<MODE, 1, CF>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 2: dense_go_nct<MODE, 2, CF>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 3: dense_go_nct<MODE, 3, CF>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 4: dense_go_nct<MODE, 4, CF>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 5: dense_go_nct<MODE, 5, CF>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    case 6: dense_go_nct<MODE, 6, CF>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
+    default: dense_go_nct<MODE, 7, CF>(nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s); break;
     }
+}
+
+// cf: the operators are complex fragments (complex maps)
+template <int MODE>
+void dense_go(bool cf, int rtw, int nct, const int32_t *recv, int n_recv, const int64_t *item_ptr,
+              const int32_t *item_src, const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom,
+              void *loc, int R, const uint8_t *acc, hipStream_t s) {
+    if (cf)
+        dense_go_cf<MODE, true>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s);
+    else
+        dense_go_cf<MODE, false>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s);
 }
 
 }  // namespace
@@ -469,6 +515,16 @@ void build_dense_ops(DenseOps &D, int dtype, int K, const void *ops, const int32
         if (!(orient_mask & (1 << o))) continue;
         auto build = [&](auto tag, DevBuf &dst) {
             using OUT = decltype(tag);
+            if (cplx) {
+                // complex fragments: half the real form's values
+                const int64_t ctot = n * (int64_t)KrT * KrT * 128;
+                dst.alloc((size_t)ctot * sizeof(OUT));
+                const unsigned cb = (unsigned)std::min<int64_t>((ctot + 255) / 256, 1 << 20);
+                hipLaunchKernelGGL((dense_cfrag_kernel<OUT>), cb, 256, 0, 0, (const double *)ops, K, Kr, KrT, n,
+                                   d_op.as<int32_t>(), dsym, perms_inv, o, dst.as<OUT>());
+                H2G_CHECK_LAUNCH();
+                return;
+            }
             dst.alloc(per * n * sizeof(OUT));
             if (cplx)
                 hipLaunchKernelGGL((dense_frag_kernel<OUT, true>), blocks, 256, 0, 0, (const double *)ops, K, Kr, KrT, n,
@@ -494,6 +550,7 @@ void build_dense_ops(DenseOps &D, int dtype, int K, const void *ops, const int32
         }
     }
     H2G_HIP(hipDeviceSynchronize());
+    D.cf = cplx;
     D.ok = true;
 }
 
@@ -514,12 +571,12 @@ void launch_dense(const DenseOps &D, int orient, int rep, const int32_t *recv, i
                              D.KrT, in, out, R, acc_flags, s);
     } else if (rep == DENSE_REP_F32) {
         if (!D.f32[orient].ptr) throw Error{H2G_EINVAL, "FP32 operators were not built for this core"};
-        dense_go<1>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.f32[orient].ptr, D.Kr, D.KrT, in, out, R,
-                    acc_flags, s);
+        dense_go<1>(D.cf, rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.f32[orient].ptr, D.Kr, D.KrT, in,
+                    out, R, acc_flags, s);
     } else {
         if (!D.f64[orient].ptr) throw Error{H2G_EINVAL, "FP64 operators were not built for this core"};
-        dense_go<0>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.f64[orient].ptr, D.Kr, D.KrT, in, out, R,
-                    acc_flags, s);
+        dense_go<0>(D.cf, rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, D.f64[orient].ptr, D.Kr, D.KrT, in,
+                    out, R, acc_flags, s);
     }
     H2G_CHECK_LAUNCH();
 }

@@ -188,12 +188,14 @@ constexpr int DENSE_MAX_RTW = 7;  // at most 4 * 7 row tiles of 16 (Kr <= 448)
 enum { DENSE_REP_F64 = 1, DENSE_REP_F32 = 2, DENSE_REP_BF16 = 4, DENSE_REP_BF16X2 = 8, DENSE_REP_BF16X3 = 16 };
 struct DenseOps {
     bool ok = false;
+    bool cf = false;  // complex maps: f64 / f32 hold complex fragments (half the real form's values)
     int Kr = 0, KrT = 0;
     int64_t n = 0;
     DevBuf f64[4], f32[4], bf[4];
     DenseOps() = default;
     DenseOps &operator=(DenseOps &&o) noexcept {
         ok = o.ok;
+        cf = o.cf;
         Kr = o.Kr;
         KrT = o.KrT;
         n = o.n;
