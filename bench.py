@@ -60,8 +60,16 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the requests (1/s)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="HBM budget of the resident maps (GB)")
     ap.add_argument("--serve-seed", type=int, default=0, help="seed of the serving map mix")
-    ap.add_argument("--serve-rate-low", type=float, default=200.0,
-                    help="second operating point below capacity (requests/s; 0: skip)")
+    ap.add_argument("--serve-ws-per-map", type=int, default=1,
+                    help="workspaces per resident map (batches of one map in flight at once; one keeps the "
+                         "spectra of one batch per map, so more maps fit the budget)")
+    ap.add_argument("--serve-helmholtz-m2l", default="toeplitz", choices=["toeplitz", "dense"],
+                    help="couplings of the Helmholtz maps of the serving mix: equispaced Toeplitz (spectra, "
+                         "as the Laplace maps) or Chebyshev dense (the Helmholtz workload's form, whose "
+                         "operator stream costs a product the same from 1 to 16 columns)")
+    ap.add_argument("--serve-rate-low", type=float, default=-1.0,
+                    help="second operating point, below capacity (requests/s; negative: 25%% of the "
+                         "throughput served at --serve-rate; 0: skip)")
     ap.add_argument("--serve-requests-low", type=int, default=1000)
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (one JSON line)")
     ap.add_argument("--workload", default="laplace", choices=["laplace", "helmholtz"],
@@ -259,10 +267,11 @@ def helmholtz_main(args):
     print(json.dumps(out), flush=True)
 
 
-def serving_mix(seed=0, n_maps=8):
+def serving_mix(seed=0, n_maps=8, helmholtz_m2l="toeplitz"):
     """The serving workload's resident maps, drawn from a seed: Laplace (equispaced p, Toeplitz
-    couplings) or Helmholtz (k in {2, 10}; Chebyshev p, dense complex couplings), N log-uniform in
-    [50k, 4M], points uniform in the unit cube or on the unit sphere, p in {4, 6}."""
+    couplings) or Helmholtz (k in {2, 10}; equispaced p with Toeplitz complex couplings, or with
+    helmholtz_m2l="dense" Chebyshev p with dense complex couplings), N log-uniform in [50k, 4M],
+    points uniform in the unit cube or on the unit sphere, p in {4, 6}."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -273,7 +282,10 @@ def serving_mix(seed=0, n_maps=8):
         n = int(round(np.exp(rng.uniform(np.log(5e4), np.log(4e6)))))
         geom = str(rng.choice(["cube", "sphere"]))
         p = int(rng.choice([4, 6]))
-        specs.append(dict(kernel=kernel, k=k if kernel == "helmholtz" else 0.0, n=n, geom=geom, p=p))
+        spec = dict(kernel=kernel, k=k if kernel == "helmholtz" else 0.0, n=n, geom=geom, p=p)
+        if kernel == "helmholtz":
+            spec["m2l"] = helmholtz_m2l
+        specs.append(spec)
     return specs
 
 
@@ -292,6 +304,9 @@ def build_spec(spec, seed):
     if spec["kernel"] == "laplace":
         return PM.build_map(pts, p=spec["p"], nodekind="equi", m2l_kind="toeplitz", leaf_size=64, plan="device",
                             ops="device", **box)
+    if spec.get("m2l", "toeplitz") == "toeplitz":
+        return PM.build_map(pts, p=spec["p"], nodekind="equi", m2l_kind="toeplitz", kernel="helmholtz",
+                            wavenumber=spec["k"], leaf_size=64, plan="device", ops="device", **box)
     return PM.build_map(pts, p=spec["p"], nodekind="cheb", m2l_kind="dense", kernel="helmholtz",
                         wavenumber=spec["k"], leaf_size=64, plan="device", ops="device", **box)
 
@@ -307,8 +322,9 @@ def serving_leg(args):
     import h2g
     from h2g.serve import AdmissionError, Request, Server
 
-    specs = serving_mix(args.serve_seed)
-    srv = Server(budget_bytes=int(args.serve_budget_gb * 1e9), max_batch=32, ws_per_map=2, precision="mixed",
+    specs = serving_mix(args.serve_seed, helmholtz_m2l=args.serve_helmholtz_m2l)
+    srv = Server(budget_bytes=int(args.serve_budget_gb * 1e9), max_batch=32, ws_per_map=args.serve_ws_per_map,
+                 precision="mixed",
                  n_streams=4, autotune=not args.no_tune)
     admitted, rejected = [], []
     t0 = time.perf_counter()
@@ -317,7 +333,9 @@ def serving_leg(args):
         name = f"m{i}"
         try:
             rm = srv.add_map(name, arrays)
-            admitted.append(dict(name=name, **s, gb=round(rm.device_bytes() / 1e9, 1), tile=rm.tile))
+            admitted.append(dict(name=name, **s, gb=round(rm.device_bytes() / 1e9, 1), tile=rm.tile,
+                                 workspaces=len(rm.slots), max_batch=rm.max_batch,
+                                 product_ms={"1": round(1e3 * rm.t1, 2), str(rm.max_batch): round(1e3 * rm.t32, 2)}))
             log(f"serving: admitted {name} {s}: {rm.device_bytes() / 1e9:.1f} GB, total {srv.used_bytes() / 1e9:.1f} GB")
         except AdmissionError as e:
             rejected.append(dict(name=name, **s, reason=str(e)))
@@ -357,15 +375,19 @@ def serving_leg(args):
     st["accounting"] = dict(srv.last_accounting)
     log(f"serving: {st}")
     low = None
-    if args.serve_rate_low > 0 and args.serve_requests_low > 0:
+    # the main rate saturates the server whenever it exceeds what the mix can serve; the second
+    # point arrives at a fraction of the measured throughput, so its latencies are service times
+    # plus batching delay rather than queueing
+    rate_low = args.serve_rate_low if args.serve_rate_low >= 0 else round(0.25 * st["matvecs_per_s"], 1)
+    if rate_low > 0 and args.serve_requests_low > 0:
         reqs_low = stream(args.serve_requests_low)
         srv.done.clear()
-        sl = srv.run_stream(reqs_low, rate_hz=args.serve_rate_low, seed=8)
-        low = {"requests": sl["requests"], "arrival_rate_hz": args.serve_rate_low,
+        sl = srv.run_stream(reqs_low, rate_hz=rate_low, seed=8)
+        low = {"requests": sl["requests"], "arrival_rate_hz": rate_low,
                "matvecs_per_s": round(sl["matvecs_per_s"], 1), "p50_ms": round(sl["p50_ms"], 1),
                "p99_ms": round(sl["p99_ms"], 1), "mean_batch": round(sl["mean_batch"], 2),
                "per_map": per_map_stats(reqs_low), "accounting": dict(srv.last_accounting)}
-        log(f"serving at {args.serve_rate_low}/s: {low}")
+        log(f"serving at {rate_low}/s: {low}")
     # plain FP64 one request at a time: service time per map; and each map's served results (two
     # per map) against the plain product
     svc, errs = {}, {}
@@ -396,6 +418,7 @@ def serving_leg(args):
     mean_svc = float(np.mean([svc[r.map_name] for r in reqs]))
     return {"maps_admitted": admitted, "maps_rejected": rejected, "budget_gb": args.serve_budget_gb,
             "requests": st["requests"], "arrival_rate_hz": args.serve_rate, "streams": 4, "max_batch": 32,
+            "workspaces_per_map": args.serve_ws_per_map,
             "matvecs_per_s": round(st["matvecs_per_s"], 1), "wall_s": round(st["wall_s"], 2),
             "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1),
             "mean_batch": round(st["mean_batch"], 2), "per_map": per_map, "accounting": st["accounting"],

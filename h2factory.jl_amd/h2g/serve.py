@@ -6,18 +6,23 @@ turns the incoming stream into multi-RHS products:
 
 * a map is admitted when its core plus its workspaces fit the HBM budget
   (``AdmissionError`` otherwise); each map owns ``ws_per_map`` workspaces of ``max_batch``
-  columns, so a batch needs no new memory once its map is admitted;
-* requests queue per (map, orientation); whenever a workspace of that map is free, up to
-  ``max_batch`` queued requests form one product -- requests that arrived while earlier
-  batches were on the GPU join the next batch (continuous batching);
+  columns, or, when those do not fit, one workspace of ``max_batch`` and then of
+  ``max_batch // 2`` columns (the spectra of a Toeplitz map's largest level dominate a
+  workspace), so a batch needs no new memory once its map is admitted;
+* requests queue per (map, orientation); whenever a stream has room (``stream_depth`` batches
+  in flight, default 1) and a workspace of some map is free, up to ``max_batch`` queued
+  requests of the (map, orientation) with the oldest waiting request form one product --
+  requests that arrive while the streams are busy join the next batch instead of waiting
+  behind it in a stream's queue (continuous batching; a map's product costs nearly the same
+  for 1 as for 8 columns, so batches formed late are the cheap ones);
 * the workspaces of all maps share a pool of ``n_streams`` HIP streams (default 4, the number
   of hardware queues a process gets by default; more streams than queues would only serialise
-  on the queues); each batch goes to the stream expected to drain first (per-map product times
-  measured at admission, a 1-column and a 32-column product), so a small map's batch does not
-  wait in order behind a large map's product (``h2g_ws_set_stream`` moves the workspace, ordered
-  after its previous product); ``h2g_matvec`` with ``H2G_HOST_PTRS | H2G_ASYNC`` on page-locked
-  buffers enqueues host->device copy, product and device->host copy back to back, so batches on
-  different streams overlap on the device while the host forms the next batch;
+  on the queues); each batch goes to the stream with room expected to drain first (per-map
+  product times measured at admission, a 1-column and a ``max_batch``-column product)
+  (``h2g_ws_set_stream`` moves the workspace, ordered after its previous product);
+  ``h2g_matvec`` with ``H2G_HOST_PTRS | H2G_ASYNC`` on page-locked buffers enqueues
+  host->device copy, product and device->host copy back to back, so batches on different
+  streams overlap on the device while the host forms the next batch;
 * completion is polled with ``h2g_ws_query`` (one completion event per workspace, so
   workspaces sharing a stream are told apart); latency = completion time - arrival time;
 * nothing else records work on a workspace's stream, and ``close`` waits for every product
@@ -53,6 +58,35 @@ def _spectrum_values(arrays):
     return L ** 3 if arrays.dtype == "c64" else (L * L + 1) // 2 + (arrays.p - 1) * L * L
 
 
+def workspace_bytes(arrays, max_rhs, precision="mixed", staging=True):
+    """Device bytes of one workspace of ``max_rhs`` columns, by the library's layout: tree-ordered
+    x and y in the core type, the moment and local pools in the compute type, the FP32 copy of x
+    that the mixed near field of real maps reads, for Toeplitz couplings the source and receiver
+    spectra of the largest level (an upper bound: every box of the level counted), and with
+    ``staging`` the device copies of host-pointer requests (FP32 / ComplexF32 x and y)."""
+    cplx = arrays.dtype == "c64"
+    mixed = precision != "plain"
+    se = 16 if cplx else 8
+    ce = se // 2 if mixed else se
+    trees = [arrays.test.tree] + ([] if arrays.galerkin else [arrays.trial.tree])
+    nmax = max(t.points.shape[0] for t in trees)
+    bmax = max(t.n_boxes for t in trees)
+    R = int(max_rhs)
+    b = 2 * nmax * R * se + 2 * bmax * arrays.K * R * ce
+    if staging:
+        b += 2 * nmax * R * (8 if cplx else 4)
+    if mixed and not cplx:
+        w = 16 if R <= 16 else 32 if R <= 32 else 64
+        b += nmax * ((R + w - 1) // w * w) * 4
+    F = _spectrum_values(arrays)
+    if F:
+        lv = max(int(np.diff(np.asarray(t.level_offsets)).max()) for t in trees)
+        cc = 8 if mixed else 16
+        rin = (R + 31) // 32 * 32 if mixed and R >= 4 else R
+        b += lv * F * (rin + R) * cc
+    return int(b)
+
+
 class AdmissionError(RuntimeError):
     """A map does not fit the server's HBM budget."""
 
@@ -80,6 +114,8 @@ class _Slot:
     keep: tuple            # the pinned allocations backing xhost / yhost
     batch: list = field(default_factory=list)
     n_out: int = 0
+    stream: int = -1       # pool index of the stream the batch in flight runs on
+    draining: object = None  # future of the copies of a finished batch's results out of yhost
 
 
 def _pinned(shape, dtype):
@@ -191,12 +227,14 @@ class Server:
     """Continuous-batching request loop over resident maps."""
 
     def __init__(self, budget_bytes: int, max_batch: int = 32, ws_per_map: int = 2, precision: str = "mixed",
-                 n_streams: int = 4, use_graph: bool = False, autotune: bool = False):
+                 n_streams: int = 4, use_graph: bool = False, autotune: bool = False, stream_depth: int = 1):
         """``use_graph``: capture each (orientation, batch width) product of a workspace in a hipGraph
         on first use (worth it for maps whose products are launch-bound; a stream of single
         requests produces every width from 1 to ``max_batch``).  ``autotune``: measure each
         admitted map's tile shapes (``h2g.autotune.tune`` at ``max_batch`` columns) before its
-        workspaces are made."""
+        workspaces are made.  ``stream_depth``: batches in flight per stream; a batch is formed
+        only when a stream has room, so requests that arrive while every stream is busy join the
+        batch rather than waiting behind it in a stream's queue."""
         self.budget = int(budget_bytes)
         self.max_batch = max_batch
         self.ws_per_map = ws_per_map
@@ -209,6 +247,13 @@ class Server:
         self._pool = None  # the streams, created with the first admitted map
         self._next_stream = 0
         self._free_at = [0.0] * self._n_streams  # expected drain time of each stream
+        self._depth = max(1, stream_depth)
+        self._inflight = [0] * self._n_streams  # batches in flight per stream
+        # results are copied out of the pinned buffers by worker threads (numpy releases the GIL
+        # for the copies), so the loop keeps forming and launching batches meanwhile
+        from concurrent.futures import ThreadPoolExecutor
+
+        self._copier = ThreadPoolExecutor(max_workers=4)
         # accounting of a stream: host seconds staging batches in / results out, and the device
         # seconds the batches were expected to take (per-map cost model)
         self.host_launch_s = self.host_harvest_s = self.device_est_s = 0.0
@@ -229,21 +274,21 @@ class Server:
     def used_bytes(self):
         return sum(m.device_bytes() for m in self.maps.values())
 
-    def add_map(self, name, arrays):
-        """Admit and build a map, or raise AdmissionError when it does not fit the budget."""
-        est = estimate_core_bytes(arrays)
-        n = arrays.test.tree.points.shape[0]
-        es = 16 if arrays.dtype == "c64" else 8
-        ws_est = self.ws_per_map * (n * self.max_batch * (4 * es) + arrays.test.tree.n_boxes * arrays.K *
-                                    self.max_batch * es + 2 * _spectrum_values(arrays) *
-                                    arrays.test.tree.n_boxes * self.max_batch * es)
-        if self.used_bytes() + est + ws_est > self.budget:
-            raise AdmissionError(f"map {name!r} needs ~{(est + ws_est) / 1e9:.1f} GB, "
-                                 f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
-        core = GpuH2Core(arrays, precisions=("plain", self.precision))
-        dtype = np.complex64 if arrays.dtype == "c64" else np.float32
-        rm = _ResidentMap(name, core, self.ws_per_map, self.max_batch, self.precision, dtype, self._stream,
-                          self.use_graph, self._tile(core))
+    def _workspace_plan(self, arrays, core_bytes):
+        """(workspaces, columns) of the first configuration that fits the budget beside a core of
+        ``core_bytes``: ``ws_per_map`` x ``max_batch``, then 1 x ``max_batch``, then 1 x
+        ``max_batch // 2``; None when none does."""
+        options = [(self.ws_per_map, self.max_batch), (1, self.max_batch), (1, max(1, self.max_batch // 2))]
+        left = self.budget - self.used_bytes() - core_bytes
+        for n_ws, nb in options:
+            if n_ws * workspace_bytes(arrays, nb, self.precision) <= left:
+                return n_ws, nb
+        return None
+
+    def _admit(self, name, core, n_ws, nb):
+        dtype = np.complex64 if core.arrays.dtype == "c64" else np.float32
+        rm = _ResidentMap(name, core, n_ws, nb, self.precision, dtype, self._stream, self.use_graph,
+                          self._tile(core, nb))
         self.maps[name] = rm
         if self.used_bytes() > self.budget:
             rm.close()
@@ -252,39 +297,64 @@ class Server:
         rm.calibrate()
         return rm
 
+    def add_map(self, name, arrays):
+        """Admit and build a map, or raise AdmissionError when it does not fit the budget."""
+        est = estimate_core_bytes(arrays)
+        plan = self._workspace_plan(arrays, est)
+        if plan is None:
+            need = est + workspace_bytes(arrays, max(1, self.max_batch // 2), self.precision)
+            raise AdmissionError(f"map {name!r} needs ~{need / 1e9:.1f} GB, "
+                                 f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
+        core = GpuH2Core(arrays, precisions=("plain", self.precision))
+        try:
+            return self._admit(name, core, *plan)
+        except AdmissionError:
+            core.close()
+            raise
+
     def add_core(self, name, core):
         """Admit an already built core (its workspaces are created here), or raise AdmissionError."""
-        a = core.arrays
-        es = 16 if a.dtype == "c64" else 8
-        n = max(core.shape)
-        ws_est = self.ws_per_map * (n * self.max_batch * (4 * es) + a.test.tree.n_boxes * a.K * self.max_batch * es +
-                                    2 * _spectrum_values(a) * a.test.tree.n_boxes * self.max_batch * es)
-        if self.used_bytes() + ws_est > self.budget:
-            raise AdmissionError(f"map {name!r} needs ~{ws_est / 1e9:.1f} GB of workspaces, "
+        plan = self._workspace_plan(core.arrays, 0)
+        if plan is None:
+            need = workspace_bytes(core.arrays, max(1, self.max_batch // 2), self.precision)
+            raise AdmissionError(f"map {name!r} needs ~{need / 1e9:.1f} GB of workspaces, "
                                  f"{(self.budget - self.used_bytes()) / 1e9:.1f} GB of the budget left")
-        dtype = np.complex64 if a.dtype == "c64" else np.float32
-        rm = _ResidentMap(name, core, self.ws_per_map, self.max_batch, self.precision, dtype, self._stream,
-                          self.use_graph, self._tile(core))
-        self.maps[name] = rm
-        rm.calibrate()
-        return rm
+        return self._admit(name, core, *plan)
 
-    def _tile(self, core):
+    def _tile(self, core, nb):
         if not self.autotune:
             return None
         from .autotune import tune
 
         # one launch per coupling level: the admission estimate sizes the spectra for the largest level
-        best, _ = tune(core, nrhs=self.max_batch, precision=self.precision, blocks=(self.max_batch,), repeats=2,
+        best, _ = tune(core, nrhs=nb, precision=self.precision, blocks=(nb,), repeats=2,
                        use_graph=self.use_graph, persistent=(0,))
         return best
 
     def remove_map(self, name):
-        self.maps.pop(name).close()
+        rm = self.maps.pop(name)
+        for slot in rm.slots:
+            if slot.draining is not None:
+                self._drained(slot)
+        rm.close()
 
     def close(self):
+        self._finish_drains()
         for name in list(self.maps):
             self.remove_map(name)
+        self._copier.shutdown(wait=True)
+
+    def _finish_drains(self):
+        for rm in self.maps.values():
+            for slot in rm.slots:
+                if slot.draining is not None:
+                    self._drained(slot)
+
+    def _drained(self, slot: _Slot):
+        self.host_harvest_s += slot.draining.result()  # (re-raises a failed copy)
+        self.done.extend(slot.batch)
+        slot.draining = None
+        slot.batch = []
 
     def submit(self, req: Request):
         if req.map_name not in self.maps:
@@ -295,20 +365,20 @@ class Server:
         return sum(len(q) for m in self.maps.values() for q in m.queues.values()) + \
             sum(len(s.batch) for m in self.maps.values() for s in m.slots)
 
-    def _launch(self, rm: _ResidentMap, slot: _Slot, op: str):
+    def _launch(self, rm: _ResidentMap, slot: _Slot, op: str, si: int):
         t_host = time.perf_counter()
         q = rm.queues[op]
-        batch = [q.popleft() for _ in range(min(self.max_batch, len(q)))]
+        batch = [q.popleft() for _ in range(min(rm.max_batch, len(q)))]
         nb = len(batch)
         n_out, n_in = rm.core.shape
         if op != "N":
             n_out, n_in = n_in, n_out
         for j, r in enumerate(batch):
             slot.xhost[:n_in, j] = r.x
-        # the stream expected to drain first
         now = time.perf_counter()
-        si = min(range(len(self._streams)), key=lambda i: max(now, self._free_at[i]))
         self._free_at[si] = max(now, self._free_at[si]) + rm.cost(nb)
+        self._inflight[si] += 1
+        slot.stream = si
         self.device_est_s += rm.cost(nb)
         slot.ws.set_stream(self._streams[si].cuda_stream)
         dt = H.H2G_C32 if rm.dtype == np.complex64 else H.H2G_F32
@@ -322,32 +392,54 @@ class Server:
         slot.n_out = n_out
         self.host_launch_s += time.perf_counter() - t_host
 
-    def _harvest(self, slot: _Slot, now: float):
-        for j, r in enumerate(slot.batch):
-            r.t_done = now
-            r.result = slot.yhost[:slot.n_out, j].copy()
-            self.done.append(r)
-        slot.batch = []
-        self.host_harvest_s += time.perf_counter() - now
+    def _harvest(self, slot: _Slot):
+        """The batch's product has finished: its stream has room again; its results are copied out
+        by a worker, after which the workspace takes a new batch."""
+        self._inflight[slot.stream] -= 1
+        slot.stream = -1
+        batch, n_out, y = slot.batch, slot.n_out, slot.yhost
+
+        def copy_out():
+            t0 = time.perf_counter()
+            for j, r in enumerate(batch):
+                r.result = y[:n_out, j].copy()
+                r.t_done = time.perf_counter()
+            return time.perf_counter() - t0
+
+        slot.draining = self._copier.submit(copy_out)
 
     def step(self):
-        """One scheduler pass: harvest finished batches, launch new ones on free workspaces."""
+        """One scheduler pass: harvest finished batches; then, while a stream has room, launch the
+        (map, orientation) whose oldest waiting request is the oldest of all, on a free workspace of
+        that map, on the stream with room expected to drain first."""
         progressed = False
         for rm in self.maps.values():
             for slot in rm.slots:
-                if slot.batch and not rm.busy(slot):
-                    self._harvest(slot, time.perf_counter())
+                if slot.draining is not None:
+                    if slot.draining.done():
+                        self._drained(slot)
+                        progressed = True
+                elif slot.batch and not rm.busy(slot):
+                    self._harvest(slot)
                     progressed = True
-            for slot in rm.slots:
-                if slot.batch:
+        while True:
+            room = [i for i in range(self._n_streams) if self._inflight[i] < self._depth]
+            if not room:
+                break
+            best = None
+            for rm in self.maps.values():
+                slot = next((sl for sl in rm.slots if not sl.batch), None)
+                if slot is None:
                     continue
-                # the orientation with the oldest waiting request goes first
-                heads = [(q[0].t_arrival, op) for op, q in rm.queues.items() if q]
-                if not heads:
-                    break
-                _, op = min(heads)
-                self._launch(rm, slot, op)
-                progressed = True
+                for op, q in rm.queues.items():
+                    if q and (best is None or q[0].t_arrival < best[0]):
+                        best = (q[0].t_arrival, rm, slot, op)
+            if best is None:
+                break
+            now = time.perf_counter()
+            si = min(room, key=lambda i: max(now, self._free_at[i]))
+            self._launch(best[1], best[2], best[3], si)
+            progressed = True
         return progressed
 
     def drain(self, timeout_s=600.0):

@@ -23,6 +23,8 @@ def test_device_operators_equal_oracle(case):
     ("symtoeplitz", 3, "equi", "laplace", 0.0, False),
     ("dense", 3, "cheb", "helmholtz", 10.0, False),
     ("symdense", 3, "cheb", "laplace", 0.0, True),
+    ("toeplitz", 4, "equi", "helmholtz", 10.0, False),
+    ("toeplitz", 3, "equi", "helmholtz", 2.0, True),
 ])
 def test_map_built_on_device_equals_oracle(kind, p, nodes, kernel, k, petrov):
     import h2g

@@ -31,6 +31,48 @@ def test_admission_rejects_over_budget():
     assert not srv.maps
 
 
+def test_admission_falls_back_to_one_workspace_then_narrower_batches():
+    """When the preferred workspaces do not fit beside the core, a map gets one workspace of the
+    full batch width, then one of half the width; below that it is refused."""
+    from h2g.serve import Server, workspace_bytes
+
+    lap, hel = _maps()
+    for arrays in (lap, hel):
+        w32, w16 = workspace_bytes(arrays, 32), workspace_bytes(arrays, 16)
+        assert w16 < w32
+        assert Server(budget_bytes=3 * w32, max_batch=32)._workspace_plan(arrays, 0) == (2, 32)
+        assert Server(budget_bytes=int(1.5 * w32), max_batch=32)._workspace_plan(arrays, 0) == (1, 32)
+        assert Server(budget_bytes=(w16 + w32) // 2, max_batch=32)._workspace_plan(arrays, 0) == (1, 16)
+        assert Server(budget_bytes=w16 - 1, max_batch=32)._workspace_plan(arrays, 0) is None
+
+
+@pytest.mark.gpu
+def test_workspace_estimate_bounds_the_library(gpu):
+    """The admission estimate of a workspace (host-pointer staging included) is at least what the
+    library allocates and within a small factor of it."""
+    import h2g
+    from h2g import _lib as H
+    from h2g.serve import workspace_bytes
+
+    for arrays in _maps():
+        core = h2g.GpuH2Core(arrays, precisions=("plain", "mixed"))
+        cplx = arrays.dtype == "c64"
+        for nb in (1, 8, 32):
+            ws = h2g.H2Workspace(core, max_rhs=nb, precision="mixed")
+            n = max(core.shape)
+            dt = np.complex64 if cplx else np.float32
+            x = np.zeros((n, nb), dtype=dt, order="F")
+            y = np.zeros((n, nb), dtype=dt, order="F")
+            code = H.H2G_C32 if cplx else H.H2G_F32
+            H.check(H.lib().h2g_matvec(ws.handle, H.H2G_OP_N, H.H2G_MUL, code, nb, H.ptr(x), n, H.ptr(y), n, None,
+                                       None, H.H2G_HOST_PTRS))
+            actual, est = ws.device_bytes(), workspace_bytes(arrays, nb)
+            print(f"{arrays.dtype} {nb} columns: library {actual} B, estimate {est} B", flush=True)
+            assert actual <= est <= 2 * actual, (arrays.dtype, nb, actual, est)
+            ws.close()
+        core.close()
+
+
 @pytest.mark.gpu
 def test_continuous_batching_same_alone_and_in_mix(gpu):
     import h2g
