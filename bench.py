@@ -165,7 +165,7 @@ def helmholtz_main(args):
     """Helmholtz workload: e^{ikr}/(4 pi r), k = 10, N points uniform on the unit sphere (seed 0),
     octree on [-1, 1]^3 with leaves of at most 64 points, Chebyshev p = 6 (K = 216), dense complex
     couplings, R complex-normal right-hand sides (seed 1).  Times the ComplexF32 form and the split
-    BF16 (hi + mid + lo) form, forward and adjoint, each against the plain ComplexF64 form one column
+    BF16 forms (hi + lo and hi + mid + lo), forward and adjoint, each against the plain ComplexF64 form one column
     at a time, and reports every form's error against the O(N^2) FP64 direct sum on 1,000 targets."""
     import numpy as np
     import torch
@@ -186,7 +186,7 @@ def helmholtz_main(args):
     t_build = time.perf_counter() - t0
     log(f"map built ({args.plan} plan) in {t_build:.1f}s: {arrays.stats}")
     t0 = time.perf_counter()
-    core = h2g.GpuH2Core(arrays, precisions=("plain", "mixed", "bf16x3"))
+    core = h2g.GpuH2Core(arrays, precisions=("plain", "mixed", "bf16x2", "bf16x3"))
     torch.cuda.synchronize()
     t_core = time.perf_counter() - t0
     log(f"core on device in {t_core:.1f}s, {core.device_bytes() / 1e9:.1f} GB")
@@ -225,7 +225,7 @@ def helmholtz_main(args):
         del diff, d, dd, G
     rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
     out_forms = {}
-    for prec in ("mixed", "bf16x3"):
+    for prec in ("mixed", "bf16x2", "bf16x3"):
         A = h2g.H2Workspace(core, max_rhs=R, precision=prec, use_graph=True)
         form = {}
         for op, M in (("N", A), ("C", h2g.adjoint(A))):
