@@ -68,8 +68,8 @@ def parse():
                          "as the Laplace maps) or Chebyshev dense (the Helmholtz workload's form, whose "
                          "operator stream costs a product the same from 1 to 16 columns)")
     ap.add_argument("--serve-rate-low", type=float, default=-1.0,
-                    help="second operating point, below capacity (requests/s; negative: 25%% of the "
-                         "throughput served at --serve-rate; 0: skip)")
+                    help="operating point below capacity (requests/s; negative: two points, 10%% and 25%% of "
+                         "the throughput served at --serve-rate; 0: skip)")
     ap.add_argument("--serve-requests-low", type=int, default=1000)
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (one JSON line)")
     ap.add_argument("--workload", default="laplace", choices=["laplace", "helmholtz"],
@@ -374,20 +374,26 @@ def serving_leg(args):
     per_map = per_map_stats(reqs)
     st["accounting"] = dict(srv.last_accounting)
     log(f"serving: {st}")
-    low = None
-    # the main rate saturates the server whenever it exceeds what the mix can serve; the second
-    # point arrives at a fraction of the measured throughput, so its latencies are service times
-    # plus batching delay rather than queueing
-    rate_low = args.serve_rate_low if args.serve_rate_low >= 0 else round(0.25 * st["matvecs_per_s"], 1)
-    if rate_low > 0 and args.serve_requests_low > 0:
+    low = []
+    # the main rate saturates the server whenever it exceeds what the mix can serve; the points
+    # below arrive at fractions of the measured throughput, so their latencies are service times
+    # plus batching and stream waits rather than queueing (batches are narrower there, and a
+    # narrow product costs nearly a wide one, so the capacity at low load is below the saturated
+    # throughput)
+    rates = [args.serve_rate_low] if args.serve_rate_low >= 0 else \
+        [round(f * st["matvecs_per_s"], 1) for f in (0.1, 0.25)]
+    for i, rate_low in enumerate(rates):
+        if rate_low <= 0 or args.serve_requests_low <= 0:
+            continue
         reqs_low = stream(args.serve_requests_low)
         srv.done.clear()
-        sl = srv.run_stream(reqs_low, rate_hz=rate_low, seed=8)
-        low = {"requests": sl["requests"], "arrival_rate_hz": rate_low,
-               "matvecs_per_s": round(sl["matvecs_per_s"], 1), "p50_ms": round(sl["p50_ms"], 1),
-               "p99_ms": round(sl["p99_ms"], 1), "mean_batch": round(sl["mean_batch"], 2),
-               "per_map": per_map_stats(reqs_low), "accounting": dict(srv.last_accounting)}
-        log(f"serving at {rate_low}/s: {low}")
+        sl = srv.run_stream(reqs_low, rate_hz=rate_low, seed=8 + i)
+        pt = {"requests": sl["requests"], "arrival_rate_hz": rate_low,
+              "matvecs_per_s": round(sl["matvecs_per_s"], 1), "p50_ms": round(sl["p50_ms"], 1),
+              "p99_ms": round(sl["p99_ms"], 1), "mean_batch": round(sl["mean_batch"], 2),
+              "per_map": per_map_stats(reqs_low), "accounting": dict(srv.last_accounting)}
+        low.append(pt)
+        log(f"serving at {rate_low}/s: {pt}")
     # plain FP64 one request at a time: service time per map; and each map's served results (two
     # per map) against the plain product
     svc, errs = {}, {}

@@ -858,13 +858,14 @@ void build_lattice_level(const Core &c, OrientPlan &P, int l, const std::vector<
 
 // Row-block tasks of one orientation: for each output leaf (CSR row t = blocks order[p[t]] ..
 // order[p[t+1]-1], all with the same output points), the source columns of all its blocks in
-// order, and one task per 64 output rows.  Tasks are grouped by row-tile count, largest first.
+// order, and one task per 4 tiles of `tile_rows` output rows (16 for real maps; 8 for complex
+// maps, whose MFMA tiles hold the real form).  Tasks are grouped by tile count, largest first.
 // Source indices must fit int32; otherwise the plan stays empty and the block kernels run.
 // With `frag`, also the padded source-point tables and each task's place in the FP32 row-major
 // copy (filled by build_near_frags once the values are set).
 void build_near_plan(NearPlan &plan, bool trans, const std::vector<int64_t> &p, const std::vector<int32_t> &order,
                      const std::vector<int64_t> &rb, const std::vector<int64_t> &cb, const std::vector<int32_t> &nr,
-                     const std::vector<int32_t> &nc, const std::vector<int64_t> &off, bool frag) {
+                     const std::vector<int32_t> &nc, const std::vector<int64_t> &off, bool frag, int tile_rows) {
     plan.task.release();
     plan.col.release();
     plan.frag.release();
@@ -900,17 +901,17 @@ void build_near_plan(NearPlan &plan, bool trans, const std::vector<int64_t> &p, 
         if (frag) {
             for (int64_t j = 0; j < near_ncolp(ncol); ++j) xrow.push_back(cols[col0 + std::min(j, ncol - 1)].xrow);
         }
-        for (int r0 = 0; r0 < n_out; r0 += 64) {
+        for (int r0 = 0; r0 < n_out; r0 += 4 * tile_rows) {
             NearTask k;
             k.out0 = out0;
             k.col0 = col0;
-            k.nrow = std::min(64, n_out - r0);
+            k.nrow = std::min(4 * tile_rows, n_out - r0);
             k.ncol = (int32_t)ncol;
             k.row0 = r0;
             k.pad = 0;
             k.foff = 0;
             k.xoff = xoff;
-            byrt[(k.nrow + 15) / 16 - 1].push_back(k);
+            byrt[(k.nrow + tile_rows - 1) / tile_rows - 1].push_back(k);
         }
     }
     std::vector<NearTask> tasks;
@@ -980,7 +981,9 @@ void build_near(Core &c, const h2g_core_desc &d) {
         // the mixed form of real maps reads the near values rounded to FP32, per task in row-major
         // order (half the HBM traffic of the near field; products and sums stay FP64)
         if (c.dtype == H2G_F64)
-            build_near_plan(plan, trans, p, order, rb, cb, nr, nc, off, c.reduced_precisions() && nb > 0);
+            build_near_plan(plan, trans, p, order, rb, cb, nr, nc, off, c.reduced_precisions() && nb > 0, 16);
+        else if (c.dtype == H2G_C64)  // tiles of 8 complex rows (16 rows of the real form)
+            build_near_plan(plan, trans, p, order, rb, cb, nr, nc, off, false, 8);
     };
     csr(d.near_test, d.test.n_boxes, c.near_fwd_tgt, c.near_fwd_ptr, c.near_fwd_blk, c.n_near_fwd_tgt,
         c.near_fwd_plan, false);

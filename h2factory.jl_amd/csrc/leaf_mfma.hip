@@ -261,6 +261,114 @@ __global__ void __launch_bounds__(TPB, 2) near_frag_kernel(const NearTask *__res
         }
 }
 
+// Near field of complex maps on FP64 MFMA, one row-block product per task as near_task_kernel,
+// in real form: a complex product Y = A X is the real product of the matrix with entries
+// [Re A, -Im A; Im A, Re A] (row 2i + c, column 2j + b) and [Re X; Im X] (row 2j + b), so a
+// 16-row tile holds 8 complex rows and a k-step 2 complex columns.  A lane's A operand is one FP64
+// part of its entry (Re when c == b, else Im, negated for c = 0 -- c = 1 for the adjoint, which
+// conjugates A) and its B operand one part of its source value; D row 2i + c is part c of row i.
+// Tasks have at most 4 tiles (32 complex rows).
+template <bool TRANS, bool CONJ, int RT, int KU, int NCT>
+__global__ void __launch_bounds__(TPB, 2)
+    near_task_cplx_kernel(const NearTask *__restrict__ tasks, int64_t t0, int64_t t1, const NearCol *__restrict__ cols,
+                          const double *__restrict__ vals, const double *__restrict__ xs, double *__restrict__ ys,
+                          int R) {
+    constexpr int KBC = 2 * KU;  // complex columns per batch
+    int wg = blockIdx.x;
+    const int n8 = gridDim.x & ~7;
+    if (wg < n8) wg = (wg & 7) * (n8 >> 3) + (wg >> 3);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t ti = t0 + (int64_t)wg * (TPB / 64) + wave;
+    if (ti >= t1) return;
+    const NearTask tk = tasks[ti];
+    const int nrow = tk.nrow, ncol = tk.ncol;
+    const NearCol *__restrict__ cl = cols + tk.col0;
+    const int lane = threadIdx.x & 63;
+    const int ar = mf_a_row(lane), ak = mf_a_k(lane);
+    // this lane's A part (Re or +-Im) and B part (Re or Im); its column within a k-step
+    const int cpart = ar & 1, bpart = ak & 1, jk = ak >> 1;
+    const int apart = cpart == bpart ? 0 : 1;
+    const double asgn = apart == 0 ? 1.0 : ((cpart == 0) != CONJ ? -1.0 : 1.0);
+    const int c0 = blockIdx.y * NCT * 16;
+    const int tl = lane & (KBC - 1);
+    int64_t arow[RT];
+    double rmask[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int r = 8 * rt + (ar >> 1);
+        arow[rt] = tk.row0 + min(r, nrow - 1);
+        rmask[rt] = r < nrow ? asgn : 0.0;
+    }
+    int xcol[NCT];
+    double cmask[NCT];
+#pragma unroll
+    for (int g = 0; g < NCT; ++g) {
+        const int col = c0 + 16 * g + ar;
+        xcol[g] = min(col, R - 1);
+        cmask[g] = col < R ? 1.0 : 0.0;
+    }
+    double a[KU][RT], b[KU][NCT], km[KU];
+    auto load_ops = [&](int k0, const NearCol &ev) {
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int src = (2 * u + jk) << 2;  // byte address of the lane holding this column's entry
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)ev.aoff);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(ev.aoff >> 32));
+            const int xr = __builtin_amdgcn_ds_bpermute(src, ev.xrow);
+            const int64_t rs = TRANS ? (int64_t)__builtin_amdgcn_ds_bpermute(src, ev.rstride) : 1;
+            km[u] = (k0 + 2 * u + jk < ncol) ? 1.0 : 0.0;
+            const double *__restrict__ ap = vals + 2 * (int64_t)(((uint64_t)hi << 32) | lo) + apart;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) a[u][rt] = ap[2 * arow[rt] * rs];
+            const double *__restrict__ xp = xs + 2 * (int64_t)xr * R + bpart;
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) b[u][g] = xp[2 * xcol[g]];
+        }
+    };
+    double4_t acc[RT][NCT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) acc[rt][g] = double4_t{0.0, 0.0, 0.0, 0.0};
+    const int nb = (ncol + KBC - 1) / KBC;
+    NearCol tb0 = cl[min(tl, ncol - 1)], tb1 = cl[min(KBC + tl, ncol - 1)];
+    load_ops(0, tb0);
+    for (int ib = 0; ib < nb; ++ib) {
+        double ac[KU][RT], bc[KU][NCT];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) ac[u][rt] = a[u][rt] * (rmask[rt] * km[u]);
+#pragma unroll
+            for (int g = 0; g < NCT; ++g) bc[u][g] = b[u][g] * (cmask[g] * km[u]);
+        }
+        const NearCol tb2 = cl[min((ib + 2) * KBC + tl, ncol - 1)];
+        load_ops((ib + 1) * KBC, tb1);
+        __asm__ volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < KU; ++u)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int g = 0; g < NCT; ++g)
+                    acc[rt][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u][rt], bc[u][g], acc[rt][g], 0, 0, 0);
+        tb1 = tb2;
+    }
+    double *__restrict__ yo = ys + 2 * (tk.out0 + tk.row0) * R + (ak & 1);
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int g = 0; g < NCT; ++g) {
+            const int col = c0 + 16 * g + ar;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                // D row q + 4v (q = lane / 16) is part q % 2 of complex row q / 2 + 2v of the tile
+                const int row = 8 * rt + (ak >> 1) + 2 * v;
+                if (row < nrow && col < R) yo[2 * ((int64_t)row * R + col)] = acc[rt][g][v];
+            }
+        }
+}
+
 // Leaf expansions on FP64 MFMA (real maps, R >= MFMA_MIN_R), FP64 accumulation.
 //   L2P: Y_leaf (n x R) += alpha C^T (n x K) L (K x R)      rows = points, contraction over k
 //   P2M: M (K x R)      = C (K x n) X (n x R)                rows = k, contraction over points
@@ -527,11 +635,50 @@ bool launch_p2m_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
 
 bool launch_near_mfma(int op, const Core &c, const void *xs, void *ys, int R, float *x32, int near_cols,
                       hipStream_t s) {
-    if (c.dtype != H2G_F64 || R < MFMA_MIN_R) return false;
+    if ((c.dtype != H2G_F64 && c.dtype != H2G_C64) || R < MFMA_MIN_R) return false;
     const bool trans = op != H2G_OP_N;
     const NearPlan &P = trans ? c.near_rev_plan : c.near_fwd_plan;
     if (!P.ok) return false;
     using std::integral_constant;
+    if (c.dtype == H2G_C64) {
+        // complex maps (both forms: the near field computes in ComplexF64), up to 32 RHS per
+        // workgroup, tasks of 4, 3, 2, 1 tiles of 8 complex rows.  Measured on Helmholtz maps of
+        // 1.2M (Toeplitz) and 1M (dense) points on the sphere: the block kernels win at 1 and 2
+        // columns (3.0 / 6.1 ms against 9.3 / 9.6 ms forward), this one from 4 columns on (9.8
+        // against 11.8 ms at 4 columns, 14 against 27 ms forward and 16 against 82 ms adjoint at
+        // 32); batches of 16 complex columns for tasks of 1-2 tiles and 8 for 3-4 tiles (8 for
+        // all: up to 10 % slower; 8 and 4: up to 35 % slower)
+        if (R < 3) return false;
+        const int nct = R > 16 ? 2 : 1;
+        const unsigned gy = (unsigned)((R + 16 * nct - 1) / (16 * nct));
+        for (int i = 0; i < 4; ++i) {
+            const int64_t t0 = P.cls[i], t1 = P.cls[i + 1];
+            if (t1 == t0) continue;
+            const dim3 grid((unsigned)((t1 - t0 + TPB / 64 - 1) / (TPB / 64)), gy);
+            auto pick = [&](auto rt_c, auto ku_c, auto nct_c) {
+                constexpr int RT = decltype(rt_c)::value, KU = decltype(ku_c)::value, NC = decltype(nct_c)::value;
+                auto kern = op == H2G_OP_N   ? near_task_cplx_kernel<false, false, RT, KU, NC>
+                            : op == H2G_OP_T ? near_task_cplx_kernel<true, false, RT, KU, NC>
+                                             : near_task_cplx_kernel<true, true, RT, KU, NC>;
+                hipLaunchKernelGGL(kern, grid, TPB, 0, s, P.task.as<NearTask>(), t0, t1, P.col.as<NearCol>(),
+                                   (const double *)c.near_vals.ptr, (const double *)xs, (double *)ys, R);
+            };
+            auto by_nct = [&](auto rt_c, auto ku_c) {
+                if (nct == 1)
+                    pick(rt_c, ku_c, integral_constant<int, 1>{});
+                else
+                    pick(rt_c, ku_c, integral_constant<int, 2>{});
+            };
+            switch (4 - i) {
+            case 1: by_nct(integral_constant<int, 1>{}, integral_constant<int, 8>{}); break;
+            case 2: by_nct(integral_constant<int, 2>{}, integral_constant<int, 8>{}); break;
+            case 3: by_nct(integral_constant<int, 3>{}, integral_constant<int, 4>{}); break;
+            default: by_nct(integral_constant<int, 4>{}, integral_constant<int, 4>{}); break;
+            }
+            H2G_CHECK_LAUNCH();
+        }
+        return true;
+    }
     if (x32 && P.frag.ptr) {
         // mixed form: FP32 source columns, then the tasks on their row-major FP32 values
         const int nct = near_x32_nct(R, near_cols);
