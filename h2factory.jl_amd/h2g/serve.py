@@ -197,9 +197,9 @@ class _ResidentMap:
 
 def estimate_core_bytes(arrays, precisions=("plain", "mixed")):
     """Device bytes of a core before it is built (admission check): near values (and for real maps
-    with a reduced precision their FP32 row-major copies per orientation, columns padded to 16),
-    collections, couplings, transfers, and for dense couplings and non-separable (complex)
-    transfers the MFMA fragment matrices per orientation and representation."""
+    with a reduced precision their FP32 row-major copies per orientation, columns padded to 16)
+    and task tables, collections, couplings, transfers, and for dense couplings and non-separable
+    (complex) transfers the MFMA fragment matrices per orientation and representation."""
     cplx = arrays.dtype == "c64"
     es = 16 if cplx else 8
     reduced = any(p != "plain" for p in precisions)
@@ -210,6 +210,16 @@ def estimate_core_bytes(arrays, precisions=("plain", "mixed")):
     b += arrays.ops.nbytes * (1.5 if reduced else 1.0)
     b += arrays.test.transfers.nbytes * 1.5
     b += len(arrays.near_pairs) * 40 + len(arrays.pairs) * 24 + n * 24
+    # the near field's row-block task tables: a 16-byte entry per source point of every task (at
+    # most 64 output rows, 32 for complex maps), in each orientation
+    if len(arrays.near_pairs):
+        t = arrays.test.tree
+        r = t if arrays.galerkin else arrays.trial.tree
+        near = np.asarray(arrays.near_pairs, dtype=np.int64)
+        st = (np.asarray(t.end) - np.asarray(t.begin))[near[:, 0]]
+        sr = (np.asarray(r.end) - np.asarray(r.begin))[near[:, 1]]
+        T = 32 if cplx else 64
+        b += 16 * int((sr * -(-st // T)).sum() + (st * -(-sr // T)).sum())
     krp2 = (16 * ((arrays.K * (2 if cplx else 1) + 15) // 16)) ** 2
     per = ("plain" in precisions) * 8 + reduced * 4 + any(p.startswith("bf16") for p in precisions) * 6
     if arrays.m2l_kind in ("dense", "symdense"):
