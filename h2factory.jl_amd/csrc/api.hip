@@ -314,6 +314,27 @@ void copy_side(Side &S, const h2g_side_desc &d, int dtype, int K, bool want32) {
                 }
             }
         }
+        // complex maps whose transfers are real (interpolation transfers are, whatever the kernel):
+        // their real factors, for the real separable kernels on planar expansion rows
+        if (dtype == H2G_C64 && p * p * p == K && p > 1 && p <= 6 && d.n_levels > 1) {
+            const double *tc = (const double *)d.transfers;  // (re, im) pairs
+            bool real = true;
+            for (size_t i = 0; i < ntr && real; ++i) real = tc[2 * i + 1] == 0.0;
+            if (real) {
+                std::vector<double> tr(ntr), fac;
+                for (size_t i = 0; i < ntr; ++i) tr[i] = tc[2 * i];
+                if (extract_factors(tr.data(), d.n_levels, K, p, fac)) {
+                    S.sep_shared = share_factors(tr.data(), d.n_levels, K, p, fac);
+                    S.factors.upload_vec(fac);
+                    if (want32) {
+                        std::vector<float> f32(fac.begin(), fac.end());
+                        S.factors32.upload_vec(f32);
+                    }
+                    S.sep_cplx = true;
+                    S.sep_p = p;
+                }
+            }
+        }
     }
     if (want32) {
         auto narrow = [&](const void *src, size_t n, DevBuf &dst) {
@@ -1037,6 +1058,7 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     const Side &src = *P.src, &dst = *P.dst;
     const bool mixed = w.opts.precision != H2G_PREC_PLAIN;
     const int cd = mixed ? (c.dtype == H2G_F64 ? H2G_F32 : H2G_C32) : c.dtype;
+    const int rd = mixed ? H2G_F32 : H2G_F64;  // the real type of the compute type
     const bool conj = op == H2G_OP_C;
     const bool cmap = c.dtype == H2G_C64;
     const int32_t *L = P.lists.as<int32_t>();
@@ -1054,11 +1076,25 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
             w.mark_begin(2, l);
             launch_p2m(cd, conj, L + P.p2m[l].begin, P.p2m[l].count, src.d_begin.as<int64_t>(),
                        src.d_end.as<int64_t>(), scoll, K, w.xs.ptr, R, w.mom.ptr, s);
+            // real transfers of a complex map: the leaves' moments become planar rows for M2M
+            if (src.sep_cplx) launch_cplx_planar(cd, L + P.p2m[l].begin, P.p2m[l].count, w.mom.ptr, R, K, true, s);
             w.mark_end();
         }
         if (P.m2m[l].count) {
             w.mark_begin(3, l);
-            if (src.separable && src.sep_shared)
+            if (src.sep_cplx) {
+                // real factors on planar rows (2 R real columns); conjugation leaves them unchanged
+                const void *fac = mixed ? src.factors32.ptr : src.factors.ptr;
+                if (src.sep_shared)
+                    launch_sep_m2m(rd, 0, P.m2m[l].count, L + P.m2m[l].begin, P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l],
+                                   P.m2m_src.as<int32_t>(), P.m2m_mat.as<int32_t>(), fac, (l + 1) * 8, src.sep_p,
+                                   w.mom.ptr, 2 * R, s);
+                else
+                    launch_sep_transfer(rd, 0, P.m2m[l].count, L + P.m2m[l].begin,
+                                        P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
+                                        P.m2m_mat.as<int32_t>(), fac, (l + 1) * 8, src.sep_p, w.mom.ptr, w.mom.ptr,
+                                        2 * R, 0, nullptr, s);
+            } else if (src.separable && src.sep_shared)
                 launch_sep_m2m(cd, conj ? 3 : 0, P.m2m[l].count, L + P.m2m[l].begin,
                                P.m2m_ptr.as<int64_t>() + P.m2m_ptr_base[l], P.m2m_src.as<int32_t>(),
                                P.m2m_mat.as<int32_t>(), mixed ? src.factors32.ptr : src.factors.ptr, (l + 1) * 8,
@@ -1080,6 +1116,15 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
                                nullptr, s);
             w.mark_end();
         }
+    }
+    // the couplings read interleaved moments
+    if (src.sep_cplx) {
+        w.mark_begin(3, -1);
+        for (int l = 0; l < src.n_levels; ++l) {
+            launch_cplx_planar(cd, L + P.p2m[l].begin, P.p2m[l].count, w.mom.ptr, R, K, false, s);
+            launch_cplx_planar(cd, L + P.m2m[l].begin, P.m2m[l].count, w.mom.ptr, R, K, false, s);
+        }
+        w.mark_end();
     }
     // couplings
     const int opA = op == H2G_OP_N ? 0 : (op == H2G_OP_T ? 1 : 2);
@@ -1189,10 +1234,23 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     }
     // downward pass
     const cplx<double> a = conj ? conjv(c.alpha) : c.alpha;
+    // real transfers of a complex map: the couplings' results become planar rows for L2L
+    if (dst.sep_cplx) {
+        w.mark_begin(7, -1);
+        for (int l = 0; l < dst.n_levels; ++l)
+            launch_cplx_planar(cd, L + P.m2l_recv[l].begin, P.m2l_recv[l].count, w.loc.ptr, R, K, true, s);
+        w.mark_end();
+    }
     for (int l = 0; l < dst.n_levels; ++l) {
         if (P.l2l[l].count) {
             w.mark_begin(7, l);
-            if (dst.separable)
+            if (dst.sep_cplx)
+                launch_sep_l2l(rd, 1, P.l2l_par[l].count, P.l2lp_box.as<int32_t>() + P.l2l_par[l].begin,
+                               P.l2lp_ptr.as<int64_t>() + P.l2lp_base[l], P.l2lp_child.as<int32_t>(),
+                               P.l2lp_mat.as<int32_t>(), P.l2lp_acc.as<uint8_t>(),
+                               mixed ? dst.factors32.ptr : dst.factors.ptr, l * 8, dst.sep_p, dst.sep_shared, w.loc.ptr,
+                               2 * R, s);
+            else if (dst.separable)
                 launch_sep_l2l(cd, conj ? 2 : 1, P.l2l_par[l].count, P.l2lp_box.as<int32_t>() + P.l2l_par[l].begin,
                                P.l2lp_ptr.as<int64_t>() + P.l2lp_base[l], P.l2lp_child.as<int32_t>(),
                                P.l2lp_mat.as<int32_t>(), P.l2lp_acc.as<uint8_t>(),
@@ -1214,6 +1272,7 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
                 join = nullptr;
             }
             w.mark_begin(8, l);
+            if (dst.sep_cplx) launch_cplx_planar(cd, L + P.l2p[l].begin, P.l2p[l].count, w.loc.ptr, R, K, false, s);
             launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(),
                        dst.d_end.as<int64_t>(), dcoll, K, w.loc.ptr, R, w.ys.ptr, a, P.l2p_avg[l], s);
             w.mark_end();
@@ -1553,7 +1612,7 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
             const int32_t *pf = c->m2l_dense.ok ? pair_fop.data() : nullptr;
             // transfers that are not separable (complex maps, general operators) on the dense kernel
             for (Side *S : {&c->test, c->galerkin ? nullptr : &c->trial_own}) {
-                if (!S || S->separable || S->n_levels < 2) continue;
+                if (!S || S->separable || S->sep_cplx || S->n_levels < 2) continue;
                 std::vector<int32_t> mats(S->n_levels * 8);
                 for (int i = 0; i < (int)mats.size(); ++i) mats[i] = i;
                 build_dense_ops(S->tr_dense, d.dtype, d.K, S->transfers.ptr, mats.data(), nullptr, nullptr,

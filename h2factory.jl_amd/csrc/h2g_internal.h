@@ -221,6 +221,10 @@ struct Side {
     DevBuf coll, coll32;            // K x n_dofs core type / reduced type
     DevBuf transfers, transfers32;  // [n_levels][8] K x K
     bool separable = false;         // every transfer is Az (x) Ay (x) Ax
+    // complex map whose transfers are real and separable: `factors` hold their real factors, and
+    // the transfers run on the real separable kernels over expansion rows made planar (all real
+    // parts, then all imaginary parts), i.e. a real pool of twice the columns
+    bool sep_cplx = false;
     bool sep_shared = false;        // ... with one factor per axis and child bit (stored per sector)
     int sep_p = 0;
     DevBuf factors, factors32;      // [n_levels*8][3][p][p]
@@ -489,6 +493,11 @@ void launch_diag_m2l(int cdtype, bool conj, int n_recv, const int32_t *recv, con
                      const int32_t *item_src, const int32_t *item_op, const void *ops, int K, const void *in, void *out,
                      int R, hipStream_t s);
 // M2M of all children of each target (shared factors: Side::sep_shared), overwriting the target
+// In place, for the boxes of a list: complex expansion rows (R per box, K values each) between the
+// interleaved layout (K complex values) and the planar one (K real parts, then K imaginary parts).
+// cdtype: H2G_C64 or H2G_C32.
+void launch_cplx_planar(int cdtype, const int32_t *boxes, int n, void *pool, int R, int K, bool to_planar,
+                        hipStream_t s);
 void launch_sep_m2m(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
                     const int32_t *item_src, const int32_t *item_mat, const void *factors, int mat_base, int p,
                     void *mom, int R, hipStream_t s);

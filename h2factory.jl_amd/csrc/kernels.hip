@@ -1065,7 +1065,8 @@ __global__ void __launch_bounds__(NT) sep_l2l_shared_kernel(int opA, int n_par, 
         const int js = e / LINES, line = e - js * LINES;  // js = j * 8 + sector
         const int box = e < ZL ? s_box[js] : -1;
         const bool acc = box >= 0 && s_acc[js] != 0;
-        const T *src = loc + ((int64_t)(box < 0 ? 0 : box) * R + s_r[js >> 3]) * K + line;
+        // (the column table is read only for lines of the workgroup's jobs: past ZL, js / 8 >= JB)
+        const T *src = loc + ((int64_t)(box < 0 ? 0 : box) * R + (e < ZL ? s_r[js >> 3] : 0)) * K + line;
 #pragma unroll
         for (int o = 0; o < p; ++o) pre[u][o] = acc ? src[LINES * o] : zero_v<T>();
     }
@@ -1282,6 +1283,47 @@ static void sep_l2l_p(int opA, int n_par, const int32_t *parents, const int64_t 
     const size_t sh = sizeof(T) * (size_t)(3 * JB * K + 8 * 3 * P * P) + sizeof(int) * (size_t)(3 * JB + 1);
     hipLaunchKernelGGL((sep_l2l_kernel<T, JB, P>), (unsigned)((njobs + JB - 1) / JB), TPB, sh, s,
         opA, n_par, parents, child_ptr, child_box, child_mat, child_acc, (const T *)factors, mat_base, (T *)loc, R);
+}
+
+// One workgroup per box of the list: RB rows at a time pass through LDS and return permuted.
+template <class Rl>
+__global__ void cplx_planar_kernel(const int32_t *__restrict__ boxes, Rl *__restrict__ pool, int R, int K, int RB,
+                                   bool to_planar) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Rl *buf = reinterpret_cast<Rl *>(smem_raw);
+    const int K2 = 2 * K;
+    Rl *base = pool + (int64_t)boxes[blockIdx.x] * R * K2;
+    for (int r0 = 0; r0 < R; r0 += RB) {
+        const int n2 = min(RB, R - r0) * K2;
+        Rl *rows = base + (int64_t)r0 * K2;
+        for (int e = threadIdx.x; e < n2; e += (int)blockDim.x) buf[e] = rows[e];
+        __syncthreads();
+        for (int e = threadIdx.x; e < n2; e += (int)blockDim.x) {
+            const int row = e / K2, q = e - row * K2;
+            // planar q = (part, k) <- interleaved 2 k + part; interleaved q = (k, part) <- part K + k
+            const int src = to_planar ? 2 * (q % K) + q / K : (q & 1) * K + (q >> 1);
+            rows[e] = buf[row * K2 + src];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_cplx_planar(int cdtype, const int32_t *boxes, int n, void *pool, int R, int K, bool to_planar,
+                        hipStream_t s) {
+    if (n == 0) return;
+    auto go = [&](auto tv) {
+        using Rl = decltype(tv);
+        const int RB = std::max(1, std::min(R, (int)(16384 / (2 * (size_t)K * sizeof(Rl)))));
+        hipLaunchKernelGGL((cplx_planar_kernel<Rl>), n, TPB, sizeof(Rl) * (size_t)RB * 2 * K, s, boxes, (Rl *)pool, R,
+                           K, RB, to_planar);
+    };
+    if (cdtype == H2G_C64)
+        go(0.0);
+    else if (cdtype == H2G_C32)
+        go(0.0f);
+    else
+        throw Error{H2G_EINVAL, "planar rows are for complex expansions"};
+    H2G_CHECK_LAUNCH();
 }
 
 void launch_sep_m2m(int cdtype, int opA, int n_targets, const int32_t *targets, const int64_t *item_ptr,
