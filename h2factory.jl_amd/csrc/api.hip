@@ -1074,10 +1074,18 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
     for (int l = src.n_levels - 1; l >= 0; --l) {
         if (P.p2m[l].count) {
             w.mark_begin(2, l);
-            launch_p2m(cd, conj, L + P.p2m[l].begin, P.p2m[l].count, src.d_begin.as<int64_t>(),
-                       src.d_end.as<int64_t>(), scoll, K, w.xs.ptr, R, w.mom.ptr, s);
-            // real transfers of a complex map: the leaves' moments become planar rows for M2M
-            if (src.sep_cplx) launch_cplx_planar(cd, L + P.p2m[l].begin, P.p2m[l].count, w.mom.ptr, R, K, true, s);
+            if (src.sep_cplx && src.coll_real) {
+                // real collection: the real leaf GEMM over x as 2 R real columns writes planar rows
+                launch_p2m(rd, false, L + P.p2m[l].begin, P.p2m[l].count, src.d_begin.as<int64_t>(),
+                           src.d_end.as<int64_t>(), mixed ? src.coll32_re.ptr : src.coll_re.ptr, K, w.xs.ptr,
+                           2 * R, w.mom.ptr, s);
+            } else {
+                launch_p2m(cd, conj, L + P.p2m[l].begin, P.p2m[l].count, src.d_begin.as<int64_t>(),
+                           src.d_end.as<int64_t>(), scoll, K, w.xs.ptr, R, w.mom.ptr, s);
+                // real transfers of a complex map: the leaves' moments become planar rows for M2M
+                if (src.sep_cplx)
+                    launch_cplx_planar(cd, L + P.p2m[l].begin, P.p2m[l].count, w.mom.ptr, R, K, true, s);
+            }
             w.mark_end();
         }
         if (P.m2m[l].count) {
@@ -1272,9 +1280,17 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
                 join = nullptr;
             }
             w.mark_begin(8, l);
-            if (dst.sep_cplx) launch_cplx_planar(cd, L + P.l2p[l].begin, P.l2p[l].count, w.loc.ptr, R, K, false, s);
-            launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(),
-                       dst.d_end.as<int64_t>(), dcoll, K, w.loc.ptr, R, w.ys.ptr, a, P.l2p_avg[l], s);
+            if (dst.sep_cplx && dst.coll_real) {
+                // real collection (and alpha): the real leaf GEMM from planar rows into y as 2 R real columns
+                launch_l2p(rd, false, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(),
+                           dst.d_end.as<int64_t>(), mixed ? dst.coll32_re.ptr : dst.coll_re.ptr, K, w.loc.ptr,
+                           2 * R, w.ys.ptr, a, P.l2p_avg[l], s);
+            } else {
+                if (dst.sep_cplx)
+                    launch_cplx_planar(cd, L + P.l2p[l].begin, P.l2p[l].count, w.loc.ptr, R, K, false, s);
+                launch_l2p(cd, conj, L + P.l2p[l].begin, P.l2p[l].count, dst.d_begin.as<int64_t>(),
+                           dst.d_end.as<int64_t>(), dcoll, K, w.loc.ptr, R, w.ys.ptr, a, P.l2p_avg[l], s);
+            }
             w.mark_end();
         }
     }
@@ -1471,7 +1487,8 @@ int64_t Core::device_bytes() const {
         for (int o = 0; o < 4; ++o) b += D.f64[o].bytes + D.f32[o].bytes + D.bf[o].bytes;
     };
     auto side = [&](const Side &S) {
-        b += S.d_perm.bytes + S.d_iperm.bytes + S.d_begin.bytes + S.d_end.bytes + S.coll.bytes + S.coll32.bytes + S.transfers.bytes +
+        b += S.d_perm.bytes + S.d_iperm.bytes + S.d_begin.bytes + S.d_end.bytes + S.coll.bytes + S.coll32.bytes +
+             S.coll_re.bytes + S.coll32_re.bytes + S.transfers.bytes +
              S.transfers32.bytes + S.factors.bytes + S.factors32.bytes;
         dense(S.tr_dense);
     };
@@ -1536,6 +1553,10 @@ int h2g_core_create(const h2g_core_desc *desc, h2g_core **out) {
                 copy_side(c->trial_own, d.trial, d.dtype, d.K, want32);
                 c->trial = &c->trial_own;
             }
+            // given collections that are real (a collection to be filled by h2g_method_fill is
+            // checked after the fill)
+            if (d.test.collection) make_real_collection(c->test, *c);
+            if (!c->galerkin && d.trial.collection) make_real_collection(c->trial_own, *c);
             const bool toep = is_toeplitz(*c);
             const bool diag = d.m2l_kind == H2G_M2L_DIAGONAL;
             const size_t opelems = toep ? (size_t)d.n_ops * c->L3 : (size_t)d.n_ops * d.K * (diag ? 1 : d.K);

@@ -219,6 +219,11 @@ struct Side {
     std::vector<int32_t> coord;  // [n_boxes][3] or empty
     DevBuf d_perm, d_iperm, d_begin, d_end;  // d_iperm: input index -> tree index
     DevBuf coll, coll32;            // K x n_dofs core type / reduced type
+    // complex map with real transfers (sep_cplx), a real collection and a real alpha: the
+    // collection kept real (FP64 / FP32, `coll` and `coll32` released), so P2M and L2P run the real
+    // leaf kernels over x, y viewed as 2R real columns and planar expansion rows
+    bool coll_real = false;
+    DevBuf coll_re, coll32_re;
     DevBuf transfers, transfers32;  // [n_levels][8] K x K
     bool separable = false;         // every transfer is Az (x) Ay (x) Ax
     // complex map whose transfers are real and separable: `factors` hold their real factors, and
@@ -493,6 +498,9 @@ void launch_diag_m2l(int cdtype, bool conj, int n_recv, const int32_t *recv, con
                      const int32_t *item_src, const int32_t *item_op, const void *ops, int K, const void *in, void *out,
                      int R, hipStream_t s);
 // M2M of all children of each target (shared factors: Side::sep_shared), overwriting the target
+// Real collection of a complex map (Side::coll_real) when `S` has real separable transfers, the core
+// a real alpha and every collection entry a zero imaginary part; otherwise nothing changes.
+void make_real_collection(Side &S, const Core &c);
 // In place, for the boxes of a list: complex expansion rows (R per box, K values each) between the
 // interleaved layout (K complex values) and the planar one (K real parts, then K imaginary parts).
 // cdtype: H2G_C64 or H2G_C32.

@@ -1308,6 +1308,41 @@ __global__ void cplx_planar_kernel(const int32_t *__restrict__ boxes, Rl *__rest
     }
 }
 
+__global__ void real_part_kernel(const double *__restrict__ c, double *__restrict__ re, float *__restrict__ re32,
+                                 int64_t n, int *__restrict__ nonreal) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r = c[2 * i];
+    re[i] = r;
+    if (re32) re32[i] = (float)r;
+    if (c[2 * i + 1] != 0.0) atomicOr(nonreal, 1);
+}
+
+void make_real_collection(Side &S, const Core &c) {
+    if (c.dtype != H2G_C64 || !S.sep_cplx || c.alpha.im != 0.0 || S.coll_real || !S.coll.ptr) return;
+    const int64_t n = (int64_t)(S.coll.bytes / (2 * sizeof(double)));
+    DevBuf flag;
+    flag.alloc(sizeof(int));
+    H2G_HIP(hipMemset(flag.ptr, 0, sizeof(int)));
+    S.coll_re.alloc((size_t)n * sizeof(double));
+    if (S.coll32.ptr) S.coll32_re.alloc((size_t)n * sizeof(float));
+    if (n > 0)
+        hipLaunchKernelGGL(real_part_kernel, (unsigned)((n + TPB - 1) / TPB), TPB, 0, 0, S.coll.as<double>(),
+                           S.coll_re.as<double>(), S.coll32_re.ptr ? S.coll32_re.as<float>() : nullptr, n,
+                           flag.as<int>());
+    H2G_CHECK_LAUNCH();
+    int nonreal = 0;
+    H2G_HIP(hipMemcpy(&nonreal, flag.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    if (nonreal) {
+        S.coll_re.release();
+        S.coll32_re.release();
+        return;
+    }
+    S.coll.release();
+    S.coll32.release();
+    S.coll_real = true;
+}
+
 void launch_cplx_planar(int cdtype, const int32_t *boxes, int n, void *pool, int R, int K, bool to_planar,
                         hipStream_t s) {
     if (n == 0) return;

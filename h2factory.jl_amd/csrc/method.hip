@@ -159,10 +159,15 @@ extern "C" int h2g_method_fill(h2g_core *core, const h2g_method_desc *desc) {
         }
         if (d.fill_collections) {
             if (d.p < 1 || d.p * d.p * d.p != c.K || !d.nodes) throw Error{H2G_EINVAL, "collections need K == p^3 and nodes"};
+            if (c.test.coll_real || c.trial->coll_real)
+                throw Error{H2G_EINVAL, "the collections were filled already (and are kept real)"};
             DevBuf nodes;
             nodes.upload(d.nodes, (size_t)d.p * sizeof(double));
             fill_side_collection(c.test, d.test, c.dtype, d.p, nodes.as<double>());
             if (!c.galerkin) fill_side_collection(*c.trial, d.trial, c.dtype, d.p, nodes.as<double>());
+            // interpolation collections are real for every kernel
+            make_real_collection(c.test, c);
+            if (!c.galerkin) make_real_collection(c.trial_own, c);
         }
         return H2G_OK;
     } catch (const Error &e) {

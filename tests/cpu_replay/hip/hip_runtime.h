@@ -60,6 +60,8 @@ inline std::common_type_t<A, B> min(A a, B b) { return a < b ? a : b; }
 template <class A, class B>
 inline std::common_type_t<A, B> max(A a, B b) { return a < b ? b : a; }
 inline void sincos(double x, double *s, double *c) { *s = std::sin(x); *c = std::cos(x); }
+// device atomics on host memory: the workgroup's threads run concurrently
+inline int atomicOr(int *p, int v) { return std::atomic_ref<int>(*p).fetch_or(v); }
 inline void sincospi(double x, double *s, double *c) { *s = std::sin(M_PI * x); *c = std::cos(M_PI * x); }
 
 // the dynamic LDS arrays the kernels declare (extern __shared__ unsigned char smem_raw[]),
@@ -122,6 +124,10 @@ inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
     *p = const_cast<void *>(symbol);
+    return hipSuccess;
+}
+inline hipError_t hipMemset(void *d, int v, size_t n) {
+    if (n) std::memset(d, v, n);
     return hipSuccess;
 }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) {
