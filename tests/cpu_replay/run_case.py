@@ -44,6 +44,20 @@ def helmholtz_sphere():
     return core, x
 
 
+def helmholtz_petrov_alpha():
+    """Helmholtz on two point sets (Chebyshev p = 3, dense couplings: the lattice kernels do not
+    run here) with a complex kernel factor: real transfers on planar rows, complex collections (the
+    complex factor keeps them complex), so P2M and L2P take the complex kernels with the row
+    conversions after P2M and before L2P."""
+    rng = np.random.default_rng(12)
+    a = rng.random((700, 3))
+    b = rng.random((500, 3))
+    core = OB.build_core(a, b, p=3, nodekind="cheb", m2l_kind="dense", kernel="helmholtz", wavenumber=5.0,
+                         leaf_size=24, alpha=0.7 - 0.2j)
+    x = rng.standard_normal((500, 2)) + 1j * rng.standard_normal((500, 2))
+    return core, x
+
+
 def random_petrov_symdense():
     """Random complex operators, separate test and trial trees, symmetry-reduced dense couplings
     (the box GEMM with grid permutations), two right-hand sides."""
@@ -195,7 +209,8 @@ def main(name):
         sys.exit(0 if ok else 1)
     else:
         core, x = {"helmholtz_sphere": helmholtz_sphere, "random_petrov_symdense": random_petrov_symdense,
-                   "random_petrov_diagonal": random_petrov_diagonal}[name]()
+                   "random_petrov_diagonal": random_petrov_diagonal,
+                   "helmholtz_petrov_alpha": helmholtz_petrov_alpha}[name]()
         c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
         A = h2g.H2Workspace(c, max_rhs=2)
         ok = _products(name, A, core, x, "NTC", ("mul", "farmul", "nearmul"))

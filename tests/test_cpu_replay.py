@@ -26,7 +26,8 @@ def replay_lib():
     return B.build(), B.asan_runtime()
 
 
-@pytest.mark.parametrize("case", ["helmholtz_sphere", "laplace_device_fill", "random_petrov_symdense",
+@pytest.mark.parametrize("case", ["helmholtz_sphere", "helmholtz_petrov_alpha", "laplace_device_fill",
+                                  "random_petrov_symdense",
                                   "random_petrov_diagonal", "noalloc", "diagnostics",
                                   "device_plan", "device_operators"])
 def test_replay_in_bounds_and_equal_to_oracle(replay_lib, case):

@@ -58,6 +58,34 @@ def test_coincident_points(kernel, kind, nodes):
     _products(core, cplx=kernel == "helmholtz")
 
 
+@pytest.mark.parametrize("alpha", [1.0, 0.7 - 0.2j])
+def test_helmholtz_real_transfers_and_collections(alpha):
+    """Helmholtz maps whose interpolation transfers are real run them on the real separable kernels
+    over planar rows; with a real kernel factor their collections are kept real too (real leaf
+    kernels), with a complex one they stay complex (the rows are converted after P2M and before
+    L2P).  Petrov, Toeplitz couplings, every orientation and the 5-argument form."""
+    import h2g
+    from convert import to_arrays
+
+    rng = np.random.default_rng(12)
+    a = rng.random((1400, 3))
+    b = rng.random((1000, 3))
+    core = OB.build_core(a, b, p=3, nodekind="equi", m2l_kind="toeplitz", kernel="helmholtz", wavenumber=5.0,
+                         leaf_size=24, alpha=alpha)
+    _products(core, cplx=True, nrhs=5)
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain",))
+    ws = h2g.H2Workspace(c, max_rhs=5)
+    s1, s2 = 1.3 - 0.4j, -0.6 + 0.2j
+    for op in "NTC":
+        M = {"N": ws, "T": h2g.transpose(ws), "C": h2g.adjoint(ws)}[op]
+        x = rng.standard_normal((M.shape[1], 5)) + 1j * rng.standard_normal((M.shape[1], 5))
+        y0 = rng.standard_normal((M.shape[0], 5)) + 1j * rng.standard_normal((M.shape[0], 5))
+        y = y0.copy()
+        h2g.mul_(y, M, x, s1, s2)
+        assert relerr(y, O.mul(core, x, op, s1, s2, y0)) < 1e-12, op
+    ws.close()
+
+
 def test_petrov_sets_without_near_pairs():
     rng = np.random.default_rng(3)
     a = 0.2 * rng.random((800, 3))
