@@ -491,7 +491,9 @@ def main():
         from h2g.autotune import tune
 
         t0 = time.perf_counter()
-        tile, tune_table = tune(core, nrhs=R, precision=args.precision, blocks=(16, R))
+        # RHS blocks 1, 4, 8, 16 and the full width (32 columns as 32 passes of 1, 8 of 4, ...)
+        tile, tune_table = tune(core, nrhs=R, precision=args.precision,
+                                blocks=tuple(sorted({b for b in (1, 4, 8, 16) if b < R} | {R})))
         log(f"autotune in {time.perf_counter() - t0:.1f}s: {tile}; " +
             ", ".join(f"{c['max_rhs']}/{c['m2l_groups']}/{c['near_cols']}/{c.get('persistent', 0)}: {ms * R:.2f} ms"
                       for c, ms in tune_table))
