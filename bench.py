@@ -530,6 +530,24 @@ def main():
     log(f"plain FP64: {1e3 * t_plain:.2f} ms per RHS ({plain_rhs_per_s:.1f} RHS/s); "
         f"batched vs plain rel err {err_vs_plain:.2e}")
 
+    # the transpose product, same columns and steps, against the plain FP64 transpose
+    AT, PT = h2g.transpose(A), h2g.transpose(P)
+    YT = torch.zeros_like(Y)
+    h2g.mul_(YT, AT, X)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        h2g.mul_(YT, AT, X)
+    torch.cuda.synchronize()
+    dt_t = (time.perf_counter() - t0) / args.steps
+    YT64 = torch.zeros_like(Y64)
+    for j in range(ncol):
+        h2g.mul_(YT64[:, j:j + 1], PT, X64[:, j:j + 1])
+    transpose = {"ms_per_product": round(1e3 * dt_t, 3), "rhs_per_s": round(R / dt_t, 2),
+                 "rel_err_vs_plain_fp64": float(torch.linalg.norm(YT[:, :ncol].double() - YT64) /
+                                                torch.linalg.norm(YT64))}
+    log(f"transpose: {transpose}")
+
     # O(N^2) direct sum on 1,000 sampled targets (FP64, on the GPU)
     rows = torch.from_numpy(np.random.default_rng(2).choice(N, 1000, replace=False)).to(dev)
     P_d = torch.from_numpy(pts).to(dev)
@@ -633,6 +651,7 @@ def main():
                    "tile": tile},
         "autotune": [{"config": c, "ms_per_product": round(ms * R, 3)} for c, ms in tune_table],
         "plain_fp64_rhs_per_s": round(plain_rhs_per_s, 2),
+        "transpose": transpose,
         "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
         "rel_err_vs_plain_fp64": err_vs_plain,
         "gate_1e-6_vs_plain": bool(err_vs_plain <= 1e-6),
