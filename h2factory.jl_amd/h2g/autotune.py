@@ -104,9 +104,9 @@ def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, 
 
     best = {"max_rhs": min(nrhs, max(blocks)), "m2l_groups": 0, "near_cols": 0, "persistent": 0}
     for axis, values in (("max_rhs", [b for b in blocks if b <= nrhs] or [nrhs]),
-                         ("m2l_groups", list(m2l_groups) if _lattice(core) and not cplx else []),
+                         ("m2l_groups", list(m2l_groups) if _lattice(core) else []),
                          ("near_cols", list(near_cols) if not cplx else []),
-                         ("persistent", list(persistent) if _lattice(core) and not cplx else [])):
+                         ("persistent", list(persistent) if _lattice(core) else [])):
         if not values:
             continue
         scores = {}
