@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--plan", default="device", choices=["device", "host"],
                     help="build the trees, pair lists and operators on the GPU (h2g_plan_build, "
                          "h2g_method_operators) or with the host builders")
-    ap.add_argument("--tile", default=None, help="RHS block / coupling groups / near-field columns [/ persistent], "
+    ap.add_argument("--tile", default=None, help="RHS block / coupling groups / near-field columns [/ persistent [/ dense pairs]], "
                                                   "e.g. 32/256/32 or 32/512/32/1: use this tile instead of autotuning "
                                                   "(0 = the library default)")
     ap.add_argument("--no-tune", action="store_true",
@@ -486,7 +486,8 @@ def main():
     tune_table = []
     if args.tile:
         v = [int(t) for t in args.tile.split("/")]
-        tile = {"max_rhs": v[0], "m2l_groups": v[1], "near_cols": v[2], "persistent": v[3] if len(v) > 3 else 0}
+        tile = {"max_rhs": v[0], "m2l_groups": v[1], "near_cols": v[2], "persistent": v[3] if len(v) > 3 else 0,
+                "dense_pairs": v[4] if len(v) > 4 else 0}
     elif not args.no_tune:
         from h2g.autotune import tune
 

@@ -551,6 +551,86 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
     P.m2l_op.upload_vec(iop);
     if (pair_sym) P.m2l_sym.upload_vec(isym);
     if (pair_fop) P.m2l_fop.upload_vec(ifop);
+    if (pair_fop) {
+        // receiver pairs for the dense pair kernel
+        std::vector<int32_t> r2, s2, f2;
+        std::vector<int64_t> p2;
+        P.m2l2_base.assign(dst.n_levels, 0);
+        P.m2l2_count.assign(dst.n_levels, 0);
+        P.m2l2_rbase.assign(dst.n_levels, 0);
+        std::vector<std::pair<int32_t, int32_t>> ea, eb;
+        auto items_of = [&](int l, int i, std::vector<std::pair<int32_t, int32_t>> &e) {
+            e.clear();
+            const int64_t q0 = iptr[P.m2l_base[l] + i], q1 = iptr[P.m2l_base[l] + i + 1];
+            for (int64_t q = q0; q < q1; ++q) e.emplace_back(ifop[q], isrc[q]);
+            std::sort(e.begin(), e.end());
+        };
+        for (int l = 0; l < dst.n_levels; ++l) {
+            P.m2l2_base[l] = (int64_t)p2.size();
+            P.m2l2_rbase[l] = (int64_t)r2.size() / 2;
+            const int n = P.m2l_recv[l].count;
+            const int32_t *rl = lists.data() + P.m2l_recv[l].begin;
+            // partner of each receiver: a free lattice neighbour (x, y, z, then the opposite sides)
+            // when the tree has coordinates, else the next free receiver
+            std::vector<int> mate(n, -1);
+            std::vector<uint8_t> used(n, 0);
+            std::unordered_map<uint64_t, int> at;
+            auto ckey = [](int64_t x, int64_t y, int64_t z) {
+                return ((uint64_t)(x & 0x1FFFFF) << 42) | ((uint64_t)(y & 0x1FFFFF) << 21) | (uint64_t)(z & 0x1FFFFF);
+            };
+            const bool geo = !dst.coord.empty();
+            if (geo)
+                for (int i = 0; i < n; ++i) {
+                    const int b = rl[i];
+                    at.emplace(ckey(dst.coord[3 * (size_t)b], dst.coord[3 * (size_t)b + 1], dst.coord[3 * (size_t)b + 2]), i);
+                }
+            static const int nb[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+            for (int i = 0; i < n; ++i) {
+                if (used[i]) continue;
+                used[i] = 1;
+                if (geo) {
+                    const int b = rl[i];
+                    for (const auto &d : nb) {
+                        auto it = at.find(ckey(dst.coord[3 * (size_t)b] + d[0], dst.coord[3 * (size_t)b + 1] + d[1],
+                                               dst.coord[3 * (size_t)b + 2] + d[2]));
+                        if (it != at.end() && !used[it->second]) {
+                            mate[i] = it->second;
+                            break;
+                        }
+                    }
+                }
+                if (mate[i] < 0)
+                    for (int j = i + 1; j < n; ++j)
+                        if (!used[j]) {
+                            mate[i] = j;
+                            break;
+                        }
+                if (mate[i] >= 0) used[mate[i]] = 1;
+                items_of(l, i, ea);
+                if (mate[i] >= 0) items_of(l, mate[i], eb);
+                else eb.clear();
+                r2.push_back(rl[i]);
+                r2.push_back(mate[i] >= 0 ? rl[mate[i]] : -1);
+                p2.push_back((int64_t)f2.size());
+                size_t x = 0, y = 0;
+                while (x < ea.size() || y < eb.size()) {
+                    if (y == eb.size() || (x < ea.size() && ea[x].first < eb[y].first)) {
+                        f2.push_back(ea[x].first), s2.push_back(ea[x].second), s2.push_back(-1), ++x;
+                    } else if (x == ea.size() || eb[y].first < ea[x].first) {
+                        f2.push_back(eb[y].first), s2.push_back(-1), s2.push_back(eb[y].second), ++y;
+                    } else {
+                        f2.push_back(ea[x].first), s2.push_back(ea[x].second), s2.push_back(eb[y].second), ++x, ++y;
+                    }
+                }
+                ++P.m2l2_count[l];
+            }
+            p2.push_back((int64_t)f2.size());
+        }
+        P.m2l2_recv.upload_vec(r2);
+        P.m2l2_ptr.upload_vec(p2);
+        P.m2l2_src.upload_vec(s2);
+        P.m2l2_fop.upload_vec(f2);
+    }
     P.lattice = std::vector<LatticeLevel>(dst.n_levels);
     if (c.m2l_kind == H2G_M2L_TOEPLITZ || c.m2l_kind == H2G_M2L_SYMTOEPLITZ)
         for (int l = 0; l < dst.n_levels; ++l) {
@@ -1193,9 +1273,14 @@ void far_pass(Workspace &w, int op, int R, hipEvent_t join) {
             w.mark_end();
         } else if (!is_toeplitz(c)) {
             w.mark_begin(5, l);
-            if (c.m2l_dense.ok)
-                launch_dense(c.m2l_dense, op == H2G_OP_N ? 0 : (op == H2G_OP_T || !cmap ? 1 : 2),
-                             dense_rep(w.opts.precision, true), recv, n, iptr, P.m2l_src.as<int32_t>(),
+            const int orient = op == H2G_OP_N ? 0 : (op == H2G_OP_T || !cmap ? 1 : 2);
+            const int rep = dense_rep(w.opts.precision, true);
+            if (c.m2l_dense.ok && P.m2l2_ptr.ptr &&
+                launch_dense_pair(c.m2l_dense, orient, rep, w.opts.dense_pairs, P.m2l2_recv.as<int32_t>() + 2 * P.m2l2_rbase[l],
+                                  P.m2l2_count[l], P.m2l2_ptr.as<int64_t>() + P.m2l2_base[l],
+                                  P.m2l2_src.as<int32_t>(), P.m2l2_fop.as<int32_t>(), w.mom.ptr, w.loc.ptr, R, s)) {
+            } else if (c.m2l_dense.ok)
+                launch_dense(c.m2l_dense, orient, rep, recv, n, iptr, P.m2l_src.as<int32_t>(),
                              P.m2l_fop.as<int32_t>(), w.mom.ptr, w.loc.ptr, R, nullptr, s);
             else
                 launch_boxgemm(cd, opA, n, recv, iptr, P.m2l_src.as<int32_t>(), P.m2l_op.as<int32_t>(),
@@ -1776,6 +1861,7 @@ int h2g_ws_create(h2g_core *core, const h2g_ws_opts *opts, h2g_ws **out) {
             H2G_HIP(hipEventCreateWithFlags(&w->ev_near, hipEventDisableTiming));
             H2G_HIP(hipEventCreateWithFlags(&w->ev_done, hipEventDisableTiming));
             if (o.persistent != 0 && o.persistent != 1) fail(H2G_EINVAL, "persistent must be 0 or 1");
+            if (o.dense_pairs < -1 || o.dense_pairs > 1) fail(H2G_EINVAL, "dense_pairs must be -1, 0 or 1");
             const WsLayout L = ws_layout(*core, o.precision != H2G_PREC_PLAIN, w->R_cap, o.near_cols, o.persistent);
             w->xs.alloc(L.xs);
             w->ys.alloc(L.ys);

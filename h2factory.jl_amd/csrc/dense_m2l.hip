@@ -298,6 +298,116 @@ __global__ void __launch_bounds__(DNT, 2)
     }
 }
 
+// Two receiving boxes per workgroup (M2L, real operators): the pair's items are merged by operator,
+// so an operator that both receivers apply (to different sources) is read once and feeds both
+// receivers' MFMAs.  Item = (operator, source of receiver 0, source of receiver 1), -1 where a
+// receiver has no partner with that operator (its MFMAs are skipped: the source index is
+// wave-uniform).  NW waves share the row tiles (wave, wave + NW, ...), each holding RTW row tiles x
+// 2 receivers of 16-column accumulators.  Workgroup x = pair x / nchunk, columns 16 (x % nchunk)
+// .. + 15: the chunks of one pair run on adjacent workgroups, so they share its operators in L2.
+template <int MODE, int NW, int RTW>
+__global__ void __launch_bounds__(64 * NW, 1)
+    dense_m2l_pair_kernel(const int32_t *__restrict__ recv2, const int64_t *__restrict__ item_ptr,
+                          const int2 *__restrict__ item_src, const int32_t *__restrict__ item_fop,
+                          const typename std::conditional<MODE == 0, double, float>::type *__restrict__ frag,
+                          int Kr, int KrT,
+                          const typename std::conditional<MODE == 0, double, float>::type *__restrict__ mom,
+                          typename std::conditional<MODE == 0, double, float>::type *__restrict__ loc, int R,
+                          int nchunk) {
+    using V = typename std::conditional<MODE == 0, double, float>::type;
+    using V4 = typename dvec<V>::type;
+    const int ti = blockIdx.x / nchunk;
+    const int r0 = (blockIdx.x - ti * nchunk) * 16;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int n = lane & 15, kq = lane >> 4;
+    const int G = KrT;
+    const int ntw = (KrT - wave + NW - 1) / NW;  // row tiles of this wave: wave, wave + NW, ...
+    const int64_t fsz = (int64_t)(16 * KrT) * (16 * KrT);
+    const int col = r0 + n;
+    double acc[2][RTW][4];
+    V4 part[2][RTW];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int j = 0; j < RTW; ++j) {
+            part[u][j] = V4{0, 0, 0, 0};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][j][v] = 0.0;
+        }
+    const int64_t i0 = item_ptr[ti], i1 = item_ptr[ti + 1];
+    for (int64_t it = i0; it < i1; ++it) {
+        const V *A0 = frag + (int64_t)item_fop[it] * fsz + (int64_t)lane * 4;
+        const int2 sp = item_src[it];
+        const int src[2] = {__builtin_amdgcn_readfirstlane(sp.x), __builtin_amdgcn_readfirstlane(sp.y)};
+        const V *Bc[2];
+        bool bval[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            bval[u] = src[u] >= 0 && col < R;
+            Bc[u] = mom + ((int64_t)(src[u] >= 0 ? src[u] : 0) * R + (col < R ? col : 0)) * (int64_t)Kr;
+        }
+        for (int g = 0; g < G; ++g) {
+            V4 a[RTW], b[2];
+#pragma unroll
+            for (int j = 0; j < RTW; ++j)
+                a[j] = j < ntw ? *reinterpret_cast<const V4 *>(A0 + ((int64_t)(wave + NW * j) * G + g) * 256)
+                               : V4{0, 0, 0, 0};
+#pragma unroll
+            for (int u = 0; u < 2; ++u) b[u] = load_k4<V>(Bc[u], 16 * g + 4 * kq, Kr, bval[u]);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (src[u] < 0) continue;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+                    for (int j = 0; j < RTW; ++j) {
+                        if (j >= ntw) continue;
+                        if (MODE == 0) {
+                            double4_d d = {acc[u][j][0], acc[u][j][1], acc[u][j][2], acc[u][j][3]};
+                            d = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[j][kk], (double)b[u][kk], d, 0, 0, 0);
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) acc[u][j][v] = d[v];
+                        } else {
+                            float4_d d = {(float)part[u][j][0], (float)part[u][j][1], (float)part[u][j][2],
+                                          (float)part[u][j][3]};
+                            d = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a[j][kk], (float)b[u][kk], d, 0, 0, 0);
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) part[u][j][v] = d[v];
+                        }
+                    }
+            }
+            if (MODE == 1 && ((g & 3) == 3 || g == G - 1)) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int j = 0; j < RTW; ++j) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[u][j][v] += (double)part[u][j][v];
+                        part[u][j] = V4{0, 0, 0, 0};
+                    }
+            }
+        }
+    }
+    // D layouts as in dense_m2l_kernel
+    if (col >= R) return;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int t = recv2[2 * ti + u];
+        if (t < 0) continue;
+        V *o = loc + ((int64_t)t * R + col) * Kr;
+#pragma unroll
+        for (int j = 0; j < RTW; ++j) {
+            if (j >= ntw) continue;
+            const int rt = wave + NW * j;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * rt + (MODE == 0 ? kq + 4 * v : 4 * kq + v);
+                if (row < Kr) o[row] = (V)acc[u][j][v];
+            }
+        }
+    }
+}
+
 typedef __bf16 bf16x8_d __attribute__((ext_vector_type(8)));
 
 // 8 consecutive k of an FP32 column (zero beyond Kr), split into NPL BF16 planes
@@ -492,6 +602,36 @@ void dense_go(bool cf, int rtw, int nct, const int32_t *recv, int n_recv, const 
         dense_go_cf<MODE, false>(rtw, nct, recv, n_recv, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, acc, s);
 }
 
+
+template <int MODE, int NW, int RTW>
+void dense_pair_go3(const int32_t *recv2, int n_pairs, const int64_t *item_ptr, const int2 *item_src,
+                    const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
+                    hipStream_t s) {
+    using V = typename std::conditional<MODE == 0, double, float>::type;
+    const int nchunk = (R + 15) / 16;
+    hipLaunchKernelGGL((dense_m2l_pair_kernel<MODE, NW, RTW>), dim3((unsigned)((int64_t)n_pairs * nchunk)), 64 * NW, 0,
+                       s, recv2, item_ptr, item_src, item_fop, (const V *)frag, Kr, KrT, (const V *)mom, (V *)loc, R,
+                       nchunk);
+}
+
+// NW = 4 waves for at most 4 row tiles, else 8; RTW = row tiles per wave (at most 4)
+template <int MODE>
+void dense_pair_go(int nw, int rtw, const int32_t *recv2, int n_pairs, const int64_t *item_ptr, const int2 *item_src,
+                   const int32_t *item_fop, const void *frag, int Kr, int KrT, const void *mom, void *loc, int R,
+                   hipStream_t s) {
+#define H2G_PAIR_GO(NW_, RTW_) \
+    dense_pair_go3<MODE, NW_, RTW_>(recv2, n_pairs, item_ptr, item_src, item_fop, frag, Kr, KrT, mom, loc, R, s)
+    switch (nw * 10 + rtw) {
+    case 41: H2G_PAIR_GO(4, 1); break;
+    case 81: H2G_PAIR_GO(8, 1); break;
+    case 82: H2G_PAIR_GO(8, 2); break;
+    case 83: H2G_PAIR_GO(8, 3); break;
+    case 84: H2G_PAIR_GO(8, 4); break;
+    default: throw Error{H2G_EINVAL, "dense pair kernel: no instantiation for this tile shape"};
+    }
+#undef H2G_PAIR_GO
+}
+
 }  // namespace
 
 void build_dense_ops(DenseOps &D, int dtype, int K, const void *ops, const int32_t *fop_op, const int32_t *fop_sym,
@@ -579,6 +719,30 @@ void launch_dense(const DenseOps &D, int orient, int rep, const int32_t *recv, i
                     out, R, acc_flags, s);
     }
     H2G_CHECK_LAUNCH();
+}
+
+bool launch_dense_pair(const DenseOps &D, int orient, int rep, int want, const int32_t *recv2, int n_pairs,
+                       const int64_t *item_ptr, const int32_t *item_src2, const int32_t *item_fop, const void *in,
+                       void *out, int R, hipStream_t s) {
+    if (want < 0 || D.cf || (rep != DENSE_REP_F32 && rep != DENSE_REP_F64)) return false;
+    const DevBuf &F = rep == DENSE_REP_F32 ? D.f32[orient] : D.f64[orient];
+    const void *frag = F.ptr;
+    if (!frag) return false;
+    // measured on dense Laplace maps at N = 2M: p = 6 (1.6k operators of 216 x 216, 0.33 GB in
+    // FP32, 0.66 GB in FP64) coupling stage from 1 to 16 columns 530 -> 230-280 ms (FP32), 536-695
+    // -> 371-620 ms (FP64), at 32 columns 415 -> 546 ms; p = 4 (operators in the caches) 10-25 %
+    // slower in pairs
+    if (want == 0 && (R > 16 || (int64_t)F.bytes <= DENSE_PAIR_MIN_BYTES)) return false;
+    const int nw = D.KrT <= 4 ? 4 : 8, rtw = (D.KrT + nw - 1) / nw;
+    if (rtw > 4) return false;
+    if (n_pairs == 0) return true;
+    const int2 *src2 = reinterpret_cast<const int2 *>(item_src2);
+    if (rep == DENSE_REP_F32)
+        dense_pair_go<1>(nw, rtw, recv2, n_pairs, item_ptr, src2, item_fop, frag, D.Kr, D.KrT, in, out, R, s);
+    else
+        dense_pair_go<0>(nw, rtw, recv2, n_pairs, item_ptr, src2, item_fop, frag, D.Kr, D.KrT, in, out, R, s);
+    H2G_CHECK_LAUNCH();
+    return true;
 }
 
 }  // namespace h2g

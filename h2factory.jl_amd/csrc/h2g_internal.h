@@ -367,6 +367,13 @@ struct OrientPlan {
     // (translation id, symmetry) operator of the dense MFMA kernel
     DevBuf m2l_ptr, m2l_src, m2l_op, m2l_sym, m2l_fop;
     std::vector<int64_t> m2l_base;
+    // dense M2L on receiver pairs (operators read once for both receivers of a pair): per level the
+    // receivers paired with a lattice neighbour where one is free, else with the next free one; per
+    // pair its items merged by operator (operator, source of each receiver or -1)
+    DevBuf m2l2_recv, m2l2_ptr, m2l2_src, m2l2_fop;
+    std::vector<int64_t> m2l2_base;   // per level: offset of the level's pairs into m2l2_ptr
+    std::vector<int64_t> m2l2_rbase;  // per level: offset of the level's pairs into m2l2_recv (2 entries each)
+    std::vector<int32_t> m2l2_count;  // per level: receiver pairs
     int min_level = 0, max_level = -1;
     int64_t max_src_level_boxes = 0, max_recv_level_boxes = 0;
     DevBuf out_mask;                    // uint8 per tree-ordered output dof: leaf initialised
@@ -553,6 +560,16 @@ void build_dense_ops(DenseOps &D, int dtype, int K, const void *ops, const int32
 void launch_dense(const DenseOps &D, int orient, int rep, const int32_t *recv, int n_recv, const int64_t *item_ptr,
                   const int32_t *item_src, const int32_t *item_fop, const void *in, void *out, int R,
                   const uint8_t *acc_flags, hipStream_t s);
+// M2L on receiver pairs: per pair of receiving boxes recv2[2 i], recv2[2 i + 1] (-1: none), items
+// item_ptr[i] .. item_ptr[i + 1] of (operator item_fop, sources item_src2[2 item], [2 item + 1], -1
+// where that receiver has no such partner).  want: 1 pairs wherever the pair kernel applies, -1
+// never, 0 for batches of at most 16 columns when the operators of the representation exceed
+// DENSE_PAIR_MIN_BYTES.  False when pairs are not wanted or the map has no pair kernel (complex
+// operators, split BF16); the caller then runs launch_dense on the unpaired items.
+constexpr int64_t DENSE_PAIR_MIN_BYTES = 192ll << 20;
+bool launch_dense_pair(const DenseOps &D, int orient, int rep, int want, const int32_t *recv2, int n_pairs,
+                       const int64_t *item_ptr, const int32_t *item_src2, const int32_t *item_fop, const void *in,
+                       void *out, int R, hipStream_t s);
 
 }  // namespace h2g
 

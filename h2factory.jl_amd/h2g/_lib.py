@@ -80,7 +80,8 @@ class MethodDesc(C.Structure):
 
 class WsOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_rhs", C.c_int32), ("use_graph", C.c_int32), ("timing", C.c_int32),
-                ("stream", VP), ("m2l_groups", C.c_int32), ("near_cols", C.c_int32), ("persistent", C.c_int32)]
+                ("stream", VP), ("m2l_groups", C.c_int32), ("near_cols", C.c_int32), ("persistent", C.c_int32),
+                ("dense_pairs", C.c_int32)]
 
 
 class TimingRecord(C.Structure):

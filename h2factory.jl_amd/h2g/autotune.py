@@ -16,11 +16,15 @@ What is tuned (each a workspace option of ``h2g_ws_opts``):
 * ``persistent`` -- the coupling stage of all lattice levels in one persistent launch (per-XCD
   task lists, the small levels filling the tail of the large ones) or one launch per level; the
   persistent form keeps the spectra of all levels at once (more workspace memory).
+* ``dense_pairs`` -- dense couplings of real maps on receiving boxes two by two (each operator
+  the two share read once) or one by one: pairs halve the operator stream where the operators
+  come from HBM, single boxes keep more waves per CU.
 
 The best values depend on the map (N, p, tree depth, near-field density, coupling kind) and the
 device, so they are measured: each candidate runs the same product on synthetic columns, timed
 on the device, and the fastest per column wins.  The search is coordinate-wise (RHS block with
-the default shapes, then the coupling groups, then the near-field columns), which needs a few
+the default shapes, then the coupling groups, the near-field columns, the persistent stage and the
+receiver pairs, each where the map has it), which needs a few
 products per candidate instead of the full grid.  Results are cached per map signature in a
 JSON file, so a serving process tunes a map once.
 """
@@ -38,6 +42,7 @@ DEFAULT_BLOCKS = (8, 16, 32)
 DEFAULT_M2L_GROUPS = (256, 512, 768)
 DEFAULT_NEAR_COLS = (16, 32)
 DEFAULT_PERSISTENT = (0, 1)
+DEFAULT_DENSE_PAIRS = (-1, 1)
 
 
 def signature(core, precision: str, nrhs: int) -> str:
@@ -49,15 +54,21 @@ def signature(core, precision: str, nrhs: int) -> str:
     return hashlib.sha1(key.encode()).hexdigest()[:16]
 
 
+def _dense_real(core):
+    """Whether the map's couplings are dense real operators (dense_pairs matters)."""
+    return core.arrays.m2l_kind in ("dense", "symdense") and core.dtype != "c64"
+
+
 def _lattice(core):
     """Whether the map's couplings run on the lattice Toeplitz kernels (m2l_groups matters)."""
     return core.arrays.m2l_kind in ("toeplitz", "symtoeplitz")
 
 
 def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, m2l_groups=DEFAULT_M2L_GROUPS,
-         near_cols=DEFAULT_NEAR_COLS, persistent=DEFAULT_PERSISTENT, repeats: int = 3, cache_path: str | None = None,
-         use_graph: bool = True):
-    """Returns (best, table): ``best`` = {"max_rhs", "m2l_groups", "near_cols", "persistent"}
+         near_cols=DEFAULT_NEAR_COLS, persistent=DEFAULT_PERSISTENT, dense_pairs=DEFAULT_DENSE_PAIRS,
+         repeats: int = 3, cache_path: str | None = None, use_graph: bool = True):
+    """Returns (best, table): ``best`` = {"max_rhs", "m2l_groups", "near_cols", "persistent",
+    "dense_pairs"}
     (0 = the library's default) and ``table`` = [(config, ms per column)] for every candidate
     timed."""
     import torch
@@ -82,7 +93,7 @@ def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, 
     timed = {}
 
     def measure(cfg):
-        key = (cfg["max_rhs"], cfg["m2l_groups"], cfg["near_cols"], cfg["persistent"])
+        key = (cfg["max_rhs"], cfg["m2l_groups"], cfg["near_cols"], cfg["persistent"], cfg["dense_pairs"])
         if key in timed:
             return timed[key]
         ws = H2Workspace(core, precision=precision, use_graph=use_graph, **cfg)
@@ -102,11 +113,12 @@ def tune(core, nrhs: int = 32, precision: str = "mixed", blocks=DEFAULT_BLOCKS, 
         table.append((dict(cfg), ms))
         return ms
 
-    best = {"max_rhs": min(nrhs, max(blocks)), "m2l_groups": 0, "near_cols": 0, "persistent": 0}
+    best = {"max_rhs": min(nrhs, max(blocks)), "m2l_groups": 0, "near_cols": 0, "persistent": 0, "dense_pairs": 0}
     for axis, values in (("max_rhs", [b for b in blocks if b <= nrhs] or [nrhs]),
                          ("m2l_groups", list(m2l_groups) if _lattice(core) else []),
                          ("near_cols", list(near_cols) if not cplx else []),
-                         ("persistent", list(persistent) if _lattice(core) else [])):
+                         ("persistent", list(persistent) if _lattice(core) else []),
+                         ("dense_pairs", list(dense_pairs) if _dense_real(core) else [])):
         if not values:
             continue
         scores = {}

@@ -209,11 +209,13 @@ class H2Workspace:
     op = H.H2G_OP_N
 
     def __init__(self, core: GpuH2Core, max_rhs=1, precision="plain", use_graph=False, timing=False, stream=None,
-                 m2l_groups=0, near_cols=0, persistent=0):
+                 m2l_groups=0, near_cols=0, persistent=0, dense_pairs=0):
         """``stream``: a HIP stream handle (int) the workspace runs on, shared with other workspaces,
         or None for streams of its own.  ``m2l_groups`` / ``near_cols``: tile shapes (0: the
         library's defaults; ``h2g.autotune.tune`` measures them for a map).  ``persistent``: the
-        lattice coupling stage of all levels in one persistent launch."""
+        lattice coupling stage of all levels in one persistent launch.  ``dense_pairs``: dense
+        couplings of real maps on receiving boxes two by two (1), one by one (-1), or the library's
+        choice (0)."""
         self.core = core
         o = H.WsOpts()
         o.stream = stream
@@ -224,8 +226,10 @@ class H2Workspace:
         o.m2l_groups = int(m2l_groups)
         o.near_cols = int(near_cols)
         o.persistent = int(persistent)
+        o.dense_pairs = int(dense_pairs)
         self.opts = dict(max_rhs=max_rhs, precision=precision, use_graph=use_graph, timing=timing, stream=stream,
-                         m2l_groups=m2l_groups, near_cols=near_cols, persistent=persistent)
+                         m2l_groups=m2l_groups, near_cols=near_cols, persistent=persistent,
+                         dense_pairs=dense_pairs)
         h = C.c_void_p()
         _raise(H.lib().h2g_ws_create(core.handle, C.byref(o), C.byref(h)))
         self.handle = h

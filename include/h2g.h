@@ -185,6 +185,13 @@ typedef struct h2g_ws_opts {
                                      workspace then keeps the spectra of all lattice levels at
                                      once (about 1.45x the spectrum memory of the per-level
                                      launches).  0: one launch per level.                        */
+    int32_t dense_pairs;          /* dense couplings of real maps: 1 runs the receiving boxes two
+                                     by two (a lattice neighbour where one is free), each operator
+                                     the two apply read once for both; -1 one receiving box per
+                                     workgroup; 0 the library's choice: pairs for batches of at
+                                     most 16 columns when the map's operators in the product's
+                                     precision exceed 192 MB (then the coupling stage streams
+                                     them from HBM)                                              */
 } h2g_ws_opts;
 
 /* One per-phase timing record, shaped like the reference's TimingRecord

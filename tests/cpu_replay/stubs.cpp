@@ -32,6 +32,10 @@ void launch_dense(const DenseOps &, int, int, const int32_t *, int, const int64_
                   const void *, void *, int, const uint8_t *, hipStream_t) {
     throw Error{H2G_EUNSUPPORTED, "host replay: the dense MFMA kernels run on the GPU only"};
 }
+bool launch_dense_pair(const DenseOps &, int, int, int, const int32_t *, int, const int64_t *, const int32_t *,
+                       const int32_t *, const void *, void *, int, hipStream_t) {
+    throw Error{H2G_EUNSUPPORTED, "host replay: the dense MFMA kernels run on the GPU only"};
+}
 
 [[noreturn]] static void lattice_refused() {
     throw Error{H2G_EUNSUPPORTED, "host replay: the lattice Toeplitz kernels run on the GPU only"};

@@ -225,7 +225,7 @@ def test_tile_shape_arguments_are_checked(gpu):
 
     _, core = _tile_case()
     c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"))
-    for kw in ({"near_cols": 24}, {"m2l_groups": -1}, {"persistent": 2}):
+    for kw in ({"near_cols": 24}, {"m2l_groups": -1}, {"persistent": 2}, {"dense_pairs": 2}):
         with pytest.raises(h2g.ArgumentError):
             h2g.H2Workspace(c, max_rhs=8, precision="mixed", **kw)
 
@@ -629,3 +629,43 @@ def test_diagnostics_and_timing_records(gpu):
     assert {"near", "m2l_apply", "l2p"} <= {r.operation for r in recs}
     dense = OB.build_core(pts[:2000], p=3, nodekind="cheb", m2l_kind="dense", leaf_size=32)
     assert D.h2summary(h2g.GpuH2Core(to_arrays(dense))).dense_mfma
+
+
+@pytest.mark.parametrize("kind,nodes,p", [("dense", "cheb", 3), ("symdense", "equi", 3), ("dense", "equi", 6)])
+@pytest.mark.parametrize("petrov", [False, True])
+@pytest.mark.parametrize("coords", [True, False])
+def test_dense_receiver_pairs(gpu, kind, nodes, p, petrov, coords):
+    """Dense couplings of real maps on receiving boxes two by two (``dense_pairs=1``: lattice
+    neighbours paired with tree coordinates, consecutive boxes without), and one by one
+    (``dense_pairs=-1``), against the oracle: plain to 1e-12 (3- and 5-argument), mixed within the
+    1e-6 gate, at 1, 5 and 17 columns (two 16-column chunks per pair), forward and transpose.  p = 3
+    runs 4 waves of one row tile, p = 6 8 waves of two."""
+    import h2g
+    from convert import to_arrays
+
+    rng = np.random.default_rng(21)
+    a = rng.random((2500, 3))
+    b = rng.random((1800, 3)) if petrov else None
+    core = OB.build_core(a, b, p=p, nodekind=nodes, m2l_kind=kind, leaf_size=48)
+    c = h2g.GpuH2Core(to_arrays(core), precisions=("plain", "mixed"), lattice=coords)
+    s1, s2 = 1.7, -0.4
+    for R in (1, 5, 17):
+        for op in "NT":
+            n_out, n_in = (c.shape if op == "N" else c.shape[::-1])
+            x = rng.standard_normal((n_in, R))
+            y0 = rng.standard_normal((n_out, R))
+            ref = O.mul(core, x, op).real
+            ref5 = O.mul(core, x, op, s1, s2, y0).real
+            for prec, tol in (("plain", 1e-12), ("mixed", 1e-6)):
+                for pairs in (1, -1):
+                    W = h2g.H2Workspace(c, max_rhs=R, precision=prec, dense_pairs=pairs)
+                    M = orient(W, op)
+                    xx = x if prec == "plain" else x.astype(np.float32)
+                    y = M @ xx
+                    err = relerr(y, ref)
+                    assert err < tol, (R, op, prec, pairs, err)
+                    if prec == "plain":
+                        y = y0.copy()
+                        h2g.mul_(y, M, x, s1, s2)
+                        assert relerr(y, ref5) < tol, (R, op, "5-arg", pairs)
+                    W.close()
