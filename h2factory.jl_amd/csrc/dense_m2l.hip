@@ -159,6 +159,15 @@ __device__ inline typename dvec<V>::type load_k4(const V *__restrict__ col, int 
     return v;
 }
 
+// Workgroup b of nb runs on XCD b % 8 (the dispatcher's round robin); this maps the workgroups of
+// each XCD to a contiguous range of [0, nb), so that boxes adjacent in the receiving list (Morton
+// order: neighbours, which share most of their operators) run on one XCD and share its L2
+// (Laplace dense, N = 2M, p = 6, 32 columns: coupling stage 416 -> 389 ms).  A bijection for any nb.
+__device__ inline int xcd_order(int b, int nb) {
+    const int per = nb >> 3, rem = nb & 7, x = b & 7, k = b >> 3;
+    return x * per + min(x, rem) + k;
+}
+
 // Accumulators started from the current output (the accumulate form, e.g. L2L into a box that
 // already holds couplings), in the D layout of the 16 x 16 MFMAs: FP32 -- D[m][n] in lane
 // n + 16 (m / 4), element m % 4; FP64 -- D[m][n] in lane n + 16 (m % 4), element m / 4.
@@ -195,7 +204,7 @@ __global__ void __launch_bounds__(DNT, 2)
                      const uint8_t *__restrict__ acc_flags) {
     using V = typename std::conditional<MODE == 0, double, float>::type;
     using V4 = typename dvec<V>::type;
-    const int ti = blockIdx.x;
+    const int ti = xcd_order(blockIdx.x, gridDim.x);
     const int t = recv[ti];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kq = lane >> 4;
@@ -303,8 +312,9 @@ __global__ void __launch_bounds__(DNT, 2)
 // receivers' MFMAs.  Item = (operator, source of receiver 0, source of receiver 1), -1 where a
 // receiver has no partner with that operator (its MFMAs are skipped: the source index is
 // wave-uniform).  NW waves share the row tiles (wave, wave + NW, ...), each holding RTW row tiles x
-// 2 receivers of 16-column accumulators.  Workgroup x = pair x / nchunk, columns 16 (x % nchunk)
-// .. + 15: the chunks of one pair run on adjacent workgroups, so they share its operators in L2.
+// 2 receivers of 16-column accumulators.  Workgroup index i (in XCD order) = pair i / nchunk,
+// columns 16 (i % nchunk) .. + 15: the chunks of one pair run on one XCD and share its operators
+// in that XCD's L2.
 template <int MODE, int NW, int RTW>
 __global__ void __launch_bounds__(64 * NW, 1)
     dense_m2l_pair_kernel(const int32_t *__restrict__ recv2, const int64_t *__restrict__ item_ptr,
@@ -316,8 +326,9 @@ __global__ void __launch_bounds__(64 * NW, 1)
                           int nchunk) {
     using V = typename std::conditional<MODE == 0, double, float>::type;
     using V4 = typename dvec<V>::type;
-    const int ti = blockIdx.x / nchunk;
-    const int r0 = (blockIdx.x - ti * nchunk) * 16;
+    const int bi = xcd_order(blockIdx.x, gridDim.x);
+    const int ti = bi / nchunk;
+    const int r0 = (bi - ti * nchunk) * 16;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int n = lane & 15, kq = lane >> 4;
     const int G = KrT;
@@ -448,7 +459,7 @@ __global__ void __launch_bounds__(DNT, 2)
                           const bf16x8_d *__restrict__ frag, int64_t plane, int Kr, int KrT,
                           const float *__restrict__ mom, float *__restrict__ loc, int R,
                           const uint8_t *__restrict__ acc_flags) {
-    const int ti = blockIdx.x;
+    const int ti = xcd_order(blockIdx.x, gridDim.x);
     const int t = recv[ti];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kq = lane >> 4;
@@ -729,9 +740,9 @@ bool launch_dense_pair(const DenseOps &D, int orient, int rep, int want, const i
     const void *frag = F.ptr;
     if (!frag) return false;
     // measured on dense Laplace maps at N = 2M: p = 6 (1.6k operators of 216 x 216, 0.33 GB in
-    // FP32, 0.66 GB in FP64) coupling stage from 1 to 16 columns 530 -> 230-280 ms (FP32), 536-695
-    // -> 371-620 ms (FP64), at 32 columns 415 -> 546 ms; p = 4 (operators in the caches) 10-25 %
-    // slower in pairs
+    // FP32, 0.66 GB in FP64) coupling stage from 1 to 16 columns 513-520 -> 222-259 ms (FP32),
+    // 553-691 -> 366-610 ms (FP64), at 32 columns 391 -> 519 ms (FP32); p = 4 (operators in the
+    // caches) 10-25 % slower in pairs
     if (want == 0 && (R > 16 || (int64_t)F.bytes <= DENSE_PAIR_MIN_BYTES)) return false;
     const int nw = D.KrT <= 4 ? 4 : 8, rtw = (D.KrT + nw - 1) / nw;
     if (rtw > 4) return false;
