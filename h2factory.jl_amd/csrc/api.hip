@@ -551,8 +551,8 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
     P.m2l_op.upload_vec(iop);
     if (pair_sym) P.m2l_sym.upload_vec(isym);
     if (pair_fop) P.m2l_fop.upload_vec(ifop);
-    if (pair_fop) {
-        // receiver pairs for the dense pair kernel
+    if (pair_fop && c.dtype != H2G_C64) {
+        // receiver pairs for the dense pair kernel (real operators only)
         std::vector<int32_t> r2, s2, f2;
         std::vector<int64_t> p2;
         P.m2l2_base.assign(dst.n_levels, 0);
