@@ -369,16 +369,38 @@ __global__ void __launch_bounds__(TPB, 2)
         }
 }
 
+template <class T> struct vec4_of;
+template <> struct vec4_of<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct vec4_of<double> { typedef double type __attribute__((ext_vector_type(4))); };
+
+// p[k0 .. k0 + 3] as doubles, zero where !valid or beyond K: one vector load when K is a multiple of
+// 4 (k0 is then one too, and rows of K values keep 16-byte alignment), else clamped scalar loads
+template <class T>
+__device__ inline void load4_k(const T *__restrict__ p, int k0, int K, bool valid, double (&o)[4]) {
+    if ((K & 3) == 0) {
+        using V4 = typename vec4_of<T>::type;
+        const V4 v = *reinterpret_cast<const V4 *>(p + min(k0, K - 4));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) o[u] = (valid && k0 + u < K) ? (double)v[u] : 0.0;
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) o[u] = (double)p[min(k0 + u, K - 1)] * ((valid && k0 + u < K) ? 1.0 : 0.0);
+    }
+}
+
 // Leaf expansions on FP64 MFMA (real maps, R >= MFMA_MIN_R), FP64 accumulation.
 //   L2P: Y_leaf (n x R) += alpha C^T (n x K) L (K x R)      rows = points, contraction over k
 //   P2M: M (K x R)      = C (K x n) X (n x R)                rows = k, contraction over points
 // A k-step's 4 contraction indices are assigned so that each lane's operands over 4 consecutive
 // k-steps are 4 consecutive values (index 16 j + 4 k' + u for lane row k' and k-step 4 j + u):
 // the contraction order does not change the product, and every lane reads its operand rows
-// contiguously.  One workgroup (4 waves) per (leaf, group of NCT 16-column RHS tiles).
+// contiguously.  P2M: one workgroup per (leaf, group of NCT 16-column RHS tiles), its waves over
+// the row tiles of k.  L2P of large leaves (STAGE): the same, its waves over the leaf's 16-point
+// row tiles; L2P of small leaves: one wave per leaf (8 leaves per workgroup), since a leaf of at
+// most 16 points has one row tile and a workgroup per leaf would leave 7 of its 8 waves idle.
 template <class T, int NCT, bool STAGE>
 __global__ void __launch_bounds__(L2P_NT)
-    l2p_mfma_kernel(const int32_t *__restrict__ leaves, const int64_t *__restrict__ begin,
+    l2p_mfma_kernel(const int32_t *__restrict__ leaves, int n_leaves, const int64_t *__restrict__ begin,
                     const int64_t *__restrict__ end, const T *__restrict__ coll, int K, const T *__restrict__ loc,
                     int R, double *__restrict__ ys, double alpha) {
     // STAGE (leaves of many points): the leaf's expansions of this workgroup's columns, [col][k]
@@ -388,7 +410,10 @@ __global__ void __launch_bounds__(L2P_NT)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lt = reinterpret_cast<T *>(smem_raw);
     const int KP = K + 1;
-    const int b = leaves[blockIdx.x];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int li = STAGE ? (int)blockIdx.x : (int)blockIdx.x * (L2P_NT / 64) + wave;
+    if (!STAGE && li >= n_leaves) return;  // no barrier below without STAGE
+    const int b = leaves[li];
     const int64_t j0 = begin[b];
     const int n = (int)(end[b] - j0);
     const int c0 = blockIdx.y * NCT * 16;
@@ -403,9 +428,9 @@ __global__ void __launch_bounds__(L2P_NT)
     }
     const T *lsrc = STAGE ? lt : lb;
     const int pitch = STAGE ? KP : K;
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const int m = lane & 15, kq = lane >> 4;
-    for (int rt = wave; rt * 16 < n; rt += (int)(blockDim.x >> 6)) {
+    for (int rt = STAGE ? wave : 0; rt * 16 < n; rt += STAGE ? (int)(blockDim.x >> 6) : 1) {
         const int row = rt * 16 + m;
         const T *crow = coll + (j0 + min(row, n - 1)) * K;
         double4_t acc[NCT];
@@ -414,18 +439,21 @@ __global__ void __launch_bounds__(L2P_NT)
         for (int jb = 0; jb < K; jb += 16) {  // uniform trip count
             const int kb = jb + 4 * kq;
             double a[4], bv[NCT][4];
-            // unconditional loads from clamped (valid) addresses, out-of-range operands multiplied
-            // by zero (no branches in the loop, so the accumulators stay in the MFMA registers)
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                a[u] = (double)crow[min(kb + u, K - 1)] * ((row < n && kb + u < K) ? 1.0 : 0.0);
+            // loads from clamped (valid) addresses, out-of-range operands zero (no divergent
+            // branches in the loop, so the accumulators stay in the MFMA registers); the global
+            // rows go as one vector load per lane and k-step group
+            load4_k(crow, kb, K, row < n, a);
 #pragma unroll
             for (int g = 0; g < NCT; ++g) {
                 const int cl = min(16 * g + m, ncol - 1);
                 const T *lc = lsrc + cl * pitch;
+                if (STAGE) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    bv[g][u] = (double)lc[min(kb + u, K - 1)] * ((16 * g + m < ncol && kb + u < K) ? 1.0 : 0.0);
+                    for (int u = 0; u < 4; ++u)
+                        bv[g][u] = (double)lc[min(kb + u, K - 1)] * ((16 * g + m < ncol && kb + u < K) ? 1.0 : 0.0);
+                } else {
+                    load4_k(lc, kb, K, 16 * g + m < ncol, bv[g]);
+                }
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -592,11 +620,13 @@ bool launch_l2p_mfma(int cdtype, const int32_t *leaves, int n, const int64_t *be
     const size_t es = cdtype == H2G_F64 ? 8 : 4;
     const int nct = std::min({4, (R + 15) / 16, stage ? (int)(65536 / (es * 16 * (size_t)(K + 1))) : 4});
     if (nct < 1) return false;
-    const dim3 grid((unsigned)n, (unsigned)((R + 16 * nct - 1) / (16 * nct)));
+    const int per_wg = stage ? 1 : L2P_NT / 64;  // leaves per workgroup
+    const dim3 grid((unsigned)((n + per_wg - 1) / per_wg), (unsigned)((R + 16 * nct - 1) / (16 * nct)));
     auto go = [&](auto kern, auto tv) {
         using T = decltype(tv);
         const size_t sh = stage ? sizeof(T) * (size_t)nct * 16 * (K + 1) : 0;
-        kern<<<grid, L2P_NT, sh, s>>>(leaves, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys, alpha);
+        kern<<<grid, L2P_NT, sh, s>>>(leaves, n, begin, end, (const T *)coll, K, (const T *)loc, R, (double *)ys,
+                                      alpha);
     };
     const bool f64 = cdtype == H2G_F64;
     switch (nct) {
