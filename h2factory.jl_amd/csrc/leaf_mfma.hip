@@ -420,9 +420,20 @@ __global__ void __launch_bounds__(L2P_NT)
     const int ncol = min(NCT * 16, R - c0);
     const T *lb = loc + ((int64_t)b * R + c0) * K;
     if (STAGE) {
-        for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
-            const int c = e / K, k = e - c * K;
-            lt[c * KP + k] = lb[e];
+        if ((K & 3) == 0) {
+            // one 16-byte (FP32) or 32-byte (FP64) load per thread: 4 values of one column
+            using V4 = typename vec4_of<T>::type;
+            for (int e = 4 * (int)threadIdx.x; e < ncol * K; e += 4 * (int)blockDim.x) {
+                const int c = e / K, k = e - c * K;
+                const V4 v = *reinterpret_cast<const V4 *>(lb + e);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) lt[c * KP + k + u] = v[u];
+            }
+        } else {
+            for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
+                const int c = e / K, k = e - c * K;
+                lt[c * KP + k] = lb[e];
+            }
         }
         __syncthreads();
     }
@@ -530,9 +541,21 @@ __global__ void __launch_bounds__(P2M_NT)
     }
     __syncthreads();
     T *mb = mom + ((int64_t)b * R + c0) * K;
-    for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
-        const int c = e / K, k = e - c * K;
-        mb[e] = mt[c * KP + k];
+    if ((K & 3) == 0) {
+        // one 16-byte (FP32) or 32-byte (FP64) store per thread: 4 values of one column
+        using V4 = typename vec4_of<T>::type;
+        for (int e = 4 * (int)threadIdx.x; e < ncol * K; e += 4 * (int)blockDim.x) {
+            const int c = e / K, k = e - c * K;
+            V4 v;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = mt[c * KP + k + u];
+            *reinterpret_cast<V4 *>(mb + e) = v;
+        }
+    } else {
+        for (int e = threadIdx.x; e < ncol * K; e += (int)blockDim.x) {
+            const int c = e / K, k = e - c * K;
+            mb[e] = mt[c * KP + k];
+        }
     }
 }
 
