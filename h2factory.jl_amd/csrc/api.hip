@@ -585,6 +585,7 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
                     at.emplace(ckey(dst.coord[3 * (size_t)b], dst.coord[3 * (size_t)b + 1], dst.coord[3 * (size_t)b + 2]), i);
                 }
             static const int nb[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+            int nxt = 0;  // the first free receiver after i (free ones only become used: monotone)
             for (int i = 0; i < n; ++i) {
                 if (used[i]) continue;
                 used[i] = 1;
@@ -599,12 +600,11 @@ void build_orient(Core &c, OrientPlan &P, const Side &src, const Side &dst, cons
                         }
                     }
                 }
-                if (mate[i] < 0)
-                    for (int j = i + 1; j < n; ++j)
-                        if (!used[j]) {
-                            mate[i] = j;
-                            break;
-                        }
+                if (mate[i] < 0) {
+                    nxt = std::max(nxt, i + 1);
+                    while (nxt < n && used[nxt]) ++nxt;
+                    if (nxt < n) mate[i] = nxt;
+                }
                 if (mate[i] >= 0) used[mate[i]] = 1;
                 items_of(l, i, ea);
                 if (mate[i] >= 0) items_of(l, mate[i], eb);
